@@ -1,0 +1,571 @@
+// vm_api.cpp -- the C ABI (include/voromesh_hip.h): the plain optimizer handle
+// with the reference's slot semantics, the Poisson sampler, and batches.
+#include "voromesh_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "vm_engine.h"
+
+using vmrt::DBuf;
+using vmrt::VmError;
+
+namespace {
+
+#define VM_REQUIRE(cond, msg)                                                                                      \
+    do {                                                                                                           \
+        if (!(cond)) throw VmError(VM_EINVAL, msg);                                                                \
+    } while (0)
+
+inline bool liveX(float x) { return x >= 0.0f; } // NaN and negative x mark an empty slot
+
+hipStream_t makeStream()
+{
+    hipStream_t s = nullptr;
+    VM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+// Boundary mesh from per-slot clipped cells (voro_optimizer.cu:798-854): every
+// edge of an inside cell whose other side is an outside cell or the box;
+// vertices are identified by the sorted generator triple they are
+// equidistant to (box sides are -1..-4), so the mesh is watertight by
+// construction.  Keys are three 32-bit fields, so slot counts are not capped.
+struct TripleKey {
+    uint32_t a, b, c;
+    bool     operator==(const TripleKey &o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct TripleHash {
+    size_t operator()(const TripleKey &k) const
+    {
+        uint64_t h = uint64_t(k.a) * 0x9E3779B97F4A7C15ull;
+        h ^= (uint64_t(k.b) + 0x632BE59BD9B4E019ull + (h << 6) + (h >> 2));
+        h ^= (uint64_t(k.c) + 0x85EBCA77C2B2AE63ull + (h << 6) + (h >> 2));
+        return size_t(h);
+    }
+};
+
+TripleKey tripleKey(int a, int b, int c)
+{
+    uint32_t v[3] = {uint32_t(a + vmk::kBoxLabels), uint32_t(b + vmk::kBoxLabels), uint32_t(c + vmk::kBoxLabels)};
+    if (v[0] > v[1]) std::swap(v[0], v[1]);
+    if (v[1] > v[2]) std::swap(v[1], v[2]);
+    if (v[0] > v[1]) std::swap(v[0], v[1]);
+    return TripleKey{v[0], v[1], v[2]};
+}
+
+} // namespace
+
+// ===========================================================================
+// Plain optimizer (voromesh::VoroOptimizer)
+// ===========================================================================
+struct vm_handle {
+    float                          W = 1, H = 1, spacing = 1;
+    vm_params                      prm{};
+    hipStream_t                    stream = nullptr;
+    std::unique_ptr<vmrt::Engine>  eng;
+    int                            slotsW = 0, slotsH = 0;
+    float                          slotCellSize = 0;
+    std::vector<vm_vec2>           slotPos;        // generator slots (host mirror)
+    std::vector<vm_vec2>           slotM, slotV;   // Adam moments per slot
+    std::vector<int>               slotOfSite, siteOfSlot;
+    std::vector<vm_vec2>           targetSlots;
+    int                            targetCapacity = 0;
+    std::vector<uint64_t>          masks;
+    DBuf<uint64_t>                 dMasks;
+    int                            tilesX = 0, tilesY = 0;
+    float                          voxel = 0;
+    int                            iteration = 0, scheduleLength = 0;
+    float                          lastBatchLoss = 0;
+    int                            rect[4] = {0, 0, 0, 0};
+    bool                           haveGenerators = false;
+    // Extract() storage
+    std::vector<vm_vec2> mGen, mVerts, mVertices;
+    std::vector<uint8_t> mInside;
+    std::vector<int32_t> mStart, mLabels, mEdges;
+
+    ~vm_handle()
+    {
+        eng.reset();
+        dMasks.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    vmrt::InstSpec &spec() { return eng->specs[0]; }
+    int             numSlots() const { return slotsW * slotsH; }
+
+    // Device -> slot mirrors.
+    void pullSlots()
+    {
+        if (!eng->deviceNewer) return;
+        eng->pull();
+        const vmrt::InstSpec &s = spec();
+        for (size_t i = 0; i < slotOfSite.size(); ++i) {
+            int sl       = slotOfSite[i];
+            slotPos[sl]  = vm_vec2{float(s.pos[2 * i]), float(s.pos[2 * i + 1])};
+            if (!s.m.empty()) slotM[sl] = vm_vec2{float(s.m[2 * i]), float(s.m[2 * i + 1])};
+            if (!s.v.empty()) slotV[sl] = vm_vec2{float(s.v[2 * i]), float(s.v[2 * i + 1])};
+        }
+    }
+
+    // Slot mirrors -> compact live sites of the instance.
+    void compactSites()
+    {
+        vmrt::InstSpec &s = spec();
+        slotOfSite.clear();
+        siteOfSlot.assign(numSlots(), -1);
+        s.pos.clear(), s.m.clear(), s.v.clear();
+        for (int sl = 0; sl < numSlots(); ++sl) {
+            if (!liveX(slotPos[sl].x)) continue;
+            siteOfSlot[sl] = int(slotOfSite.size());
+            slotOfSite.push_back(sl);
+            s.pos.push_back(slotPos[sl].x), s.pos.push_back(slotPos[sl].y);
+            s.m.push_back(slotM[sl].x), s.m.push_back(slotM[sl].y);
+            s.v.push_back(slotV[sl].x), s.v.push_back(slotV[sl].y);
+        }
+        if (prm.grid_cell <= 0.0f || prm.stage_rings <= 0) {
+            float cell;
+            int   rs;
+            vmrt::tunedChoice(std::max(int(slotOfSite.size()), 1), W, H, s.K, &cell, &rs);
+            s.cell   = prm.grid_cell > 0.0f ? prm.grid_cell : cell;
+            s.stageR = prm.stage_rings > 0 ? prm.stage_rings : rs;
+        }
+        eng->dirty = true;
+    }
+
+    void compactTargets()
+    {
+        vmrt::InstSpec &s = spec();
+        s.tgt.clear(), s.tslot.clear();
+        for (int t = 0; t < int(targetSlots.size()); ++t) {
+            if (!liveX(targetSlots[t].x)) continue;
+            s.tgt.push_back(targetSlots[t].x), s.tgt.push_back(targetSlots[t].y);
+            s.tslot.push_back(t);
+        }
+        eng->dirty = true;
+    }
+
+    void beginSchedule(int iterations, int cx0, int cy0, int cx1, int cy1)
+    {
+        cx0 = std::max(cx0, 0), cy0 = std::max(cy0, 0);
+        cx1 = std::min(cx1, slotsW), cy1 = std::min(cy1, slotsH);
+        pullSlots();
+        vmrt::InstSpec &s = spec();
+        s.lr    = double(prm.lr_per_spacing * spacing);
+        s.gamma = prm.gamma, s.beta1 = 0.9, s.beta2 = 0.999, s.eps = 1e-8;
+        for (int k = 0; k < 4; ++k)
+            s.milestones[k] = prm.iterations > 0
+                                  ? int(std::lround(double(prm.milestones[k]) * iterations / prm.iterations))
+                                  : prm.milestones[k];
+        float margin = 2.0f * spacing;
+        s.useBox     = 1;
+        s.box[0] = float(cx0) * slotCellSize - margin, s.box[1] = float(cy0) * slotCellSize - margin;
+        s.box[2] = float(cx1) * slotCellSize + margin, s.box[3] = float(cy1) * slotCellSize + margin;
+        rect[0] = cx0, rect[1] = cy0, rect[2] = cx1, rect[3] = cy1;
+        bool all = cx0 <= 0 && cy0 <= 0 && cx1 >= slotsW && cy1 >= slotsH;
+        if (cx0 < cx1 && cy0 < cy1)
+            for (int y = cy0; y < cy1; ++y)
+                for (int x = cx0; x < cx1; ++x) slotM[y * slotsW + x] = vm_vec2{0, 0}, slotV[y * slotsW + x] = vm_vec2{0, 0};
+        compactSites();
+        s.frozen.clear();
+        if (!all) {
+            s.frozen.assign(slotOfSite.size(), 1);
+            for (size_t i = 0; i < slotOfSite.size(); ++i) {
+                int sl = slotOfSite[i], x = sl % slotsW, y = sl / slotsW;
+                if (x >= cx0 && x < cx1 && y >= cy0 && y < cy1) s.frozen[i] = 0;
+            }
+        }
+        s.state        = vmk::InstState{};
+        scheduleLength = iterations;
+        iteration      = 0;
+        lastBatchLoss  = 0.0f;
+        eng->dirty     = true;
+    }
+};
+
+extern "C" {
+
+void vm_default_params(vm_params *p)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->iterations       = 400;
+    p->local_iterations = 100;
+    p->lr_per_spacing   = 0.09f;
+    p->sample_fraction  = 0.2f;
+    p->gamma            = 0.5f;
+    p->milestones[0] = 80, p->milestones[1] = 150, p->milestones[2] = 200, p->milestones[3] = 250;
+    p->seed        = 1;
+    p->loss_k      = 12;
+    p->grid_cell   = 0.0f;
+    p->stage_rings = 0;
+    p->precision   = 0;
+}
+
+const char *vm_last_error(void) { return vmrt::lastError(); }
+const char *vm_version(void) { return "voromesh-hip 0.1 (gfx950)"; }
+
+int vm_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int vm_create(float width, float height, float spacing, const vm_params *params, vm_handle **out)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(out, "out is null");
+        VM_REQUIRE(width > 0 && height > 0 && spacing > 0, "width, height and spacing must be positive");
+        auto h = std::make_unique<vm_handle>();
+        h->W = width, h->H = height, h->spacing = spacing;
+        if (params) h->prm = *params;
+        else vm_default_params(&h->prm);
+        VM_REQUIRE(vmk::lossKSupported(h->prm.loss_k), "unsupported loss_k");
+        h->stream = makeStream();
+        h->eng    = vmrt::Engine::create(h->prm.precision, h->stream);
+        vmrt::InstSpec s;
+        s.W = width, s.H = height;
+        s.K        = h->prm.loss_k;
+        s.fraction = h->prm.sample_fraction;
+        s.seed     = h->prm.seed;
+        float cell;
+        int   rs;
+        vmrt::tunedChoice(1, width, height, s.K, &cell, &rs);
+        s.cell   = h->prm.grid_cell > 0.0f ? h->prm.grid_cell : 2.0f * spacing;
+        s.stageR = h->prm.stage_rings > 0 ? h->prm.stage_rings : rs;
+        h->eng->add(std::move(s));
+        *out = h.release();
+    });
+}
+
+int vm_destroy(vm_handle *h)
+{
+    return vmrt::guardedCall([&] { delete h; });
+}
+
+int vm_set_targets(vm_handle *h, const vm_vec2 *slots, int n)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h && n >= 0 && (slots || n == 0), "invalid arguments");
+        h->pullSlots();
+        h->targetCapacity = std::max(n, 1);
+        h->targetSlots.assign(slots, slots + n);
+        h->compactTargets();
+    });
+}
+
+int vm_update_targets(vm_handle *h, const int32_t *dirty, int numDirty, const vm_vec2 *slots, int numSlots)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h && numDirty >= 0 && numSlots >= 0, "invalid arguments");
+        if (numDirty == 0) return;
+        VM_REQUIRE(dirty && slots, "null arrays");
+        h->pullSlots();
+        if (numSlots > h->targetCapacity) {
+            // Grow with slack and re-upload everything (voro_optimizer.cu:566-571).
+            size_t cap = std::max(size_t(numSlots) * 3 / 2, size_t(numSlots) + 64);
+            h->targetSlots.assign(slots, slots + numSlots);
+            h->targetSlots.resize(cap, vm_vec2{-1.0f, -1.0f});
+            h->targetCapacity = int(cap);
+        } else {
+            if (int(h->targetSlots.size()) < h->targetCapacity)
+                h->targetSlots.resize(h->targetCapacity, vm_vec2{NAN, NAN});
+            for (int k = 0; k < numDirty; ++k) {
+                int d = dirty[k];
+                VM_REQUIRE(d >= 0 && d < numSlots && d < h->targetCapacity, "dirty slot out of range");
+                h->targetSlots[d] = slots[d];
+            }
+        }
+        h->compactTargets();
+    });
+}
+
+int vm_set_occupancy(vm_handle *h, const uint64_t *masks, int tilesX, int tilesY, float voxelSize)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h && tilesX >= 0 && tilesY >= 0 && voxelSize > 0, "invalid arguments");
+        size_t n = size_t(tilesX) * tilesY;
+        VM_REQUIRE(masks || n == 0, "null masks");
+        h->masks.assign(masks, masks + n);
+        h->tilesX = tilesX, h->tilesY = tilesY, h->voxel = voxelSize;
+        h->dMasks.reserve(n);
+        if (n) VM_HIP(hipMemcpy(h->dMasks.p, masks, n * 8, hipMemcpyHostToDevice));
+    });
+}
+
+int vm_update_occupancy(vm_handle *h, const uint64_t *masks, int tx0, int ty0, int tx1, int ty1)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h, "null handle");
+        tx0 = std::max(tx0, 0), ty0 = std::max(ty0, 0);
+        tx1 = std::min(tx1, h->tilesX), ty1 = std::min(ty1, h->tilesY);
+        if (tx0 >= tx1 || ty0 >= ty1) return;
+        VM_REQUIRE(masks, "null masks");
+        for (int y = ty0; y < ty1; ++y)
+            for (int x = tx0; x < tx1; ++x) h->masks[size_t(y) * h->tilesX + x] = masks[size_t(y) * h->tilesX + x];
+        size_t pitch = size_t(h->tilesX) * 8, off = size_t(ty0) * h->tilesX + tx0;
+        VM_HIP(hipMemcpy2D(h->dMasks.p + off, pitch, h->masks.data() + off, pitch, size_t(tx1 - tx0) * 8, ty1 - ty0,
+                           hipMemcpyHostToDevice));
+    });
+}
+
+int vm_set_generators(vm_handle *h, const vm_vec2 *slots, int slotsW, int slotsH, float slotCellSize)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h && slotsW >= 0 && slotsH >= 0, "invalid arguments");
+        size_t n = size_t(slotsW) * slotsH;
+        VM_REQUIRE(slots || n == 0, "null slots");
+        h->slotsW = slotsW, h->slotsH = slotsH, h->slotCellSize = slotCellSize;
+        h->slotPos.assign(slots, slots + n);
+        h->slotM.assign(n, vm_vec2{0, 0});
+        h->slotV.assign(n, vm_vec2{0, 0});
+        h->eng->deviceNewer = false;
+        h->haveGenerators   = true;
+        h->beginSchedule(h->prm.iterations, 0, 0, slotsW, slotsH);
+    });
+}
+
+int vm_update_generators(vm_handle *h, const vm_vec2 *slots, int cx0, int cy0, int cx1, int cy1)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h, "null handle");
+        cx0 = std::max(cx0, 0), cy0 = std::max(cy0, 0);
+        cx1 = std::min(cx1, h->slotsW), cy1 = std::min(cy1, h->slotsH);
+        if (cx0 >= cx1 || cy0 >= cy1) return;
+        VM_REQUIRE(slots, "null slots");
+        h->pullSlots();
+        for (int y = cy0; y < cy1; ++y)
+            for (int x = cx0; x < cx1; ++x) h->slotPos[size_t(y) * h->slotsW + x] = slots[size_t(y) * h->slotsW + x];
+        h->compactSites();
+        // Frozen mask follows the current schedule rectangle.
+        vmrt::InstSpec &s = h->spec();
+        bool all = h->rect[0] <= 0 && h->rect[1] <= 0 && h->rect[2] >= h->slotsW && h->rect[3] >= h->slotsH;
+        s.frozen.clear();
+        if (!all) {
+            s.frozen.assign(h->slotOfSite.size(), 1);
+            for (size_t i = 0; i < h->slotOfSite.size(); ++i) {
+                int sl = h->slotOfSite[i], x = sl % h->slotsW, y = sl / h->slotsW;
+                if (x >= h->rect[0] && x < h->rect[2] && y >= h->rect[1] && y < h->rect[3]) s.frozen[i] = 0;
+            }
+        }
+    });
+}
+
+int vm_begin_schedule(vm_handle *h, int iterations, int cx0, int cy0, int cx1, int cy1)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h, "null handle");
+        h->beginSchedule(iterations, cx0, cy0, cx1, cy1);
+    });
+}
+
+int vm_step(vm_handle *h, int iterations, float *gpuMs)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h, "null handle");
+        float ms = 0.0f;
+        if (iterations > 0 && h->numSlots() > 0 && h->targetCapacity > 0) {
+            ms = h->eng->step(iterations);
+            h->eng->readStates();
+            const vmk::InstState &st = h->spec().state;
+            h->iteration     = st.iter;
+            h->lastBatchLoss = st.count > 0 ? float(st.loss / double(st.count)) : 0.0f;
+        }
+        if (gpuMs) *gpuMs = ms;
+    });
+}
+
+int vm_get_state(vm_handle *h, int *iteration, int *scheduleLength, float *lastBatchLoss, int *numSlots)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h, "null handle");
+        if (iteration) *iteration = h->iteration;
+        if (scheduleLength) *scheduleLength = h->scheduleLength;
+        if (lastBatchLoss) *lastBatchLoss = h->lastBatchLoss;
+        if (numSlots) *numSlots = h->numSlots();
+    });
+}
+
+int vm_evaluate_loss(vm_handle *h, double *mean)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h && mean, "invalid arguments");
+        *mean = 0.0;
+        if (h->numSlots() == 0 || h->targetCapacity == 0) return;
+        int64_t cnt = 0;
+        double  sum = h->eng->evaluate(0, &cnt);
+        *mean       = cnt > 0 ? sum / double(cnt) : 0.0;
+    });
+}
+
+int vm_get_generators(vm_handle *h, vm_vec2 *slots, int n)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h && slots && n == h->numSlots(), "n must equal the number of slots");
+        h->pullSlots();
+        std::copy(h->slotPos.begin(), h->slotPos.end(), slots);
+    });
+}
+
+int vm_extract(vm_handle *h, vm_mesh *out)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h && out, "invalid arguments");
+        std::memset(out, 0, sizeof(*out));
+        const int n = h->numSlots();
+        h->mGen.clear(), h->mVerts.clear(), h->mVertices.clear(), h->mInside.clear();
+        h->mStart.assign(1, 0), h->mLabels.clear(), h->mEdges.clear();
+        if (n == 0) {
+            out->cell_start = h->mStart.data();
+            return;
+        }
+        vmrt::ClipOut c;
+        h->eng->clip(0, h->masks.empty() ? nullptr : h->dMasks.p, h->tilesX, h->tilesY, h->voxel, c);
+        auto t0 = std::chrono::steady_clock::now();
+        h->pullSlots();
+        h->mGen = h->slotPos;
+        h->mInside.assign(n, 0);
+        std::vector<int> cnt(n, 0);
+        for (size_t i = 0; i < h->slotOfSite.size(); ++i) {
+            int sl          = h->slotOfSite[i];
+            cnt[sl]         = c.count[i];
+            h->mInside[sl]  = c.inside[i];
+        }
+        h->mStart.assign(n + 1, 0);
+        for (int s = 0; s < n; ++s) h->mStart[s + 1] = h->mStart[s] + cnt[s];
+        h->mVerts.resize(h->mStart[n]);
+        h->mLabels.resize(h->mStart[n]);
+        for (size_t i = 0; i < h->slotOfSite.size(); ++i) {
+            int sl = h->slotOfSite[i], o = h->mStart[sl];
+            for (int k = 0; k < c.count[i]; ++k) {
+                size_t src       = size_t(i) * vmk::kMaxCellVerts + k;
+                h->mVerts[o + k] = vm_vec2{c.verts[2 * src], c.verts[2 * src + 1]};
+                int l            = c.labels[src];
+                h->mLabels[o + k] = l >= 0 ? h->slotOfSite[l] : l;
+            }
+        }
+        // Boundary assembly in slot space.
+        std::unordered_map<TripleKey, int, TripleHash> vertexOf;
+        auto vertex = [&](TripleKey k, vm_vec2 p) {
+            auto it = vertexOf.find(k);
+            if (it != vertexOf.end()) return it->second;
+            int id = int(h->mVertices.size());
+            vertexOf.emplace(k, id);
+            h->mVertices.push_back(p);
+            return id;
+        };
+        for (int i = 0; i < n; ++i) {
+            if (!h->mInside[i]) continue;
+            int            cc  = cnt[i];
+            const int32_t *lab = h->mLabels.data() + h->mStart[i];
+            const vm_vec2 *pos = h->mVerts.data() + h->mStart[i];
+            for (int k = 0; k < cc; ++k) {
+                int l = lab[k];
+                if (l >= 0 && h->mInside[l]) continue;
+                int prev = lab[(k + cc - 1) % cc], next = lab[(k + 1) % cc];
+                int a = vertex(tripleKey(i, prev, l), pos[k]);
+                int b = vertex(tripleKey(i, l, next), pos[(k + 1) % cc]);
+                h->mEdges.push_back(a), h->mEdges.push_back(b);
+            }
+        }
+        const int        V = int(h->mVertices.size()), E = int(h->mEdges.size() / 2);
+        std::vector<int> nxt(V, -1), inDeg(V, 0);
+        bool             wt = true;
+        for (int e = 0; e < E; ++e) {
+            int a = h->mEdges[2 * e], b = h->mEdges[2 * e + 1];
+            if (nxt[a] != -1) wt = false;
+            nxt[a] = b;
+            ++inDeg[b];
+        }
+        for (int v = 0; v < V; ++v)
+            if (nxt[v] == -1 || inDeg[v] != 1) wt = false;
+        int loops = 0;
+        if (wt) {
+            std::vector<uint8_t> seen(V, 0);
+            for (int v = 0; v < V; ++v) {
+                if (seen[v]) continue;
+                ++loops;
+                for (int w = v; !seen[w]; w = nxt[w]) seen[w] = 1;
+            }
+        }
+        out->num_slots      = n;
+        out->generators     = h->mGen.data();
+        out->inside         = h->mInside.data();
+        out->cell_start     = h->mStart.data();
+        out->cell_verts     = h->mVerts.data();
+        out->cell_labels    = h->mLabels.data();
+        out->num_vertices   = V;
+        out->vertices       = h->mVertices.data();
+        out->num_edges      = E;
+        out->edges          = h->mEdges.data();
+        out->watertight     = wt ? 1 : 0;
+        out->loops          = loops;
+        out->unsafe_cells   = c.unsafe;
+        out->overflow_cells = c.overflow;
+        out->gpu_ms         = c.gpuMs;
+        out->host_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+int vm_knn(vm_handle *h, const vm_vec2 *queries, int nq, int k, int32_t *idx, float *d2)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h && nq >= 0 && (queries || nq == 0) && idx && d2, "invalid arguments");
+        VM_REQUIRE(k >= 1 && k <= 32, "k must be in [1, 32]");
+        std::vector<double> q(2 * size_t(nq)), dd(size_t(nq) * k);
+        for (int i = 0; i < nq; ++i) q[2 * i] = queries[i].x, q[2 * i + 1] = queries[i].y;
+        h->eng->knnQuery(0, q.data(), nq, k, idx, dd.data());
+        for (size_t i = 0; i < dd.size(); ++i) {
+            d2[i]  = float(dd[i]);
+            idx[i] = idx[i] >= 0 ? h->slotOfSite[idx[i]] : -1;
+        }
+    });
+}
+
+int vm_loss_pass(vm_handle *h, double *gradXY, double *lossSum, int64_t *count, int32_t *knn, int32_t *nearest,
+                 int32_t *partner, double *lossTerms)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h, "null handle");
+        bool         wantDbg = knn || nearest || partner || lossTerms;
+        vmrt::DiagOut d;
+        h->eng->diag(0, gradXY != nullptr, wantDbg, d);
+        if (lossSum) *lossSum = d.lossSum;
+        if (count) *count = d.count;
+        if (gradXY) {
+            std::fill(gradXY, gradXY + 2 * size_t(h->numSlots()), 0.0);
+            for (size_t i = 0; i < h->slotOfSite.size(); ++i) {
+                gradXY[2 * size_t(h->slotOfSite[i])]     = d.grad[2 * i];
+                gradXY[2 * size_t(h->slotOfSite[i]) + 1] = d.grad[2 * i + 1];
+            }
+        }
+        if (wantDbg) {
+            const vmrt::InstSpec &s = h->spec();
+            const int K = s.K, nT = int(h->targetSlots.size());
+            if (knn) std::fill(knn, knn + size_t(nT) * K, -1);
+            if (nearest) std::fill(nearest, nearest + nT, -1);
+            if (partner) std::fill(partner, partner + nT, -1);
+            if (lossTerms) std::fill(lossTerms, lossTerms + nT, 0.0);
+            auto toSlot = [&](int i) { return i >= 0 ? h->slotOfSite[i] : -1; };
+            for (size_t t = 0; t < s.tslot.size(); ++t) {
+                int ts = s.tslot[t];
+                if (knn)
+                    for (int k = 0; k < K; ++k) knn[size_t(ts) * K + k] = toSlot(d.knn[t * K + k]);
+                if (nearest) nearest[ts] = toSlot(d.terms[t].i0);
+                if (partner) partner[ts] = toSlot(d.terms[t].j);
+                if (lossTerms) lossTerms[ts] = d.terms[t].loss;
+            }
+        }
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,514 @@
+// vm_engine.cpp -- pooled layout, work tables and launch sequencing for a batch
+// of VoroMesh instances (see vm_engine.h).
+#include "vm_engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+namespace vmrt {
+
+void hipCheck(hipError_t e, const char *what, const char *file, int line)
+{
+    if (e == hipSuccess) return;
+    throw VmError(e == hipErrorOutOfMemory ? 3 : 2, std::string(what) + ": " + hipGetErrorString(e) + " (" + file +
+                                                        ":" + std::to_string(line) + ")");
+}
+
+namespace {
+thread_local std::string t_lastError;
+}
+
+int guardedCall(const std::function<void()> &f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const VmError &e) {
+        t_lastError = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        t_lastError = "host allocation failed";
+        return 3;
+    } catch (const std::exception &e) {
+        t_lastError = e.what();
+        return 1;
+    }
+}
+
+const char *lastError() { return t_lastError.c_str(); }
+
+void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR)
+{
+    // Mean site spacing a = sqrt(area / S).  A cell of 2a holds ~4 sites; the
+    // K-NN radius is ~a*sqrt(K/pi), so two staged rings cover it for K <= 32.
+    double a = std::sqrt(double(W) * double(H) / double(std::max(numSites, 1)));
+    double f = 2.0;
+    *cell    = float(f * a);
+    *stageR  = 2;
+    (void)K;
+}
+
+namespace {
+
+inline int cellCoordF(float v, float inv, int n)
+{
+    float f = v * inv;
+    int   c = (f >= 0.0f) ? (f < 2147483520.0f ? int(f) : n - 1) : 0;
+    return std::min(std::max(c, 0), n - 1);
+}
+
+inline int64_t align64(int64_t x) { return (x + 63) & ~int64_t(63); }
+
+constexpr int kMaxLossGrid = 16384;
+constexpr int kMaxSiteGrid = 8192;
+constexpr int kMaxScanGrid = 4096;
+
+template <class R> class EngineT final : public Engine {
+    using V = vmk::V2<R>;
+
+public:
+    explicit EngineT(hipStream_t s) { stream = s; }
+    ~EngineT() override { destroyGraph(); }
+
+    // ---- layout ---------------------------------------------------------------
+    struct Layout {
+        std::vector<vmk::Inst>       inst;
+        std::vector<vmk::SiteWork>   siteWork;
+        std::vector<vmk::TargetWork> targetWork; // grouped by K
+        struct Group { int K, begin, count; };
+        std::vector<Group>            groups;
+        std::vector<int>              instList;
+        std::vector<std::vector<int>> perm; // per instance: sorted position -> input target index
+        int64_t sites = 0, cells = 0, targets = 0;
+    } L;
+
+    DBuf<V>              dPos, dM, dV, dGrad, dSpos, dTpos;
+    DBuf<int>            dCellOf, dRank, dSidx, dCounts, dStart, dTslot, dInstList, dCnt;
+    DBuf<uint8_t>        dFrozen;
+    DBuf<vmk::Inst>      dInst;
+    DBuf<vmk::InstState> dState;
+    DBuf<vmk::SiteWork>  dSiteWork;
+    DBuf<vmk::TargetWork> dTargetWork;
+    hipGraphExec_t       graph = nullptr;
+    hipEvent_t           ev[8] = {};
+    int                  gridScan = 1, gridSite = 1;
+    std::vector<int>     gridLoss;
+
+    void destroyGraph()
+    {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e), e = nullptr;
+    }
+
+    void ensureEvents()
+    {
+        for (auto &e : ev)
+            if (!e) VM_HIP(hipEventCreate(&e));
+    }
+
+    size_t deviceBytes() const override
+    {
+        size_t b = 0;
+        b += (dPos.n + dM.n + dV.n + dGrad.n + dSpos.n + dTpos.n) * sizeof(V);
+        b += (dCellOf.n + dRank.n + dSidx.n + dCounts.n + dStart.n + dTslot.n + dInstList.n + dCnt.n) * sizeof(int);
+        b += dFrozen.n + dInst.n * sizeof(vmk::Inst) + dState.n * sizeof(vmk::InstState);
+        b += dSiteWork.n * sizeof(vmk::SiteWork) + dTargetWork.n * sizeof(vmk::TargetWork);
+        return b;
+    }
+
+    void buildLayout()
+    {
+        L = Layout();
+        const int n = int(specs.size());
+        L.inst.resize(n);
+        L.perm.resize(n);
+        std::vector<std::vector<vmk::TargetWork>> twByInst(n);
+        for (int i = 0; i < n; ++i) {
+            InstSpec &s  = specs[i];
+            vmk::Inst  &in = L.inst[i];
+            std::memset(&in, 0, sizeof(in));
+            const int S = int(s.pos.size() / 2), T = int(s.tgt.size() / 2);
+            if (S >= vmk::kMaxSitesPerInst) throw VmError(1, "instance has too many sites");
+            if (!vmk::lossKSupported(s.K)) throw VmError(1, "unsupported loss_k " + std::to_string(s.K));
+            if (!(s.cell > 0.0f)) throw VmError(1, "grid cell must be positive");
+            in.W = s.W, in.H = s.H;
+            in.cell = s.cell, in.inv = 1.0f / s.cell, in.ext = std::max(s.W, s.H);
+            double gwd = std::ceil(double(s.W) / s.cell), ghd = std::ceil(double(s.H) / s.cell);
+            if (gwd * ghd > double(1 << 28)) throw VmError(1, "hash grid too large");
+            in.gw = std::max(1, int(gwd)), in.gh = std::max(1, int(ghd));
+            in.S = S, in.T = T, in.K = s.K;
+            in.stageR     = std::min(std::max(s.stageR, 0), 2);
+            in.allMovable = s.frozen.empty() ? 1 : 0;
+            in.siteOff = L.sites, in.cellOff = L.cells, in.targetOff = L.targets;
+            in.lr = s.lr, in.gamma = s.gamma, in.beta1 = s.beta1, in.beta2 = s.beta2, in.eps = s.eps;
+            for (int k = 0; k < 4; ++k) in.milestones[k] = s.milestones[k];
+            in.fraction = s.fraction, in.seed = s.seed;
+            in.useBox = s.useBox, in.bx0 = s.box[0], in.by0 = s.box[1], in.bx1 = s.box[2], in.by1 = s.box[3];
+            in.maxIter = s.maxIter, in.tol = s.tol;
+            L.sites += align64(S);
+            L.cells += align64(int64_t(in.gw) * in.gh + 1);
+            L.targets += align64(T);
+
+            for (int b = 0; b < S; b += vmk::kSiteChunk) L.siteWork.push_back({i, b, std::min(vmk::kSiteChunk, S - b), 0});
+            L.instList.push_back(i);
+
+            // Targets sorted by hash cell (stable), chunked by cell.
+            std::vector<int> cellOfT(T);
+            for (int t = 0; t < T; ++t)
+                cellOfT[t] = cellCoordF(float(s.tgt[2 * t + 1]), in.inv, in.gh) * in.gw +
+                             cellCoordF(float(s.tgt[2 * t]), in.inv, in.gw);
+            std::vector<int> &perm = L.perm[i];
+            perm.resize(T);
+            std::iota(perm.begin(), perm.end(), 0);
+            std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return cellOfT[a] < cellOfT[b]; });
+            for (int t = 0; t < T;) {
+                int c = cellOfT[perm[t]], e = t;
+                while (e < T && cellOfT[perm[e]] == c) ++e;
+                for (int b = t; b < e; b += vmk::kTargetChunk)
+                    twByInst[i].push_back({i, b, std::min(vmk::kTargetChunk, e - b), c});
+                t = e;
+            }
+        }
+        // Group target work by K (one loss launch per distinct K).
+        std::vector<int> ks;
+        for (auto &s : specs) ks.push_back(s.K);
+        std::sort(ks.begin(), ks.end());
+        ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+        for (int K : ks) {
+            typename Layout::Group g{K, int(L.targetWork.size()), 0};
+            for (int i = 0; i < n; ++i)
+                if (specs[i].K == K) L.targetWork.insert(L.targetWork.end(), twByInst[i].begin(), twByInst[i].end());
+            g.count = int(L.targetWork.size()) - g.begin;
+            L.groups.push_back(g);
+        }
+    }
+
+    void commit() override
+    {
+        destroyGraph();
+        buildLayout();
+        const int n = int(specs.size());
+        dPos.reserve(L.sites), dM.reserve(L.sites), dV.reserve(L.sites), dGrad.reserve(L.sites);
+        dSpos.reserve(L.sites), dCellOf.reserve(L.sites), dRank.reserve(L.sites), dSidx.reserve(L.sites);
+        dFrozen.reserve(L.sites);
+        dCounts.reserve(L.cells), dStart.reserve(L.cells);
+        dTpos.reserve(L.targets), dTslot.reserve(L.targets);
+        dInst.reserve(n), dState.reserve(n), dInstList.reserve(n);
+        dSiteWork.reserve(L.siteWork.size()), dTargetWork.reserve(L.targetWork.size());
+        dCnt.reserve(2 + L.groups.size());
+
+        std::vector<V>       hp(L.sites), hm(L.sites), hv(L.sites), ht(L.targets);
+        std::vector<int>     hts(L.targets, 0);
+        std::vector<uint8_t> hf(L.sites, 0);
+        std::vector<vmk::InstState> hs(n);
+        for (int i = 0; i < n; ++i) {
+            const InstSpec  &s  = specs[i];
+            const vmk::Inst &in = L.inst[i];
+            for (int k = 0; k < in.S; ++k) {
+                hp[in.siteOff + k] = V{R(s.pos[2 * k]), R(s.pos[2 * k + 1])};
+                hm[in.siteOff + k] = s.m.empty() ? V{0, 0} : V{R(s.m[2 * k]), R(s.m[2 * k + 1])};
+                hv[in.siteOff + k] = s.v.empty() ? V{0, 0} : V{R(s.v[2 * k]), R(s.v[2 * k + 1])};
+                if (!s.frozen.empty()) hf[in.siteOff + k] = s.frozen[k];
+            }
+            for (int t = 0; t < in.T; ++t) {
+                int o = L.perm[i][t];
+                ht[in.targetOff + t]  = V{R(s.tgt[2 * o]), R(s.tgt[2 * o + 1])};
+                hts[in.targetOff + t] = s.tslot.empty() ? o : s.tslot[o];
+            }
+            hs[i] = s.state;
+        }
+        std::vector<int> cnt(2 + L.groups.size());
+        cnt[0] = n, cnt[1] = int(L.siteWork.size());
+        for (size_t g = 0; g < L.groups.size(); ++g) cnt[2 + g] = L.groups[g].count;
+
+        auto up = [&](auto *dst, const auto &src) {
+            if (!src.empty())
+                VM_HIP(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice, stream));
+        };
+        up(dPos.p, hp), up(dM.p, hm), up(dV.p, hv), up(dTpos.p, ht), up(dTslot.p, hts), up(dFrozen.p, hf);
+        up(dInst.p, L.inst), up(dState.p, hs), up(dInstList.p, L.instList), up(dSiteWork.p, L.siteWork);
+        up(dTargetWork.p, L.targetWork), up(dCnt.p, cnt);
+        VM_HIP(hipMemsetAsync(dGrad.p, 0, dGrad.n * sizeof(V), stream));
+        VM_HIP(vmk::launchFillInt(stream, dCounts.p, int64_t(dCounts.n), 0));
+        gridScan = std::max(1, std::min(n, kMaxScanGrid));
+        gridSite = std::max(1, std::min(int(L.siteWork.size()), kMaxSiteGrid));
+        gridLoss.clear();
+        for (auto &g : L.groups) gridLoss.push_back(std::max(1, std::min(g.count, kMaxLossGrid)));
+        bin();
+        VM_HIP(hipStreamSynchronize(stream));
+        dirty       = false;
+        deviceNewer = false;
+    }
+
+    // ---- launch helpers -------------------------------------------------------
+    vmk::LossArgs<R> lossArgs(int g, int mode, bool wantGrad)
+    {
+        vmk::LossArgs<R> a{};
+        a.insts = dInst.p, a.state = dState.p, a.work = dTargetWork.p + L.groups[g].begin, a.nWork = dCnt.p + 2 + g;
+        a.pos = dPos.p, a.spos = dSpos.p, a.sidx = dSidx.p, a.start = dStart.p, a.tpos = dTpos.p, a.tslot = dTslot.p;
+        a.grad = wantGrad ? dGrad.p : nullptr, a.mode = mode;
+        return a;
+    }
+
+    vmk::AdamArgs<R> adamArgs(int doAdam)
+    {
+        vmk::AdamArgs<R> a{};
+        a.insts = dInst.p, a.state = dState.p, a.work = dSiteWork.p, a.nWork = dCnt.p + 1;
+        a.pos = dPos.p, a.m = dM.p, a.v = dV.p, a.grad = dGrad.p, a.frozen = dFrozen.p;
+        a.cellOf = dCellOf.p, a.rank = dRank.p, a.counts = dCounts.p, a.doAdam = doAdam;
+        return a;
+    }
+
+    void bin() { VM_HIP(vmk::launchAdamBin<R>(gridSite, stream, adamArgs(0))); }
+
+    void gridBuild(int mode)
+    {
+        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dInstList.p, dCnt.p, dCounts.p, dStart.p, mode));
+        VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p, dCellOf.p,
+                                     dRank.p, dStart.p, dSpos.p, dSidx.p, mode));
+    }
+
+    void enqueueIteration(hipEvent_t *marks)
+    {
+        if (marks) VM_HIP(hipEventRecord(marks[0], stream));
+        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
+                               vmk::kModeIterate));
+        if (marks) VM_HIP(hipEventRecord(marks[1], stream));
+        VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p, dCellOf.p,
+                                     dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate));
+        if (marks) VM_HIP(hipEventRecord(marks[2], stream));
+        for (size_t g = 0; g < L.groups.size(); ++g)
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, lossArgs(int(g), vmk::kModeIterate, true)));
+        if (marks) VM_HIP(hipEventRecord(marks[3], stream));
+        VM_HIP(vmk::launchAdamBin<R>(gridSite, stream, adamArgs(1)));
+        if (marks) VM_HIP(hipEventRecord(marks[4], stream));
+    }
+
+    void buildGraph()
+    {
+        hipGraph_t g;
+        VM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        try {
+            enqueueIteration(nullptr);
+        } catch (...) {
+            hipGraph_t dummy;
+            (void)hipStreamEndCapture(stream, &dummy);
+            throw;
+        }
+        VM_HIP(hipStreamEndCapture(stream, &g));
+        hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        VM_HIP(e);
+    }
+
+    float step(int iterations) override
+    {
+        if (dirty) commit();
+        if (iterations <= 0) return 0.0f;
+        if (!graph) buildGraph();
+        ensureEvents();
+        VM_HIP(hipEventRecord(ev[5], stream));
+        for (int k = 0; k < iterations; ++k) VM_HIP(hipGraphLaunch(graph, stream));
+        VM_HIP(hipEventRecord(ev[6], stream));
+        VM_HIP(hipEventSynchronize(ev[6]));
+        float ms = 0;
+        VM_HIP(hipEventElapsedTime(&ms, ev[5], ev[6]));
+        deviceNewer = true;
+        return ms;
+    }
+
+    void profile(int iterations, float *phaseMs) override
+    {
+        if (dirty) commit();
+        ensureEvents();
+        double acc[4] = {0, 0, 0, 0};
+        for (int k = 0; k < iterations; ++k) {
+            enqueueIteration(ev);
+            VM_HIP(hipEventSynchronize(ev[4]));
+            for (int p = 0; p < 4; ++p) {
+                float ms = 0;
+                VM_HIP(hipEventElapsedTime(&ms, ev[p], ev[p + 1]));
+                acc[p] += ms;
+            }
+        }
+        for (int p = 0; p < 4; ++p) phaseMs[p] = float(acc[p] / std::max(iterations, 1));
+        deviceNewer = true;
+    }
+
+    void readStates() override
+    {
+        if (dirty) return;
+        std::vector<vmk::InstState> hs(specs.size());
+        if (!hs.empty())
+            VM_HIP(hipMemcpyAsync(hs.data(), dState.p, hs.size() * sizeof(vmk::InstState), hipMemcpyDeviceToHost,
+                                  stream));
+        VM_HIP(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < hs.size(); ++i) specs[i].state = hs[i];
+    }
+
+    void pull() override
+    {
+        if (dirty || !deviceNewer) return;
+        std::vector<V> hp(L.sites), hm(L.sites), hv(L.sites);
+        if (L.sites) {
+            VM_HIP(hipMemcpyAsync(hp.data(), dPos.p, L.sites * sizeof(V), hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(hm.data(), dM.p, L.sites * sizeof(V), hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(hv.data(), dV.p, L.sites * sizeof(V), hipMemcpyDeviceToHost, stream));
+        }
+        readStates();
+        for (size_t i = 0; i < specs.size(); ++i) {
+            InstSpec        &s  = specs[i];
+            const vmk::Inst &in = L.inst[i];
+            s.m.resize(2 * size_t(in.S)), s.v.resize(2 * size_t(in.S));
+            for (int k = 0; k < in.S; ++k) {
+                s.pos[2 * k] = double(hp[in.siteOff + k].x), s.pos[2 * k + 1] = double(hp[in.siteOff + k].y);
+                s.m[2 * k] = double(hm[in.siteOff + k].x), s.m[2 * k + 1] = double(hm[in.siteOff + k].y);
+                s.v[2 * k] = double(hv[in.siteOff + k].x), s.v[2 * k + 1] = double(hv[in.siteOff + k].y);
+            }
+        }
+        deviceNewer = false;
+    }
+
+    double evaluate(int id, int64_t *count) override
+    {
+        if (dirty) commit();
+        gridBuild(vmk::kModeEvaluate);
+        for (size_t g = 0; g < L.groups.size(); ++g)
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, lossArgs(int(g), vmk::kModeEvaluate, false)));
+        bin();
+        vmk::InstState st;
+        VM_HIP(hipMemcpyAsync(&st, dState.p + id, sizeof(st), hipMemcpyDeviceToHost, stream));
+        VM_HIP(hipStreamSynchronize(stream));
+        if (count) *count = st.evalCount;
+        return st.evalLoss;
+    }
+
+    void diag(int id, bool wantGrad, bool wantDbg, DiagOut &out) override
+    {
+        if (dirty) commit();
+        const vmk::Inst &in = L.inst[id];
+        DBuf<int>          dK;
+        DBuf<vmk::TermDbg> dT;
+        int                K = in.K;
+        if (wantDbg) {
+            dK.reserve(size_t(L.targets) * K);
+            dT.reserve(size_t(L.targets));
+        }
+        gridBuild(vmk::kModeEvaluate);
+        for (size_t g = 0; g < L.groups.size(); ++g) {
+            auto a = lossArgs(int(g), vmk::kModeEvaluate, wantGrad);
+            if (wantDbg && L.groups[g].K == K) a.dbgKnn = dK.p, a.dbgTerm = dT.p;
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, a));
+        }
+        std::vector<V> hg;
+        if (wantGrad) {
+            hg.resize(in.S);
+            if (in.S)
+                VM_HIP(hipMemcpyAsync(hg.data(), dGrad.p + in.siteOff, in.S * sizeof(V), hipMemcpyDeviceToHost,
+                                      stream));
+        }
+        std::vector<int>          hk;
+        std::vector<vmk::TermDbg> ht;
+        if (wantDbg) {
+            hk.resize(size_t(in.T) * K), ht.resize(in.T);
+            if (in.T) {
+                VM_HIP(hipMemcpyAsync(hk.data(), dK.p + in.targetOff * K, hk.size() * sizeof(int),
+                                      hipMemcpyDeviceToHost, stream));
+                VM_HIP(hipMemcpyAsync(ht.data(), dT.p + in.targetOff, ht.size() * sizeof(vmk::TermDbg),
+                                      hipMemcpyDeviceToHost, stream));
+            }
+        }
+        vmk::InstState st;
+        VM_HIP(hipMemcpyAsync(&st, dState.p + id, sizeof(st), hipMemcpyDeviceToHost, stream));
+        if (wantGrad) VM_HIP(hipMemsetAsync(dGrad.p, 0, dGrad.n * sizeof(V), stream));
+        bin();
+        VM_HIP(hipStreamSynchronize(stream));
+        out.lossSum = st.evalLoss, out.count = st.evalCount;
+        out.grad.assign(2 * size_t(in.S), 0.0);
+        for (int k = 0; k < (wantGrad ? in.S : 0); ++k) out.grad[2 * k] = double(hg[k].x), out.grad[2 * k + 1] = double(hg[k].y);
+        if (wantDbg) {
+            out.knn.assign(size_t(in.T) * K, -1);
+            out.terms.assign(in.T, vmk::TermDbg{-1, -1, 0.0});
+            for (int t = 0; t < in.T; ++t) {
+                int o = L.perm[id][t];
+                std::copy_n(hk.begin() + size_t(t) * K, K, out.knn.begin() + size_t(o) * K);
+                out.terms[o] = ht[t];
+            }
+        }
+    }
+
+    void knnQuery(int id, const double *q, int nq, int K, int *idx, double *d2) override
+    {
+        if (dirty) commit();
+        if (K < 1 || K > 32) throw VmError(1, "k must be in [1, 32]");
+        gridBuild(vmk::kModeEvaluate);
+        DBuf<V>   dq;
+        DBuf<int> di;
+        DBuf<R>   dd;
+        dq.reserve(nq), di.reserve(size_t(nq) * K), dd.reserve(size_t(nq) * K);
+        std::vector<V> hq(nq);
+        for (int i = 0; i < nq; ++i) hq[i] = V{R(q[2 * i]), R(q[2 * i + 1])};
+        if (nq) VM_HIP(hipMemcpyAsync(dq.p, hq.data(), nq * sizeof(V), hipMemcpyHostToDevice, stream));
+        VM_HIP(vmk::launchKnnQuery<R>(K, stream, dInst.p, id, dSpos.p, dSidx.p, dStart.p, dq.p, nq, di.p, dd.p));
+        std::vector<R> hd(size_t(nq) * K);
+        if (nq) {
+            VM_HIP(hipMemcpyAsync(idx, di.p, size_t(nq) * K * sizeof(int), hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(hd.data(), dd.p, hd.size() * sizeof(R), hipMemcpyDeviceToHost, stream));
+        }
+        bin();
+        VM_HIP(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < hd.size(); ++i) d2[i] = double(hd[i]);
+    }
+
+    void clip(int id, const uint64_t *dMasks, int tilesX, int tilesY, float voxel, ClipOut &out) override
+    {
+        if (dirty) commit();
+        ensureEvents();
+        const vmk::Inst &in = L.inst[id];
+        const int        S  = in.S;
+        DBuf<float>      dv;
+        DBuf<int>        dl, dc, df;
+        DBuf<uint8_t>    di;
+        dv.reserve(size_t(S) * vmk::kMaxCellVerts * 2), dl.reserve(size_t(S) * vmk::kMaxCellVerts);
+        dc.reserve(S), di.reserve(S), df.reserve(2);
+        VM_HIP(hipEventRecord(ev[5], stream));
+        gridBuild(vmk::kModeEvaluate);
+        VM_HIP(vmk::launchFillInt(stream, df.p, 2, 0));
+        vmk::ClipArgs<R> a{};
+        a.insts = dInst.p, a.inst = id, a.pos = dPos.p, a.spos = dSpos.p, a.sidx = dSidx.p, a.start = dStart.p;
+        a.masks = dMasks, a.tilesX = tilesX, a.tilesY = tilesY, a.invVoxel = voxel > 0 ? 1.0f / voxel : 0.0f;
+        a.verts = dv.p, a.labels = dl.p, a.count = dc.p, a.inside = di.p, a.flags = df.p;
+        VM_HIP(vmk::launchClip<R>(S, stream, a));
+        VM_HIP(hipEventRecord(ev[6], stream));
+        bin();
+        out.verts.assign(size_t(S) * vmk::kMaxCellVerts * 2, 0.0f);
+        out.labels.assign(size_t(S) * vmk::kMaxCellVerts, 0);
+        out.count.assign(S, 0), out.inside.assign(S, 0);
+        int flags[2] = {0, 0};
+        if (S) {
+            VM_HIP(hipMemcpyAsync(out.verts.data(), dv.p, out.verts.size() * 4, hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(out.labels.data(), dl.p, out.labels.size() * 4, hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(out.count.data(), dc.p, S * 4, hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(out.inside.data(), di.p, S, hipMemcpyDeviceToHost, stream));
+        }
+        VM_HIP(hipMemcpyAsync(flags, df.p, 8, hipMemcpyDeviceToHost, stream));
+        VM_HIP(hipStreamSynchronize(stream));
+        VM_HIP(hipEventElapsedTime(&out.gpuMs, ev[5], ev[6]));
+        out.unsafe = flags[0], out.overflow = flags[1];
+    }
+};
+
+} // namespace
+
+std::unique_ptr<Engine> Engine::create(int precision, hipStream_t stream)
+{
+    if (precision == 1) return std::unique_ptr<Engine>(new EngineT<double>(stream));
+    if (precision == 0) return std::unique_ptr<Engine>(new EngineT<float>(stream));
+    throw VmError(1, "precision must be 0 (fp32) or 1 (fp64)");
+}
+
+} // namespace vmrt

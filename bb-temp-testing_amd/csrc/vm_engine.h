@@ -1,0 +1,127 @@
+// vm_engine.h -- host runtime of the HIP optimizer: a batch of instances in
+// pooled device arrays, driven by the four per-iteration launches of
+// vm_kernels.hip, one iteration captured as a HIP graph.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "vm_kernels.h"
+
+namespace vmrt {
+
+struct VmError : std::runtime_error {
+    int code;
+    VmError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char *what, const char *file, int line);
+
+// Runs f, mapping exceptions to a vm_status code and the thread's last error.
+int         guardedCall(const std::function<void()> &f);
+const char *lastError();
+#define VM_HIP(x) ::vmrt::hipCheck((x), #x, __FILE__, __LINE__)
+
+template <class T> struct DBuf {
+    T     *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    // Reallocates (contents lost) when the capacity is too small.
+    void reserve(size_t cnt)
+    {
+        if (cnt <= n && p) return;
+        release();
+        size_t c = cnt ? cnt : 1;
+        VM_HIP(hipMalloc(reinterpret_cast<void **>(&p), c * sizeof(T)));
+        n = c;
+    }
+};
+
+// Host description of one instance.  Positions and moments are kept in double
+// on the host so either device precision round-trips exactly.
+struct InstSpec {
+    float               W = 1, H = 1;
+    std::vector<double> pos, m, v;   // xy interleaved, live sites
+    std::vector<double> tgt;         // xy interleaved, live targets
+    std::vector<int>    tslot;       // slot number of each live target (subsampling hash index)
+    std::vector<uint8_t> frozen;     // empty: all sites move
+    double lr = 1e-3, gamma = 0.5, beta1 = 0.9, beta2 = 0.999, eps = 1e-8;
+    int    milestones[4] = {80, 150, 200, 250};
+    float  fraction = 1.0f;
+    uint32_t seed   = 1;
+    int    K = 16;
+    float  cell = 0.0f;      // hash-grid cell
+    int    stageR = 2;
+    int    useBox = 0;
+    float  box[4] = {0, 0, 0, 0};
+    int    maxIter = 0;
+    float  tol = 0.0f;
+    vmk::InstState state{};  // host copy of the device state
+};
+
+struct DiagOut {
+    std::vector<double> grad;      // [2S]
+    double              lossSum = 0;
+    int64_t             count   = 0;
+    std::vector<int>    knn;       // [T*K] per live target (input order)
+    std::vector<vmk::TermDbg> terms; // [T] per live target (input order)
+};
+
+struct ClipOut {
+    std::vector<float>   verts;  // [S*24*2]
+    std::vector<int>     labels; // [S*24]
+    std::vector<int>     count;  // [S]
+    std::vector<uint8_t> inside; // [S]
+    int                  unsafe = 0, overflow = 0;
+    float                gpuMs = 0;
+};
+
+class Engine {
+public:
+    virtual ~Engine() = default;
+    static std::unique_ptr<Engine> create(int precision, hipStream_t stream);
+
+    std::vector<InstSpec> specs;
+
+    int  add(InstSpec s)
+    {
+        specs.push_back(std::move(s));
+        dirty = true;
+        return int(specs.size()) - 1;
+    }
+    // Brings host specs up to date with the device (positions, moments, state).
+    virtual void pull() = 0;
+    // Lays out, uploads and bins every instance; rebuilds the iteration graph.
+    virtual void commit() = 0;
+    virtual float step(int iterations) = 0;                 // GPU ms
+    virtual void  profile(int iterations, float *phaseMs) = 0; // mean ms per phase
+    virtual void  readStates() = 0;                          // device -> specs[i].state
+    virtual double evaluate(int id, int64_t *count) = 0;     // loss sum over all live targets of `id`
+    virtual void  diag(int id, bool wantGrad, bool wantDbg, DiagOut &out) = 0;
+    virtual void  knnQuery(int id, const double *q, int nq, int K, int *idx, double *d2) = 0;
+    virtual void  clip(int id, const uint64_t *dMasks, int tilesX, int tilesY, float voxel, ClipOut &out) = 0;
+    virtual size_t deviceBytes() const = 0;
+
+    bool        dirty = true;
+    bool        deviceNewer = false;
+    hipStream_t stream = nullptr;
+};
+
+// Default per-size-class choices (hash cell in mean spacings, staged rings).
+void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR);
+
+} // namespace vmrt

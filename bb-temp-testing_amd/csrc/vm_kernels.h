@@ -1,0 +1,82 @@
+// vm_kernels.h -- kernel argument blocks and host-side launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "vm_types.h"
+
+namespace vmk {
+
+// Point / vector in the kernel's real type (layout of float2 / double2).
+template <class R> struct V2 {
+    R x, y;
+};
+
+// Per-target diagnostics written by the loss kernel when requested.
+struct TermDbg {
+    int    i0, j;  // nearest site and bisector partner (instance-local), -1 if unused
+    double loss;   // VoroLoss term (in the kernel's real type, widened)
+};
+
+template <class R> struct LossArgs {
+    const Inst       *insts;
+    InstState        *state;
+    const TargetWork *work;
+    const int        *nWork;
+    const V2<R>      *pos;    // current positions (instance-local order)
+    const V2<R>      *spos;   // positions sorted by cell
+    const int        *sidx;   // instance-local site index of each sorted entry
+    const int        *start;  // cell starts
+    const V2<R>      *tpos;   // targets sorted by cell
+    const int        *tslot;  // their slot numbers (subsampling hash index)
+    V2<R>            *grad;   // accumulated gradient, null: loss only
+    int               mode;   // kModeIterate / kModeEvaluate
+    int              *dbgKnn; // optional [T*K] k-NN lists (instance-local indices)
+    TermDbg          *dbgTerm;
+};
+
+template <class R> struct AdamArgs {
+    const Inst          *insts;
+    const InstState     *state;
+    const SiteWork      *work;
+    const int           *nWork;
+    V2<R>               *pos, *m, *v, *grad;
+    const unsigned char *frozen;
+    int                 *cellOf, *rank, *counts;
+    int                  doAdam; // 0: only bin the current positions
+};
+
+template <class R> struct ClipArgs {
+    const Inst     *insts;
+    int             inst;
+    const V2<R>    *pos, *spos;
+    const int      *sidx, *start;
+    const uint64_t *masks;
+    int             tilesX, tilesY;
+    float           invVoxel;
+    float          *verts;   // [S*24*2]
+    int            *labels;  // [S*24]
+    int            *count;   // [S]
+    unsigned char  *inside;  // [S]
+    int            *flags;   // {unsafe, overflow}
+};
+
+hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, const int *instList,
+                      const int *nInst, int *counts, int *start, int mode);
+template <class R>
+hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstState *state, const SiteWork *work,
+                         const int *nWork, const void *pos, const int *cellOf, const int *rank, const int *start,
+                         void *spos, int *sidx, int mode);
+template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const LossArgs<R> &a);
+template <class R> hipError_t launchAdamBin(int grid, hipStream_t s, const AdamArgs<R> &a);
+template <class R>
+hipError_t launchKnnQuery(int K, hipStream_t s, const Inst *insts, int inst, const void *spos, const int *sidx,
+                          const int *start, const void *q, int nq, int *outIdx, void *outD2);
+template <class R> hipError_t launchClip(int nSites, hipStream_t s, const ClipArgs<R> &a);
+hipError_t launchPoissonClear(int grid, hipStream_t s, const PoiInst *insts, const PoiWork *work, int nWork, float *g);
+hipError_t launchPoissonPhase(int grid, hipStream_t s, const PoiInst *insts, const PoiWork *work, int nWork, float *g,
+                              int px, int py, int round);
+hipError_t launchFillInt(hipStream_t s, int *p, int64_t n, int v);
+
+inline bool lossKSupported(int K) { return K == 2 || K == 4 || K == 6 || K == 8 || K == 12 || K == 16 || K == 24 || K == 32; }
+
+} // namespace vmk

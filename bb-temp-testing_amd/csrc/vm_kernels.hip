@@ -1,0 +1,909 @@
+// vm_kernels.hip -- HIP kernels of the VoroMesh optimizer step for gfx950.
+//
+// One optimizer iteration over a packed table of instances is four launches:
+//   scan     one workgroup per instance: exclusive scan of the per-cell site
+//            counts into cell starts (and zeroes the counts), begins the step
+//            (iter += 1, clears the loss accumulators, convergence test);
+//   scatter  site chunks: counting-sort scatter into cell order;
+//   loss     target chunks (one wavefront = up to 64 targets of one hash
+//            cell): stage the sites of the surrounding cell block into LDS,
+//            k-NN by ring search with a register top-K, VoroLoss and its
+//            gradient accumulated in LDS and flushed with one global atomic
+//            per touched site;
+//   adam     site chunks: Adam + MultiStepLR + clamp, zero the gradient, and
+//            bin the new position (count + rank) for the next scan.
+// Reference: src/voromesh/voro_optimizer.cu (binKernel :168, scatterKernel
+// :179, vorolossKernel :206, adamKernel :271, voronoiCellKernel :361) and
+// src/poisson/poisson_disk.cu (dartPhaseKernel :138, clearRegionKernel :173).
+//
+// Arithmetic contract (shared with the CPU oracle): squared distances are
+// fma(dy, dy, dx*dx), dot products fma(a.y, b.y, a.x*b.x), k-NN lists are
+// ordered by (d2, site index); the file is built with -ffp-contract=off and
+// correctly rounded division and square root, so every per-target term is
+// bit-identical to the oracle's.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "vm_kernels.h"
+
+namespace vmk {
+
+// ---------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t pcg_hash(uint32_t v)
+{
+    uint32_t state = v * 747796405u + 2891336453u;
+    uint32_t word  = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+__device__ __forceinline__ float rand01_loss(uint32_t index, uint32_t iter, uint32_t seed)
+{
+    uint32_t h = pcg_hash(index ^ pcg_hash(iter * 2654435761u ^ pcg_hash(seed)));
+    return float(h & 0x00FFFFFFu) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ float rand01_poisson(uint32_t cell, uint32_t round, uint32_t seed, uint32_t stream)
+{
+    uint32_t h = pcg_hash(cell ^ pcg_hash(round * 2654435761u ^ pcg_hash(seed + stream * 0x9E3779B9u)));
+    return float(h & 0x00FFFFFFu) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ float  fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float  sqrtR(float a) { return sqrtf(a); }
+__device__ __forceinline__ double sqrtR(double a) { return sqrt(a); }
+
+template <class R> __device__ __forceinline__ R dist2(V2<R> p, V2<R> q)
+{
+    R dx = p.x - q.x, dy = p.y - q.y;
+    return fmaR(dy, dy, dx * dx);
+}
+
+template <class R> __device__ __forceinline__ R dotR(V2<R> a, V2<R> b) { return fmaR(a.y, b.y, a.x * b.x); }
+
+template <class R> __device__ __forceinline__ int cellCoord(R v, R inv, int n)
+{
+    R   f = v * inv;
+    int c = (f >= R(0)) ? (f < R(2147483520.0) ? int(f) : n - 1) : 0;
+    return min(max(c, 0), n - 1);
+}
+
+template <class R> __device__ __forceinline__ R ringRel();
+template <> __device__ __forceinline__ float  ringRel<float>() { return 1e-6f; }
+template <> __device__ __forceinline__ double ringRel<double>() { return 1e-13; }
+
+__device__ __forceinline__ double powi(double b, int e)
+{
+    double r = 1.0;
+    while (e > 0) {
+        if (e & 1) r = r * b;
+        b = b * b;
+        e >>= 1;
+    }
+    return r;
+}
+
+__device__ __forceinline__ void gatomicAdd(float *p, float v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void gatomicAdd(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// ---------------------------------------------------------------------------
+// k-NN keys: total order (d2, site index).  The low word of a key is
+// (site index << kLdsSlotBits) | LDS slot, so ties on d2 are broken by the
+// site index and the slot rides along for free.
+// ---------------------------------------------------------------------------
+template <class R> struct Key;
+
+template <> struct Key<float> {
+    uint64_t k;
+    static __device__ __forceinline__ Key make(float d2, uint32_t low)
+    {
+        return Key{(uint64_t(__float_as_uint(d2)) << 32) | low};
+    }
+    static __device__ __forceinline__ Key none() { return Key{~0ull}; }
+    __device__ __forceinline__ bool     lt(const Key &o) const { return k < o.k; }
+    __device__ __forceinline__ bool     valid() const { return k != ~0ull; }
+    __device__ __forceinline__ uint32_t low() const { return uint32_t(k); }
+    __device__ __forceinline__ float    d2() const { return __uint_as_float(uint32_t(k >> 32)); }
+};
+
+template <> struct Key<double> {
+    double   d;
+    uint32_t i;
+    static __device__ __forceinline__ Key make(double d2, uint32_t low) { return Key{d2, low}; }
+    static __device__ __forceinline__ Key none() { return Key{DBL_MAX, 0xFFFFFFFFu}; }
+    __device__ __forceinline__ bool     lt(const Key &o) const { return d < o.d || (d == o.d && i < o.i); }
+    __device__ __forceinline__ bool     valid() const { return i != 0xFFFFFFFFu; }
+    __device__ __forceinline__ uint32_t low() const { return i; }
+    __device__ __forceinline__ double   d2() const { return d; }
+};
+
+__device__ __forceinline__ int keySite(uint32_t low) { return int(low >> kLdsSlotBits); }
+__device__ __forceinline__ int keySlot(uint32_t low) { return int(low & uint32_t(kLdsNone)); }
+
+// Static-index insertion into an ascending top-K list held in registers.
+template <int K, class KeyT> __device__ __forceinline__ void topkInsert(KeyT (&keys)[K], KeyT nk)
+{
+    if (!nk.lt(keys[K - 1])) return;
+#pragma unroll
+    for (int k = K - 1; k > 0; --k) {
+        bool before = nk.lt(keys[k - 1]);
+        keys[k]     = before ? keys[k - 1] : (nk.lt(keys[k]) ? nk : keys[k]);
+    }
+    keys[0] = nk.lt(keys[0]) ? nk : keys[0];
+}
+
+// Per-thread ring search over the global sorted grid of one instance (used by
+// the clipping and the diagnostic k-NN kernels).  Same stop rule as the loss
+// kernel and the oracle: conservative, so the list equals brute force.
+template <int K, class R>
+__device__ void knnGlobal(const Inst &in, const V2<R> *spos, const int *sidx, const int *start, V2<R> q,
+                          Key<R> (&keys)[K])
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) keys[k] = Key<R>::none();
+    const R   cell   = R(in.cell);
+    const R   inv    = R(1) / cell;
+    const R   rel    = ringRel<R>();
+    const R   extent = fmax(R(in.ext), fmax(fabs(q.x), fabs(q.y)));
+    const R   margin = rel * extent;
+    const int cx = cellCoord(q.x, inv, in.gw), cy = cellCoord(q.y, inv, in.gh);
+    for (int ring = 0;; ++ring) {
+        int x0 = cx - ring, x1 = cx + ring, y0 = cy - ring, y1 = cy + ring;
+        for (int y = max(y0, 0); y <= min(y1, in.gh - 1); ++y) {
+            int stride = (y == y0 || y == y1) ? 1 : max(x1 - x0, 1);
+            for (int x = x0; x <= x1; x += stride) {
+                if (x < 0 || x >= in.gw) continue;
+                int c = y * in.gw + x;
+                for (int s = start[c]; s < start[c + 1]; ++s) {
+                    V2<R> p = spos[s];
+                    topkInsert<K>(keys, Key<R>::make(dist2(p, q), (uint32_t(sidx[s]) << kLdsSlotBits) | kLdsNone));
+                }
+            }
+        }
+        if (x0 <= 0 && y0 <= 0 && x1 >= in.gw - 1 && y1 >= in.gh - 1) break;
+        R safe = fmin(fmin(q.x - R(x0) * cell, R(x1 + 1) * cell - q.x), fmin(q.y - R(y0) * cell, R(y1 + 1) * cell - q.y));
+        safe   = fmax(safe - margin, R(0));
+        if (keys[K - 1].valid() && keys[K - 1].d2() * (R(1) + rel) < safe * safe) break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1. Scan: one workgroup per instance (also begins the step)
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 1024;
+
+__global__ __launch_bounds__(kScanThreads) void scanKernel(const Inst *__restrict__ insts, InstState *state,
+                                                          const int *__restrict__ instList, const int *nInstPtr,
+                                                          int *counts, int *start, int mode)
+{
+    __shared__ int waveSum[kScanThreads / 64];
+    __shared__ int skip;
+    const int nInst = *nInstPtr;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int w = blockIdx.x; w < nInst; w += gridDim.x) {
+        const int   id = instList[w];
+        const Inst &in = insts[id];
+        if (threadIdx.x == 0) {
+            InstState *s = state + id;
+            int        d = s->done;
+            if (mode == kModeIterate) {
+                if (!d) {
+                    if (s->iter > 0 && s->count > 0 && in.tol > 0.0f) {
+                        double mean = s->loss / double(s->count);
+                        if (s->iter > 1 && fabs(mean - s->prevMean) < double(in.tol) * fabs(s->prevMean)) d = 1;
+                        s->prevMean = mean;
+                    }
+                    if (in.maxIter > 0 && s->iter >= in.maxIter) d = 1;
+                    if (!d) {
+                        s->iter += 1;
+                        s->count = 0;
+                        s->loss  = 0.0;
+                    }
+                    s->done = d;
+                }
+            } else {
+                s->evalLoss  = 0.0;
+                s->evalCount = 0;
+                d            = 0;
+            }
+            skip = d;
+        }
+        __syncthreads();
+        if (skip) {
+            __syncthreads();
+            continue;
+        }
+        const int n   = in.gw * in.gh + 1;
+        int      *cnt = counts + in.cellOff;
+        int      *st  = start + in.cellOff;
+        int       running = 0;
+        for (int base = 0; base < n; base += kScanThreads * 4) {
+            int i0 = base + threadIdx.x * 4;
+            int v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = (i0 + j < n) ? cnt[i0 + j] : 0;
+            int tot = v[0] + v[1] + v[2] + v[3];
+            int inc = tot;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                int o = __shfl_up(inc, off, 64);
+                if (lane >= off) inc += o;
+            }
+            if (lane == 63) waveSum[wave] = inc;
+            __syncthreads();
+            int wpre = 0, btot = 0;
+#pragma unroll
+            for (int k = 0; k < kScanThreads / 64; ++k) {
+                int s = waveSum[k];
+                wpre += (k < wave) ? s : 0;
+                btot += s;
+            }
+            int ex = running + wpre + inc - tot;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (i0 + j < n) {
+                    st[i0 + j]  = ex;
+                    cnt[i0 + j] = 0;
+                }
+                ex += v[j];
+            }
+            running += btot;
+            __syncthreads();
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 2. Scatter into cell order (voro_optimizer.cu:179-187)
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(kSiteChunk) void scatterKernel(const Inst *__restrict__ insts,
+                                                            const InstState *__restrict__ state,
+                                                            const SiteWork *__restrict__ work, const int *nWorkPtr,
+                                                            const V2<R> *__restrict__ pos,
+                                                            const int *__restrict__ cellOf,
+                                                            const int *__restrict__ rank,
+                                                            const int *__restrict__ start, V2<R> *spos, int *sidx,
+                                                            int mode)
+{
+    const int nWork = *nWorkPtr;
+    for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
+        const SiteWork wk = work[w];
+        const Inst    &in = insts[wk.inst];
+        if (mode == kModeIterate && state[wk.inst].done) continue;
+        int i = wk.begin + int(threadIdx.x);
+        if (i >= wk.begin + wk.count) continue;
+        int64_t g = in.siteOff + i;
+        int     s = start[in.cellOff + cellOf[g]] + rank[g];
+        spos[in.siteOff + s] = pos[g];
+        sidx[in.siteOff + s] = i;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3. VoroLoss + gradient (voro_optimizer.cu:199-266)
+// ---------------------------------------------------------------------------
+constexpr int kLossThreads = 64;
+constexpr int kStageMaxCells = 25; // (2*2+1)^2
+
+template <class R> struct LossLds {
+    V2<R> pos[kStageCap];
+    int   idx[kStageCap];
+    V2<R> grad[kStageCap];
+    int   cellBeg[kStageMaxCells + 1];
+    int   cellEnd[kStageMaxCells + 1];
+    int   total;
+};
+
+template <int K, class R>
+__global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
+{
+    __shared__ LossLds<R> lds;
+    const int lane  = threadIdx.x;
+    const int nWork = *a.nWork;
+    for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
+        const TargetWork wk = a.work[w];
+        const Inst      &in = a.insts[wk.inst];
+        InstState       *st = a.state + wk.inst;
+        if (a.mode == kModeIterate && st->done) continue;
+        const int      iter = st->iter;
+        const int64_t  so   = in.siteOff;
+        const V2<R>   *spos = a.spos + so;
+        const int     *sidx = a.sidx + so;
+        const int     *strt = a.start + in.cellOff;
+        const int      gw = in.gw, gh = in.gh;
+        const int      cx = wk.cell % gw, cy = wk.cell / gw;
+        const int      rs = in.stageR;
+        const int      sx0 = max(cx - rs, 0), sx1 = min(cx + rs, gw - 1);
+        const int      sy0 = max(cy - rs, 0), sy1 = min(cy + rs, gh - 1);
+        const int      snx = sx1 - sx0 + 1, sny = sy1 - sy0 + 1;
+        const int      ncell = snx * sny;
+
+        // ---- stage the cell block [sx0,sx1] x [sy0,sy1] into LDS ----------
+        int b = 0, e = 0;
+        if (lane < ncell) {
+            int c = (sy0 + lane / snx) * gw + sx0 + lane % snx;
+            b     = strt[c];
+            e     = strt[c + 1];
+        }
+        int cnt = e - b, inc = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            int o = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += o;
+        }
+        const int total  = __shfl(inc, 63, 64);
+        const bool staged = total <= kStageCap;
+        if (staged) {
+            if (lane < ncell) {
+                lds.cellBeg[lane] = inc - cnt;
+                lds.cellEnd[lane] = inc;
+            }
+            // Rows of the block are contiguous ranges of the sorted arrays.
+            for (int r = 0; r < sny; ++r) {
+                int rb   = __shfl(b, r * snx, 64);
+                int re   = __shfl(e, r * snx + snx - 1, 64);
+                int roff = __shfl(inc - cnt, r * snx, 64);
+                for (int s = lane; s < re - rb; s += 64) {
+                    lds.pos[roff + s]  = spos[rb + s];
+                    lds.idx[roff + s]  = sidx[rb + s];
+                    lds.grad[roff + s] = V2<R>{R(0), R(0)};
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- this lane's target -----------------------------------------
+        const int t    = wk.begin + lane;
+        V2<R>     p    = V2<R>{R(-1), R(-1)};
+        bool      take = false;
+        if (lane < wk.count) {
+            p    = a.tpos[in.targetOff + t];
+            take = p.x >= R(0);
+            if (take && in.useBox && a.mode == kModeIterate)
+                take = p.x >= R(in.bx0) && p.x <= R(in.bx1) && p.y >= R(in.by0) && p.y <= R(in.by1);
+            if (take && a.mode == kModeIterate && in.fraction < 1.0f)
+                take = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(iter), in.seed) < in.fraction;
+        }
+
+        // ---- k-NN by ring search (the ring geometry is wave-uniform) -------
+        Key<R> keys[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) keys[k] = Key<R>::none();
+        const R    cell   = R(in.cell);
+        const R    rel    = ringRel<R>();
+        const R    margin = rel * fmax(R(in.ext), fmax(fabs(p.x), fabs(p.y)));
+        bool       active = take;
+        for (int ring = 0;; ++ring) {
+            if (!__any(active)) break;
+            const int x0 = cx - ring, x1 = cx + ring, y0 = cy - ring, y1 = cy + ring;
+            for (int y = max(y0, 0); y <= min(y1, gh - 1); ++y) {
+                const int stride = (y == y0 || y == y1) ? 1 : max(x1 - x0, 1);
+                for (int x = x0; x <= x1; x += stride) {
+                    if (x < 0 || x >= gw) continue;
+                    if (staged && x >= sx0 && x <= sx1 && y >= sy0 && y <= sy1) {
+                        const int li = (y - sy0) * snx + (x - sx0);
+                        const int lb = lds.cellBeg[li], le = lds.cellEnd[li];
+                        for (int s = lb; s < le; ++s) {
+                            if (active) {
+                                V2<R> q = lds.pos[s];
+                                topkInsert<K>(keys, Key<R>::make(dist2(q, p), (uint32_t(lds.idx[s]) << kLdsSlotBits) |
+                                                                                 uint32_t(s)));
+                            }
+                        }
+                    } else {
+                        const int c  = y * gw + x;
+                        const int gb = strt[c], ge = strt[c + 1];
+                        for (int s = gb; s < ge; ++s) {
+                            if (active) {
+                                V2<R> q = spos[s];
+                                topkInsert<K>(keys, Key<R>::make(dist2(q, p), (uint32_t(sidx[s]) << kLdsSlotBits) |
+                                                                                 uint32_t(kLdsNone)));
+                            }
+                        }
+                    }
+                }
+            }
+            if (x0 <= 0 && y0 <= 0 && x1 >= gw - 1 && y1 >= gh - 1) break;
+            if (active) {
+                R safe = fmin(fmin(p.x - R(x0) * cell, R(x1 + 1) * cell - p.x),
+                              fmin(p.y - R(y0) * cell, R(y1 + 1) * cell - p.y));
+                safe   = fmax(safe - margin, R(0));
+                if (keys[K - 1].valid() && keys[K - 1].d2() * (R(1) + rel) < safe * safe) active = false;
+            }
+        }
+
+        if (a.dbgKnn && lane < wk.count) {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                a.dbgKnn[(in.targetOff + t) * int64_t(K) + k] = (take && keys[k].valid()) ? keySite(keys[k].low()) : -1;
+        }
+
+        // ---- VoroLoss term and gradient -----------------------------------
+        R    lossT = R(0);
+        int  used  = 0;
+        int  i0 = -1, bj = -1;
+        if (take && keys[0].valid()) {
+            const uint32_t lo0 = keys[0].low();
+            i0                 = keySite(lo0);
+            const int sl0      = keySlot(lo0);
+            V2<R>     c0       = sl0 != kLdsNone ? lds.pos[sl0] : a.pos[so + i0];
+            V2<R>     u{p.x - c0.x, p.y - c0.y};
+            R         best  = sizeof(R) == 4 ? R(FLT_MAX) : R(DBL_MAX);
+            R         bestD = R(0);
+            int       bestK = -1;
+            const R   tiny  = sizeof(R) == 4 ? R(1e-12f) : R(1e-12);
+#pragma unroll
+            for (int k = 1; k < K; ++k) {
+                if (!keys[k].valid()) break;
+                const uint32_t lo = keys[k].low();
+                const int      sl = keySlot(lo);
+                V2<R>          cj = sl != kLdsNone ? lds.pos[sl] : a.pos[so + keySite(lo)];
+                V2<R>          ev{cj.x - c0.x, cj.y - c0.y};
+                R              l = sqrtR(dotR(ev, ev));
+                if (l < tiny) continue;
+                R q  = dotR(u, ev) / l;
+                R d  = fmaR(R(-0.5), l, q);
+                R dd = d * d;
+                if (dd < best) {
+                    best  = dd;
+                    bestD = d;
+                    bestK = k;
+                }
+            }
+            if (bestK >= 0) {
+                uint32_t lob = 0;
+#pragma unroll
+                for (int k = 1; k < K; ++k)
+                    if (k == bestK) lob = keys[k].low();
+                bj      = keySite(lob);
+                lossT   = best;
+                used    = 1;
+                if (a.grad) {
+                    const int slj = keySlot(lob);
+                    V2<R>     cj  = slj != kLdsNone ? lds.pos[slj] : a.pos[so + bj];
+                    V2<R>     ev{cj.x - c0.x, cj.y - c0.y};
+                    R         l  = sqrtR(dotR(ev, ev));
+                    R         rl = R(1) / l;
+                    V2<R>     n{rl * ev.x, rl * ev.y};
+                    R         un = dotR(u, n);
+                    R         tx = fmaR(-un, n.x, u.x), ty = fmaR(-un, n.y, u.y);
+                    V2<R>     dcj{fmaR(rl, tx, R(-0.5) * n.x), fmaR(rl, ty, R(-0.5) * n.y)};
+                    R         s = R(2) * bestD;
+                    V2<R>     gj{s * dcj.x, s * dcj.y};
+                    V2<R>     g0{-(s * (n.x + dcj.x)), -(s * (n.y + dcj.y))};
+                    if (slj != kLdsNone) {
+                        atomicAdd(&lds.grad[slj].x, gj.x);
+                        atomicAdd(&lds.grad[slj].y, gj.y);
+                    } else {
+                        gatomicAdd(&a.grad[so + bj].x, gj.x);
+                        gatomicAdd(&a.grad[so + bj].y, gj.y);
+                    }
+                    if (sl0 != kLdsNone) {
+                        atomicAdd(&lds.grad[sl0].x, g0.x);
+                        atomicAdd(&lds.grad[sl0].y, g0.y);
+                    } else {
+                        gatomicAdd(&a.grad[so + i0].x, g0.x);
+                        gatomicAdd(&a.grad[so + i0].y, g0.y);
+                    }
+                }
+            }
+        }
+        if (a.dbgTerm && lane < wk.count) {
+            int64_t o = in.targetOff + t;
+            a.dbgTerm[o].i0   = used ? i0 : -1;
+            a.dbgTerm[o].j    = used ? bj : -1;
+            a.dbgTerm[o].loss = double(lossT);
+        }
+
+        // ---- FP64 loss reduction: one atomic pair per wavefront ----------
+        double ls = double(lossT);
+        int    uc = used;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            ls += __shfl_down(ls, off, 64);
+            uc += __shfl_down(uc, off, 64);
+        }
+        if (lane == 0 && uc > 0) {
+            if (a.mode == kModeIterate) {
+                unsafeAtomicAdd(&st->loss, ls);
+                atomicAdd((unsigned long long *)&st->count, (unsigned long long)uc);
+            } else {
+                unsafeAtomicAdd(&st->evalLoss, ls);
+                atomicAdd((unsigned long long *)&st->evalCount, (unsigned long long)uc);
+            }
+        }
+
+        // ---- flush the LDS gradient: one atomic per touched site ----------
+        __syncthreads();
+        if (a.grad && staged) {
+            for (int s = lane; s < total; s += 64) {
+                V2<R> g = lds.grad[s];
+                if (g.x != R(0) || g.y != R(0)) {
+                    int i = lds.idx[s];
+                    gatomicAdd(&a.grad[so + i].x, g.x);
+                    gatomicAdd(&a.grad[so + i].y, g.y);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 4. Adam + MultiStepLR + clamp (voro_optimizer.cu:268-301), then bin the new
+//    position for the next step (voro_optimizer.cu:168-177).
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
+{
+    const int nWork = *a.nWork;
+    for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
+        const SiteWork   wk = a.work[w];
+        const Inst      &in = a.insts[wk.inst];
+        const InstState *st = a.state + wk.inst;
+        if (a.doAdam && st->done) continue;
+        int i = wk.begin + int(threadIdx.x);
+        if (i >= wk.begin + wk.count) continue;
+        const int64_t g = in.siteOff + i;
+        V2<R>         p = a.pos[g];
+        if (a.doAdam) {
+            V2<R> gr = a.grad[g];
+            a.grad[g] = V2<R>{R(0), R(0)};
+            bool movable = in.allMovable || !a.frozen[g];
+            if (movable) {
+                const int t  = st->iter;
+                R         lr = R(in.lr);
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (t - 1 >= in.milestones[k]) lr = lr * R(in.gamma);
+                const R b1 = R(in.beta1), b2 = R(in.beta2), eps = R(in.eps);
+                const R bc1 = R(1.0 - powi(double(b1), t));
+                const R bc2 = R(1.0 - powi(double(b2), t));
+                const int64_t cnt = st->count;
+                const R inv = R(1) / R(cnt > 1 ? cnt : 1);
+                const R ob1 = R(1) - b1, ob2 = R(1) - b2;
+                V2<R>   gg{inv * gr.x, inv * gr.y};
+                V2<R>   mo = a.m[g], vo = a.v[g];
+                V2<R>   mi{b1 * mo.x + ob1 * gg.x, b1 * mo.y + ob1 * gg.y};
+                V2<R>   vi{b2 * vo.x + ob2 * (gg.x * gg.x), b2 * vo.y + ob2 * (gg.y * gg.y)};
+                a.m[g] = mi;
+                a.v[g] = vi;
+                p.x    = p.x - (lr * (mi.x / bc1)) / (sqrtR(vi.x / bc2) + eps);
+                p.y    = p.y - (lr * (mi.y / bc1)) / (sqrtR(vi.y / bc2) + eps);
+                p.x    = fmin(fmax(p.x, R(0)), R(in.W));
+                p.y    = fmin(fmax(p.y, R(0)), R(in.H));
+                a.pos[g] = p;
+            }
+        }
+        const R   inv = R(1) / R(in.cell);
+        const int c   = cellCoord(p.y, inv, in.gh) * in.gw + cellCoord(p.x, inv, in.gw);
+        a.cellOf[g]   = c;
+        a.rank[g]     = atomicAdd(&a.counts[in.cellOff + c], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostic k-NN of arbitrary queries against one instance's current grid.
+// ---------------------------------------------------------------------------
+template <int K, class R>
+__global__ void knnQueryKernel(const Inst *__restrict__ insts, int inst, const V2<R> *__restrict__ spos,
+                               const int *__restrict__ sidx, const int *__restrict__ start,
+                               const V2<R> *__restrict__ q, int nq, int kOut, int *outIdx, R *outD2)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const Inst &in = insts[inst];
+    Key<R>      keys[K];
+    knnGlobal<K>(in, spos + in.siteOff, sidx + in.siteOff, start + in.cellOff, q[i], keys);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (k < kOut) {
+            outIdx[int64_t(i) * kOut + k] = keys[k].valid() ? keySite(keys[k].low()) : -1;
+            outD2[int64_t(i) * kOut + k]  = keys[k].valid() ? keys[k].d2() : R(FLT_MAX);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cell clipping (voro_optimizer.cu:312-437).  FP64 predicates and
+// intersections; the polygon is rebuilt in place through a small ring buffer.
+// ---------------------------------------------------------------------------
+struct CellPoly {
+    double vx[kMaxCellVerts], vy[kMaxCellVerts];
+    int    e[kMaxCellVerts];
+    int    n;
+};
+
+__device__ bool clipByBisector(CellPoly &poly, double rx, double ry, int label)
+{
+    double c      = 0.5 * (rx * rx + ry * ry);
+    bool   anyOut = false;
+    for (int k = 0; k < poly.n; ++k) anyOut |= poly.vx[k] * rx + poly.vy[k] * ry - c > 0.0;
+    if (!anyOut) return true;
+    CellPoly out;
+    out.n   = 0;
+    bool ok = true;
+    for (int k = 0; k < poly.n; ++k) {
+        int    k1 = (k + 1 == poly.n) ? 0 : k + 1;
+        double ax = poly.vx[k], ay = poly.vy[k], bx = poly.vx[k1], by = poly.vy[k1];
+        double fa = ax * rx + ay * ry - c;
+        double fb = bx * rx + by * ry - c;
+        bool   inA = fa <= 0.0, inB = fb <= 0.0;
+        if (inA) {
+            if (out.n < kMaxCellVerts) {
+                out.vx[out.n] = ax, out.vy[out.n] = ay, out.e[out.n] = poly.e[k];
+                ++out.n;
+            } else
+                ok = false;
+        }
+        if (inA != inB) {
+            double t = fa / (fa - fb);
+            if (out.n < kMaxCellVerts) {
+                out.vx[out.n] = ax + t * (bx - ax), out.vy[out.n] = ay + t * (by - ay);
+                out.e[out.n]  = inA ? label : poly.e[k];
+                ++out.n;
+            } else
+                ok = false;
+        }
+    }
+    poly = out;
+    return ok;
+}
+
+__device__ double maxRadius2(const CellPoly &p)
+{
+    double r2 = 0.0;
+    for (int k = 0; k < p.n; ++k) r2 = fmax(r2, p.vx[k] * p.vx[k] + p.vy[k] * p.vy[k]);
+    return r2;
+}
+
+__device__ bool occupied(const uint64_t *masks, int tilesX, int tilesY, float invVoxel, float px, float py)
+{
+    if (!masks) return false;
+    int i = int(floorf(px * invVoxel)), j = int(floorf(py * invVoxel));
+    if (i < 0 || j < 0) return false;
+    int tx = i >> 3, ty = j >> 3;
+    if (tx >= tilesX || ty >= tilesY) return false;
+    return (masks[int64_t(ty) * tilesX + tx] >> (((j & 7) << 3) | (i & 7))) & 1ull;
+}
+
+template <class R>
+__global__ __launch_bounds__(64) void clipKernel(ClipArgs<R> a)
+{
+    const Inst &in = a.insts[a.inst];
+    int         i  = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= in.S) return;
+    const V2<R> *pos = a.pos + in.siteOff;
+    V2<R>        ci  = pos[i];
+    Key<R>       keys[kCellK];
+    knnGlobal<kCellK>(in, a.spos + in.siteOff, a.sidx + in.siteOff, a.start + in.cellOff, ci, keys);
+
+    CellPoly poly;
+    double   x0 = -double(ci.x), y0 = -double(ci.y), x1 = double(in.W) - double(ci.x), y1 = double(in.H) - double(ci.y);
+    poly.vx[0] = x0, poly.vy[0] = y0, poly.e[0] = -1;
+    poly.vx[1] = x1, poly.vy[1] = y0, poly.e[1] = -2;
+    poly.vx[2] = x1, poly.vy[2] = y1, poly.e[2] = -3;
+    poly.vx[3] = x0, poly.vy[3] = y1, poly.e[3] = -4;
+    poly.n     = 4;
+    bool   ok = true, certified = false;
+    double r2max = maxRadius2(poly);
+    int    idx[kCellK];
+#pragma unroll
+    for (int k = 0; k < kCellK; ++k) idx[k] = keys[k].valid() ? keySite(keys[k].low()) : -1;
+    for (int k = 0; k < kCellK; ++k) {
+        int j = idx[k];
+        if (j < 0) { certified = true; break; }
+        if (j == i) continue;
+        if (double(keys[k].d2()) > 4.0 * r2max) { certified = true; break; }
+        V2<R> cj = pos[j];
+        ok &= clipByBisector(poly, double(cj.x) - double(ci.x), double(cj.y) - double(ci.y), j);
+        r2max = maxRadius2(poly);
+    }
+    if (!certified && double(keys[kCellK - 1].d2()) > 4.0 * r2max) certified = true;
+    if (!certified) {
+        atomicAdd(&a.flags[0], 1);
+        for (int j = 0; j < in.S; ++j) {
+            if (j == i) continue;
+            bool seen = false;
+            for (int k = 0; k < kCellK; ++k) seen |= idx[k] == j;
+            if (seen) continue;
+            V2<R>  cj = pos[j];
+            double rx = double(cj.x) - double(ci.x), ry = double(cj.y) - double(ci.y);
+            if (rx * rx + ry * ry > 4.0 * r2max) continue;
+            ok &= clipByBisector(poly, rx, ry, j);
+            r2max = maxRadius2(poly);
+        }
+    }
+    if (!ok) atomicAdd(&a.flags[1], 1);
+
+    double area = 0.0, cx = 0.0, cy = 0.0;
+    for (int k = 0; k < poly.n; ++k) {
+        int    k1 = (k + 1 == poly.n) ? 0 : k + 1;
+        double cr = poly.vx[k] * poly.vy[k1] - poly.vx[k1] * poly.vy[k];
+        area += cr;
+        cx += (poly.vx[k] + poly.vx[k1]) * cr;
+        cy += (poly.vy[k] + poly.vy[k1]) * cr;
+    }
+    float cenx = float(ci.x), ceny = float(ci.y);
+    if (fabs(area) > 1e-12) {
+        cenx = cenx + float(cx / (3.0 * area));
+        ceny = ceny + float(cy / (3.0 * area));
+    }
+    a.inside[i] = occupied(a.masks, a.tilesX, a.tilesY, a.invVoxel, cenx, ceny) ? 1 : 0;
+    a.count[i]  = poly.n;
+    for (int k = 0; k < poly.n; ++k) {
+        a.verts[(int64_t(i) * kMaxCellVerts + k) * 2 + 0] = float(double(ci.x) + poly.vx[k]);
+        a.verts[(int64_t(i) * kMaxCellVerts + k) * 2 + 1] = float(double(ci.y) + poly.vy[k]);
+        a.labels[int64_t(i) * kMaxCellVerts + k]          = poly.e[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Poisson-disk dart throwing over a table of instance grids
+// (poisson_disk.cu:138-177).  Cells of one phase group are >= 3 apart, so
+// every thread writes only its own cell and the result is deterministic.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void poissonClearKernel(const PoiInst *__restrict__ insts,
+                                                          const PoiWork *__restrict__ work, int nWork, float *grid)
+{
+    // Clear chunks cover the whole region, one cell per thread.
+    for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
+        PoiWork        wk = work[w];
+        const PoiInst &pi = insts[wk.inst];
+        int            rw = pi.x1 - pi.x0, rh = pi.y1 - pi.y0;
+        int            c  = wk.base + int(threadIdx.x);
+        if (c >= rw * rh) continue;
+        int     cx = pi.x0 + c % rw, cy = pi.y0 + c / rw;
+        float2 *g  = reinterpret_cast<float2 *>(grid) + pi.gridOff;
+        g[int64_t(cy) * pi.gw + cx] = make_float2(-1.0f, -1.0f);
+    }
+}
+
+__global__ __launch_bounds__(256) void poissonPhaseKernel(const PoiInst *__restrict__ insts,
+                                                          const PoiWork *__restrict__ work, int nWork, float *grid,
+                                                          int px, int py, int round)
+{
+    for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
+        PoiWork        wk = work[w];
+        const PoiInst &pi = insts[wk.inst];
+        int            rw = pi.x1 - pi.x0, rh = pi.y1 - pi.y0;
+        int            nx = (rw - px + 2) / 3, ny = (rh - py + 2) / 3;
+        int            k  = wk.base + int(threadIdx.x);
+        if (nx <= 0 || ny <= 0 || k >= nx * ny) continue;
+        int     cx = pi.x0 + 3 * (k % nx) + px, cy = pi.y0 + 3 * (k / nx) + py;
+        float2 *g  = reinterpret_cast<float2 *>(grid) + pi.gridOff;
+        int     cellId = cy * pi.gw + cx;
+        if (g[cellId].x >= 0.0f) continue;
+        uint32_t rnd = pi.roundBase + uint32_t(round);
+        float    x   = (float(cx) + rand01_poisson(uint32_t(cellId), rnd, pi.seed, 0)) * pi.cellSize;
+        float    y   = (float(cy) + rand01_poisson(uint32_t(cellId), rnd, pi.seed, 1)) * pi.cellSize;
+        if (x >= pi.W || y >= pi.H) continue;
+        float r2 = pi.radius * pi.radius;
+        int   nx0 = max(cx - 2, 0), nx1 = min(cx + 2, pi.gw - 1);
+        int   ny0 = max(cy - 2, 0), ny1 = min(cy + 2, pi.gh - 1);
+        bool  ok  = true;
+        for (int yy = ny0; yy <= ny1 && ok; ++yy)
+            for (int xx = nx0; xx <= nx1; ++xx) {
+                float2 q = g[yy * pi.gw + xx];
+                if (q.x < 0.0f) continue;
+                float dx = q.x - x, dy = q.y - y;
+                if (__builtin_fmaf(dy, dy, dx * dx) < r2) { ok = false; break; }
+            }
+        if (ok) g[cellId] = make_float2(x, y);
+    }
+}
+
+__global__ void fillIntKernel(int *p, int64_t n, int v)
+{
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+// ===========================================================================
+// Host-side launchers
+// ===========================================================================
+hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, const int *instList,
+                      const int *nInst, int *counts, int *start, int mode)
+{
+    hipLaunchKernelGGL(scanKernel, dim3(grid), dim3(kScanThreads), 0, s, insts, state, instList, nInst, counts, start,
+                       mode);
+    return hipGetLastError();
+}
+
+template <class R>
+hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstState *state, const SiteWork *work,
+                         const int *nWork, const void *pos, const int *cellOf, const int *rank, const int *start,
+                         void *spos, int *sidx, int mode)
+{
+    hipLaunchKernelGGL(scatterKernel<R>, dim3(grid), dim3(kSiteChunk), 0, s, insts, state, work, nWork,
+                       static_cast<const V2<R> *>(pos), cellOf, rank, start, static_cast<V2<R> *>(spos), sidx, mode);
+    return hipGetLastError();
+}
+
+template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const LossArgs<R> &a)
+{
+    switch (K) {
+#define VM_K(k)                                                                                                   \
+    case k: hipLaunchKernelGGL((lossKernel<k, R>), dim3(grid), dim3(kLossThreads), 0, s, a); break;
+        VM_K(2) VM_K(4) VM_K(6) VM_K(8) VM_K(12) VM_K(16) VM_K(24) VM_K(32)
+#undef VM_K
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class R> hipError_t launchAdamBin(int grid, hipStream_t s, const AdamArgs<R> &a)
+{
+    hipLaunchKernelGGL(adamBinKernel<R>, dim3(grid), dim3(kSiteChunk), 0, s, a);
+    return hipGetLastError();
+}
+
+template <class R>
+hipError_t launchKnnQuery(int K, hipStream_t s, const Inst *insts, int inst, const void *spos, const int *sidx,
+                          const int *start, const void *q, int nq, int *outIdx, void *outD2)
+{
+    dim3 grid((nq + 63) / 64), block(64);
+    if (nq <= 0) return hipSuccess;
+    auto *sp = static_cast<const V2<R> *>(spos);
+    auto *qq = static_cast<const V2<R> *>(q);
+    auto *od = static_cast<R *>(outD2);
+    if (K <= 8) hipLaunchKernelGGL((knnQueryKernel<8, R>), grid, block, 0, s, insts, inst, sp, sidx, start, qq, nq, K, outIdx, od);
+    else if (K <= 16) hipLaunchKernelGGL((knnQueryKernel<16, R>), grid, block, 0, s, insts, inst, sp, sidx, start, qq, nq, K, outIdx, od);
+    else if (K <= 32) hipLaunchKernelGGL((knnQueryKernel<32, R>), grid, block, 0, s, insts, inst, sp, sidx, start, qq, nq, K, outIdx, od);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <class R> hipError_t launchClip(int nSites, hipStream_t s, const ClipArgs<R> &a)
+{
+    if (nSites <= 0) return hipSuccess;
+    hipLaunchKernelGGL(clipKernel<R>, dim3((nSites + 63) / 64), dim3(64), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchPoissonClear(int grid, hipStream_t s, const PoiInst *insts, const PoiWork *work, int nWork, float *g)
+{
+    if (nWork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(poissonClearKernel, dim3(grid), dim3(256), 0, s, insts, work, nWork, g);
+    return hipGetLastError();
+}
+
+hipError_t launchPoissonPhase(int grid, hipStream_t s, const PoiInst *insts, const PoiWork *work, int nWork, float *g,
+                              int px, int py, int round)
+{
+    if (nWork <= 0) return hipSuccess;
+    hipLaunchKernelGGL(poissonPhaseKernel, dim3(grid), dim3(256), 0, s, insts, work, nWork, g, px, py, round);
+    return hipGetLastError();
+}
+
+hipError_t launchFillInt(hipStream_t s, int *p, int64_t n, int v)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(fillIntKernel, dim3(unsigned(blocks > 4096 ? 4096 : blocks)), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+template hipError_t launchScatter<float>(int, hipStream_t, const Inst *, const InstState *, const SiteWork *,
+                                         const int *, const void *, const int *, const int *, const int *, void *,
+                                         int *, int);
+template hipError_t launchScatter<double>(int, hipStream_t, const Inst *, const InstState *, const SiteWork *,
+                                          const int *, const void *, const int *, const int *, const int *, void *,
+                                          int *, int);
+template hipError_t launchLoss<float>(int, int, hipStream_t, const LossArgs<float> &);
+template hipError_t launchLoss<double>(int, int, hipStream_t, const LossArgs<double> &);
+template hipError_t launchAdamBin<float>(int, hipStream_t, const AdamArgs<float> &);
+template hipError_t launchAdamBin<double>(int, hipStream_t, const AdamArgs<double> &);
+template hipError_t launchKnnQuery<float>(int, hipStream_t, const Inst *, int, const void *, const int *, const int *,
+                                          const void *, int, int *, void *);
+template hipError_t launchKnnQuery<double>(int, hipStream_t, const Inst *, int, const void *, const int *,
+                                           const int *, const void *, int, int *, void *);
+template hipError_t launchClip<float>(int, hipStream_t, const ClipArgs<float> &);
+template hipError_t launchClip<double>(int, hipStream_t, const ClipArgs<double> &);
+
+} // namespace vmk

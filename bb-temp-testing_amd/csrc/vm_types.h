@@ -1,0 +1,81 @@
+// vm_types.h -- layouts shared by the HIP kernels and the host runtime.
+//
+// Every optimizer instance ("fit") lives in pooled device arrays and is
+// described by one Inst record in a packed instance table.  The per-iteration
+// kernels walk work tables (site chunks, target chunks, instance ids), so the
+// same four launches serve one fit or hundreds of them.
+#pragma once
+#include <stdint.h>
+
+namespace vmk {
+
+constexpr int kMaxCellVerts = 24; // clipped cell vertex budget (voro_optimizer.cu:56)
+constexpr int kCellK        = 32; // clip neighbours before brute force (voro_optimizer.cu:55)
+constexpr int kBoxLabels    = 4;  // edge labels -1..-4 are the box sides (voro_optimizer.cu:58-59)
+constexpr int kSiteChunk    = 256;
+constexpr int kTargetChunk  = 64;  // one wavefront of targets, all in one hash cell
+constexpr int kLdsSlotBits  = 10;  // low bits of a k-NN key: slot of the site in the staged LDS block
+constexpr int kLdsNone      = (1 << kLdsSlotBits) - 1;
+constexpr int kStageCap     = kLdsNone - 1; // staged sites per target chunk
+constexpr int kMaxSitesPerInst = 1 << 22;  // instance-local site index fits above the LDS slot bits
+
+enum : int { kModeIterate = 0, kModeEvaluate = 1 };
+
+// One optimizer instance.  Offsets are element offsets into the pooled arrays.
+struct Inst {
+    float   W, H;         // domain [0,W] x [0,H]
+    float   cell, inv;    // hash-grid cell size and 1/cell (FP32, as the kernels bin with it)
+    float   ext;          // max(W, H): scale of the conservative ring-stop margin
+    int     gw, gh;       // hash grid dimensions
+    int     S, T;         // live sites, live targets
+    int     K;            // loss neighbours
+    int     stageR;       // staged ring radius of the loss kernel (block of (2r+1)^2 cells)
+    int     allMovable;   // 1: every site moves; 0: consult the frozen mask
+    int64_t siteOff;      // sites, m, v, grad, cellOf, rank, sorted pos/idx, frozen
+    int64_t cellOff;      // counts and start, gw*gh + 1 entries
+    int64_t targetOff;    // sorted targets and their slot numbers
+    double  lr, gamma, beta1, beta2, eps; // rounded to the kernel's real type before use
+    int     milestones[4];
+    float   fraction;     // target subsampling fraction
+    uint32_t seed;
+    int     useBox;       // targets outside [bx0,bx1]x[by0,by1] are ignored
+    float   bx0, by0, bx1, by1;
+    int     maxIter;      // 0: no limit
+    float   tol;          // relative loss-change convergence threshold (0: off)
+    int     pad[3];
+};
+
+// Written by the device every iteration (voro_optimizer.cu:81-85 IterState,
+// with the loss sum kept in FP64 and convergence bookkeeping added).
+struct InstState {
+    int     iter;      // Adam step count, 1-based once a step has started
+    int     done;      // converged or reached maxIter: kernels skip the instance
+    int64_t count;     // targets used by the loss of the current step
+    double  loss;      // sum of their VoroLoss
+    double  prevMean;  // mean loss of the previous step (convergence test)
+    double  evalLoss;  // EvaluateLoss accumulators
+    int64_t evalCount;
+};
+
+struct SiteWork {
+    int inst, begin, count, pad;
+};
+
+struct TargetWork {
+    int inst, begin, count, cell; // targets [begin, begin+count) of `inst`, all in hash cell `cell`
+};
+
+// Poisson dart thrower: one record per instance grid being (re)sampled.
+struct PoiInst {
+    float    W, H, radius, cellSize;
+    int      gw, gh;
+    int      x0, y0, x1, y1; // region being sampled
+    uint32_t seed, roundBase;
+    int64_t  gridOff;        // float2 offset of the instance grid
+};
+
+struct PoiWork {
+    int inst, base; // phase-cell indices [base, base + blockDim) of instance `inst`
+};
+
+} // namespace vmk

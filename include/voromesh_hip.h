@@ -1,0 +1,227 @@
+/* voromesh_hip.h -- C ABI of libvoromesh_hip.so, the MI355X (gfx950) VoroMesh
+ * optimizer.
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success and a
+ * nonzero vm_status otherwise; vm_last_error() gives the message of the last
+ * failure on the calling thread.  Host arrays are read or written during the
+ * call and never retained (as in the reference, which copies synchronously).
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *   vm_create ... vm_extract     voromesh::VoroOptimizer, include/voromesh/voro_optimizer.h:70-138
+ *                                 (implementation src/voromesh/voro_optimizer.cu:451-860)
+ *   vm_sampler_*                 poisson::Sampler, include/poisson/poisson_disk.h:21-62
+ *                                 (implementation src/poisson/poisson_disk.cu:117-201)
+ *   vm_batch_*, vm_serve         no counterpart in the reference: many fits in one set of
+ *                                 launches, and a continuous-batching request loop.
+ */
+#ifndef VOROMESH_HIP_H
+#define VOROMESH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define VM_API __attribute__((visibility("default")))
+#else
+#define VM_API
+#endif
+
+typedef enum vm_status {
+    VM_OK            = 0,
+    VM_EINVAL        = 1, /* invalid argument */
+    VM_EHIP          = 2, /* HIP runtime error */
+    VM_ENOMEM        = 3, /* allocation failed or over budget */
+    VM_ESTATE        = 4, /* call not valid in the current state */
+    VM_EUNSUPPORTED  = 5
+} vm_status;
+
+/* Layout-identical to voromesh::Vec2 / poisson::Sample / float2
+ * (include/voromesh/shape2d.h:11-14, include/poisson/poisson_disk.h:16-19). */
+typedef struct vm_vec2 {
+    float x, y;
+} vm_vec2;
+
+/* voromesh::OptimizerParams (include/voromesh/voro_optimizer.h:34-44) plus the
+ * knobs the reference fixes at compile time. */
+typedef struct vm_params {
+    int      iterations;        /* 400: schedule length the milestones are given for */
+    int      local_iterations;  /* 100 */
+    float    lr_per_spacing;    /* 0.09: Adam lr in units of the spacing */
+    float    sample_fraction;   /* 0.2: targets used per step */
+    float    gamma;             /* 0.5: MultiStepLR decay */
+    int      milestones[4];     /* {80, 150, 200, 250} */
+    unsigned seed;              /* 1: subsampling hash seed */
+    int      loss_k;            /* 12: neighbours per target (reference kLossK, voro_optimizer.cu:54) */
+    float    grid_cell;         /* hash-grid cell in world units; 0: 2 x spacing */
+    int      stage_rings;       /* cell rings staged in LDS by the loss kernel (1 or 2); 0: auto */
+    int      precision;         /* 0: FP32 sites/gradients/Adam; 1: all FP64 */
+} vm_params;
+
+typedef struct vm_handle vm_handle;
+
+/* Clipped diagram and boundary mesh (voromesh::VoroMesh2D,
+ * include/voromesh/voro_optimizer.h:46-68).  Arrays are owned by the handle
+ * and stay valid until the next vm_extract or vm_destroy. */
+typedef struct vm_mesh {
+    int            num_slots;
+    const vm_vec2 *generators;  /* [num_slots] (x < 0: empty slot) */
+    const uint8_t *inside;      /* [num_slots] */
+    const int32_t *cell_start;  /* [num_slots + 1] CSR offsets into cell_verts */
+    const vm_vec2 *cell_verts;  /* [cell_start[num_slots]] */
+    const int32_t *cell_labels; /* [cell_start[num_slots]]: label of edge v[k] -> v[k+1]; >= 0 neighbour slot, -1..-4 box */
+    int            num_vertices;
+    const vm_vec2 *vertices;
+    int            num_edges;
+    const int32_t *edges;       /* [num_edges * 2] */
+    int            watertight;
+    int            loops;
+    int            unsafe_cells;
+    int            overflow_cells;
+    float          gpu_ms;
+    float          host_ms;
+} vm_mesh;
+
+VM_API void        vm_default_params(vm_params *p);
+VM_API const char *vm_last_error(void);
+VM_API const char *vm_version(void);
+/* Number of HIP devices visible (0 without a GPU); never fails. */
+VM_API int         vm_device_count(void);
+
+/* ---- voromesh::VoroOptimizer --------------------------------------------- */
+/* VoroOptimizer(width, height, spacing, params)  voro_optimizer.h:75, .cu:512-532 */
+VM_API int vm_create(float width, float height, float spacing, const vm_params *params, vm_handle **out);
+/* ~VoroOptimizer  .cu:534-547 */
+VM_API int vm_destroy(vm_handle *h);
+/* SetTargets(slots)  voro_optimizer.h:82, .cu:549-560; slots with x < 0 or NaN are empty */
+VM_API int vm_set_targets(vm_handle *h, const vm_vec2 *slots, int n);
+/* UpdateTargets(dirty, slots)  voro_optimizer.h:84, .cu:562-587 */
+VM_API int vm_update_targets(vm_handle *h, const int32_t *dirty, int num_dirty, const vm_vec2 *slots, int num_slots);
+/* SetOccupancy(masks, tilesX, tilesY, voxelSize)  voro_optimizer.h:87, .cu:589-599 */
+VM_API int vm_set_occupancy(vm_handle *h, const uint64_t *masks, int tiles_x, int tiles_y, float voxel_size);
+/* UpdateOccupancy(masks, tx0, ty0, tx1, ty1)  voro_optimizer.h:89, .cu:601-611 */
+VM_API int vm_update_occupancy(vm_handle *h, const uint64_t *masks, int tx0, int ty0, int tx1, int ty1);
+/* SetGenerators(slots, slotsW, slotsH, slotCellSize)  voro_optimizer.h:94, .cu:613-638 */
+VM_API int vm_set_generators(vm_handle *h, const vm_vec2 *slots, int slots_w, int slots_h, float slot_cell_size);
+/* UpdateGenerators(slots, cx0, cy0, cx1, cy1)  voro_optimizer.h:96, .cu:640-650 */
+VM_API int vm_update_generators(vm_handle *h, const vm_vec2 *slots, int cx0, int cy0, int cx1, int cy1);
+/* BeginSchedule(iterations, cx0, cy0, cx1, cy1)  voro_optimizer.h:102-103, .cu:652-687 */
+VM_API int vm_begin_schedule(vm_handle *h, int iterations, int cx0, int cy0, int cx1, int cy1);
+/* Step(iterations) -> GPU ms  voro_optimizer.h:106, .cu:724-744 */
+VM_API int vm_step(vm_handle *h, int iterations, float *gpu_ms);
+/* Iteration(), ScheduleLength(), LastBatchLoss(), NumSlots()  voro_optimizer.h:111,118-120 */
+VM_API int vm_get_state(vm_handle *h, int *iteration, int *schedule_length, float *last_batch_loss, int *num_slots);
+/* EvaluateLoss()  voro_optimizer.h:113, .cu:746-761 (mean over all targets, FP64 sum) */
+VM_API int vm_evaluate_loss(vm_handle *h, double *mean_loss);
+/* Extract()  voro_optimizer.h:116, .cu:763-858 */
+VM_API int vm_extract(vm_handle *h, vm_mesh *out);
+/* Current generator slots (x < 0: empty).  n must be NumSlots(). */
+VM_API int vm_get_generators(vm_handle *h, vm_vec2 *slots, int n);
+
+/* ---- diagnostics (parity tests) --------------------------------------------- */
+/* k nearest live generator slots of each query by the device ring search; d2 in the
+ * optimizer's arithmetic.  Missing entries: idx -1. */
+VM_API int vm_knn(vm_handle *h, const vm_vec2 *queries, int num_queries, int k, int32_t *idx, float *d2);
+/* One loss pass over all live targets at the current generators, as the loss kernel
+ * runs it (no subsampling, no target box), without touching the schedule.  Outputs
+ * (each optional): summed gradient per slot [num_slots*2], loss sum and count, and per
+ * target slot: the k-NN list [n_target_slots*loss_k], nearest slot, bisector partner slot
+ * and the loss term. */
+VM_API int vm_loss_pass(vm_handle *h, double *grad_xy, double *loss_sum, int64_t *count, int32_t *knn,
+                        int32_t *nearest, int32_t *partner, double *loss_terms);
+
+/* ---- poisson::Sampler ------------------------------------------------------ */
+typedef struct vm_sampler vm_sampler;
+/* Sampler(width, height, radius, trials, seed)  poisson_disk.h:24, .cu:119-132 (runs ResampleAll) */
+VM_API int vm_sampler_create(float width, float height, float radius, int trials, unsigned seed, vm_sampler **out);
+VM_API int vm_sampler_destroy(vm_sampler *s);
+/* Resample(cx0, cy0, cx1, cy1) -> GPU ms  poisson_disk.h:33, .cu:141-177 */
+VM_API int vm_sampler_resample(vm_sampler *s, int cx0, int cy0, int cx1, int cy1, float *gpu_ms);
+/* Verify(cx0, cy0, cx1, cy1)  poisson_disk.h:38, .cu:179-199; *ok = 1 if valid */
+VM_API int vm_sampler_verify(vm_sampler *s, int cx0, int cy0, int cx1, int cy1, int *ok);
+/* GridW(), GridH(), CellSize()  poisson_disk.h:44-50 */
+VM_API int vm_sampler_dims(vm_sampler *s, int *grid_w, int *grid_h, float *cell_size);
+/* Cells(): host mirror, grid_w * grid_h entries  poisson_disk.h:42 */
+VM_API int vm_sampler_cells(vm_sampler *s, vm_vec2 *cells, int n);
+
+/* Many independent samplers in one set of launches (batched initialisation).
+ * Instance i samples [0,w[i]]x[0,h[i]] with radius r[i] and seed seed[i], `trials`
+ * rounds; its grid (dims from vm_poisson_grid_dims) is written to cells + offsets[i]. */
+VM_API int vm_poisson_grid_dims(float width, float height, float radius, int *grid_w, int *grid_h, float *cell_size);
+VM_API int vm_poisson_batch(int n, const float *w, const float *h, const float *r, const unsigned *seed, int trials,
+                            const int64_t *offsets, vm_vec2 *cells, float *gpu_ms);
+
+/* ---- batched fits: one set of launches per iteration for many instances ------ */
+typedef struct vm_fit_desc {
+    float          width, height;   /* domain [0,width] x [0,height] */
+    const vm_vec2 *sites;           /* live sites (dense) */
+    int            num_sites;
+    const vm_vec2 *targets;         /* live targets */
+    int            num_targets;
+    float          lr;              /* Adam learning rate */
+    float          gamma;
+    int            milestones[4];   /* already scaled to this schedule */
+    float          sample_fraction;
+    unsigned       seed;
+    int            loss_k;          /* 0: tuner choice for the size class */
+    float          grid_cell;       /* 0: tuner choice */
+    int            stage_rings;     /* 0: tuner choice */
+    int            max_iterations;  /* 0: no limit */
+    float          tol;             /* stop when |dL/L| < tol between steps (0: off) */
+} vm_fit_desc;
+
+typedef struct vm_batch vm_batch;
+VM_API int vm_batch_create(int precision, vm_batch **out);
+VM_API int vm_batch_destroy(vm_batch *b);
+/* Adds an instance; returns its id.  Valid until vm_batch_commit. */
+VM_API int vm_batch_add(vm_batch *b, const vm_fit_desc *desc, int *id);
+/* Lays out, uploads and bins every instance and captures one iteration as a graph. */
+VM_API int vm_batch_commit(vm_batch *b);
+/* Runs `iterations` optimizer iterations of every instance (graph replays). */
+VM_API int vm_batch_step(vm_batch *b, int iterations, float *gpu_ms);
+VM_API int vm_batch_get_sites(vm_batch *b, int id, vm_vec2 *out, int n);
+/* Per instance: completed iterations, done flag and the mean loss of its last step. */
+VM_API int vm_batch_status(vm_batch *b, int id, int *iteration, int *done, double *last_mean_loss);
+/* Resolved per-instance choices (after commit): k, grid cell, staged rings. */
+VM_API int vm_batch_choice(vm_batch *b, int id, int *loss_k, float *grid_cell, int *stage_rings);
+/* Mean of the kernel durations (ms) of the last vm_batch_step, measured with HIP events
+ * around each kernel on the batch's stream; phase 0 scan, 1 scatter, 2 loss, 3 adam. */
+VM_API int vm_batch_profile(vm_batch *b, int iterations, float *phase_ms);
+
+/* ---- continuous-batching request loop ---------------------------------------- */
+typedef struct vm_serve_config {
+    double   hbm_budget_bytes; /* device bytes available to resident fit state */
+    int      max_resident;     /* instance slots (0: unlimited) */
+    int      max_iterations;   /* per request (e.g. 300) */
+    float    tol;              /* per request relative loss-change stop */
+    int      poll_every;       /* iterations between completion polls */
+    int      precision;
+} vm_serve_config;
+
+typedef struct vm_serve_result {
+    double latency_s;     /* completion - arrival */
+    double queue_s;       /* admission - arrival */
+    int    iterations;
+    double final_mean_loss;
+} vm_serve_result;
+
+typedef struct vm_serve_stats {
+    double wall_s;
+    double site_iters;
+    int    completed;
+    int    max_resident_seen;
+    double peak_bytes;
+} vm_serve_stats;
+
+/* Serves requests[i] arriving at arrival_s[i] seconds after the call starts
+ * (non-decreasing).  Returns when every request has completed. */
+VM_API int vm_serve(const vm_fit_desc *requests, const double *arrival_s, int n, const vm_serve_config *cfg,
+                    vm_serve_result *results, vm_serve_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOROMESH_HIP_H */
