@@ -1,0 +1,441 @@
+"""Python front end of libvoromesh_hip.so (ctypes over the C ABI in include/voromesh_hip.h).
+
+Mirrors the reference's C++ interfaces for the VoroMesh path:
+  * ``Optimizer``  -- voromesh::VoroOptimizer (include/voromesh/voro_optimizer.h:70-138)
+  * ``Sampler``    -- poisson::Sampler (include/poisson/poisson_disk.h:21-62)
+and adds ``Batch`` (many fits per launch) and ``poisson_batch``.
+
+The library is required: importing this module raises if it has not been built
+(``python -c "import __graft_entry__ as g; g.build()"``).  There is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "libvoromesh_hip.so"))
+
+MAX_CELL_VERTS = 24
+
+
+class VmError(RuntimeError):
+    pass
+
+
+class Vec2(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+class Params(C.Structure):
+    """vm_params: voromesh::OptimizerParams plus the reference's compile-time knobs."""
+    _fields_ = [("iterations", C.c_int), ("local_iterations", C.c_int), ("lr_per_spacing", C.c_float),
+                ("sample_fraction", C.c_float), ("gamma", C.c_float), ("milestones", C.c_int * 4),
+                ("seed", C.c_uint), ("loss_k", C.c_int), ("grid_cell", C.c_float), ("stage_rings", C.c_int),
+                ("precision", C.c_int)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("num_slots", C.c_int), ("generators", C.POINTER(Vec2)), ("inside", C.POINTER(C.c_uint8)),
+                ("cell_start", C.POINTER(C.c_int32)), ("cell_verts", C.POINTER(Vec2)),
+                ("cell_labels", C.POINTER(C.c_int32)), ("num_vertices", C.c_int),
+                ("vertices", C.POINTER(Vec2)), ("num_edges", C.c_int), ("edges", C.POINTER(C.c_int32)),
+                ("watertight", C.c_int), ("loops", C.c_int), ("unsafe_cells", C.c_int),
+                ("overflow_cells", C.c_int), ("gpu_ms", C.c_float), ("host_ms", C.c_float)]
+
+
+class FitDesc(C.Structure):
+    _fields_ = [("width", C.c_float), ("height", C.c_float), ("sites", C.c_void_p), ("num_sites", C.c_int),
+                ("targets", C.c_void_p), ("num_targets", C.c_int), ("lr", C.c_float), ("gamma", C.c_float),
+                ("milestones", C.c_int * 4), ("sample_fraction", C.c_float), ("seed", C.c_uint),
+                ("loss_k", C.c_int), ("grid_cell", C.c_float), ("stage_rings", C.c_int),
+                ("max_iterations", C.c_int), ("tol", C.c_float)]
+
+
+class ServeConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_double), ("max_resident", C.c_int), ("max_iterations", C.c_int),
+                ("tol", C.c_float), ("poll_every", C.c_int), ("precision", C.c_int)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("latency_s", C.c_double), ("queue_s", C.c_double), ("iterations", C.c_int),
+                ("final_mean_loss", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("site_iters", C.c_double), ("completed", C.c_int),
+                ("max_resident_seen", C.c_int), ("peak_bytes", C.c_double)]
+
+
+# Every symbol include/voromesh_hip.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "vm_default_params", "vm_last_error", "vm_version", "vm_device_count",
+    "vm_create", "vm_destroy", "vm_set_targets", "vm_update_targets", "vm_set_occupancy", "vm_update_occupancy",
+    "vm_set_generators", "vm_update_generators", "vm_begin_schedule", "vm_step", "vm_get_state",
+    "vm_evaluate_loss", "vm_extract", "vm_get_generators", "vm_knn", "vm_loss_pass",
+    "vm_sampler_create", "vm_sampler_destroy", "vm_sampler_resample", "vm_sampler_verify", "vm_sampler_dims",
+    "vm_sampler_cells", "vm_poisson_grid_dims", "vm_poisson_batch",
+    "vm_batch_create", "vm_batch_destroy", "vm_batch_add", "vm_batch_commit", "vm_batch_step",
+    "vm_batch_get_sites", "vm_batch_status", "vm_batch_choice", "vm_batch_profile", "vm_serve",
+]
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise VmError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, f32, f64 = C.c_void_p, C.c_int, C.c_float, C.c_double
+    P = C.POINTER
+    L.vm_last_error.restype = C.c_char_p
+    L.vm_version.restype = C.c_char_p
+    L.vm_default_params.argtypes = [P(Params)]
+    L.vm_device_count.restype = i32
+    sig = {
+        "vm_create": [f32, f32, f32, P(Params), P(vp)],
+        "vm_destroy": [vp],
+        "vm_set_targets": [vp, vp, i32],
+        "vm_update_targets": [vp, vp, i32, vp, i32],
+        "vm_set_occupancy": [vp, vp, i32, i32, f32],
+        "vm_update_occupancy": [vp, vp, i32, i32, i32, i32],
+        "vm_set_generators": [vp, vp, i32, i32, f32],
+        "vm_update_generators": [vp, vp, i32, i32, i32, i32],
+        "vm_begin_schedule": [vp, i32, i32, i32, i32, i32],
+        "vm_step": [vp, i32, P(f32)],
+        "vm_get_state": [vp, P(i32), P(i32), P(f32), P(i32)],
+        "vm_evaluate_loss": [vp, P(f64)],
+        "vm_extract": [vp, P(Mesh)],
+        "vm_get_generators": [vp, vp, i32],
+        "vm_knn": [vp, vp, i32, i32, vp, vp],
+        "vm_loss_pass": [vp, vp, vp, vp, vp, vp, vp, vp],
+        "vm_sampler_create": [f32, f32, f32, i32, C.c_uint, P(vp)],
+        "vm_sampler_destroy": [vp],
+        "vm_sampler_resample": [vp, i32, i32, i32, i32, P(f32)],
+        "vm_sampler_verify": [vp, i32, i32, i32, i32, P(i32)],
+        "vm_sampler_dims": [vp, P(i32), P(i32), P(f32)],
+        "vm_sampler_cells": [vp, vp, i32],
+        "vm_poisson_grid_dims": [f32, f32, f32, P(i32), P(i32), P(f32)],
+        "vm_poisson_batch": [i32, vp, vp, vp, vp, i32, vp, vp, P(f32)],
+        "vm_batch_create": [i32, P(vp)],
+        "vm_batch_destroy": [vp],
+        "vm_batch_add": [vp, P(FitDesc), P(i32)],
+        "vm_batch_commit": [vp],
+        "vm_batch_step": [vp, i32, P(f32)],
+        "vm_batch_get_sites": [vp, i32, vp, i32],
+        "vm_batch_status": [vp, i32, P(i32), P(i32), P(f64)],
+        "vm_batch_choice": [vp, i32, P(i32), P(f32), P(i32)],
+        "vm_batch_profile": [vp, i32, vp],
+        "vm_serve": [P(FitDesc), vp, i32, P(ServeConfig), vp, P(ServeStats)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = i32
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise VmError(f"status {rc}: {lib().vm_last_error().decode(errors='replace')}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _pts(a):
+    return np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1, 2))
+
+
+def device_count() -> int:
+    return int(lib().vm_device_count())
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().vm_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "milestones":
+            for i in range(4):
+                p.milestones[i] = int(v[i])
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Optimizer:
+    """voromesh::VoroOptimizer over the HIP library (same methods, same semantics)."""
+
+    def __init__(self, width, height, spacing, params: Params | None = None, **kw):
+        self._h = C.c_void_p()
+        self.params = params if params is not None else default_params(**kw)
+        _check(lib().vm_create(width, height, spacing, C.byref(self.params), C.byref(self._h)))
+        self.width, self.height, self.spacing = width, height, spacing
+        self.slots_w = self.slots_h = 0
+
+    def close(self):
+        if self._h:
+            lib().vm_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_targets(self, slots):
+        s = _pts(slots)
+        _check(lib().vm_set_targets(self._h, _ptr(s), len(s)))
+
+    def update_targets(self, dirty, slots):
+        d = np.ascontiguousarray(dirty, np.int32)
+        s = _pts(slots)
+        _check(lib().vm_update_targets(self._h, _ptr(d), len(d), _ptr(s), len(s)))
+
+    def set_occupancy(self, masks, tiles_x, tiles_y, voxel):
+        m = np.ascontiguousarray(masks, np.uint64)
+        _check(lib().vm_set_occupancy(self._h, _ptr(m), tiles_x, tiles_y, voxel))
+
+    def update_occupancy(self, masks, tx0, ty0, tx1, ty1):
+        m = np.ascontiguousarray(masks, np.uint64)
+        _check(lib().vm_update_occupancy(self._h, _ptr(m), tx0, ty0, tx1, ty1))
+
+    def set_generators(self, slots, slots_w, slots_h, slot_cell):
+        s = _pts(slots)
+        assert len(s) == slots_w * slots_h
+        self.slots_w, self.slots_h = slots_w, slots_h
+        _check(lib().vm_set_generators(self._h, _ptr(s), slots_w, slots_h, slot_cell))
+
+    def update_generators(self, slots, cx0, cy0, cx1, cy1):
+        s = _pts(slots)
+        _check(lib().vm_update_generators(self._h, _ptr(s), cx0, cy0, cx1, cy1))
+
+    def begin_schedule(self, iterations, rect=None):
+        if rect is None:
+            rect = (0, 0, self.slots_w, self.slots_h)
+        _check(lib().vm_begin_schedule(self._h, iterations, *rect))
+
+    def step(self, iterations) -> float:
+        ms = C.c_float()
+        _check(lib().vm_step(self._h, iterations, C.byref(ms)))
+        return ms.value
+
+    def state(self):
+        it, sl, ls, ns = C.c_int(), C.c_int(), C.c_float(), C.c_int()
+        _check(lib().vm_get_state(self._h, C.byref(it), C.byref(sl), C.byref(ls), C.byref(ns)))
+        return dict(iteration=it.value, schedule_length=sl.value, last_batch_loss=ls.value, num_slots=ns.value)
+
+    def optimize(self) -> float:
+        st = self.state()
+        return self.step(st["schedule_length"] - st["iteration"])
+
+    def evaluate_loss(self) -> float:
+        v = C.c_double()
+        _check(lib().vm_evaluate_loss(self._h, C.byref(v)))
+        return v.value
+
+    def generators(self):
+        n = self.slots_w * self.slots_h
+        out = np.zeros((n, 2), np.float32)
+        _check(lib().vm_get_generators(self._h, _ptr(out), n))
+        return out
+
+    def knn(self, queries, k):
+        q = _pts(queries)
+        idx = np.zeros((len(q), k), np.int32)
+        d2 = np.zeros((len(q), k), np.float32)
+        _check(lib().vm_knn(self._h, _ptr(q), len(q), k, _ptr(idx), _ptr(d2)))
+        return idx, d2
+
+    def loss_pass(self, n_target_slots, want_grad=True, want_terms=True):
+        n = self.slots_w * self.slots_h
+        K = self.params.loss_k
+        grad = np.zeros((n, 2), np.float64) if want_grad else None
+        ls, cnt = np.zeros(1, np.float64), np.zeros(1, np.int64)
+        knn = near = part = terms = None
+        if want_terms:
+            knn = np.zeros((n_target_slots, K), np.int32)
+            near = np.zeros(n_target_slots, np.int32)
+            part = np.zeros(n_target_slots, np.int32)
+            terms = np.zeros(n_target_slots, np.float64)
+        _check(lib().vm_loss_pass(self._h, _ptr(grad), _ptr(ls), _ptr(cnt), _ptr(knn), _ptr(near), _ptr(part),
+                                  _ptr(terms)))
+        return dict(grad=grad, loss_sum=float(ls[0]), count=int(cnt[0]), knn=knn, nearest=near, partner=part,
+                    loss_terms=terms)
+
+    def extract(self):
+        m = Mesh()
+        _check(lib().vm_extract(self._h, C.byref(m)))
+        n = m.num_slots
+
+        def arr(p, count, dt, shape=None):
+            if count == 0 or not p:
+                return np.zeros(shape if shape else (0,), dt)
+            a = np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(count,))
+            a = a.copy()
+            return a.reshape(shape) if shape else a
+
+        start = arr(m.cell_start, n + 1, np.int32)
+        nv = int(start[-1]) if n else 0
+        return dict(
+            generators=arr(C.cast(m.generators, C.POINTER(C.c_float)), 2 * n, np.float32, (n, 2)),
+            inside=arr(m.inside, n, np.uint8), cell_start=start,
+            cell_verts=arr(C.cast(m.cell_verts, C.POINTER(C.c_float)), 2 * nv, np.float32, (nv, 2)),
+            cell_labels=arr(m.cell_labels, nv, np.int32),
+            vertices=arr(C.cast(m.vertices, C.POINTER(C.c_float)), 2 * m.num_vertices, np.float32,
+                         (m.num_vertices, 2)),
+            edges=arr(m.edges, 2 * m.num_edges, np.int32, (m.num_edges, 2)),
+            watertight=bool(m.watertight), loops=m.loops, unsafe_cells=m.unsafe_cells,
+            overflow_cells=m.overflow_cells, gpu_ms=m.gpu_ms, host_ms=m.host_ms)
+
+
+class Sampler:
+    """poisson::Sampler over the HIP dart thrower."""
+
+    def __init__(self, width, height, radius, trials=30, seed=1):
+        self._s = C.c_void_p()
+        _check(lib().vm_sampler_create(width, height, radius, trials, seed, C.byref(self._s)))
+        gw, gh, cs = C.c_int(), C.c_int(), C.c_float()
+        _check(lib().vm_sampler_dims(self._s, C.byref(gw), C.byref(gh), C.byref(cs)))
+        self.grid_w, self.grid_h, self.cell_size = gw.value, gh.value, cs.value
+        self.width, self.height, self.radius = width, height, radius
+
+    def close(self):
+        if self._s:
+            lib().vm_sampler_destroy(self._s)
+            self._s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def resample(self, cx0, cy0, cx1, cy1) -> float:
+        ms = C.c_float()
+        _check(lib().vm_sampler_resample(self._s, cx0, cy0, cx1, cy1, C.byref(ms)))
+        return ms.value
+
+    def verify(self, cx0=0, cy0=0, cx1=None, cy1=None) -> bool:
+        ok = C.c_int()
+        cx1 = self.grid_w if cx1 is None else cx1
+        cy1 = self.grid_h if cy1 is None else cy1
+        _check(lib().vm_sampler_verify(self._s, cx0, cy0, cx1, cy1, C.byref(ok)))
+        return bool(ok.value)
+
+    def cells(self):
+        n = self.grid_w * self.grid_h
+        out = np.zeros((n, 2), np.float32)
+        _check(lib().vm_sampler_cells(self._s, _ptr(out), n))
+        return out
+
+
+def poisson_grid_dims(width, height, radius):
+    gw, gh, cs = C.c_int(), C.c_int(), C.c_float()
+    _check(lib().vm_poisson_grid_dims(width, height, radius, C.byref(gw), C.byref(gh), C.byref(cs)))
+    return gw.value, gh.value, cs.value
+
+
+def poisson_batch(widths, heights, radii, seeds, trials=30):
+    """Samples many Poisson-disk grids in one set of launches; returns (list of (gh*gw,2) grids, gpu_ms)."""
+    n = len(radii)
+    w = np.ascontiguousarray(widths, np.float32)
+    h = np.ascontiguousarray(heights, np.float32)
+    r = np.ascontiguousarray(radii, np.float32)
+    s = np.ascontiguousarray(seeds, np.uint32)
+    dims = [poisson_grid_dims(float(w[i]), float(h[i]), float(r[i])) for i in range(n)]
+    offs = np.zeros(n, np.int64)
+    total = 0
+    for i, (gw, gh, _) in enumerate(dims):
+        offs[i] = total
+        total += gw * gh
+    cells = np.zeros((max(total, 1), 2), np.float32)
+    ms = C.c_float()
+    _check(lib().vm_poisson_batch(n, _ptr(w), _ptr(h), _ptr(r), _ptr(s), trials, _ptr(offs), _ptr(cells),
+                                  C.byref(ms)))
+    grids = [cells[offs[i]:offs[i] + dims[i][0] * dims[i][1]] for i in range(n)]
+    return grids, dims, ms.value
+
+
+def fit_desc(width, height, sites, targets, lr, milestones=(80, 150, 200, 250), gamma=0.5, sample_fraction=1.0,
+             seed=1, loss_k=16, grid_cell=0.0, stage_rings=0, max_iterations=0, tol=0.0):
+    """A FitDesc plus the arrays it points to (keep the pair alive together)."""
+    s, t = _pts(sites), _pts(targets)
+    d = FitDesc()
+    d.width, d.height = width, height
+    d.sites, d.num_sites = s.ctypes.data, len(s)
+    d.targets, d.num_targets = t.ctypes.data, len(t)
+    d.lr, d.gamma = lr, gamma
+    for i in range(4):
+        d.milestones[i] = int(milestones[i])
+    d.sample_fraction, d.seed, d.loss_k = sample_fraction, seed, loss_k
+    d.grid_cell, d.stage_rings, d.max_iterations, d.tol = grid_cell, stage_rings, max_iterations, tol
+    return d, (s, t)
+
+
+class Batch:
+    """Many independent fits advanced by one set of launches per iteration."""
+
+    def __init__(self, precision=0):
+        self._b = C.c_void_p()
+        _check(lib().vm_batch_create(precision, C.byref(self._b)))
+        self._keep = []
+        self.sizes = []
+
+    def close(self):
+        if self._b:
+            lib().vm_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, width, height, sites, targets, lr, **kw) -> int:
+        d, keep = fit_desc(width, height, sites, targets, lr, **kw)
+        i = C.c_int()
+        _check(lib().vm_batch_add(self._b, C.byref(d), C.byref(i)))
+        self._keep.append(keep)
+        self.sizes.append(int((keep[0][:, 0] >= 0).sum()))
+        return i.value
+
+    def commit(self):
+        _check(lib().vm_batch_commit(self._b))
+        self._keep = []
+
+    def step(self, iterations) -> float:
+        ms = C.c_float()
+        _check(lib().vm_batch_step(self._b, iterations, C.byref(ms)))
+        return ms.value
+
+    def profile(self, iterations):
+        out = np.zeros(4, np.float32)
+        _check(lib().vm_batch_profile(self._b, iterations, _ptr(out)))
+        return out
+
+    def sites(self, i):
+        out = np.zeros((self.sizes[i], 2), np.float32)
+        _check(lib().vm_batch_get_sites(self._b, i, _ptr(out), self.sizes[i]))
+        return out
+
+    def status(self, i):
+        it, done, loss = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().vm_batch_status(self._b, i, C.byref(it), C.byref(done), C.byref(loss)))
+        return dict(iteration=it.value, done=bool(done.value), last_mean_loss=loss.value)
+
+    def choice(self, i):
+        k, cell, rings = C.c_int(), C.c_float(), C.c_int()
+        _check(lib().vm_batch_choice(self._b, i, C.byref(k), C.byref(cell), C.byref(rings)))
+        return dict(loss_k=k.value, grid_cell=cell.value, stage_rings=rings.value)

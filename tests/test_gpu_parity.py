@@ -1,0 +1,297 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: k-NN lists, nearest site / bisector partner, per-target loss terms,
+clipped cells and Poisson grids bit-exact against the oracle's FP32 semantics;
+gradient sums within 1e-5 relative (norm-wise) of the FP64 oracle; one Adam
+step within 1e-6 of the oracle's step; the FP64 mode within 1e-12 of the FP64
+oracle.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+vmhip = pytest.importorskip("vmhip")
+from vmhip import workloads as wl  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+W = H = 1.0
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    if vmhip.device_count() == 0:
+        pytest.skip("no HIP device")
+
+
+def _config0(r=0.025, n_targets=8192, seed=0):
+    grid = O.poisson(W, H, r, 30, seed)
+    gw, gh, cs = O.poisson_dims(W, H, r)
+    return grid, (gw, gh, cs), wl.star(n_targets, seed=seed)
+
+
+def _optimizer(r, K, precision=0, iterations=200, fraction=1.0, grid_cell=0.0, rings=0):
+    return vmhip.Optimizer(W, H, r, loss_k=K, sample_fraction=fraction, lr_per_spacing=1e-3 / r,
+                           iterations=iterations, precision=precision, grid_cell=grid_cell, stage_rings=rings)
+
+
+def _lr(r):
+    """lr exactly as the optimizer forms it: float(lr_per_spacing) * float(spacing) in FP32."""
+    return float(np.float32(np.float32(1e-3 / r) * np.float32(r)))
+
+
+def _slots(grid):
+    return np.nonzero(grid[:, 0] >= 0)[0]
+
+
+def _map(idx, slot_of):
+    """Instance-local site indices -> slot numbers; negative labels (missing, box sides) pass through."""
+    idx = np.asarray(idx)
+    return np.where(idx >= 0, slot_of[np.maximum(idx, 0)], idx)
+
+
+@pytest.mark.parametrize("K", [12, 16, 32])
+def test_knn_query_bitexact_config0(K):
+    r = 0.025
+    grid, (gw, gh, cs), targets = _config0(r)
+    opt = _optimizer(r, K)
+    opt.set_targets(targets)
+    opt.set_generators(grid, gw, gh, cs)
+    sites = wl.live_sites(grid)
+    idx, d2 = opt.knn(targets, K)
+    ridx, rd2 = O.knn(sites, targets, K, W, H, 0.1, brute=True)
+    assert np.array_equal(idx, _map(ridx, _slots(grid)))
+    assert np.array_equal(d2, rd2)
+
+
+@pytest.mark.parametrize("cell_factor,rings", [(2.0, 2), (1.0, 1), (3.0, 1), (0.5, 2)])
+def test_loss_kernel_terms_bitexact(cell_factor, rings):
+    """k-NN inside the loss kernel, nearest site, partner and loss term, for several grid shapes."""
+    r, K = 0.025, 16
+    grid, (gw, gh, cs), targets = _config0(r)
+    sites = wl.live_sites(grid)
+    spacing = (1.0 / len(sites)) ** 0.5
+    opt = _optimizer(r, K, grid_cell=cell_factor * spacing, rings=rings)
+    opt.set_targets(targets)
+    opt.set_generators(grid, gw, gh, cs)
+    d = opt.loss_pass(len(targets))
+    ref = O.loss_grad(sites, targets, K, W, H, 0.1, per_target=True)
+    ridx, _ = O.knn(sites, targets, K, W, H, 0.1)
+    so = _slots(grid)
+    assert np.array_equal(d["knn"], _map(ridx, so))
+    assert np.array_equal(d["nearest"], _map(ref["c0"], so))
+    assert np.array_equal(d["partner"], _map(ref["bestj"], so))
+    assert np.array_equal(d["loss_terms"], ref["loss_t"].astype(np.float64))
+    assert d["count"] == ref["count"]
+    assert d["loss_sum"] == pytest.approx(ref["loss_sum"], rel=1e-12)
+
+
+def test_knn_lattice_ties_bitexact():
+    """Jittered lattice (+-1e-7): many exactly tied squared distances; order is (d2, index)."""
+    sites = wl.lattice(64, 1e-7, seed=3)
+    targets = wl.circle(16384, radius=0.3, sigma=1e-4, seed=3)
+    # Sites as slots of a 64x64 slot grid, all live.
+    opt = _optimizer(1 / 64, 16)
+    opt.set_targets(targets)
+    opt.set_generators(sites, 64, 64, 1 / 64)
+    idx, d2 = opt.knn(targets, 16)
+    ridx, rd2 = O.knn(sites, targets, 16, W, H, 0.1, brute=True)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(d2, rd2)
+    # Query the sites themselves: exact ties between lattice neighbours.
+    idx2, _ = opt.knn(sites, 9)
+    ridx2, _ = O.knn(sites, sites, 9, W, H, 0.1, brute=True)
+    assert np.array_equal(idx2, ridx2)
+    d = opt.loss_pass(len(targets))
+    ref = O.loss_grad(sites, targets, 16, W, H, 0.1, per_target=True)
+    assert np.array_equal(d["partner"], ref["bestj"])
+    assert np.array_equal(d["loss_terms"], ref["loss_t"].astype(np.float64))
+
+
+def test_gradient_vs_fp64_oracle():
+    r, K = 0.025, 16
+    grid, (gw, gh, cs), targets = _config0(r)
+    sites = wl.live_sites(grid)
+    opt = _optimizer(r, K)
+    opt.set_targets(targets)
+    opt.set_generators(grid, gw, gh, cs)
+    g = opt.loss_pass(len(targets), want_terms=False)["grad"][_slots(grid)]
+    r32 = O.loss_grad(sites, targets, K, W, H, 0.1)
+    r64 = O.loss_grad(sites.astype(np.float64), targets.astype(np.float64), K, W, H, 0.1, real=1)
+    e32 = np.linalg.norm(g - r32["grad"]) / np.linalg.norm(r32["grad"])
+    e64 = np.linalg.norm(g - r64["grad"]) / np.linalg.norm(r64["grad"])
+    print(f"grad rel err vs FP32-sum oracle {e32:.3e}, vs FP64 oracle {e64:.3e}")
+    assert e32 < 1e-6
+    assert e64 < 1e-5
+
+
+def test_one_step_matches_oracle():
+    r, K = 0.025, 16
+    grid, (gw, gh, cs), targets = _config0(r)
+    sites = wl.live_sites(grid)
+    opt = _optimizer(r, K)
+    opt.set_targets(targets)
+    opt.set_generators(grid, gw, gh, cs)
+    opt.step(1)
+    got = opt.generators()[_slots(grid)]
+    sc = O.make_sched(_lr(r), W, H)
+    ref, _, _, lh, ch = O.fit(sites, targets, K, 0.1, sc, 1, box=(-2 * r, -2 * r, gw * cs + 2 * r, gh * cs + 2 * r))
+    assert np.abs(got - ref).max() < 1e-6
+    assert opt.state()["iteration"] == 1
+
+
+def test_subsampled_steps_count_exact():
+    """With sample_fraction 0.2 the per-step target count is an exact integer parity check."""
+    r, K = 0.025, 12
+    grid, (gw, gh, cs), targets = _config0(r)
+    sites = wl.live_sites(grid)
+    opt = _optimizer(r, K, fraction=0.2, iterations=400)
+    opt.set_targets(targets)
+    opt.set_generators(grid, gw, gh, cs)
+    opt.step(1)
+    st = opt.state()
+    sc = O.make_sched(_lr(r), W, H)
+    _, _, _, lh, ch = O.fit(sites, targets, K, 0.1, sc, 1, fraction=0.2, seed=1,
+                            box=(-2 * r, -2 * r, gw * cs + 2 * r, gh * cs + 2 * r))
+    assert st["last_batch_loss"] == pytest.approx(lh[0] / ch[0], rel=1e-5)
+
+
+def test_fit_config0_final_loss_close_to_oracle():
+    r, K, iters = 0.025, 16, 200
+    grid, (gw, gh, cs), targets = _config0(r)
+    sites = wl.live_sites(grid)
+    opt = _optimizer(r, K, iterations=iters)
+    opt.set_targets(targets)
+    opt.set_generators(grid, gw, gh, cs)
+    opt.optimize()
+    gpu_loss = opt.evaluate_loss()
+    sc = O.make_sched(_lr(r), W, H)
+    ref, _, _, lh, ch = O.fit(sites, targets, K, 0.1, sc, iters, box=(-2 * r, -2 * r, gw * cs + 2 * r,
+                                                                     gh * cs + 2 * r))
+    r_loss = O.loss_grad(ref, targets, K, W, H, 0.1)
+    ref_mean = r_loss["loss_sum"] / r_loss["count"]
+    l0 = O.loss_grad(sites, targets, K, W, H, 0.1)
+    print(f"initial {l0['loss_sum'] / l0['count']:.4e}  gpu {gpu_loss:.4e}  oracle {ref_mean:.4e}")
+    assert gpu_loss < 0.5 * l0["loss_sum"] / l0["count"]
+    assert gpu_loss == pytest.approx(ref_mean, rel=0.05)
+
+
+def test_clip_bitexact_and_watertight():
+    r, K = 0.025, 16
+    grid, (gw, gh, cs), targets = _config0(r)
+    sites = wl.live_sites(grid)
+    opt = _optimizer(r, K)
+    opt.set_targets(targets)
+    opt.set_generators(grid, gw, gh, cs)
+    opt.step(20)
+    moved = opt.generators()
+    so = _slots(grid)
+    live = moved[so]
+    # occupancy: voxel centre inside the star polygon (64x64 voxels of 1/64)
+    masks, tx, ty, vox = _star_masks(64)
+    opt.set_occupancy(masks, tx, ty, vox)
+    mesh = opt.extract()
+    ref = O.clip(live, W, H, 0.1, masks=masks, tiles=(tx, ty), voxel=vox)
+    cnt = mesh["cell_start"][1:] - mesh["cell_start"][:-1]
+    assert np.array_equal(cnt[so], ref["count"])
+    for k, s in enumerate(so):
+        b = mesh["cell_start"][s]
+        n = ref["count"][k]
+        assert np.array_equal(mesh["cell_verts"][b:b + n], ref["verts"][k, :n])
+        assert np.array_equal(mesh["cell_labels"][b:b + n], _map(ref["labels"][k, :n], so) if n else [])
+    assert np.array_equal(mesh["inside"][so], ref["inside"])
+    assert mesh["unsafe_cells"] == ref["unsafe"] and mesh["overflow_cells"] == ref["overflow"] == 0
+    b = O.boundary(ref["verts"], ref["labels"], ref["count"], ref["inside"])
+    assert mesh["watertight"] and b["watertight"]
+    assert mesh["loops"] == b["loops"] >= 1
+    assert len(mesh["vertices"]) == len(b["vertices"]) and len(mesh["edges"]) == len(b["edges"])
+    area = 0.0
+    for k in range(len(so)):
+        v = ref["verts"][k, :ref["count"][k]].astype(np.float64)
+        area += 0.5 * np.sum(v[:, 0] * np.roll(v[:, 1], -1) - np.roll(v[:, 0], -1) * v[:, 1])
+    assert area == pytest.approx(1.0, abs=1e-5)
+
+
+def _star_masks(vox_n):
+    """Occupancy of the config-0 star: bit set when the voxel centre is inside (even-odd)."""
+    import math
+    ang = math.pi / 2 + np.arange(10) * math.pi / 5
+    rad = np.where(np.arange(10) % 2 == 0, 0.40, 0.18)
+    poly = np.stack([0.5 + rad * np.cos(ang), 0.5 + rad * np.sin(ang)], axis=1)
+    vox = 1.0 / vox_n
+    tx = ty = (vox_n + 7) // 8
+    masks = np.zeros(tx * ty, np.uint64)
+    for j in range(vox_n):
+        for i in range(vox_n):
+            x, y = (i + 0.5) * vox, (j + 0.5) * vox
+            inside = False
+            for s in range(10):
+                a, b = poly[s], poly[s - 1]
+                if (a[1] > y) != (b[1] > y) and x < a[0] + (y - a[1]) * (b[0] - a[0]) / (b[1] - a[1]):
+                    inside = not inside
+            if inside:
+                masks[(j >> 3) * tx + (i >> 3)] |= np.uint64(1) << np.uint64(((j & 7) << 3) | (i & 7))
+    return masks, tx, ty, np.float32(vox)
+
+
+@pytest.mark.parametrize("r,seed", [(0.025, 0), (0.01, 7)])
+def test_poisson_sampler_bitexact(r, seed):
+    s = vmhip.Sampler(W, H, r, 30, seed)
+    ref = O.poisson(W, H, r, 30, seed)
+    assert np.array_equal(s.cells(), ref)
+    assert s.verify()
+    # Region resample continues the round counter (fresh darts).
+    s.resample(3, 4, 20, 17)
+    ref = O.poisson(W, H, r, 30, seed, region=(3, 4, 20, 17), round_base=30, grid=ref)
+    assert np.array_equal(s.cells(), ref)
+    assert s.verify()
+
+
+def test_poisson_batch_matches_single():
+    radii = [0.05, 0.02, 0.03]
+    grids, dims, _ = vmhip.poisson_batch([1, 1, 1], [1, 1, 1], radii, [5, 6, 7])
+    for g, r, sd in zip(grids, radii, [5, 6, 7]):
+        assert np.array_equal(g, O.poisson(W, H, r, 30, sd))
+
+
+def test_fp64_mode_vs_fp64_oracle():
+    r, K = 0.025, 16
+    grid, (gw, gh, cs), targets = _config0(r)
+    sites = wl.live_sites(grid)
+    opt = _optimizer(r, K, precision=1)
+    opt.set_targets(targets)
+    opt.set_generators(grid, gw, gh, cs)
+    d = opt.loss_pass(len(targets))
+    r64 = O.loss_grad(sites.astype(np.float64), targets.astype(np.float64), K, W, H, 0.1, real=1,
+                      per_target=True)
+    so = _slots(grid)
+    assert np.array_equal(d["partner"], _map(r64["bestj"], so))
+    assert np.array_equal(d["loss_terms"], r64["loss_t"])
+    g = d["grad"][so]
+    assert np.linalg.norm(g - r64["grad"]) / np.linalg.norm(r64["grad"]) < 1e-12
+
+
+def test_batch_instances_match_plain():
+    """A fit in a batch advances like the same fit alone (within FP32 atomic-order noise)."""
+    r, K, iters = 0.05, 16, 30
+    b = vmhip.Batch()
+    cases = []
+    for i in range(4):
+        grid = O.poisson(W, H, r * (1 + 0.2 * i), 30, 10 + i)
+        sites = wl.live_sites(grid)
+        tg = wl.polygon(8 * len(sites), seed=20 + i)
+        cases.append((sites, tg))
+        b.add(W, H, sites, tg, 1e-3, loss_k=K)
+    b.commit()
+    b.step(iters)
+    for i, (sites, tg) in enumerate(cases):
+        sc = O.make_sched(1e-3, W, H)
+        ref, _, _, lh, ch = O.fit(sites, tg, K, 0.1, sc, iters)
+        got = b.sites(i)
+        diff = np.abs(got - ref).max(axis=1)
+        # FP32 atomic summation order differs; Adam's normalised step keeps most sites identical.
+        assert np.median(diff) < 1e-6 and (diff > 1e-5).mean() < 0.02
+        st = b.status(i)
+        assert st["iteration"] == iters
+        assert st["last_mean_loss"] == pytest.approx(lh[-1] / ch[-1], rel=0.05)
