@@ -41,13 +41,13 @@ const char *lastError() { return t_lastError.c_str(); }
 
 void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR)
 {
-    // Mean site spacing a = sqrt(area / S).  A cell of 2a holds ~4 sites; the
-    // K-NN radius is ~a*sqrt(K/pi), so two staged rings cover it for K <= 32.
+    // Mean site spacing a = sqrt(area / S).
+    // The loss kernel's fast path needs two staged rings to cover the K-NN
+    // radius (~a*sqrt(K/pi)) plus the chunk extent (<= 0.7a diagonal).
     double a = std::sqrt(double(W) * double(H) / double(std::max(numSites, 1)));
-    double f = 2.0;
+    double f = std::max(1.5, 0.5 * (std::sqrt(double(K) / 3.141592653589793) + 0.8));
     *cell    = float(f * a);
     *stageR  = 2;
-    (void)K;
 }
 
 namespace {
@@ -156,20 +156,42 @@ public:
             for (int b = 0; b < S; b += vmk::kSiteChunk) L.siteWork.push_back({i, b, std::min(vmk::kSiteChunk, S - b), 0});
             L.instList.push_back(i);
 
-            // Targets sorted by hash cell (stable), chunked by cell.
-            std::vector<int> cellOfT(T);
-            for (int t = 0; t < T; ++t)
-                cellOfT[t] = cellCoordF(float(s.tgt[2 * t + 1]), in.inv, in.gh) * in.gw +
-                             cellCoordF(float(s.tgt[2 * t]), in.inv, in.gw);
+            // Targets sorted by hash cell, then by a 4x4-bit Morton key inside the cell,
+            // then by input order; a chunk (one wavefront) stays in one cell, holds at
+            // most 64 targets and its bounding box stays within ~0.35 mean spacings.
+            std::vector<int>      cellOfT(T);
+            std::vector<uint32_t> keyT(T);
+            for (int t = 0; t < T; ++t) {
+                float fx = float(s.tgt[2 * t]) * in.inv, fy = float(s.tgt[2 * t + 1]) * in.inv;
+                int   ccx = cellCoordF(float(s.tgt[2 * t]), in.inv, in.gw);
+                int   ccy = cellCoordF(float(s.tgt[2 * t + 1]), in.inv, in.gh);
+                int   sx = std::min(std::max(int((fx - float(ccx)) * 16.0f), 0), 15);
+                int   sy = std::min(std::max(int((fy - float(ccy)) * 16.0f), 0), 15);
+                uint32_t mort = 0;
+                for (int bit = 0; bit < 4; ++bit)
+                        mort |= ((uint32_t(sx >> bit) & 1u) << (2 * bit)) | ((uint32_t(sy >> bit) & 1u) << (2 * bit + 1));
+                cellOfT[t] = ccy * in.gw + ccx;
+                keyT[t]    = mort;
+            }
             std::vector<int> &perm = L.perm[i];
             perm.resize(T);
             std::iota(perm.begin(), perm.end(), 0);
-            std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return cellOfT[a] < cellOfT[b]; });
+            std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+                return cellOfT[a] != cellOfT[b] ? cellOfT[a] < cellOfT[b] : keyT[a] < keyT[b];
+            });
+            const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(S, 1)));
+            const double maxDiag = 0.7 * spacing;
             for (int t = 0; t < T;) {
-                int c = cellOfT[perm[t]], e = t;
-                while (e < T && cellOfT[perm[e]] == c) ++e;
-                for (int b = t; b < e; b += vmk::kTargetChunk)
-                    twByInst[i].push_back({i, b, std::min(vmk::kTargetChunk, e - b), c});
+                int    c = cellOfT[perm[t]], e = t;
+                double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
+                while (e < T && cellOfT[perm[e]] == c && e - t < vmk::kTargetChunk) {
+                    double px = s.tgt[2 * perm[e]], py = s.tgt[2 * perm[e] + 1];
+                    double nx0 = std::min(x0, px), ny0 = std::min(y0, py), nx1 = std::max(x1, px), ny1 = std::max(y1, py);
+                    if (e > t && std::hypot(nx1 - nx0, ny1 - ny0) > maxDiag) break;
+                    x0 = nx0, y0 = ny0, x1 = nx1, y1 = ny1;
+                    ++e;
+                }
+                twByInst[i].push_back({i, t, e - t, c});
                 t = e;
             }
         }

@@ -23,6 +23,7 @@
 // bit-identical to the oracle's.
 #include <hip/hip_runtime.h>
 #include <float.h>
+#include <limits.h>
 #include <stdint.h>
 
 #include "vm_kernels.h"
@@ -286,18 +287,66 @@ __global__ __launch_bounds__(kSiteChunk) void scatterKernel(const Inst *__restri
 
 // ---------------------------------------------------------------------------
 // 3. VoroLoss + gradient (voro_optimizer.cu:199-266)
+//
+// One wavefront takes up to 64 targets of one hash cell whose bounding box B
+// is small (the host cuts chunks by extent).  The sites of the (2r+1)^2 cell
+// block around the cell are staged in LDS.
+//
+// Fast path.  Let q be the centre of B, rho its half-diagonal and D >= the
+// distance from q to its K-th nearest staged site.  For every target p in B,
+// D_K(p) <= D + rho, so p's K nearest sites all lie within D + 2 rho of q; if
+// the staged block provably contains that disc, those candidates (q's own
+// nearest first) are compacted in LDS and every lane selects its exact top-K
+// under the (d2, index) order as an unsorted set with a tracked maximum.
+// When c0 is common to the wave, the (c0, cj) bisector terms are computed once
+// per wave.  Otherwise the lanes fall back to the ring search.
 // ---------------------------------------------------------------------------
-constexpr int kLossThreads = 64;
-constexpr int kStageMaxCells = 25; // (2*2+1)^2
+constexpr int kLossThreads   = 64;
+constexpr int kStageMaxCells = 25;  // (2*2+1)^2
+constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane
 
 template <class R> struct LossLds {
-    V2<R> pos[kStageCap];
-    int   idx[kStageCap];
-    V2<R> grad[kStageCap];
-    int   cellBeg[kStageMaxCells + 1];
-    int   cellEnd[kStageMaxCells + 1];
-    int   total;
+    V2<R> pos[kLossStage];
+    int   idx[kLossStage];
+    V2<R> grad[kLossStage];
+    R     ex[kLossStage], ey[kLossStage], el[kLossStage]; // cj - c0 and |cj - c0| for the wave's common c0
+    int   cellBeg[kStageMaxCells];
+    int   cellEnd[kStageMaxCells];
 };
+
+template <class R> __device__ __forceinline__ R relMargin();
+template <> __device__ __forceinline__ float  relMargin<float>() { return 4e-6f; }
+template <> __device__ __forceinline__ double relMargin<double>() { return 1e-12; }
+
+template <class R> __device__ __forceinline__ R rInf();
+template <> __device__ __forceinline__ float  rInf<float>() { return __builtin_huge_valf(); }
+template <> __device__ __forceinline__ double rInf<double>() { return __builtin_huge_val(); }
+
+template <class T> __device__ __forceinline__ T waveMin(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v   = w < v ? w : v;
+    }
+    return v;
+}
+
+template <class T> __device__ __forceinline__ T waveMax(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v   = w > v ? w : v;
+    }
+    return v;
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ int laneRank(uint64_t mask)
+{
+    return int(__builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u)));
+}
 
 template <int K, class R>
 __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
@@ -322,6 +371,7 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         const int      sy0 = max(cy - rs, 0), sy1 = min(cy + rs, gh - 1);
         const int      snx = sx1 - sx0 + 1, sny = sy1 - sy0 + 1;
         const int      ncell = snx * sny;
+        const R        cell  = R(in.cell);
 
         // ---- stage the cell block [sx0,sx1] x [sy0,sy1] into LDS ----------
         int b = 0, e = 0;
@@ -336,8 +386,8 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
             int o = __shfl_up(inc, off, 64);
             if (lane >= off) inc += o;
         }
-        const int total  = __shfl(inc, 63, 64);
-        const bool staged = total <= kStageCap;
+        const int  total  = __shfl(inc, 63, 64);
+        const bool staged = total <= kLossStage;
         if (staged) {
             if (lane < ncell) {
                 lds.cellBeg[lane] = inc - cnt;
@@ -369,113 +419,240 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
             if (take && a.mode == kModeIterate && in.fraction < 1.0f)
                 take = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(iter), in.seed) < in.fraction;
         }
+        const uint64_t takeMask = __ballot(take);
+        if (takeMask == 0) {
+            __syncthreads();
+            continue;
+        }
 
-        // ---- k-NN by ring search (the ring geometry is wave-uniform) -------
         Key<R> keys[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) keys[k] = Key<R>::none();
-        const R    cell   = R(in.cell);
-        const R    rel    = ringRel<R>();
-        const R    margin = rel * fmax(R(in.ext), fmax(fabs(p.x), fabs(p.y)));
-        bool       active = take;
-        for (int ring = 0;; ++ring) {
-            if (!__any(active)) break;
-            const int x0 = cx - ring, x1 = cx + ring, y0 = cy - ring, y1 = cy + ring;
-            for (int y = max(y0, 0); y <= min(y1, gh - 1); ++y) {
-                const int stride = (y == y0 || y == y1) ? 1 : max(x1 - x0, 1);
-                for (int x = x0; x <= x1; x += stride) {
-                    if (x < 0 || x >= gw) continue;
-                    if (staged && x >= sx0 && x <= sx1 && y >= sy0 && y <= sy1) {
-                        const int li = (y - sy0) * snx + (x - sx0);
-                        const int lb = lds.cellBeg[li], le = lds.cellEnd[li];
-                        for (int s = lb; s < le; ++s) {
-                            if (active) {
-                                V2<R> q = lds.pos[s];
-                                topkInsert<K>(keys, Key<R>::make(dist2(q, p), (uint32_t(lds.idx[s]) << kLdsSlotBits) |
-                                                                                 uint32_t(s)));
+        bool fast  = staged && total >= K;
+        int  m     = 0; // compacted candidates (fast path)
+        const R ext = fmax(R(in.ext), fmax(fabs(p.x), fabs(p.y)));
+        if (fast) {
+            // Bounding box of the wave's targets.
+            R  bx0 = waveMin(take ? p.x : rInf<R>()), by0 = waveMin(take ? p.y : rInf<R>());
+            R  bx1 = waveMax(take ? p.x : -rInf<R>()), by1 = waveMax(take ? p.y : -rInf<R>());
+            V2<R> q{R(0.5) * (bx0 + bx1), R(0.5) * (by0 + by1)};
+            R  ddx = bx1 - bx0, ddy = by1 - by0;
+            R  mg  = relMargin<R>() * waveMax(ext);
+            R  rho = R(0.5) * sqrtR(ddx * ddx + ddy * ddy) + mg;
+            // Squared distance from q to each staged site (two per lane).
+            const int e0 = lane, e1 = lane + 64;
+            R   d0 = e0 < total ? dist2(lds.pos[e0], q) : rInf<R>();
+            R   d1 = e1 < total ? dist2(lds.pos[e1], q) : rInf<R>();
+            // D^2 >= q's K-th smallest by bisection on the value range.
+            R lo = waveMin(d0 < d1 ? d0 : d1);
+            R hi = waveMax(fmax(e0 < total ? d0 : R(0), e1 < total ? d1 : R(0)));
+            for (int it = 0; it < 14; ++it) {
+                R   mid = lo + R(0.5) * (hi - lo);
+                int c   = __popcll(__ballot(d0 <= mid)) + __popcll(__ballot(d1 <= mid));
+                if (c >= K) hi = mid;
+                else lo = mid;
+            }
+            const R D    = sqrtR(hi) + mg;
+            const R thr  = D + R(2) * rho + mg;
+            const R thr2 = thr * thr * (R(1) + relMargin<R>());
+            // The block must contain the disc of radius thr around q.
+            R edge = rInf<R>();
+            if (sx0 > 0) edge = fmin(edge, q.x - R(sx0) * cell);
+            if (sx1 < gw - 1) edge = fmin(edge, R(sx1 + 1) * cell - q.x);
+            if (sy0 > 0) edge = fmin(edge, q.y - R(sy0) * cell);
+            if (sy1 < gh - 1) edge = fmin(edge, R(sy1 + 1) * cell - q.y);
+            fast = edge - R(2) * mg > thr;
+            if (fast) {
+                // Compact the candidates into LDS: q's nearest (d2 <= D^2) first.
+                const uint64_t in0 = __ballot(d0 <= hi), in1 = __ballot(d1 <= hi);
+                const uint64_t c0m = __ballot(d0 <= thr2 && !(d0 <= hi)), c1m = __ballot(d1 <= thr2 && !(d1 <= hi));
+                const int      nIn = __popcll(in0) + __popcll(in1);
+                m                  = nIn + __popcll(c0m) + __popcll(c1m);
+                int   dst0 = -1, dst1 = -1;
+                if (d0 <= hi) dst0 = laneRank(in0);
+                else if (d0 <= thr2) dst0 = nIn + laneRank(c0m);
+                if (d1 <= hi) dst1 = __popcll(in0) + laneRank(in1);
+                else if (d1 <= thr2) dst1 = nIn + __popcll(c0m) + laneRank(c1m);
+                V2<R> p0{}, p1{};
+                int   i0v = 0, i1v = 0;
+                if (dst0 >= 0) p0 = lds.pos[e0], i0v = lds.idx[e0];
+                if (dst1 >= 0) p1 = lds.pos[e1], i1v = lds.idx[e1];
+                __syncthreads();
+                if (dst0 >= 0) lds.pos[dst0] = p0, lds.idx[dst0] = i0v;
+                if (dst1 >= 0) lds.pos[dst1] = p1, lds.idx[dst1] = i1v;
+                __syncthreads();
+                // Exact top-K of each lane as an unsorted set with a tracked maximum.
+                if (take) {
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+                        keys[k] = Key<R>::make(dist2(lds.pos[k], p), (uint32_t(lds.idx[k]) << kLdsSlotBits) | uint32_t(k));
+                }
+                Key<R> kmax = keys[0];
+                int    amax = 0;
+#pragma unroll
+                for (int k = 1; k < K; ++k)
+                    if (kmax.lt(keys[k])) kmax = keys[k], amax = k;
+                for (int j = K; j < m; ++j) {
+                    Key<R> nk = Key<R>::make(dist2(lds.pos[j], p), (uint32_t(lds.idx[j]) << kLdsSlotBits) | uint32_t(j));
+                    if (take && nk.lt(kmax)) {
+#pragma unroll
+                        for (int k = 0; k < K; ++k)
+                            if (k == amax) keys[k] = nk;
+                        kmax = keys[0], amax = 0;
+#pragma unroll
+                        for (int k = 1; k < K; ++k)
+                            if (kmax.lt(keys[k])) kmax = keys[k], amax = k;
+                    }
+                }
+            }
+        }
+        if (!fast) {
+            // ---- fallback: ring search (the ring geometry is wave-uniform) ---
+            const R rel    = ringRel<R>();
+            const R margin = rel * ext;
+            bool    active = take;
+            for (int ring = 0;; ++ring) {
+                if (!__any(active)) break;
+                const int x0 = cx - ring, x1 = cx + ring, y0 = cy - ring, y1 = cy + ring;
+                for (int y = max(y0, 0); y <= min(y1, gh - 1); ++y) {
+                    const int stride = (y == y0 || y == y1) ? 1 : max(x1 - x0, 1);
+                    for (int x = x0; x <= x1; x += stride) {
+                        if (x < 0 || x >= gw) continue;
+                        if (staged && x >= sx0 && x <= sx1 && y >= sy0 && y <= sy1) {
+                            const int li = (y - sy0) * snx + (x - sx0);
+                            const int lb = lds.cellBeg[li], le = lds.cellEnd[li];
+                            for (int s = lb; s < le; ++s) {
+                                if (active)
+                                    topkInsert<K>(keys, Key<R>::make(dist2(lds.pos[s], p),
+                                                                     (uint32_t(lds.idx[s]) << kLdsSlotBits) | uint32_t(s)));
                             }
-                        }
-                    } else {
-                        const int c  = y * gw + x;
-                        const int gb = strt[c], ge = strt[c + 1];
-                        for (int s = gb; s < ge; ++s) {
-                            if (active) {
-                                V2<R> q = spos[s];
-                                topkInsert<K>(keys, Key<R>::make(dist2(q, p), (uint32_t(sidx[s]) << kLdsSlotBits) |
-                                                                                 uint32_t(kLdsNone)));
+                        } else {
+                            const int c  = y * gw + x;
+                            const int gb = strt[c], ge = strt[c + 1];
+                            for (int s = gb; s < ge; ++s) {
+                                if (active)
+                                    topkInsert<K>(keys, Key<R>::make(dist2(spos[s], p),
+                                                                     (uint32_t(sidx[s]) << kLdsSlotBits) | uint32_t(kLdsNone)));
                             }
                         }
                     }
                 }
+                if (x0 <= 0 && y0 <= 0 && x1 >= gw - 1 && y1 >= gh - 1) break;
+                if (active) {
+                    R safe = fmin(fmin(p.x - R(x0) * cell, R(x1 + 1) * cell - p.x),
+                                  fmin(p.y - R(y0) * cell, R(y1 + 1) * cell - p.y));
+                    safe   = fmax(safe - margin, R(0));
+                    if (keys[K - 1].valid() && keys[K - 1].d2() * (R(1) + rel) < safe * safe) active = false;
+                }
             }
-            if (x0 <= 0 && y0 <= 0 && x1 >= gw - 1 && y1 >= gh - 1) break;
-            if (active) {
-                R safe = fmin(fmin(p.x - R(x0) * cell, R(x1 + 1) * cell - p.x),
-                              fmin(p.y - R(y0) * cell, R(y1 + 1) * cell - p.y));
-                safe   = fmax(safe - margin, R(0));
-                if (keys[K - 1].valid() && keys[K - 1].d2() * (R(1) + rel) < safe * safe) active = false;
+        }
+
+        // ---- nearest site c0 -------------------------------------------------
+        Key<R> kmin = keys[0];
+#pragma unroll
+        for (int k = 1; k < K; ++k)
+            if (keys[k].lt(kmin)) kmin = keys[k];
+        const bool   have = take && kmin.valid();
+        const int    i0   = have ? keySite(kmin.low()) : -1;
+        const int    sl0  = have ? keySlot(kmin.low()) : kLdsNone;
+        V2<R>        c0   = V2<R>{R(0), R(0)};
+        if (have) c0 = sl0 != kLdsNone ? lds.pos[sl0] : a.pos[so + i0];
+        // Wave-common c0: tabulate cj - c0 and |cj - c0| once for the candidates.
+        bool useTab = false;
+        if (fast) {
+            int lo0 = waveMin(have ? i0 : INT_MAX), hi0 = waveMax(have ? i0 : -1);
+            useTab  = lo0 == hi0;
+            if (useTab) {
+                const int   csl = waveMin(have ? sl0 : INT_MAX);
+                const V2<R> cc  = lds.pos[csl];
+                for (int j = lane; j < m; j += 64) {
+                    V2<R> ev{lds.pos[j].x - cc.x, lds.pos[j].y - cc.y};
+                    lds.ex[j] = ev.x;
+                    lds.ey[j] = ev.y;
+                    lds.el[j] = sqrtR(dotR(ev, ev));
+                }
             }
+            __syncthreads();
         }
 
         if (a.dbgKnn && lane < wk.count) {
+            // Lists in (d2, index) order for the parity tests.
+            Key<R> sk[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) sk[k] = keys[k];
+#pragma unroll
+            for (int i = 0; i < K; ++i)
+#pragma unroll
+                for (int j = 0; j + 1 < K - i; ++j)
+                    if (sk[j + 1].lt(sk[j])) {
+                        Key<R> tmp = sk[j];
+                        sk[j]      = sk[j + 1];
+                        sk[j + 1]  = tmp;
+                    }
 #pragma unroll
             for (int k = 0; k < K; ++k)
-                a.dbgKnn[(in.targetOff + t) * int64_t(K) + k] = (take && keys[k].valid()) ? keySite(keys[k].low()) : -1;
+                a.dbgKnn[(in.targetOff + t) * int64_t(K) + k] = (take && sk[k].valid()) ? keySite(sk[k].low()) : -1;
         }
 
-        // ---- VoroLoss term and gradient -----------------------------------
-        R    lossT = R(0);
-        int  used  = 0;
-        int  i0 = -1, bj = -1;
-        if (take && keys[0].valid()) {
-            const uint32_t lo0 = keys[0].low();
-            i0                 = keySite(lo0);
-            const int sl0      = keySlot(lo0);
-            V2<R>     c0       = sl0 != kLdsNone ? lds.pos[sl0] : a.pos[so + i0];
-            V2<R>     u{p.x - c0.x, p.y - c0.y};
-            R         best  = sizeof(R) == 4 ? R(FLT_MAX) : R(DBL_MAX);
-            R         bestD = R(0);
-            int       bestK = -1;
-            const R   tiny  = sizeof(R) == 4 ? R(1e-12f) : R(1e-12);
+        // ---- VoroLoss term: closest bisector among the other K-1 ------------
+        R   lossT = R(0);
+        int used  = 0, bj = -1;
+        if (have) {
+            V2<R>   u{p.x - c0.x, p.y - c0.y};
+            R       best  = sizeof(R) == 4 ? R(FLT_MAX) : R(DBL_MAX);
+            R       bestD = R(0);
+            Key<R>  bestKey = Key<R>::none();
+            const R tiny  = sizeof(R) == 4 ? R(1e-12f) : R(1e-12);
 #pragma unroll
-            for (int k = 1; k < K; ++k) {
-                if (!keys[k].valid()) break;
+            for (int k = 0; k < K; ++k) {
+                if (!keys[k].valid() || !kmin.lt(keys[k])) continue; // skip c0 and missing entries
                 const uint32_t lo = keys[k].low();
                 const int      sl = keySlot(lo);
-                V2<R>          cj = sl != kLdsNone ? lds.pos[sl] : a.pos[so + keySite(lo)];
-                V2<R>          ev{cj.x - c0.x, cj.y - c0.y};
-                R              l = sqrtR(dotR(ev, ev));
+                V2<R>          ev;
+                R              l;
+                if (useTab) {
+                    ev = V2<R>{lds.ex[sl], lds.ey[sl]};
+                    l  = lds.el[sl];
+                } else {
+                    V2<R> cj = sl != kLdsNone ? lds.pos[sl] : a.pos[so + keySite(lo)];
+                    ev       = V2<R>{cj.x - c0.x, cj.y - c0.y};
+                    l        = sqrtR(dotR(ev, ev));
+                }
                 if (l < tiny) continue;
                 R q  = dotR(u, ev) / l;
                 R d  = fmaR(R(-0.5), l, q);
                 R dd = d * d;
-                if (dd < best) {
-                    best  = dd;
-                    bestD = d;
-                    bestK = k;
+                if (dd < best || (dd == best && keys[k].lt(bestKey))) {
+                    best    = dd;
+                    bestD   = d;
+                    bestKey = keys[k];
                 }
             }
-            if (bestK >= 0) {
-                uint32_t lob = 0;
-#pragma unroll
-                for (int k = 1; k < K; ++k)
-                    if (k == bestK) lob = keys[k].low();
-                bj      = keySite(lob);
-                lossT   = best;
-                used    = 1;
+            if (bestKey.valid()) {
+                const uint32_t lob = bestKey.low();
+                bj    = keySite(lob);
+                lossT = best;
+                used  = 1;
                 if (a.grad) {
                     const int slj = keySlot(lob);
-                    V2<R>     cj  = slj != kLdsNone ? lds.pos[slj] : a.pos[so + bj];
-                    V2<R>     ev{cj.x - c0.x, cj.y - c0.y};
-                    R         l  = sqrtR(dotR(ev, ev));
-                    R         rl = R(1) / l;
-                    V2<R>     n{rl * ev.x, rl * ev.y};
-                    R         un = dotR(u, n);
-                    R         tx = fmaR(-un, n.x, u.x), ty = fmaR(-un, n.y, u.y);
-                    V2<R>     dcj{fmaR(rl, tx, R(-0.5) * n.x), fmaR(rl, ty, R(-0.5) * n.y)};
-                    R         s = R(2) * bestD;
-                    V2<R>     gj{s * dcj.x, s * dcj.y};
-                    V2<R>     g0{-(s * (n.x + dcj.x)), -(s * (n.y + dcj.y))};
+                    V2<R>     ev;
+                    R         l;
+                    if (useTab) {
+                        ev = V2<R>{lds.ex[slj], lds.ey[slj]};
+                        l  = lds.el[slj];
+                    } else {
+                        V2<R> cj = slj != kLdsNone ? lds.pos[slj] : a.pos[so + bj];
+                        ev       = V2<R>{cj.x - c0.x, cj.y - c0.y};
+                        l        = sqrtR(dotR(ev, ev));
+                    }
+                    R     rl = R(1) / l;
+                    V2<R> n{rl * ev.x, rl * ev.y};
+                    R     un = dotR(u, n);
+                    R     tx = fmaR(-un, n.x, u.x), ty = fmaR(-un, n.y, u.y);
+                    V2<R> dcj{fmaR(rl, tx, R(-0.5) * n.x), fmaR(rl, ty, R(-0.5) * n.y)};
+                    R     s = R(2) * bestD;
+                    V2<R> gj{s * dcj.x, s * dcj.y};
+                    V2<R> g0{-(s * (n.x + dcj.x)), -(s * (n.y + dcj.y))};
                     if (slj != kLdsNone) {
                         atomicAdd(&lds.grad[slj].x, gj.x);
                         atomicAdd(&lds.grad[slj].y, gj.y);
@@ -520,8 +697,9 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
 
         // ---- flush the LDS gradient: one atomic per touched site ----------
         __syncthreads();
-        if (a.grad && staged) {
-            for (int s = lane; s < total; s += 64) {
+        const int nflush = fast ? m : (staged ? total : 0);
+        if (a.grad) {
+            for (int s = lane; s < nflush; s += 64) {
                 V2<R> g = lds.grad[s];
                 if (g.x != R(0) || g.y != R(0)) {
                     int i = lds.idx[s];

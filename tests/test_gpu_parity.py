@@ -65,11 +65,13 @@ def test_knn_query_bitexact_config0(K):
     assert np.array_equal(d2, rd2)
 
 
-@pytest.mark.parametrize("cell_factor,rings", [(2.0, 2), (1.0, 1), (3.0, 1), (0.5, 2)])
-def test_loss_kernel_terms_bitexact(cell_factor, rings):
-    """k-NN inside the loss kernel, nearest site, partner and loss term, for several grid shapes."""
-    r, K = 0.025, 16
-    grid, (gw, gh, cs), targets = _config0(r)
+@pytest.mark.parametrize("cell_factor,rings,K,r", [(2.0, 2, 16, 0.025), (1.0, 1, 16, 0.025), (3.0, 1, 16, 0.025),
+                                                   (0.5, 2, 16, 0.025), (0.0, 0, 16, 0.025), (0.0, 0, 12, 0.025),
+                                                   (0.0, 0, 8, 0.01), (0.0, 0, 24, 0.01), (0.0, 0, 16, 0.006)])
+def test_loss_kernel_terms_bitexact(cell_factor, rings, K, r):
+    """k-NN inside the loss kernel, nearest site, partner and loss term, for several grid shapes
+    (cell_factor 0: the tuner's choice, which exercises the wavefront bounding-box path)."""
+    grid, (gw, gh, cs), targets = _config0(r, n_targets=8192 if r > 0.01 else 65536)
     sites = wl.live_sites(grid)
     spacing = (1.0 / len(sites)) ** 0.5
     opt = _optimizer(r, K, grid_cell=cell_factor * spacing, rings=rings)
