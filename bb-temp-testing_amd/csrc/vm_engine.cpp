@@ -61,6 +61,98 @@ inline int cellCoordF(float v, float inv, int n)
 
 inline int64_t align64(int64_t x) { return (x + 63) & ~int64_t(63); }
 
+} // namespace
+
+// ceil(log2(x)) for x > 0, exact (frexp gives x = m 2^e with m in [0.5, 1)).
+static int ceilLog2(double x)
+{
+    int    e;
+    double m = std::frexp(x, &e);
+    return m == 0.5 ? e - 1 : e;
+}
+
+int gradShiftFor(int numTargets, float ext)
+{
+    // A site collects at most T terms of magnitude ~ few * ext, so sums stay
+    // below 2^60 with this shift; clamped to [8, 52].
+    int g = 58 - ceilLog2(double(std::max(numTargets, 1))) - ceilLog2(double(std::max(ext, 1e-30f)));
+    return std::min(std::max(g, 8), 52);
+}
+
+void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff, int64_t targetOff, vmk::Inst &in,
+                    std::vector<vmk::SiteWork> &siteWork, std::vector<vmk::TargetWork> &targetWork,
+                    std::vector<int> &perm)
+{
+    std::memset(&in, 0, sizeof(in));
+    const int S = int(s.pos.size() / 2), T = int(s.tgt.size() / 2);
+    if (S >= vmk::kMaxSitesPerInst) throw VmError(1, "instance has too many sites");
+    if (!vmk::lossKSupported(s.K)) throw VmError(1, "unsupported loss_k " + std::to_string(s.K));
+    if (!(s.cell > 0.0f)) throw VmError(1, "grid cell must be positive");
+    in.W = s.W, in.H = s.H;
+    in.cell = s.cell, in.inv = 1.0f / s.cell, in.ext = std::max(s.W, s.H);
+    double gwd = std::ceil(double(s.W) / s.cell), ghd = std::ceil(double(s.H) / s.cell);
+    if (gwd * ghd > double(1 << 28)) throw VmError(1, "hash grid too large");
+    in.gw = std::max(1, int(gwd)), in.gh = std::max(1, int(ghd));
+    in.S = S, in.T = T, in.K = s.K;
+    in.stageR     = std::min(std::max(s.stageR, 0), 2);
+    in.allMovable = s.frozen.empty() ? 1 : 0;
+    in.siteOff = siteOff, in.cellOff = cellOff, in.targetOff = targetOff;
+    in.lr = s.lr, in.gamma = s.gamma, in.beta1 = s.beta1, in.beta2 = s.beta2, in.eps = s.eps;
+    for (int k = 0; k < 4; ++k) in.milestones[k] = s.milestones[k];
+    in.fraction = s.fraction, in.seed = s.seed;
+    in.useBox = s.useBox, in.bx0 = s.box[0], in.by0 = s.box[1], in.bx1 = s.box[2], in.by1 = s.box[3];
+    in.maxIter = s.maxIter, in.tol = s.tol;
+    in.gradShift = gradShiftFor(T, in.ext);
+
+    for (int b = 0; b < S; b += vmk::kSiteChunk) siteWork.push_back({id, b, std::min(vmk::kSiteChunk, S - b), 0});
+
+    // Targets sorted by hash cell, then by a 4x4-bit Morton key inside the cell,
+    // then by input order; a chunk (one wavefront) stays in one cell, holds at
+    // most 64 targets and its bounding-box diagonal stays within 0.7 mean spacings.
+    std::vector<int>      cellOfT(T);
+    std::vector<uint32_t> keyT(T);
+    for (int t = 0; t < T; ++t) {
+        float fx = float(s.tgt[2 * t]) * in.inv, fy = float(s.tgt[2 * t + 1]) * in.inv;
+        int   ccx = cellCoordF(float(s.tgt[2 * t]), in.inv, in.gw);
+        int   ccy = cellCoordF(float(s.tgt[2 * t + 1]), in.inv, in.gh);
+        int   sx  = std::min(std::max(int((fx - float(ccx)) * 16.0f), 0), 15);
+        int   sy  = std::min(std::max(int((fy - float(ccy)) * 16.0f), 0), 15);
+        uint32_t mort = 0;
+        for (int bit = 0; bit < 4; ++bit)
+            mort |= ((uint32_t(sx >> bit) & 1u) << (2 * bit)) | ((uint32_t(sy >> bit) & 1u) << (2 * bit + 1));
+        cellOfT[t] = ccy * in.gw + ccx;
+        keyT[t]    = mort;
+    }
+    perm.resize(T);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+        return cellOfT[a] != cellOfT[b] ? cellOfT[a] < cellOfT[b] : keyT[a] < keyT[b];
+    });
+    const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(S, 1)));
+    const double maxDiag = 0.7 * spacing;
+    for (int t = 0; t < T;) {
+        int    c = cellOfT[perm[t]], e = t;
+        double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
+        while (e < T && cellOfT[perm[e]] == c && e - t < vmk::kTargetChunk) {
+            double px = s.tgt[2 * perm[e]], py = s.tgt[2 * perm[e] + 1];
+            double nx0 = std::min(x0, px), ny0 = std::min(y0, py), nx1 = std::max(x1, px), ny1 = std::max(y1, py);
+            if (e > t && std::hypot(nx1 - nx0, ny1 - ny0) > maxDiag) break;
+            x0 = nx0, y0 = ny0, x1 = nx1, y1 = ny1;
+            ++e;
+        }
+        // Chunk centre (FP32) and an upper bound on the distance of its targets to it.
+        float  qx = float(0.5 * (x0 + x1)), qy = float(0.5 * (y0 + y1));
+        double rho = 0.0;
+        for (int k = t; k < e; ++k)
+            rho = std::max(rho, std::hypot(s.tgt[2 * perm[k]] - double(qx), s.tgt[2 * perm[k] + 1] - double(qy)));
+        float rhof = std::nextafter(std::nextafter(float(rho), 1e30f), 1e30f);
+        targetWork.push_back({id, t, e - t, c, qx, qy, rhof, 0.0f});
+        t = e;
+    }
+}
+
+namespace {
+
 constexpr int kMaxLossGrid = 16384;
 constexpr int kMaxSiteGrid = 8192;
 constexpr int kMaxScanGrid = 4096;
@@ -84,7 +176,8 @@ public:
         int64_t sites = 0, cells = 0, targets = 0;
     } L;
 
-    DBuf<V>              dPos, dM, dV, dGrad, dSpos, dTpos;
+    DBuf<V>              dPos, dM, dV, dSpos, dTpos;
+    DBuf<vmk::GradT<R>>  dGrad; // two per site
     DBuf<int>            dCellOf, dRank, dSidx, dCounts, dStart, dTslot, dInstList, dCnt;
     DBuf<uint8_t>        dFrozen;
     DBuf<vmk::Inst>      dInst;
@@ -113,7 +206,7 @@ public:
     size_t deviceBytes() const override
     {
         size_t b = 0;
-        b += (dPos.n + dM.n + dV.n + dGrad.n + dSpos.n + dTpos.n) * sizeof(V);
+        b += (dPos.n + dM.n + dV.n + dSpos.n + dTpos.n) * sizeof(V) + dGrad.n * sizeof(vmk::GradT<R>);
         b += (dCellOf.n + dRank.n + dSidx.n + dCounts.n + dStart.n + dTslot.n + dInstList.n + dCnt.n) * sizeof(int);
         b += dFrozen.n + dInst.n * sizeof(vmk::Inst) + dState.n * sizeof(vmk::InstState);
         b += dSiteWork.n * sizeof(vmk::SiteWork) + dTargetWork.n * sizeof(vmk::TargetWork);
@@ -130,70 +223,11 @@ public:
         for (int i = 0; i < n; ++i) {
             InstSpec &s  = specs[i];
             vmk::Inst  &in = L.inst[i];
-            std::memset(&in, 0, sizeof(in));
-            const int S = int(s.pos.size() / 2), T = int(s.tgt.size() / 2);
-            if (S >= vmk::kMaxSitesPerInst) throw VmError(1, "instance has too many sites");
-            if (!vmk::lossKSupported(s.K)) throw VmError(1, "unsupported loss_k " + std::to_string(s.K));
-            if (!(s.cell > 0.0f)) throw VmError(1, "grid cell must be positive");
-            in.W = s.W, in.H = s.H;
-            in.cell = s.cell, in.inv = 1.0f / s.cell, in.ext = std::max(s.W, s.H);
-            double gwd = std::ceil(double(s.W) / s.cell), ghd = std::ceil(double(s.H) / s.cell);
-            if (gwd * ghd > double(1 << 28)) throw VmError(1, "hash grid too large");
-            in.gw = std::max(1, int(gwd)), in.gh = std::max(1, int(ghd));
-            in.S = S, in.T = T, in.K = s.K;
-            in.stageR     = std::min(std::max(s.stageR, 0), 2);
-            in.allMovable = s.frozen.empty() ? 1 : 0;
-            in.siteOff = L.sites, in.cellOff = L.cells, in.targetOff = L.targets;
-            in.lr = s.lr, in.gamma = s.gamma, in.beta1 = s.beta1, in.beta2 = s.beta2, in.eps = s.eps;
-            for (int k = 0; k < 4; ++k) in.milestones[k] = s.milestones[k];
-            in.fraction = s.fraction, in.seed = s.seed;
-            in.useBox = s.useBox, in.bx0 = s.box[0], in.by0 = s.box[1], in.bx1 = s.box[2], in.by1 = s.box[3];
-            in.maxIter = s.maxIter, in.tol = s.tol;
-            L.sites += align64(S);
+            layoutInstance(s, i, L.sites, L.cells, L.targets, in, L.siteWork, twByInst[i], L.perm[i]);
+            L.sites += align64(in.S);
             L.cells += align64(int64_t(in.gw) * in.gh + 1);
-            L.targets += align64(T);
-
-            for (int b = 0; b < S; b += vmk::kSiteChunk) L.siteWork.push_back({i, b, std::min(vmk::kSiteChunk, S - b), 0});
+            L.targets += align64(in.T);
             L.instList.push_back(i);
-
-            // Targets sorted by hash cell, then by a 4x4-bit Morton key inside the cell,
-            // then by input order; a chunk (one wavefront) stays in one cell, holds at
-            // most 64 targets and its bounding box stays within ~0.35 mean spacings.
-            std::vector<int>      cellOfT(T);
-            std::vector<uint32_t> keyT(T);
-            for (int t = 0; t < T; ++t) {
-                float fx = float(s.tgt[2 * t]) * in.inv, fy = float(s.tgt[2 * t + 1]) * in.inv;
-                int   ccx = cellCoordF(float(s.tgt[2 * t]), in.inv, in.gw);
-                int   ccy = cellCoordF(float(s.tgt[2 * t + 1]), in.inv, in.gh);
-                int   sx = std::min(std::max(int((fx - float(ccx)) * 16.0f), 0), 15);
-                int   sy = std::min(std::max(int((fy - float(ccy)) * 16.0f), 0), 15);
-                uint32_t mort = 0;
-                for (int bit = 0; bit < 4; ++bit)
-                        mort |= ((uint32_t(sx >> bit) & 1u) << (2 * bit)) | ((uint32_t(sy >> bit) & 1u) << (2 * bit + 1));
-                cellOfT[t] = ccy * in.gw + ccx;
-                keyT[t]    = mort;
-            }
-            std::vector<int> &perm = L.perm[i];
-            perm.resize(T);
-            std::iota(perm.begin(), perm.end(), 0);
-            std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
-                return cellOfT[a] != cellOfT[b] ? cellOfT[a] < cellOfT[b] : keyT[a] < keyT[b];
-            });
-            const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(S, 1)));
-            const double maxDiag = 0.7 * spacing;
-            for (int t = 0; t < T;) {
-                int    c = cellOfT[perm[t]], e = t;
-                double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
-                while (e < T && cellOfT[perm[e]] == c && e - t < vmk::kTargetChunk) {
-                    double px = s.tgt[2 * perm[e]], py = s.tgt[2 * perm[e] + 1];
-                    double nx0 = std::min(x0, px), ny0 = std::min(y0, py), nx1 = std::max(x1, px), ny1 = std::max(y1, py);
-                    if (e > t && std::hypot(nx1 - nx0, ny1 - ny0) > maxDiag) break;
-                    x0 = nx0, y0 = ny0, x1 = nx1, y1 = ny1;
-                    ++e;
-                }
-                twByInst[i].push_back({i, t, e - t, c});
-                t = e;
-            }
         }
         // Group target work by K (one loss launch per distinct K).
         std::vector<int> ks;
@@ -214,7 +248,7 @@ public:
         destroyGraph();
         buildLayout();
         const int n = int(specs.size());
-        dPos.reserve(L.sites), dM.reserve(L.sites), dV.reserve(L.sites), dGrad.reserve(L.sites);
+        dPos.reserve(L.sites), dM.reserve(L.sites), dV.reserve(L.sites), dGrad.reserve(2 * L.sites);
         dSpos.reserve(L.sites), dCellOf.reserve(L.sites), dRank.reserve(L.sites), dSidx.reserve(L.sites);
         dFrozen.reserve(L.sites);
         dCounts.reserve(L.cells), dStart.reserve(L.cells);
@@ -254,7 +288,7 @@ public:
         up(dPos.p, hp), up(dM.p, hm), up(dV.p, hv), up(dTpos.p, ht), up(dTslot.p, hts), up(dFrozen.p, hf);
         up(dInst.p, L.inst), up(dState.p, hs), up(dInstList.p, L.instList), up(dSiteWork.p, L.siteWork);
         up(dTargetWork.p, L.targetWork), up(dCnt.p, cnt);
-        VM_HIP(hipMemsetAsync(dGrad.p, 0, dGrad.n * sizeof(V), stream));
+        VM_HIP(hipMemsetAsync(dGrad.p, 0, dGrad.n * sizeof(vmk::GradT<R>), stream));
         VM_HIP(vmk::launchFillInt(stream, dCounts.p, int64_t(dCounts.n), 0));
         gridScan = std::max(1, std::min(n, kMaxScanGrid));
         gridSite = std::max(1, std::min(int(L.siteWork.size()), kMaxSiteGrid));
@@ -426,12 +460,12 @@ public:
             if (wantDbg && L.groups[g].K == K) a.dbgKnn = dK.p, a.dbgTerm = dT.p;
             VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, a));
         }
-        std::vector<V> hg;
+        std::vector<vmk::GradT<R>> hg;
         if (wantGrad) {
-            hg.resize(in.S);
+            hg.resize(2 * size_t(in.S));
             if (in.S)
-                VM_HIP(hipMemcpyAsync(hg.data(), dGrad.p + in.siteOff, in.S * sizeof(V), hipMemcpyDeviceToHost,
-                                      stream));
+                VM_HIP(hipMemcpyAsync(hg.data(), dGrad.p + 2 * in.siteOff, hg.size() * sizeof(vmk::GradT<R>),
+                                      hipMemcpyDeviceToHost, stream));
         }
         std::vector<int>          hk;
         std::vector<vmk::TermDbg> ht;
@@ -446,12 +480,14 @@ public:
         }
         vmk::InstState st;
         VM_HIP(hipMemcpyAsync(&st, dState.p + id, sizeof(st), hipMemcpyDeviceToHost, stream));
-        if (wantGrad) VM_HIP(hipMemsetAsync(dGrad.p, 0, dGrad.n * sizeof(V), stream));
+        if (wantGrad) VM_HIP(hipMemsetAsync(dGrad.p, 0, dGrad.n * sizeof(vmk::GradT<R>), stream));
         bin();
         VM_HIP(hipStreamSynchronize(stream));
         out.lossSum = st.evalLoss, out.count = st.evalCount;
         out.grad.assign(2 * size_t(in.S), 0.0);
-        for (int k = 0; k < (wantGrad ? in.S : 0); ++k) out.grad[2 * k] = double(hg[k].x), out.grad[2 * k + 1] = double(hg[k].y);
+        // FP32 runs: exact value of the fixed-point sums, sum * 2^-gradShift.
+        const double inv = sizeof(R) == 4 ? std::ldexp(1.0, -in.gradShift) : 1.0;
+        for (size_t k = 0; k < hg.size(); ++k) out.grad[k] = double(hg[k]) * inv;
         if (wantDbg) {
             out.knn.assign(size_t(in.T) * K, -1);
             out.terms.assign(in.T, vmk::TermDbg{-1, -1, 0.0});

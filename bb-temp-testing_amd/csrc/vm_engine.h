@@ -121,6 +121,17 @@ public:
     hipStream_t stream = nullptr;
 };
 
+// Lays out one instance at the given pooled offsets: its Inst record, site
+// chunks, target chunks (one wavefront each) and the sort permutation of its
+// targets (sorted position -> input index).
+void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff, int64_t targetOff, vmk::Inst &in,
+                    std::vector<vmk::SiteWork> &siteWork, std::vector<vmk::TargetWork> &targetWork,
+                    std::vector<int> &perm);
+
+// Fixed-point shift of the FP32 gradient sums of an instance with T targets in
+// a domain of extent `ext`.
+int gradShiftFor(int numTargets, float ext);
+
 // Default per-size-class choices (hash cell in mean spacings, staged rings).
 void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR);
 

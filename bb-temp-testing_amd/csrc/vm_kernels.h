@@ -11,6 +11,17 @@ template <class R> struct V2 {
     R x, y;
 };
 
+// Gradient accumulator element (two per site, x then y).  FP32 runs sum the
+// gradient terms in 64-bit fixed point, round(term * 2^gradShift), so the sum
+// does not depend on the order the atomics land in; FP64 runs sum in double.
+template <class R> struct GradAcc {
+    using T = double;
+};
+template <> struct GradAcc<float> {
+    using T = long long;
+};
+template <class R> using GradT = typename GradAcc<R>::T;
+
 // Per-target diagnostics written by the loss kernel when requested.
 struct TermDbg {
     int    i0, j;  // nearest site and bisector partner (instance-local), -1 if unused
@@ -28,7 +39,7 @@ template <class R> struct LossArgs {
     const int        *start;  // cell starts
     const V2<R>      *tpos;   // targets sorted by cell
     const int        *tslot;  // their slot numbers (subsampling hash index)
-    V2<R>            *grad;   // accumulated gradient, null: loss only
+    GradT<R>         *grad;   // accumulated gradient [2 per site], null: loss only
     int               mode;   // kModeIterate / kModeEvaluate
     int              *dbgKnn; // optional [T*K] k-NN lists (instance-local indices)
     TermDbg          *dbgTerm;
@@ -39,7 +50,8 @@ template <class R> struct AdamArgs {
     const InstState     *state;
     const SiteWork      *work;
     const int           *nWork;
-    V2<R>               *pos, *m, *v, *grad;
+    V2<R>               *pos, *m, *v;
+    GradT<R>            *grad;
     const unsigned char *frozen;
     int                 *cellOf, *rank, *counts;
     int                  doAdam; // 0: only bin the current positions

@@ -87,8 +87,37 @@ __device__ __forceinline__ double powi(double b, int e)
     return r;
 }
 
-__device__ __forceinline__ void gatomicAdd(float *p, float v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void gatomicAdd(double *p, double v) { unsafeAtomicAdd(p, v); }
+// Gradient accumulation.  FP32: a term c becomes round(c * 2^shift) (double
+// scaling is exact, rounding to nearest even), clamped to +-2^56, and is summed
+// with 64-bit integer atomics; the sum is order-independent.  FP64: double atomics.
+constexpr double kQuantClamp = 72057594037927936.0; // 2^56
+
+__device__ __forceinline__ long long quantize(float c, double scale)
+{
+    double x = double(c) * scale;
+    x        = fmin(fmax(x, -kQuantClamp), kQuantClamp);
+    return __double2ll_rn(x);
+}
+
+template <class R> struct GradOps;
+template <> struct GradOps<float> {
+    using T = long long;
+    static __device__ __forceinline__ void add(T *p, float c, double scale)
+    {
+        atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)quantize(c, scale));
+    }
+    static __device__ __forceinline__ void addRaw(T *p, T v)
+    {
+        atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
+    }
+    static __device__ __forceinline__ float toReal(T v, double inv) { return float(double(v) * inv); }
+};
+template <> struct GradOps<double> {
+    using T = double;
+    static __device__ __forceinline__ void add(T *p, double c, double) { unsafeAtomicAdd(p, c); }
+    static __device__ __forceinline__ void addRaw(T *p, T v) { unsafeAtomicAdd(p, v); }
+    static __device__ __forceinline__ double toReal(T v, double) { return v; }
+};
 
 // ---------------------------------------------------------------------------
 // k-NN keys: total order (d2, site index).  The low word of a key is
@@ -288,30 +317,32 @@ __global__ __launch_bounds__(kSiteChunk) void scatterKernel(const Inst *__restri
 // ---------------------------------------------------------------------------
 // 3. VoroLoss + gradient (voro_optimizer.cu:199-266)
 //
-// One wavefront takes up to 64 targets of one hash cell whose bounding box B
-// is small (the host cuts chunks by extent).  The sites of the (2r+1)^2 cell
-// block around the cell are staged in LDS.
+// A block holds four independent wavefronts; each takes up to 64 targets of
+// one hash cell that lie within rho of a chunk centre q (both precomputed on
+// the host: targets do not move).  The sites of the (2r+1)^2 cell block around
+// the cell are staged in the wavefront's LDS slice.
 //
-// Fast path.  Let q be the centre of B, rho its half-diagonal and D >= the
-// distance from q to its K-th nearest staged site.  For every target p in B,
-// D_K(p) <= D + rho, so p's K nearest sites all lie within D + 2 rho of q; if
-// the staged block provably contains that disc, those candidates (q's own
-// nearest first) are compacted in LDS and every lane selects its exact top-K
-// under the (d2, index) order as an unsorted set with a tracked maximum.
-// When c0 is common to the wave, the (c0, cj) bisector terms are computed once
-// per wave.  Otherwise the lanes fall back to the ring search.
+// Fast path.  With D >= the distance from q to its K-th nearest staged site,
+// every target p of the chunk has D_K(p) <= D + rho, so its K nearest sites lie
+// within D + 2 rho of q.  When the staged block provably contains that disc,
+// the candidates (q's own nearest first) are compacted in LDS and each lane
+// selects its exact top-K under the (d2, index) order as an unsorted set with a
+// tracked maximum.  When c0 is common to the wave, the (c0, cj) bisector terms
+// are computed once per wave.  Otherwise the lanes fall back to a ring search.
+// Wave-wide reductions use DPP (no LDS round trips).
 // ---------------------------------------------------------------------------
-constexpr int kLossThreads   = 64;
+constexpr int kLossWaves     = 4;
+constexpr int kLossThreads   = 64 * kLossWaves;
 constexpr int kStageMaxCells = 25;  // (2*2+1)^2
 constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane
 
 template <class R> struct LossLds {
-    V2<R> pos[kLossStage];
-    int   idx[kLossStage];
-    V2<R> grad[kLossStage];
-    R     ex[kLossStage], ey[kLossStage], el[kLossStage]; // cj - c0 and |cj - c0| for the wave's common c0
-    int   cellBeg[kStageMaxCells];
-    int   cellEnd[kStageMaxCells];
+    V2<R>    pos[kLossStage];
+    int      idx[kLossStage];
+    GradT<R> grad[2 * kLossStage];
+    R        ex[kLossStage], ey[kLossStage], el[kLossStage]; // cj - c0 and |cj - c0| for the wave's common c0
+    int      cellBeg[kStageMaxCells];
+    int      cellEnd[kStageMaxCells];
 };
 
 template <class R> __device__ __forceinline__ R relMargin();
@@ -322,24 +353,78 @@ template <class R> __device__ __forceinline__ R rInf();
 template <> __device__ __forceinline__ float  rInf<float>() { return __builtin_huge_valf(); }
 template <> __device__ __forceinline__ double rInf<double>() { return __builtin_huge_val(); }
 
-template <class T> __device__ __forceinline__ T waveMin(T v)
+// Orders LDS traffic of one wavefront (its lanes execute in lockstep).
+__device__ __forceinline__ void waveSync()
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        T w = __shfl_xor(v, o, 64);
-        v   = w < v ? w : v;
-    }
-    return v;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-template <class T> __device__ __forceinline__ T waveMax(T v)
+// DPP controls (gfx9 encoding).
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+constexpr int kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+
+template <int Ctrl, int RowMask> __device__ __forceinline__ uint32_t dpp32(uint32_t identity, uint32_t v)
+{
+    return uint32_t(__builtin_amdgcn_update_dpp(int(identity), int(v), Ctrl, RowMask, 0xF, false));
+}
+
+// Wave-wide reduction of a 32-bit value with a commutative op; every lane of
+// the wave must be active.  Result valid in all lanes (read from lane 63).
+template <class T, class Op> __device__ __forceinline__ T waveReduce32(T v, T identity, Op op)
+{
+    static_assert(sizeof(T) == 4, "32-bit values only");
+    auto bits = [](T x) { return __builtin_bit_cast(uint32_t, x); };
+    auto val  = [](uint32_t b) { return __builtin_bit_cast(T, b); };
+    const uint32_t id = bits(identity);
+    v = op(v, val(dpp32<kDppXor1, 0xF>(id, bits(v))));
+    v = op(v, val(dpp32<kDppXor2, 0xF>(id, bits(v))));
+    v = op(v, val(dpp32<kDppHalfMirror, 0xF>(id, bits(v))));
+    v = op(v, val(dpp32<kDppMirror, 0xF>(id, bits(v))));
+    v = op(v, val(dpp32<kDppBcast15, 0xA>(id, bits(v))));
+    v = op(v, val(dpp32<kDppBcast31, 0xC>(id, bits(v))));
+    return val(uint32_t(__builtin_amdgcn_readlane(int(bits(v)), 63)));
+}
+
+__device__ __forceinline__ double dppDouble(double identity, double v, int which)
+{
+    uint64_t b = __builtin_bit_cast(uint64_t, v), i = __builtin_bit_cast(uint64_t, identity);
+    uint32_t lo = uint32_t(b), hi = uint32_t(b >> 32), ilo = uint32_t(i), ihi = uint32_t(i >> 32);
+    switch (which) {
+    case 0: lo = dpp32<kDppXor1, 0xF>(ilo, lo), hi = dpp32<kDppXor1, 0xF>(ihi, hi); break;
+    case 1: lo = dpp32<kDppXor2, 0xF>(ilo, lo), hi = dpp32<kDppXor2, 0xF>(ihi, hi); break;
+    case 2: lo = dpp32<kDppHalfMirror, 0xF>(ilo, lo), hi = dpp32<kDppHalfMirror, 0xF>(ihi, hi); break;
+    case 3: lo = dpp32<kDppMirror, 0xF>(ilo, lo), hi = dpp32<kDppMirror, 0xF>(ihi, hi); break;
+    case 4: lo = dpp32<kDppBcast15, 0xA>(ilo, lo), hi = dpp32<kDppBcast15, 0xA>(ihi, hi); break;
+    default: lo = dpp32<kDppBcast31, 0xC>(ilo, lo), hi = dpp32<kDppBcast31, 0xC>(ihi, hi); break;
+    }
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ double waveSumDouble(double v)
 {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        T w = __shfl_xor(v, o, 64);
-        v   = w > v ? w : v;
-    }
-    return v;
+    for (int k = 0; k < 6; ++k) v = v + dppDouble(0.0, v, k);
+    uint64_t b = __builtin_bit_cast(uint64_t, v);
+    uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b)), 63));
+    uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b >> 32)), 63));
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
+}
+
+template <class R> __device__ __forceinline__ R waveMaxR(R v);
+template <> __device__ __forceinline__ float waveMaxR<float>(float v)
+{
+    return waveReduce32<float>(v, -__builtin_huge_valf(), [](float a, float b) { return fmaxf(a, b); });
+}
+template <> __device__ __forceinline__ double waveMaxR<double>(double v)
+{
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v = fmax(v, dppDouble(-__builtin_huge_val(), v, k));
+    uint64_t b = __builtin_bit_cast(uint64_t, v);
+    uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b)), 63));
+    uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(b >> 32)), 63));
+    return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
 // Number of set bits of `mask` below this lane.
@@ -351,10 +436,11 @@ __device__ __forceinline__ int laneRank(uint64_t mask)
 template <int K, class R>
 __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
 {
-    __shared__ LossLds<R> lds;
-    const int lane  = threadIdx.x;
-    const int nWork = *a.nWork;
-    for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
+    __shared__ LossLds<R> ldsAll[kLossWaves];
+    const int  lane  = threadIdx.x & 63;
+    LossLds<R> &lds  = ldsAll[threadIdx.x >> 6];
+    const int  nWork = *a.nWork;
+    for (int w = blockIdx.x * kLossWaves + (threadIdx.x >> 6); w < nWork; w += gridDim.x * kLossWaves) {
         const TargetWork wk = a.work[w];
         const Inst      &in = a.insts[wk.inst];
         InstState       *st = a.state + wk.inst;
@@ -372,40 +458,44 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         const int      snx = sx1 - sx0 + 1, sny = sy1 - sy0 + 1;
         const int      ncell = snx * sny;
         const R        cell  = R(in.cell);
+        const double   gscale = __builtin_ldexp(1.0, in.gradShift);
 
         // ---- stage the cell block [sx0,sx1] x [sy0,sy1] into LDS ----------
+        // Rows of the block are contiguous ranges of the sorted arrays.
         int b = 0, e = 0;
         if (lane < ncell) {
             int c = (sy0 + lane / snx) * gw + sx0 + lane % snx;
             b     = strt[c];
             e     = strt[c + 1];
         }
-        int cnt = e - b, inc = cnt;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            int o = __shfl_up(inc, off, 64);
-            if (lane >= off) inc += o;
+        const int myRow = lane < ncell ? lane / snx : -1;
+        int       total = 0, myOff = 0;
+        for (int r = 0; r < sny; ++r) {
+            const int rb = __builtin_amdgcn_readlane(b, r * snx);
+            const int re = __builtin_amdgcn_readlane(e, r * snx + snx - 1);
+            if (myRow == r) myOff = total + b - rb;
+            total += re - rb;
         }
-        const int  total  = __shfl(inc, 63, 64);
         const bool staged = total <= kLossStage;
         if (staged) {
             if (lane < ncell) {
-                lds.cellBeg[lane] = inc - cnt;
-                lds.cellEnd[lane] = inc;
+                lds.cellBeg[lane] = myOff;
+                lds.cellEnd[lane] = myOff + e - b;
             }
-            // Rows of the block are contiguous ranges of the sorted arrays.
+            int roff = 0;
             for (int r = 0; r < sny; ++r) {
-                int rb   = __shfl(b, r * snx, 64);
-                int re   = __shfl(e, r * snx + snx - 1, 64);
-                int roff = __shfl(inc - cnt, r * snx, 64);
+                const int rb = __builtin_amdgcn_readlane(b, r * snx);
+                const int re = __builtin_amdgcn_readlane(e, r * snx + snx - 1);
                 for (int s = lane; s < re - rb; s += 64) {
-                    lds.pos[roff + s]  = spos[rb + s];
-                    lds.idx[roff + s]  = sidx[rb + s];
-                    lds.grad[roff + s] = V2<R>{R(0), R(0)};
+                    lds.pos[roff + s]          = spos[rb + s];
+                    lds.idx[roff + s]          = sidx[rb + s];
+                    lds.grad[2 * (roff + s)]     = GradT<R>(0);
+                    lds.grad[2 * (roff + s) + 1] = GradT<R>(0);
                 }
+                roff += re - rb;
             }
         }
-        __syncthreads();
+        waveSync();
 
         // ---- this lane's target -----------------------------------------
         const int t    = wk.begin + lane;
@@ -419,33 +509,28 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
             if (take && a.mode == kModeIterate && in.fraction < 1.0f)
                 take = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(iter), in.seed) < in.fraction;
         }
-        const uint64_t takeMask = __ballot(take);
-        if (takeMask == 0) {
-            __syncthreads();
+        if (__ballot(take) == 0) {
+            waveSync();
             continue;
         }
 
         Key<R> keys[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) keys[k] = Key<R>::none();
-        bool fast  = staged && total >= K;
-        int  m     = 0; // compacted candidates (fast path)
-        const R ext = fmax(R(in.ext), fmax(fabs(p.x), fabs(p.y)));
+        const R ext  = R(in.ext);
+        bool    fast = staged && total >= K;
+        int     m    = 0; // compacted candidates (fast path)
         if (fast) {
-            // Bounding box of the wave's targets.
-            R  bx0 = waveMin(take ? p.x : rInf<R>()), by0 = waveMin(take ? p.y : rInf<R>());
-            R  bx1 = waveMax(take ? p.x : -rInf<R>()), by1 = waveMax(take ? p.y : -rInf<R>());
-            V2<R> q{R(0.5) * (bx0 + bx1), R(0.5) * (by0 + by1)};
-            R  ddx = bx1 - bx0, ddy = by1 - by0;
-            R  mg  = relMargin<R>() * waveMax(ext);
-            R  rho = R(0.5) * sqrtR(ddx * ddx + ddy * ddy) + mg;
+            const V2<R> q{R(wk.qx), R(wk.qy)};
+            const R     mg  = relMargin<R>() * fmax(ext, fmax(fabs(q.x), fabs(q.y)) + R(wk.rho));
+            const R     rho = R(wk.rho) + mg;
             // Squared distance from q to each staged site (two per lane).
             const int e0 = lane, e1 = lane + 64;
-            R   d0 = e0 < total ? dist2(lds.pos[e0], q) : rInf<R>();
-            R   d1 = e1 < total ? dist2(lds.pos[e1], q) : rInf<R>();
-            // D^2 >= q's K-th smallest by bisection on the value range.
-            R lo = waveMin(d0 < d1 ? d0 : d1);
-            R hi = waveMax(fmax(e0 < total ? d0 : R(0), e1 < total ? d1 : R(0)));
+            R d0 = e0 < total ? dist2(lds.pos[e0], q) : rInf<R>();
+            R d1 = e1 < total ? dist2(lds.pos[e1], q) : rInf<R>();
+            // D^2 >= q's K-th smallest by bisection on [0, max].
+            R lo = R(0);
+            R hi = waveMaxR<R>(fmax(e0 < total ? d0 : R(0), e1 < total ? d1 : R(0)));
             for (int it = 0; it < 14; ++it) {
                 R   mid = lo + R(0.5) * (hi - lo);
                 int c   = __popcll(__ballot(d0 <= mid)) + __popcll(__ballot(d1 <= mid));
@@ -468,7 +553,7 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                 const uint64_t c0m = __ballot(d0 <= thr2 && !(d0 <= hi)), c1m = __ballot(d1 <= thr2 && !(d1 <= hi));
                 const int      nIn = __popcll(in0) + __popcll(in1);
                 m                  = nIn + __popcll(c0m) + __popcll(c1m);
-                int   dst0 = -1, dst1 = -1;
+                int dst0 = -1, dst1 = -1;
                 if (d0 <= hi) dst0 = laneRank(in0);
                 else if (d0 <= thr2) dst0 = nIn + laneRank(c0m);
                 if (d1 <= hi) dst1 = __popcll(in0) + laneRank(in1);
@@ -477,10 +562,10 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                 int   i0v = 0, i1v = 0;
                 if (dst0 >= 0) p0 = lds.pos[e0], i0v = lds.idx[e0];
                 if (dst1 >= 0) p1 = lds.pos[e1], i1v = lds.idx[e1];
-                __syncthreads();
+                waveSync();
                 if (dst0 >= 0) lds.pos[dst0] = p0, lds.idx[dst0] = i0v;
                 if (dst1 >= 0) lds.pos[dst1] = p1, lds.idx[dst1] = i1v;
-                __syncthreads();
+                waveSync();
                 // Exact top-K of each lane as an unsorted set with a tracked maximum.
                 if (take) {
 #pragma unroll
@@ -509,7 +594,7 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         if (!fast) {
             // ---- fallback: ring search (the ring geometry is wave-uniform) ---
             const R rel    = ringRel<R>();
-            const R margin = rel * ext;
+            const R margin = rel * fmax(ext, fmax(fabs(p.x), fabs(p.y)));
             bool    active = take;
             for (int ring = 0;; ++ring) {
                 if (!__any(active)) break;
@@ -552,18 +637,20 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
 #pragma unroll
         for (int k = 1; k < K; ++k)
             if (keys[k].lt(kmin)) kmin = keys[k];
-        const bool   have = take && kmin.valid();
-        const int    i0   = have ? keySite(kmin.low()) : -1;
-        const int    sl0  = have ? keySlot(kmin.low()) : kLdsNone;
-        V2<R>        c0   = V2<R>{R(0), R(0)};
+        const bool have = take && kmin.valid();
+        const int  i0   = have ? keySite(kmin.low()) : -1;
+        const int  sl0  = have ? keySlot(kmin.low()) : kLdsNone;
+        V2<R>      c0   = V2<R>{R(0), R(0)};
         if (have) c0 = sl0 != kLdsNone ? lds.pos[sl0] : a.pos[so + i0];
         // Wave-common c0: tabulate cj - c0 and |cj - c0| once for the candidates.
         bool useTab = false;
         if (fast) {
-            int lo0 = waveMin(have ? i0 : INT_MAX), hi0 = waveMax(have ? i0 : -1);
-            useTab  = lo0 == hi0;
+            const uint64_t haveMask = __ballot(have);
+            const int      first    = haveMask ? __builtin_ctzll(haveMask) : 0;
+            const int      f0       = __builtin_amdgcn_readlane(i0, first);
+            useTab                  = haveMask != 0 && __ballot(have && i0 != f0) == 0;
             if (useTab) {
-                const int   csl = waveMin(have ? sl0 : INT_MAX);
+                const int   csl = __builtin_amdgcn_readlane(sl0, first);
                 const V2<R> cc  = lds.pos[csl];
                 for (int j = lane; j < m; j += 64) {
                     V2<R> ev{lds.pos[j].x - cc.x, lds.pos[j].y - cc.y};
@@ -572,7 +659,7 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                     lds.el[j] = sqrtR(dotR(ev, ev));
                 }
             }
-            __syncthreads();
+            waveSync();
         }
 
         if (a.dbgKnn && lane < wk.count) {
@@ -599,10 +686,10 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         int used  = 0, bj = -1;
         if (have) {
             V2<R>   u{p.x - c0.x, p.y - c0.y};
-            R       best  = sizeof(R) == 4 ? R(FLT_MAX) : R(DBL_MAX);
-            R       bestD = R(0);
+            R       best    = sizeof(R) == 4 ? R(FLT_MAX) : R(DBL_MAX);
+            R       bestD   = R(0);
             Key<R>  bestKey = Key<R>::none();
-            const R tiny  = sizeof(R) == 4 ? R(1e-12f) : R(1e-12);
+            const R tiny    = sizeof(R) == 4 ? R(1e-12f) : R(1e-12);
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 if (!keys[k].valid() || !kmin.lt(keys[k])) continue; // skip c0 and missing entries
@@ -653,20 +740,12 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                     R     s = R(2) * bestD;
                     V2<R> gj{s * dcj.x, s * dcj.y};
                     V2<R> g0{-(s * (n.x + dcj.x)), -(s * (n.y + dcj.y))};
-                    if (slj != kLdsNone) {
-                        atomicAdd(&lds.grad[slj].x, gj.x);
-                        atomicAdd(&lds.grad[slj].y, gj.y);
-                    } else {
-                        gatomicAdd(&a.grad[so + bj].x, gj.x);
-                        gatomicAdd(&a.grad[so + bj].y, gj.y);
-                    }
-                    if (sl0 != kLdsNone) {
-                        atomicAdd(&lds.grad[sl0].x, g0.x);
-                        atomicAdd(&lds.grad[sl0].y, g0.y);
-                    } else {
-                        gatomicAdd(&a.grad[so + i0].x, g0.x);
-                        gatomicAdd(&a.grad[so + i0].y, g0.y);
-                    }
+                    GradT<R> *dj = slj != kLdsNone ? &lds.grad[2 * slj] : &a.grad[2 * (so + bj)];
+                    GradT<R> *d0 = sl0 != kLdsNone ? &lds.grad[2 * sl0] : &a.grad[2 * (so + i0)];
+                    GradOps<R>::add(dj, gj.x, gscale);
+                    GradOps<R>::add(dj + 1, gj.y, gscale);
+                    GradOps<R>::add(d0, g0.x, gscale);
+                    GradOps<R>::add(d0 + 1, g0.y, gscale);
                 }
             }
         }
@@ -678,13 +757,8 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         }
 
         // ---- FP64 loss reduction: one atomic pair per wavefront ----------
-        double ls = double(lossT);
-        int    uc = used;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            ls += __shfl_down(ls, off, 64);
-            uc += __shfl_down(uc, off, 64);
-        }
+        const double ls = waveSumDouble(double(lossT));
+        const int    uc = __popcll(__ballot(used != 0));
         if (lane == 0 && uc > 0) {
             if (a.mode == kModeIterate) {
                 unsafeAtomicAdd(&st->loss, ls);
@@ -696,19 +770,19 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         }
 
         // ---- flush the LDS gradient: one atomic per touched site ----------
-        __syncthreads();
+        waveSync();
         const int nflush = fast ? m : (staged ? total : 0);
         if (a.grad) {
             for (int s = lane; s < nflush; s += 64) {
-                V2<R> g = lds.grad[s];
-                if (g.x != R(0) || g.y != R(0)) {
+                GradT<R> gx = lds.grad[2 * s], gy = lds.grad[2 * s + 1];
+                if (gx != GradT<R>(0) || gy != GradT<R>(0)) {
                     int i = lds.idx[s];
-                    gatomicAdd(&a.grad[so + i].x, g.x);
-                    gatomicAdd(&a.grad[so + i].y, g.y);
+                    GradOps<R>::addRaw(&a.grad[2 * (so + i)], gx);
+                    GradOps<R>::addRaw(&a.grad[2 * (so + i) + 1], gy);
                 }
             }
         }
-        __syncthreads();
+        waveSync();
     }
 }
 
@@ -730,8 +804,10 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
         const int64_t g = in.siteOff + i;
         V2<R>         p = a.pos[g];
         if (a.doAdam) {
-            V2<R> gr = a.grad[g];
-            a.grad[g] = V2<R>{R(0), R(0)};
+            const double ginv = __builtin_ldexp(1.0, -in.gradShift);
+            V2<R>        gr{GradOps<R>::toReal(a.grad[2 * g], ginv), GradOps<R>::toReal(a.grad[2 * g + 1], ginv)};
+            a.grad[2 * g]     = GradT<R>(0);
+            a.grad[2 * g + 1] = GradT<R>(0);
             bool movable = in.allMovable || !a.frozen[g];
             if (movable) {
                 const int t  = st->iter;
@@ -1007,7 +1083,7 @@ template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const L
 {
     switch (K) {
 #define VM_K(k)                                                                                                   \
-    case k: hipLaunchKernelGGL((lossKernel<k, R>), dim3(grid), dim3(kLossThreads), 0, s, a); break;
+    case k: hipLaunchKernelGGL((lossKernel<k, R>), dim3((grid + kLossWaves - 1) / kLossWaves), dim3(kLossThreads), 0, s, a); break;
         VM_K(2) VM_K(4) VM_K(6) VM_K(8) VM_K(12) VM_K(16) VM_K(24) VM_K(32)
 #undef VM_K
     default: return hipErrorInvalidValue;

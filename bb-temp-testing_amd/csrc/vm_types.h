@@ -42,7 +42,8 @@ struct Inst {
     float   bx0, by0, bx1, by1;
     int     maxIter;      // 0: no limit
     float   tol;          // relative loss-change convergence threshold (0: off)
-    int     pad[3];
+    int     gradShift;    // FP32 runs: gradient terms are summed as round(term * 2^gradShift) in int64
+    int     pad[2];
 };
 
 // Written by the device every iteration (voro_optimizer.cu:81-85 IterState,
@@ -61,8 +62,11 @@ struct SiteWork {
     int inst, begin, count, pad;
 };
 
+// Targets [begin, begin+count) of `inst` (sorted order), all in hash cell
+// `cell`; every one lies within `rho` of (qx, qy) (static: targets do not move).
 struct TargetWork {
-    int inst, begin, count, cell; // targets [begin, begin+count) of `inst`, all in hash cell `cell`
+    int   inst, begin, count, cell;
+    float qx, qy, rho, pad;
 };
 
 // Poisson dart thrower: one record per instance grid being (re)sampled.

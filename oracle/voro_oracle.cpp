@@ -10,8 +10,11 @@
 //   * k-nearest lists are ordered by the total order (d2, index) with
 //     d2 = fma(dy, dy, dx*dx); the reference orders ties by visit order
 //     (voro_optimizer.cu:149-151), which depends on atomic ranks (:176).
-//   * gradient contributions are summed in double in target order (the
-//     reference uses FP32 atomics in arbitrary order, :250-253).
+//   * gradient contributions are summed order-independently (the reference
+//     uses FP32 atomics in arbitrary order, :250-253): in FP32 mode each term
+//     c becomes round(c * 2^g) in 64-bit integers (g from the target count and
+//     domain extent, see gradShift), and the sum is read back as sum * 2^-g;
+//     in FP64 mode terms are summed in double in target order.
 //   * the Adam bias corrections 1 - beta^t are evaluated in double by
 //     exponentiation by squaring and rounded once (the reference uses powf,
 //     :287-288, whose rounding is library specific).
@@ -253,6 +256,29 @@ bool takeTarget(V2<R> p, uint32_t slot, const double *box, float fraction, uint3
     return take;
 }
 
+// ceil(log2(x)) for x > 0, exact.
+inline int ceilLog2(double x)
+{
+    int    e;
+    double m = std::frexp(x, &e);
+    return m == 0.5 ? e - 1 : e;
+}
+
+// Fixed-point shift of the FP32 gradient sums: a site collects at most T
+// terms of magnitude ~ few * extent, so the sums stay below 2^60.
+inline int gradShift(int liveTargets, float extent)
+{
+    int g = 58 - ceilLog2(double(std::max(liveTargets, 1))) - ceilLog2(double(std::max(extent, 1e-30f)));
+    return std::min(std::max(g, 8), 52);
+}
+
+inline int64_t quantizeTerm(float c, int shift)
+{
+    double x = std::ldexp(double(c), shift);
+    x        = std::fmin(std::fmax(x, -72057594037927936.0), 72057594037927936.0); // +-2^56
+    return std::llrint(x);
+}
+
 struct LossOut {
     double  lossSum = 0;
     int64_t count   = 0;
@@ -263,7 +289,8 @@ struct LossOut {
 template <class R>
 LossOut lossPass(const V2<R> *sites, int S, const Grid<R> &g, const V2<R> *targets, int T,
                  const int32_t *slotOf, int K, const double *box, float fraction, uint32_t iter, uint32_t seed,
-                 double *grad /* 2*S or null */, int32_t *c0Out, int32_t *bjOut, R *lossOut, R *termGrad /* 4*T */)
+                 float extent, double *grad /* 2*S or null */, int32_t *c0Out, int32_t *bjOut, R *lossOut,
+                 R *termGrad /* 4*T */)
 {
     std::vector<TargetTerm<R>> terms(T);
     std::vector<uint8_t>       used(T, 0);
@@ -279,6 +306,11 @@ LossOut lossPass(const V2<R> *sites, int S, const Grid<R> &g, const V2<R> *targe
     }
     LossOut out;
     if (grad) std::fill(grad, grad + 2 * size_t(S), 0.0);
+    const bool           fixedPoint = sizeof(R) == 4;
+    int                  live       = 0;
+    for (int t = 0; t < T; ++t) live += targets[t].x >= R(0);
+    const int            shift = gradShift(live, extent);
+    std::vector<int64_t> isum(fixedPoint && grad ? 2 * size_t(S) : 0, 0);
     for (int t = 0; t < T; ++t) {
         if (c0Out) c0Out[t] = used[t] ? terms[t].i0 : -1;
         if (bjOut) bjOut[t] = used[t] ? terms[t].j : -1;
@@ -292,13 +324,20 @@ LossOut lossPass(const V2<R> *sites, int S, const Grid<R> &g, const V2<R> *targe
         if (!used[t]) continue;
         out.lossSum += double(terms[t].loss);
         out.count += 1;
-        if (grad) {
+        if (grad && fixedPoint) {
+            isum[2 * size_t(terms[t].j) + 0] += quantizeTerm(float(terms[t].gj.x), shift);
+            isum[2 * size_t(terms[t].j) + 1] += quantizeTerm(float(terms[t].gj.y), shift);
+            isum[2 * size_t(terms[t].i0) + 0] += quantizeTerm(float(terms[t].g0.x), shift);
+            isum[2 * size_t(terms[t].i0) + 1] += quantizeTerm(float(terms[t].g0.y), shift);
+        } else if (grad) {
             grad[2 * size_t(terms[t].j) + 0] += double(terms[t].gj.x);
             grad[2 * size_t(terms[t].j) + 1] += double(terms[t].gj.y);
             grad[2 * size_t(terms[t].i0) + 0] += double(terms[t].g0.x);
             grad[2 * size_t(terms[t].i0) + 1] += double(terms[t].g0.y);
         }
     }
+    if (grad && fixedPoint)
+        for (size_t k = 0; k < isum.size(); ++k) grad[k] = std::ldexp(double(isum[k]), -shift);
     return out;
 }
 
@@ -385,8 +424,8 @@ static int lossImpl(const void *sitesV, int S, const void *targetsV, int T, cons
     auto   *targets = static_cast<const V2<R> *>(targetsV);
     int     gw = std::max(1, int(std::ceil(W / cell))), gh = std::max(1, int(std::ceil(H / cell)));
     Grid<R> g = buildGrid(sites, S, gw, gh, R(cell), R(std::max(W, H)));
-    LossOut o = lossPass(sites, S, g, targets, T, slotOf, K, box, fraction, iter, seed, grad, c0Out, bjOut,
-                         static_cast<R *>(lossT), static_cast<R *>(termGrad));
+    LossOut o = lossPass(sites, S, g, targets, T, slotOf, K, box, fraction, iter, seed, float(std::max(W, H)), grad,
+                         c0Out, bjOut, static_cast<R *>(lossT), static_cast<R *>(termGrad));
     if (lossSum) *lossSum = o.lossSum;
     if (count) *count = o.count;
     return 0;
@@ -468,8 +507,9 @@ static int fitImpl(void *sitesV, void *mV, void *vV, const uint8_t *movable, int
     for (int it = 0; it < iters; ++it) {
         int     t = tStart + it;
         Grid<R> g = buildGrid(sites, S, gw, gh, R(cell), R(std::max(sc->width, sc->height)));
-        LossOut o = lossPass(sites, S, g, targets, T, slotOf, K, box, fraction, uint32_t(t), seed, grad.data(),
-                             nullptr, nullptr, static_cast<R *>(nullptr), static_cast<R *>(nullptr));
+        LossOut o = lossPass(sites, S, g, targets, T, slotOf, K, box, fraction, uint32_t(t), seed,
+                             float(std::max(sc->width, sc->height)), grad.data(), nullptr, nullptr,
+                             static_cast<R *>(nullptr), static_cast<R *>(nullptr));
         if (lossHist) lossHist[it] = o.lossSum;
         if (countHist) countHist[it] = o.count;
         adamImpl<R>(sites, m, v, grad.data(), movable, S, t, o.count, *sc);
