@@ -372,9 +372,4 @@ int vm_batch_choice(vm_batch *b, int id, int *k, float *cell, int *rings)
     });
 }
 
-int vm_serve(const vm_fit_desc *, const double *, int, const vm_serve_config *, vm_serve_result *, vm_serve_stats *)
-{
-    return vmrt::guardedCall([&] { throw VmError(VM_EUNSUPPORTED, "vm_serve is not built yet"); });
-}
-
 } // extern "C"

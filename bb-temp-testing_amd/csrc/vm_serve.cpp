@@ -1,0 +1,492 @@
+// vm_serve.cpp -- continuous-batching request loop over resident fit state.
+//
+// Requests arrive on a schedule.  Worker threads prepare each arrived request
+// (live sites/targets, target sort and wavefront chunking).  The GPU loop
+// admits prepared requests in arrival order whenever an instance slot and
+// room in the device pools (bounded by the HBM budget) are free, uploads and
+// bins them, and keeps replaying one captured iteration over every resident
+// instance.  Every `poll_every` iterations it reads the per-instance states:
+// instances the scan kernel marked done (relative loss change below tol, or
+// the iteration cap) are retired, their pool ranges freed, and new requests
+// join the very next iteration.  Sites, Adam moments, hash grids and targets
+// stay on the device for the whole life of a request.
+#include "voromesh_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "vm_engine.h"
+
+using vmrt::DBuf;
+using vmrt::VmError;
+
+namespace vmrt {
+InstSpec specFromDesc(const vm_fit_desc &d);
+}
+
+namespace {
+
+// First-fit allocator over [0, cap) in units of 64 elements.
+class RangeAlloc {
+public:
+    void init(int64_t cap)
+    {
+        free_.clear();
+        if (cap > 0) free_[0] = cap;
+        cap_ = cap;
+    }
+    int64_t alloc(int64_t n)
+    {
+        n = round(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            int64_t off = it->first, len = it->second;
+            free_.erase(it);
+            if (len > n) free_[off + n] = len - n;
+            used_ += n;
+            return off;
+        }
+        return -1;
+    }
+    void release(int64_t off, int64_t n)
+    {
+        n = round(n);
+        used_ -= n;
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + n == next->first) {
+            n += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += n;
+                return;
+            }
+        }
+        free_[off] = n;
+    }
+    int64_t used() const { return used_; }
+    static int64_t round(int64_t n) { return ((n > 0 ? n : 1) + 63) & ~int64_t(63); }
+
+private:
+    std::map<int64_t, int64_t> free_;
+    int64_t                    cap_ = 0, used_ = 0;
+};
+
+using Clock = std::chrono::steady_clock;
+
+struct Prepared {
+    vmrt::InstSpec                spec;
+    vmk::Inst                     in{};
+    std::vector<vmk::SiteWork>    sw;
+    std::vector<vmk::TargetWork>  tw;
+    std::vector<int>              perm;
+    int64_t                       cells = 0;
+};
+
+template <class R> class Server {
+    using V = vmk::V2<R>;
+
+public:
+    Server(const vm_serve_config &cfg, int nReq, int64_t needSites, int64_t needTargets, int64_t needCells)
+        : cfg_(cfg)
+    {
+        VM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        slots_ = cfg.max_resident > 0 ? cfg.max_resident : nReq;
+        // Pool capacities: what the budget allows, but no more than the whole
+        // request stream could ever hold at once.
+        const double perSite = 4.0 * sizeof(V) + 2.0 * sizeof(vmk::GradT<R>) + 3 * sizeof(int) + 1;
+        const double perTgt = sizeof(V) + sizeof(int), perCell = 2 * sizeof(int);
+        const double need = perSite * needSites + perTgt * needTargets + perCell * needCells;
+        const double scale = std::min(1.0, cfg.hbm_budget_bytes / std::max(need, 1.0));
+        capSites_   = RangeAlloc::round(int64_t(needSites * scale) + 64 * slots_);
+        capTargets_ = RangeAlloc::round(int64_t(needTargets * scale) + 64 * slots_);
+        capCells_   = RangeAlloc::round(int64_t(needCells * scale) + 64 * slots_);
+        budgetBytes_ = cfg.hbm_budget_bytes;
+        sites_.init(capSites_), targets_.init(capTargets_), cells_.init(capCells_);
+        dPos.reserve(capSites_), dM.reserve(capSites_), dV.reserve(capSites_), dSpos.reserve(capSites_);
+        dGrad.reserve(2 * capSites_), dCellOf.reserve(capSites_), dRank.reserve(capSites_), dSidx.reserve(capSites_);
+        dFrozen.reserve(1);
+        dTpos.reserve(capTargets_), dTslot.reserve(capTargets_);
+        dCounts.reserve(capCells_), dStart.reserve(capCells_);
+        dInst.reserve(slots_), dState.reserve(slots_), dInstList.reserve(slots_);
+        dSiteWork.reserve(capSites_ / vmk::kSiteChunk + slots_), dNewWork.reserve(capSites_ / vmk::kSiteChunk + slots_);
+        dCnt.reserve(4), dNewCnt.reserve(1);
+        twCap_ = 0;
+        VM_HIP(hipEventCreate(&ev0_));
+        VM_HIP(hipEventCreate(&ev1_));
+        slotReq_.assign(slots_, -1);
+        slotInfo_.resize(slots_);
+        VM_HIP(hipMemsetAsync(dState.p, 0, slots_ * sizeof(vmk::InstState), stream_));
+        VM_HIP(hipStreamSynchronize(stream_));
+    }
+    ~Server()
+    {
+        if (graph_) (void)hipGraphExecDestroy(graph_);
+        (void)hipEventDestroy(ev0_);
+        (void)hipEventDestroy(ev1_);
+        (void)hipStreamDestroy(stream_);
+    }
+
+    double peakBytes() const { return peakBytes_; }
+    int    maxResident() const { return maxResident_; }
+
+    // Tries to place a prepared request; returns false when no slot or pool room.
+    bool admit(int req, Prepared &p)
+    {
+        int slot = -1;
+        for (int s = 0; s < slots_; ++s)
+            if (slotReq_[s] < 0) { slot = s; break; }
+        if (slot < 0) return false;
+        const int S = p.in.S, T = p.in.T;
+        const double bytes = bytesFor(S, T, p.cells);
+        if (residentBytes_ + bytes > budgetBytes_) return false;
+        int64_t so = sites_.alloc(S), to = targets_.alloc(T), co = cells_.alloc(p.cells);
+        if (so < 0 || to < 0 || co < 0) {
+            if (so >= 0) sites_.release(so, S);
+            if (to >= 0) targets_.release(to, T);
+            if (co >= 0) cells_.release(co, p.cells);
+            return false;
+        }
+        vmk::Inst in = p.in;
+        in.siteOff = so, in.cellOff = co, in.targetOff = to;
+        SlotInfo &si = slotInfo_[slot];
+        si       = SlotInfo{so, to, co, S, T, p.cells, bytes, {}, {}};
+        for (auto w : p.sw) si.sw.push_back({slot, w.begin, w.count, 0});
+        for (auto w : p.tw) {
+            w.inst = slot;
+            si.tw.push_back(w);
+        }
+        // Upload: positions, zero moments/gradient/counts, sorted targets, record, state.
+        std::vector<V>   hp(S), ht(T);
+        std::vector<int> hs(T);
+        for (int k = 0; k < S; ++k) hp[k] = V{R(p.spec.pos[2 * k]), R(p.spec.pos[2 * k + 1])};
+        for (int t = 0; t < T; ++t) {
+            int o = p.perm[t];
+            ht[t] = V{R(p.spec.tgt[2 * o]), R(p.spec.tgt[2 * o + 1])};
+            hs[t] = p.spec.tslot.empty() ? o : p.spec.tslot[o];
+        }
+        if (S) {
+            VM_HIP(hipMemcpyAsync(dPos.p + so, hp.data(), S * sizeof(V), hipMemcpyHostToDevice, stream_));
+            VM_HIP(hipMemsetAsync(dM.p + so, 0, S * sizeof(V), stream_));
+            VM_HIP(hipMemsetAsync(dV.p + so, 0, S * sizeof(V), stream_));
+            VM_HIP(hipMemsetAsync(dGrad.p + 2 * so, 0, 2 * S * sizeof(vmk::GradT<R>), stream_));
+        }
+        if (T) {
+            VM_HIP(hipMemcpyAsync(dTpos.p + to, ht.data(), T * sizeof(V), hipMemcpyHostToDevice, stream_));
+            VM_HIP(hipMemcpyAsync(dTslot.p + to, hs.data(), T * sizeof(int), hipMemcpyHostToDevice, stream_));
+        }
+        VM_HIP(hipMemsetAsync(dCounts.p + co, 0, p.cells * sizeof(int), stream_));
+        VM_HIP(hipMemcpyAsync(dInst.p + slot, &in, sizeof(in), hipMemcpyHostToDevice, stream_));
+        vmk::InstState z{};
+        VM_HIP(hipMemcpyAsync(dState.p + slot, &z, sizeof(z), hipMemcpyHostToDevice, stream_));
+        // The host copies above must stay alive until the stream consumed them.
+        VM_HIP(hipStreamSynchronize(stream_));
+        newWork_.insert(newWork_.end(), si.sw.begin(), si.sw.end());
+        slotReq_[slot] = req;
+        residentBytes_ += bytes;
+        ++resident_;
+        maxResident_ = std::max(maxResident_, resident_);
+        peakBytes_   = std::max(peakBytes_, residentBytes_);
+        changed_     = true;
+        return true;
+    }
+
+    // Uploads the work tables after admissions/retirements and bins new instances.
+    void refresh()
+    {
+        if (!changed_) return;
+        std::vector<vmk::SiteWork>   sw;
+        std::vector<vmk::TargetWork> tw;
+        std::vector<int>             il;
+        for (int s = 0; s < slots_; ++s) {
+            if (slotReq_[s] < 0) continue;
+            il.push_back(s);
+            sw.insert(sw.end(), slotInfo_[s].sw.begin(), slotInfo_[s].sw.end());
+            tw.insert(tw.end(), slotInfo_[s].tw.begin(), slotInfo_[s].tw.end());
+        }
+        if (tw.size() > twCap_) {
+            twCap_ = std::max(tw.size() * 3 / 2, size_t(4096));
+            dTargetWork.release();
+            dTargetWork.reserve(twCap_);
+            if (graph_) (void)hipGraphExecDestroy(graph_), graph_ = nullptr;
+        }
+        int cnt[3] = {int(il.size()), int(sw.size()), int(tw.size())};
+        if (!il.empty()) VM_HIP(hipMemcpyAsync(dInstList.p, il.data(), il.size() * 4, hipMemcpyHostToDevice, stream_));
+        if (!sw.empty())
+            VM_HIP(hipMemcpyAsync(dSiteWork.p, sw.data(), sw.size() * sizeof(sw[0]), hipMemcpyHostToDevice, stream_));
+        if (!tw.empty())
+            VM_HIP(hipMemcpyAsync(dTargetWork.p, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice, stream_));
+        VM_HIP(hipMemcpyAsync(dCnt.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, stream_));
+        if (!newWork_.empty()) {
+            int nn = int(newWork_.size());
+            VM_HIP(hipMemcpyAsync(dNewWork.p, newWork_.data(), newWork_.size() * sizeof(newWork_[0]),
+                                  hipMemcpyHostToDevice, stream_));
+            VM_HIP(hipMemcpyAsync(dNewCnt.p, &nn, sizeof(nn), hipMemcpyHostToDevice, stream_));
+            auto a = adamArgs(0);
+            a.work = dNewWork.p, a.nWork = dNewCnt.p;
+            VM_HIP(vmk::launchAdamBin<R>(std::min(nn, 8192), stream_, a));
+        }
+        VM_HIP(hipStreamSynchronize(stream_));
+        newWork_.clear();
+        changed_ = false;
+    }
+
+    // `iters` iterations of every resident instance; returns GPU ms.
+    float iterate(int iters, int K)
+    {
+        if (!graph_) buildGraph(K);
+        VM_HIP(hipEventRecord(ev0_, stream_));
+        for (int k = 0; k < iters; ++k) VM_HIP(hipGraphLaunch(graph_, stream_));
+        VM_HIP(hipEventRecord(ev1_, stream_));
+        VM_HIP(hipEventSynchronize(ev1_));
+        float ms = 0;
+        VM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
+        return ms;
+    }
+
+    // Reads states; retires finished instances.  Calls done(req, state, slot).
+    template <class F> int retire(F &&done)
+    {
+        std::vector<vmk::InstState> st(slots_);
+        VM_HIP(hipMemcpyAsync(st.data(), dState.p, slots_ * sizeof(vmk::InstState), hipMemcpyDeviceToHost, stream_));
+        VM_HIP(hipStreamSynchronize(stream_));
+        int n = 0;
+        for (int s = 0; s < slots_; ++s) {
+            if (slotReq_[s] < 0 || !st[s].done) continue;
+            done(slotReq_[s], st[s], s);
+            SlotInfo &si = slotInfo_[s];
+            sites_.release(si.so, si.S), targets_.release(si.to, si.T), cells_.release(si.co, si.C);
+            residentBytes_ -= si.bytes;
+            slotReq_[s] = -1;
+            --resident_;
+            changed_ = true;
+            ++n;
+        }
+        return n;
+    }
+
+    void downloadSites(int slot, vm_vec2 *out)
+    {
+        const SlotInfo &si = slotInfo_[slot];
+        std::vector<V>  h(si.S);
+        if (si.S) VM_HIP(hipMemcpyAsync(h.data(), dPos.p + si.so, si.S * sizeof(V), hipMemcpyDeviceToHost, stream_));
+        VM_HIP(hipStreamSynchronize(stream_));
+        for (int k = 0; k < si.S; ++k) out[k] = vm_vec2{float(h[k].x), float(h[k].y)};
+    }
+
+    int resident() const { return resident_; }
+
+    double bytesFor(int S, int T, int64_t C) const
+    {
+        return double(RangeAlloc::round(S)) * (4.0 * sizeof(V) + 2.0 * sizeof(vmk::GradT<R>) + 12 + 1) +
+               double(RangeAlloc::round(T)) * (sizeof(V) + 4) + double(RangeAlloc::round(C)) * 8;
+    }
+
+private:
+    struct SlotInfo {
+        int64_t                      so = 0, to = 0, co = 0;
+        int                          S = 0, T = 0;
+        int64_t                      C = 0;
+        double                       bytes = 0;
+        std::vector<vmk::SiteWork>   sw;
+        std::vector<vmk::TargetWork> tw;
+    };
+
+    vmk::AdamArgs<R> adamArgs(int doAdam)
+    {
+        vmk::AdamArgs<R> a{};
+        a.insts = dInst.p, a.state = dState.p, a.work = dSiteWork.p, a.nWork = dCnt.p + 1;
+        a.pos = dPos.p, a.m = dM.p, a.v = dV.p, a.grad = dGrad.p, a.frozen = dFrozen.p;
+        a.cellOf = dCellOf.p, a.rank = dRank.p, a.counts = dCounts.p, a.doAdam = doAdam;
+        return a;
+    }
+
+    void buildGraph(int K)
+    {
+        const int gScan = std::min(slots_, 4096), gSite = 8192, gLoss = 16384;
+        hipGraph_t g;
+        VM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+            VM_HIP(vmk::launchScan(gScan, stream_, dInst.p, dState.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
+                                   vmk::kModeIterate));
+            VM_HIP(vmk::launchScatter<R>(gSite, stream_, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
+                                         dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate));
+            vmk::LossArgs<R> la{};
+            la.insts = dInst.p, la.state = dState.p, la.work = dTargetWork.p, la.nWork = dCnt.p + 2;
+            la.pos = dPos.p, la.spos = dSpos.p, la.sidx = dSidx.p, la.start = dStart.p, la.tpos = dTpos.p;
+            la.tslot = dTslot.p, la.grad = dGrad.p, la.mode = vmk::kModeIterate;
+            VM_HIP(vmk::launchLoss<R>(K, gLoss, stream_, la));
+            VM_HIP(vmk::launchAdamBin<R>(gSite, stream_, adamArgs(1)));
+        } catch (...) {
+            hipGraph_t dummy;
+            (void)hipStreamEndCapture(stream_, &dummy);
+            throw;
+        }
+        VM_HIP(hipStreamEndCapture(stream_, &g));
+        hipError_t e = hipGraphInstantiate(&graph_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        VM_HIP(e);
+    }
+
+    vm_serve_config cfg_;
+    hipStream_t     stream_ = nullptr;
+    hipEvent_t      ev0_ = nullptr, ev1_ = nullptr;
+    hipGraphExec_t  graph_ = nullptr;
+    int             slots_ = 0, resident_ = 0, maxResident_ = 0;
+    int64_t         capSites_ = 0, capTargets_ = 0, capCells_ = 0;
+    double          budgetBytes_ = 0, residentBytes_ = 0, peakBytes_ = 0;
+    size_t          twCap_ = 0;
+    bool            changed_ = false;
+    RangeAlloc      sites_, targets_, cells_;
+    std::vector<int>      slotReq_;
+    std::vector<SlotInfo> slotInfo_;
+    std::vector<vmk::SiteWork> newWork_;
+
+    DBuf<V>              dPos, dM, dV, dSpos, dTpos;
+    DBuf<vmk::GradT<R>>  dGrad;
+    DBuf<int>            dCellOf, dRank, dSidx, dCounts, dStart, dTslot, dInstList, dCnt, dNewCnt;
+    DBuf<uint8_t>        dFrozen;
+    DBuf<vmk::Inst>      dInst;
+    DBuf<vmk::InstState> dState;
+    DBuf<vmk::SiteWork>  dSiteWork, dNewWork;
+    DBuf<vmk::TargetWork> dTargetWork;
+};
+
+template <class R>
+void serve(const vm_fit_desc *reqs, const double *arrival, int n, const vm_serve_config &cfg, vm_serve_result *res,
+           vm_serve_stats *stats)
+{
+    // Sizes of every request (for pool sizing and K).
+    int     K = 0;
+    int64_t needS = 0, needT = 0, needC = 0;
+    for (int i = 0; i < n; ++i) {
+        int k = reqs[i].loss_k > 0 ? reqs[i].loss_k : 16;
+        if (K == 0) K = k;
+        if (k != K) throw VmError(VM_EINVAL, "all requests of one serving run must share loss_k");
+        if (i > 0 && arrival[i] < arrival[i - 1]) throw VmError(VM_EINVAL, "arrival times must be non-decreasing");
+        needS += RangeAlloc::round(reqs[i].num_sites);
+        needT += RangeAlloc::round(reqs[i].num_targets);
+        float cell = reqs[i].grid_cell;
+        int   rs;
+        if (!(cell > 0.0f)) vmrt::tunedChoice(std::max(reqs[i].num_sites, 1), reqs[i].width, reqs[i].height, k, &cell, &rs);
+        needC += RangeAlloc::round(int64_t(std::ceil(reqs[i].width / cell)) * int64_t(std::ceil(reqs[i].height / cell)) + 1);
+    }
+    Server<R> srv(cfg, n, needS, needT, needC);
+
+    // Preparation workers: request i is prepared once it has arrived.
+    std::vector<std::unique_ptr<Prepared>> prep(n);
+    std::vector<std::atomic<int>>          ready(n);
+    for (auto &r : ready) r.store(0);
+    std::atomic<int>  nextPrep{0};
+    std::atomic<bool> stop{false};
+    std::mutex        errMu;
+    std::string       prepError;
+    const auto        t0 = Clock::now();
+    auto elapsed = [&] { return std::chrono::duration<double>(Clock::now() - t0).count(); };
+    unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> workers;
+    for (unsigned w = 0; w < hw; ++w)
+        workers.emplace_back([&] {
+            for (;;) {
+                int i = nextPrep.fetch_add(1);
+                if (i >= n || stop.load()) return;
+                while (!stop.load() && elapsed() < arrival[i])
+                    std::this_thread::sleep_for(std::chrono::microseconds(200));
+                if (stop.load()) return;
+                try {
+                    auto p  = std::make_unique<Prepared>();
+                    p->spec = vmrt::specFromDesc(reqs[i]);
+                    p->spec.maxIter = cfg.max_iterations > 0 ? cfg.max_iterations : reqs[i].max_iterations;
+                    p->spec.tol     = cfg.tol > 0 ? cfg.tol : reqs[i].tol;
+                    vmrt::layoutInstance(p->spec, -1, 0, 0, 0, p->in, p->sw, p->tw, p->perm);
+                    p->cells = int64_t(p->in.gw) * p->in.gh + 1;
+                    prep[i]  = std::move(p);
+                    ready[i].store(1);
+                } catch (const std::exception &e) {
+                    std::lock_guard<std::mutex> lk(errMu);
+                    prepError = e.what();
+                    ready[i].store(2);
+                }
+            }
+        });
+    struct Join {
+        std::vector<std::thread> &w;
+        std::atomic<bool>        &stop;
+        ~Join()
+        {
+            stop.store(true);
+            for (auto &t : w)
+                if (t.joinable()) t.join();
+        }
+    } join{workers, stop};
+
+    std::vector<double> admitT(n, -1.0);
+    int    nextAdmit = 0, completed = 0;
+    double siteIters = 0;
+    const int poll = std::max(1, cfg.poll_every);
+    while (completed < n) {
+        // Admit prepared requests in arrival order.
+        while (nextAdmit < n && ready[nextAdmit].load() != 0) {
+            if (ready[nextAdmit].load() == 2) throw VmError(VM_EINVAL, "request preparation failed: " + prepError);
+            if (!srv.admit(nextAdmit, *prep[nextAdmit])) {
+                if (srv.resident() == 0) throw VmError(VM_ENOMEM, "request does not fit the HBM budget");
+                break;
+            }
+            admitT[nextAdmit] = elapsed();
+            prep[nextAdmit].reset();
+            ++nextAdmit;
+        }
+        srv.refresh();
+        if (srv.resident() == 0) {
+            std::this_thread::sleep_for(std::chrono::microseconds(100));
+            continue;
+        }
+        srv.iterate(poll, K);
+        srv.retire([&](int req, const vmk::InstState &st, int slot) {
+            double now = elapsed();
+            vm_serve_result &r = res[req];
+            r.latency_s        = now - arrival[req];
+            r.queue_s          = admitT[req] - arrival[req];
+            r.iterations       = st.iter;
+            r.final_mean_loss  = st.count > 0 ? st.loss / double(st.count) : 0.0;
+            siteIters += double(reqs[req].num_sites) * st.iter;
+            if (r.final_sites) srv.downloadSites(slot, r.final_sites);
+            ++completed;
+        });
+    }
+    if (stats) {
+        stats->wall_s            = elapsed();
+        stats->site_iters        = siteIters;
+        stats->completed         = completed;
+        stats->max_resident_seen = srv.maxResident();
+        stats->peak_bytes        = srv.peakBytes();
+    }
+}
+
+} // namespace
+
+extern "C" int vm_serve(const vm_fit_desc *requests, const double *arrival, int n, const vm_serve_config *cfg,
+                        vm_serve_result *results, vm_serve_stats *stats)
+{
+    return vmrt::guardedCall([&] {
+        if (n < 0 || (n > 0 && (!requests || !arrival || !results)) || !cfg)
+            throw VmError(VM_EINVAL, "invalid arguments");
+        if (cfg->hbm_budget_bytes <= 0) throw VmError(VM_EINVAL, "hbm budget must be positive");
+        if (n == 0) return;
+        if (cfg->precision == 1) serve<double>(requests, arrival, n, *cfg, results, stats);
+        else if (cfg->precision == 0) serve<float>(requests, arrival, n, *cfg, results, stats);
+        else throw VmError(VM_EINVAL, "precision must be 0 or 1");
+    });
+}

@@ -61,7 +61,7 @@ class ServeConfig(C.Structure):
 
 
 class ServeResult(C.Structure):
-    _fields_ = [("latency_s", C.c_double), ("queue_s", C.c_double), ("iterations", C.c_int),
+    _fields_ = [("final_sites", C.c_void_p), ("latency_s", C.c_double), ("queue_s", C.c_double), ("iterations", C.c_int),
                 ("final_mean_loss", C.c_double)]
 
 
@@ -439,3 +439,44 @@ class Batch:
         k, cell, rings = C.c_int(), C.c_float(), C.c_int()
         _check(lib().vm_batch_choice(self._b, i, C.byref(k), C.byref(cell), C.byref(rings)))
         return dict(loss_k=k.value, grid_cell=cell.value, stage_rings=rings.value)
+
+
+def serve(requests, arrivals, hbm_budget_bytes=200e9, max_resident=0, max_iterations=300, tol=1e-6, poll_every=10,
+          precision=0, want_sites=False):
+    """Continuous-batching request loop (vm_serve).
+
+    requests: list of dicts with keys width, height, sites, targets, lr and optional fit_desc keywords.
+    Returns (per-request dicts, stats dict).
+    """
+    n = len(requests)
+    descs = (FitDesc * max(n, 1))()
+    keep = []
+    for i, r in enumerate(requests):
+        kw = {k: v for k, v in r.items() if k not in ("width", "height", "sites", "targets", "lr")}
+        d, k = fit_desc(r["width"], r["height"], r["sites"], r["targets"], r["lr"], **kw)
+        descs[i] = d
+        keep.append(k)
+    arr = np.ascontiguousarray(arrivals, np.float64)
+    assert len(arr) == n
+    results = (ServeResult * max(n, 1))()
+    bufs = []
+    if want_sites:
+        for i in range(n):
+            b = np.zeros((descs[i].num_sites, 2), np.float32)
+            bufs.append(b)
+            results[i].final_sites = b.ctypes.data
+    cfg = ServeConfig()
+    cfg.hbm_budget_bytes, cfg.max_resident, cfg.max_iterations = hbm_budget_bytes, max_resident, max_iterations
+    cfg.tol, cfg.poll_every, cfg.precision = tol, poll_every, precision
+    stats = ServeStats()
+    _check(lib().vm_serve(descs, _ptr(arr), n, C.byref(cfg), C.cast(results, C.c_void_p), C.byref(stats)))
+    out = []
+    for i in range(n):
+        r = results[i]
+        o = dict(latency_s=r.latency_s, queue_s=r.queue_s, iterations=r.iterations,
+                 final_mean_loss=r.final_mean_loss)
+        if want_sites:
+            o["sites"] = bufs[i]
+        out.append(o)
+    return out, dict(wall_s=stats.wall_s, site_iters=stats.site_iters, completed=stats.completed,
+                     max_resident_seen=stats.max_resident_seen, peak_bytes=stats.peak_bytes)

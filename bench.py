@@ -106,6 +106,33 @@ def run_plain(insts, K, iters, lr=1e-3):
     return site_iters / (total_ms / 1e3), total_ms
 
 
+def run_serving(seconds, rate, seed=4, lo=512, hi=131072, budget=200e9, max_iters=300, tol=1e-6, K=16):
+    """Poisson arrivals at `rate`/s for `seconds`; each request a fit of log-uniform size with 8
+    targets per site on a random polygon, run to |dL/L| < tol or max_iters; 200 GB budget."""
+    import vmhip
+    from vmhip import workloads as wl
+    arr = wl.poisson_arrivals(rate, seconds, seed)
+    n = len(arr)
+    sizes = wl.log_uniform_sizes(n, lo, hi, seed + 1)
+    radii = np.array([wl.poisson_radius_for(int(s)) for s in sizes], np.float32)
+    seeds = (4 * (1 << 20) + np.arange(n)).astype(np.uint32)
+    grids, _, _ = vmhip.poisson_batch(np.ones(n), np.ones(n), radii, seeds)
+    reqs = []
+    for i, g in enumerate(grids):
+        sites = wl.live_sites(g)
+        reqs.append(dict(width=1.0, height=1.0, sites=sites, targets=wl.polygon(8 * len(sites), sigma=5e-4,
+                                                                              seed=int(seeds[i])),
+                         lr=1e-3, loss_k=K))
+    res, st = vmhip.serve(reqs, arr, hbm_budget_bytes=budget, max_iterations=max_iters, tol=tol, poll_every=10)
+    lat = np.array([r["latency_s"] for r in res])
+    its = np.array([r["iterations"] for r in res])
+    return dict(requests=n, arrival_rate=rate, window_s=seconds, wall_s=st["wall_s"],
+                p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
+                requests_per_s=n / st["wall_s"], site_iters_per_s=st["site_iters"] / st["wall_s"],
+                mean_iterations=float(its.mean()), max_resident=st["max_resident_seen"],
+                peak_resident_GB=st["peak_bytes"] / 1e9, sites_total=int(sum(len(r["sites"]) for r in reqs)))
+
+
 def cpu_baseline(insts, K, iters=10, budget_sites=500_000, lr=1e-3):
     import oracle as O
     threads = O.num_threads()
@@ -140,6 +167,8 @@ def main():
     ap.add_argument("--plain-instances", type=int, default=0, help="0: all instances")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--serve-seconds", type=float, default=0.25, help="arrival window of the serving run (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is a design "
@@ -196,6 +225,13 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "lossKernel<16,float>",
                      "algorithmic_bytes": lb, "kernel_ms": loss_ms},
     }
+    if args.serve_seconds > 0:
+        log(f"serving: Poisson arrivals {args.serve_rate}/s for {args.serve_seconds} s ...")
+        sv = run_serving(args.serve_seconds, args.serve_rate)
+        sv["vs_plain_one_at_a_time"] = sv["site_iters_per_s"] / plain_rate
+        out["serving"] = sv
+        log(f"  {sv['requests']} requests, p50 {sv['p50_latency_s']:.3f} s, p99 {sv['p99_latency_s']:.3f} s, "
+            f"{sv['site_iters_per_s']:.3e} site-iters/s")
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(insts, args.k)

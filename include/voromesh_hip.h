@@ -201,10 +201,11 @@ typedef struct vm_serve_config {
 } vm_serve_config;
 
 typedef struct vm_serve_result {
-    double latency_s;     /* completion - arrival */
-    double queue_s;       /* admission - arrival */
-    int    iterations;
-    double final_mean_loss;
+    vm_vec2 *final_sites;     /* in: NULL or a buffer of num_sites entries for the fitted sites */
+    double   latency_s;       /* completion - arrival */
+    double   queue_s;         /* admission - arrival */
+    int      iterations;
+    double   final_mean_loss; /* mean loss of the last step */
 } vm_serve_result;
 
 typedef struct vm_serve_stats {
@@ -216,7 +217,12 @@ typedef struct vm_serve_stats {
 } vm_serve_stats;
 
 /* Serves requests[i] arriving at arrival_s[i] seconds after the call starts
- * (non-decreasing).  Returns when every request has completed. */
+ * (non-decreasing).  Continuous batching: a request is prepared once it has
+ * arrived, admitted in arrival order as soon as an instance slot and room
+ * within hbm_budget_bytes are free, advanced together with every other
+ * resident request by one set of launches per iteration, and retired when
+ * its relative loss change drops below tol or it reaches max_iterations.
+ * All requests share loss_k.  Returns when every request has completed. */
 VM_API int vm_serve(const vm_fit_desc *requests, const double *arrival_s, int n, const vm_serve_config *cfg,
                     vm_serve_result *results, vm_serve_stats *stats);
 
