@@ -361,6 +361,14 @@ int vm_batch_status(vm_batch *b, int id, int *iteration, int *done, double *last
     });
 }
 
+int vm_batch_loss_stats(vm_batch *b, uint64_t *counters)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(b && b->committed && counters, "invalid arguments");
+        b->eng->lossStats(counters);
+    });
+}
+
 int vm_batch_choice(vm_batch *b, int id, int *k, float *cell, int *rings)
 {
     return vmrt::guardedCall([&] {

@@ -182,6 +182,7 @@ public:
     DBuf<uint8_t>        dFrozen;
     DBuf<vmk::Inst>      dInst;
     DBuf<vmk::InstState> dState;
+    DBuf<vmk::LossPartial> dPartial;
     DBuf<vmk::SiteWork>  dSiteWork;
     DBuf<vmk::TargetWork> dTargetWork;
     hipGraphExec_t       graph = nullptr;
@@ -208,7 +209,8 @@ public:
         size_t b = 0;
         b += (dPos.n + dM.n + dV.n + dSpos.n + dTpos.n) * sizeof(V) + dGrad.n * sizeof(vmk::GradT<R>);
         b += (dCellOf.n + dRank.n + dSidx.n + dCounts.n + dStart.n + dTslot.n + dInstList.n + dCnt.n) * sizeof(int);
-        b += dFrozen.n + dInst.n * sizeof(vmk::Inst) + dState.n * sizeof(vmk::InstState);
+        b += dFrozen.n + dInst.n * sizeof(vmk::Inst) + dState.n * sizeof(vmk::InstState) +
+             dPartial.n * sizeof(vmk::LossPartial);
         b += dSiteWork.n * sizeof(vmk::SiteWork) + dTargetWork.n * sizeof(vmk::TargetWork);
         return b;
     }
@@ -253,7 +255,7 @@ public:
         dFrozen.reserve(L.sites);
         dCounts.reserve(L.cells), dStart.reserve(L.cells);
         dTpos.reserve(L.targets), dTslot.reserve(L.targets);
-        dInst.reserve(n), dState.reserve(n), dInstList.reserve(n);
+        dInst.reserve(n), dState.reserve(n), dInstList.reserve(n), dPartial.reserve(size_t(n) * vmk::kLossPartials);
         dSiteWork.reserve(L.siteWork.size()), dTargetWork.reserve(L.targetWork.size());
         dCnt.reserve(2 + L.groups.size());
 
@@ -289,6 +291,7 @@ public:
         up(dInst.p, L.inst), up(dState.p, hs), up(dInstList.p, L.instList), up(dSiteWork.p, L.siteWork);
         up(dTargetWork.p, L.targetWork), up(dCnt.p, cnt);
         VM_HIP(hipMemsetAsync(dGrad.p, 0, dGrad.n * sizeof(vmk::GradT<R>), stream));
+        VM_HIP(hipMemsetAsync(dPartial.p, 0, dPartial.n * sizeof(vmk::LossPartial), stream));
         VM_HIP(vmk::launchFillInt(stream, dCounts.p, int64_t(dCounts.n), 0));
         gridScan = std::max(1, std::min(n, kMaxScanGrid));
         gridSite = std::max(1, std::min(int(L.siteWork.size()), kMaxSiteGrid));
@@ -306,7 +309,7 @@ public:
         vmk::LossArgs<R> a{};
         a.insts = dInst.p, a.state = dState.p, a.work = dTargetWork.p + L.groups[g].begin, a.nWork = dCnt.p + 2 + g;
         a.pos = dPos.p, a.spos = dSpos.p, a.sidx = dSidx.p, a.start = dStart.p, a.tpos = dTpos.p, a.tslot = dTslot.p;
-        a.grad = wantGrad ? dGrad.p : nullptr, a.mode = mode;
+        a.grad = wantGrad ? dGrad.p : nullptr, a.partial = dPartial.p, a.mode = mode;
         return a;
     }
 
@@ -314,7 +317,7 @@ public:
     {
         vmk::AdamArgs<R> a{};
         a.insts = dInst.p, a.state = dState.p, a.work = dSiteWork.p, a.nWork = dCnt.p + 1;
-        a.pos = dPos.p, a.m = dM.p, a.v = dV.p, a.grad = dGrad.p, a.frozen = dFrozen.p;
+        a.pos = dPos.p, a.m = dM.p, a.v = dV.p, a.grad = dGrad.p, a.frozen = dFrozen.p, a.partial = dPartial.p;
         a.cellOf = dCellOf.p, a.rank = dRank.p, a.counts = dCounts.p, a.doAdam = doAdam;
         return a;
     }
@@ -323,7 +326,7 @@ public:
 
     void gridBuild(int mode)
     {
-        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dInstList.p, dCnt.p, dCounts.p, dStart.p, mode));
+        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dInstList.p, dCnt.p, dCounts.p, dStart.p, mode));
         VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p, dCellOf.p,
                                      dRank.p, dStart.p, dSpos.p, dSidx.p, mode));
     }
@@ -331,7 +334,7 @@ public:
     void enqueueIteration(hipEvent_t *marks)
     {
         if (marks) VM_HIP(hipEventRecord(marks[0], stream));
-        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
+        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
                                vmk::kModeIterate));
         if (marks) VM_HIP(hipEventRecord(marks[1], stream));
         VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p, dCellOf.p,
@@ -497,6 +500,25 @@ public:
                 out.terms[o] = ht[t];
             }
         }
+    }
+
+    void lossStats(uint64_t out[6]) override
+    {
+        if (dirty) commit();
+        DBuf<unsigned long long> dS;
+        dS.reserve(6);
+        VM_HIP(hipMemsetAsync(dS.p, 0, 6 * sizeof(unsigned long long), stream));
+        gridBuild(vmk::kModeEvaluate);
+        for (size_t g = 0; g < L.groups.size(); ++g) {
+            auto a  = lossArgs(int(g), vmk::kModeEvaluate, false);
+            a.stats = dS.p;
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, a));
+        }
+        bin();
+        unsigned long long h[6];
+        VM_HIP(hipMemcpyAsync(h, dS.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+        VM_HIP(hipStreamSynchronize(stream));
+        for (int k = 0; k < 6; ++k) out[k] = h[k];
     }
 
     void knnQuery(int id, const double *q, int nq, int K, int *idx, double *d2) override

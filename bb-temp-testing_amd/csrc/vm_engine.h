@@ -115,6 +115,8 @@ public:
     virtual void  knnQuery(int id, const double *q, int nq, int K, int *idx, double *d2) = 0;
     virtual void  clip(int id, const uint64_t *dMasks, int tilesX, int tilesY, float voxel, ClipOut &out) = 0;
     virtual size_t deviceBytes() const = 0;
+    // One evaluation-mode loss pass with the loss kernel's path counters.
+    virtual void lossStats(uint64_t out[6]) = 0;
 
     bool        dirty = true;
     bool        deviceNewer = false;

@@ -40,19 +40,24 @@ template <class R> struct LossArgs {
     const V2<R>      *tpos;   // targets sorted by cell
     const int        *tslot;  // their slot numbers (subsampling hash index)
     GradT<R>         *grad;   // accumulated gradient [2 per site], null: loss only
+    LossPartial      *partial; // [instances * kLossPartials]: step loss sums (iterate mode)
     int               mode;   // kModeIterate / kModeEvaluate
     int              *dbgKnn; // optional [T*K] k-NN lists (instance-local indices)
     TermDbg          *dbgTerm;
+    unsigned long long *stats; // optional path counters: fast, ring (staged), ring (global), idle,
+                               // candidates, staged sites
+
 };
 
 template <class R> struct AdamArgs {
     const Inst          *insts;
-    const InstState     *state;
+    InstState           *state;
     const SiteWork      *work;
     const int           *nWork;
     V2<R>               *pos, *m, *v;
     GradT<R>            *grad;
     const unsigned char *frozen;
+    const LossPartial   *partial; // folded into InstState (count, loss) before the update
     int                 *cellOf, *rank, *counts;
     int                  doAdam; // 0: only bin the current positions
 };
@@ -72,8 +77,8 @@ template <class R> struct ClipArgs {
     int            *flags;   // {unsafe, overflow}
 };
 
-hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, const int *instList,
-                      const int *nInst, int *counts, int *start, int mode);
+hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, LossPartial *partial,
+                      const int *instList, const int *nInst, int *counts, int *start, int mode);
 template <class R>
 hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstState *state, const SiteWork *work,
                          const int *nWork, const void *pos, const int *cellOf, const int *rank, const int *start,

@@ -166,20 +166,25 @@ template <int K, class KeyT> __device__ __forceinline__ void topkInsert(KeyT (&k
 }
 
 // Per-thread ring search over the global sorted grid of one instance (used by
-// the clipping and the diagnostic k-NN kernels).  Same stop rule as the loss
-// kernel and the oracle: conservative, so the list equals brute force.
-template <int K, class R>
-__device__ void knnGlobal(const Inst &in, const V2<R> *spos, const int *sidx, const int *start, V2<R> q,
-                          Key<R> (&keys)[K])
+// the clipping and the diagnostic k-NN kernels).  Sites are stored in R and
+// were binned in R arithmetic; distances and keys are in KR (the clipping
+// orders its neighbours by FP64 distances).  Same stop rule as the loss kernel
+// and the oracle: conservative for R-precision binning, so the list equals
+// brute force.
+template <int K, class R, class KR = R>
+__device__ void knnGlobal(const Inst &in, const V2<R> *spos, const int *sidx, const int *start, V2<R> qr,
+                          Key<KR> (&keys)[K])
 {
 #pragma unroll
-    for (int k = 0; k < K; ++k) keys[k] = Key<R>::none();
-    const R   cell   = R(in.cell);
-    const R   inv    = R(1) / cell;
-    const R   rel    = ringRel<R>();
-    const R   extent = fmax(R(in.ext), fmax(fabs(q.x), fabs(q.y)));
-    const R   margin = rel * extent;
-    const int cx = cellCoord(q.x, inv, in.gw), cy = cellCoord(q.y, inv, in.gh);
+    for (int k = 0; k < K; ++k) keys[k] = Key<KR>::none();
+    const R      cellR  = R(in.cell);
+    const R      inv    = R(1) / cellR;
+    const KR     cell   = KR(cellR);
+    const KR     rel    = KR(ringRel<R>());
+    const V2<KR> q{KR(qr.x), KR(qr.y)};
+    const KR     extent = fmax(KR(in.ext), fmax(fabs(q.x), fabs(q.y)));
+    const KR     margin = rel * extent;
+    const int    cx = cellCoord(qr.x, inv, in.gw), cy = cellCoord(qr.y, inv, in.gh);
     for (int ring = 0;; ++ring) {
         int x0 = cx - ring, x1 = cx + ring, y0 = cy - ring, y1 = cy + ring;
         for (int y = max(y0, 0); y <= min(y1, in.gh - 1); ++y) {
@@ -188,15 +193,15 @@ __device__ void knnGlobal(const Inst &in, const V2<R> *spos, const int *sidx, co
                 if (x < 0 || x >= in.gw) continue;
                 int c = y * in.gw + x;
                 for (int s = start[c]; s < start[c + 1]; ++s) {
-                    V2<R> p = spos[s];
-                    topkInsert<K>(keys, Key<R>::make(dist2(p, q), (uint32_t(sidx[s]) << kLdsSlotBits) | kLdsNone));
+                    V2<KR> p{KR(spos[s].x), KR(spos[s].y)};
+                    topkInsert<K>(keys, Key<KR>::make(dist2(p, q), (uint32_t(sidx[s]) << kLdsSlotBits) | kLdsNone));
                 }
             }
         }
         if (x0 <= 0 && y0 <= 0 && x1 >= in.gw - 1 && y1 >= in.gh - 1) break;
-        R safe = fmin(fmin(q.x - R(x0) * cell, R(x1 + 1) * cell - q.x), fmin(q.y - R(y0) * cell, R(y1 + 1) * cell - q.y));
-        safe   = fmax(safe - margin, R(0));
-        if (keys[K - 1].valid() && keys[K - 1].d2() * (R(1) + rel) < safe * safe) break;
+        KR safe = fmin(fmin(q.x - KR(x0) * cell, KR(x1 + 1) * cell - q.x), fmin(q.y - KR(y0) * cell, KR(y1 + 1) * cell - q.y));
+        safe    = fmax(safe - margin, KR(0));
+        if (keys[K - 1].valid() && keys[K - 1].d2() * (KR(1) + rel) < safe * safe) break;
     }
 }
 
@@ -206,8 +211,8 @@ __device__ void knnGlobal(const Inst &in, const V2<R> *spos, const int *sidx, co
 constexpr int kScanThreads = 1024;
 
 __global__ __launch_bounds__(kScanThreads) void scanKernel(const Inst *__restrict__ insts, InstState *state,
-                                                          const int *__restrict__ instList, const int *nInstPtr,
-                                                          int *counts, int *start, int mode)
+                                                          LossPartial *partial, const int *__restrict__ instList,
+                                                          const int *nInstPtr, int *counts, int *start, int mode)
 {
     __shared__ int waveSum[kScanThreads / 64];
     __shared__ int skip;
@@ -245,6 +250,10 @@ __global__ __launch_bounds__(kScanThreads) void scanKernel(const Inst *__restric
         if (skip) {
             __syncthreads();
             continue;
+        }
+        if (mode == kModeIterate && threadIdx.x < kLossPartials) {
+            partial[int64_t(id) * kLossPartials + threadIdx.x].loss  = 0.0;
+            partial[int64_t(id) * kLossPartials + threadIdx.x].count = 0;
         }
         const int n   = in.gw * in.gh + 1;
         int      *cnt = counts + in.cellOff;
@@ -510,6 +519,7 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                 take = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(iter), in.seed) < in.fraction;
         }
         if (__ballot(take) == 0) {
+            if (a.stats && lane == 0) atomicAdd(&a.stats[3], 1ull);
             waveSync();
             continue;
         }
@@ -590,6 +600,11 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                     }
                 }
             }
+        }
+        if (a.stats && lane == 0) {
+            atomicAdd(&a.stats[fast ? 0 : (staged ? 1 : 2)], 1ull);
+            atomicAdd(&a.stats[4], (unsigned long long)m);
+            atomicAdd(&a.stats[5], (unsigned long long)total);
         }
         if (!fast) {
             // ---- fallback: ring search (the ring geometry is wave-uniform) ---
@@ -761,8 +776,9 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         const int    uc = __popcll(__ballot(used != 0));
         if (lane == 0 && uc > 0) {
             if (a.mode == kModeIterate) {
-                unsafeAtomicAdd(&st->loss, ls);
-                atomicAdd((unsigned long long *)&st->count, (unsigned long long)uc);
+                LossPartial *pp = a.partial + int64_t(wk.inst) * kLossPartials + (blockIdx.x & (kLossPartials - 1));
+                unsafeAtomicAdd(&pp->loss, ls);
+                atomicAdd(&pp->count, (unsigned long long)uc);
             } else {
                 unsafeAtomicAdd(&st->evalLoss, ls);
                 atomicAdd((unsigned long long *)&st->evalCount, (unsigned long long)uc);
@@ -793,14 +809,31 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
 template <class R>
 __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
 {
+    __shared__ long long sCount;
     const int nWork = *a.nWork;
     for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
-        const SiteWork   wk = a.work[w];
-        const Inst      &in = a.insts[wk.inst];
-        const InstState *st = a.state + wk.inst;
+        const SiteWork wk = a.work[w];
+        const Inst    &in = a.insts[wk.inst];
+        InstState     *st = a.state + wk.inst;
         if (a.doAdam && st->done) continue;
+        if (a.doAdam) {
+            // Fold the step's loss partials: every chunk needs the target count,
+            // the instance's first chunk also publishes the totals.
+            if (threadIdx.x < 64) {
+                const LossPartial &pp = a.partial[int64_t(wk.inst) * kLossPartials + threadIdx.x];
+                const uint32_t     c  = waveReduce32<uint32_t>(uint32_t(pp.count), 0u,
+                                                               [](uint32_t x, uint32_t y) { return x + y; });
+                const double       l  = waveSumDouble(pp.loss);
+                if (threadIdx.x == 0) {
+                    sCount = (long long)c;
+                    if (wk.begin == 0) st->count = (long long)c, st->loss = l;
+                }
+            }
+            __syncthreads();
+        }
+        const long long cnt = a.doAdam ? sCount : 0;
         int i = wk.begin + int(threadIdx.x);
-        if (i >= wk.begin + wk.count) continue;
+        if (i < wk.begin + wk.count) {
         const int64_t g = in.siteOff + i;
         V2<R>         p = a.pos[g];
         if (a.doAdam) {
@@ -818,7 +851,6 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
                 const R b1 = R(in.beta1), b2 = R(in.beta2), eps = R(in.eps);
                 const R bc1 = R(1.0 - powi(double(b1), t));
                 const R bc2 = R(1.0 - powi(double(b2), t));
-                const int64_t cnt = st->count;
                 const R inv = R(1) / R(cnt > 1 ? cnt : 1);
                 const R ob1 = R(1) - b1, ob2 = R(1) - b2;
                 V2<R>   gg{inv * gr.x, inv * gr.y};
@@ -838,6 +870,8 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
         const int c   = cellCoord(p.y, inv, in.gh) * in.gw + cellCoord(p.x, inv, in.gw);
         a.cellOf[g]   = c;
         a.rank[g]     = atomicAdd(&a.counts[in.cellOff + c], 1);
+        }
+        __syncthreads();
     }
 }
 
@@ -934,8 +968,8 @@ __global__ __launch_bounds__(64) void clipKernel(ClipArgs<R> a)
     if (i >= in.S) return;
     const V2<R> *pos = a.pos + in.siteOff;
     V2<R>        ci  = pos[i];
-    Key<R>       keys[kCellK];
-    knnGlobal<kCellK>(in, a.spos + in.siteOff, a.sidx + in.siteOff, a.start + in.cellOff, ci, keys);
+    Key<double>  keys[kCellK]; // neighbours ordered by FP64 distance: the whole clip is FP64
+    knnGlobal<kCellK, R, double>(in, a.spos + in.siteOff, a.sidx + in.siteOff, a.start + in.cellOff, ci, keys);
 
     CellPoly poly;
     double   x0 = -double(ci.x), y0 = -double(ci.y), x1 = double(in.W) - double(ci.x), y1 = double(in.H) - double(ci.y);
@@ -1061,11 +1095,11 @@ __global__ void fillIntKernel(int *p, int64_t n, int v)
 // ===========================================================================
 // Host-side launchers
 // ===========================================================================
-hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, const int *instList,
-                      const int *nInst, int *counts, int *start, int mode)
+hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, LossPartial *partial,
+                      const int *instList, const int *nInst, int *counts, int *start, int mode)
 {
-    hipLaunchKernelGGL(scanKernel, dim3(grid), dim3(kScanThreads), 0, s, insts, state, instList, nInst, counts, start,
-                       mode);
+    hipLaunchKernelGGL(scanKernel, dim3(grid), dim3(kScanThreads), 0, s, insts, state, partial, instList, nInst,
+                       counts, start, mode);
     return hipGetLastError();
 }
 

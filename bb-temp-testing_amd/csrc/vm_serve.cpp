@@ -121,6 +121,7 @@ public:
         dTpos.reserve(capTargets_), dTslot.reserve(capTargets_);
         dCounts.reserve(capCells_), dStart.reserve(capCells_);
         dInst.reserve(slots_), dState.reserve(slots_), dInstList.reserve(slots_);
+        dPartial.reserve(size_t(slots_) * vmk::kLossPartials);
         dSiteWork.reserve(capSites_ / vmk::kSiteChunk + slots_), dNewWork.reserve(capSites_ / vmk::kSiteChunk + slots_);
         dCnt.reserve(4), dNewCnt.reserve(1);
         twCap_ = 0;
@@ -129,6 +130,7 @@ public:
         slotReq_.assign(slots_, -1);
         slotInfo_.resize(slots_);
         VM_HIP(hipMemsetAsync(dState.p, 0, slots_ * sizeof(vmk::InstState), stream_));
+        VM_HIP(hipMemsetAsync(dPartial.p, 0, size_t(slots_) * vmk::kLossPartials * sizeof(vmk::LossPartial), stream_));
         VM_HIP(hipStreamSynchronize(stream_));
     }
     ~Server()
@@ -308,7 +310,7 @@ private:
     {
         vmk::AdamArgs<R> a{};
         a.insts = dInst.p, a.state = dState.p, a.work = dSiteWork.p, a.nWork = dCnt.p + 1;
-        a.pos = dPos.p, a.m = dM.p, a.v = dV.p, a.grad = dGrad.p, a.frozen = dFrozen.p;
+        a.pos = dPos.p, a.m = dM.p, a.v = dV.p, a.grad = dGrad.p, a.frozen = dFrozen.p, a.partial = dPartial.p;
         a.cellOf = dCellOf.p, a.rank = dRank.p, a.counts = dCounts.p, a.doAdam = doAdam;
         return a;
     }
@@ -319,14 +321,14 @@ private:
         hipGraph_t g;
         VM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         try {
-            VM_HIP(vmk::launchScan(gScan, stream_, dInst.p, dState.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
+            VM_HIP(vmk::launchScan(gScan, stream_, dInst.p, dState.p, dPartial.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
                                    vmk::kModeIterate));
             VM_HIP(vmk::launchScatter<R>(gSite, stream_, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
                                          dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate));
             vmk::LossArgs<R> la{};
             la.insts = dInst.p, la.state = dState.p, la.work = dTargetWork.p, la.nWork = dCnt.p + 2;
             la.pos = dPos.p, la.spos = dSpos.p, la.sidx = dSidx.p, la.start = dStart.p, la.tpos = dTpos.p;
-            la.tslot = dTslot.p, la.grad = dGrad.p, la.mode = vmk::kModeIterate;
+            la.tslot = dTslot.p, la.grad = dGrad.p, la.partial = dPartial.p, la.mode = vmk::kModeIterate;
             VM_HIP(vmk::launchLoss<R>(K, gLoss, stream_, la));
             VM_HIP(vmk::launchAdamBin<R>(gSite, stream_, adamArgs(1)));
         } catch (...) {
@@ -360,6 +362,7 @@ private:
     DBuf<uint8_t>        dFrozen;
     DBuf<vmk::Inst>      dInst;
     DBuf<vmk::InstState> dState;
+    DBuf<vmk::LossPartial> dPartial;
     DBuf<vmk::SiteWork>  dSiteWork, dNewWork;
     DBuf<vmk::TargetWork> dTargetWork;
 };

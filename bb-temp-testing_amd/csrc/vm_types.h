@@ -58,6 +58,17 @@ struct InstState {
     int64_t evalCount;
 };
 
+// Per-instance loss accumulators of one step, spread over kLossPartials cache
+// lines so the loss kernel's wavefronts of one large instance do not all hit
+// one address.  The Adam kernel folds them into InstState (it needs the
+// count); the next scan clears them.
+constexpr int kLossPartials = 64;
+struct LossPartial {
+    double             loss;
+    unsigned long long count;
+    char               pad[112];
+};
+
 struct SiteWork {
     int inst, begin, count, pad;
 };

@@ -79,7 +79,8 @@ EXPORTS = [
     "vm_sampler_create", "vm_sampler_destroy", "vm_sampler_resample", "vm_sampler_verify", "vm_sampler_dims",
     "vm_sampler_cells", "vm_poisson_grid_dims", "vm_poisson_batch",
     "vm_batch_create", "vm_batch_destroy", "vm_batch_add", "vm_batch_commit", "vm_batch_step",
-    "vm_batch_get_sites", "vm_batch_status", "vm_batch_choice", "vm_batch_profile", "vm_serve",
+    "vm_batch_get_sites", "vm_batch_status", "vm_batch_choice", "vm_batch_profile", "vm_batch_loss_stats",
+    "vm_serve",
 ]
 
 _lib = None
@@ -133,6 +134,7 @@ def lib():
         "vm_batch_status": [vp, i32, P(i32), P(i32), P(f64)],
         "vm_batch_choice": [vp, i32, P(i32), P(f32), P(i32)],
         "vm_batch_profile": [vp, i32, vp],
+        "vm_batch_loss_stats": [vp, vp],
         "vm_serve": [P(FitDesc), vp, i32, P(ServeConfig), vp, P(ServeStats)],
     }
     for name, args in sig.items():
@@ -434,6 +436,12 @@ class Batch:
         it, done, loss = C.c_int(), C.c_int(), C.c_double()
         _check(lib().vm_batch_status(self._b, i, C.byref(it), C.byref(done), C.byref(loss)))
         return dict(iteration=it.value, done=bool(done.value), last_mean_loss=loss.value)
+
+    def loss_stats(self):
+        c = np.zeros(6, np.uint64)
+        _check(lib().vm_batch_loss_stats(self._b, _ptr(c)))
+        keys = ["fast_waves", "ring_staged_waves", "ring_global_waves", "idle_waves", "candidates", "staged_sites"]
+        return {k: int(v) for k, v in zip(keys, c)}
 
     def choice(self, i):
         k, cell, rings = C.c_int(), C.c_float(), C.c_int()

@@ -83,6 +83,7 @@ def run_batched(insts, K, steps, warmup, lr=1e-3):
         b.step(warmup)
     ms = b.step(steps)
     phase = b.profile(5)
+    b.paths = b.loss_stats()
     cells = 0
     for i in range(len(insts)):
         ch = b.choice(i)
@@ -133,6 +134,118 @@ def run_serving(seconds, rate, seed=4, lo=512, hi=131072, budget=200e9, max_iter
                 peak_resident_GB=st["peak_bytes"] / 1e9, sites_total=int(sum(len(r["sites"]) for r in reqs)))
 
 
+def run_large(iters=500, K=16, n_sites=262144, n_targets=4194304):
+    """One large instance: Poisson sites (~262k) and 4.2M targets on 64 ellipses (sigma 2e-4)."""
+    import vmhip
+    from vmhip import workloads as wl
+    r = wl.poisson_radius_for(n_sites)
+    grids, _, _ = vmhip.poisson_batch([1.0], [1.0], [r], [1])
+    sites = wl.live_sites(grids[0])
+    tg = wl.ellipses(n_targets, count=64, sigma=2e-4, seed=1)
+    b = vmhip.Batch()
+    b.add(1.0, 1.0, sites, tg, 1e-3, loss_k=K)
+    b.commit()
+    stats0 = b.loss_stats()
+    b.step(2)
+    ms = b.step(iters)
+    phase = b.profile(10)
+    stats1 = b.loss_stats()
+    ch = b.choice(0)
+    g = math.ceil(1.0 / ch["grid_cell"])
+    S, T, C = len(sites), len(tg), g * g + 1
+    lb = loss_bytes(S, T, C)
+    ib = iter_bytes(S, T, C)
+    return dict(sites=S, targets=T, iterations=iters, ms_per_iter=ms / iters,
+                site_iters_per_s=S * iters / (ms / 1e3), phase_ms=[float(x) for x in phase],
+                loss_kernel_GBs=lb / (phase[2] / 1e3) / 1e9, loss_kernel_frac=lb / (phase[2] / 1e3) / 1e9 / HBM_PEAK_GBS,
+                iteration_GBs=ib / (ms / iters / 1e3) / 1e9,
+                iteration_frac=ib / (ms / iters / 1e3) / 1e9 / HBM_PEAK_GBS, grid_cell=ch["grid_cell"],
+                loss_paths_start=stats0, loss_paths_end=stats1,
+                final_mean_loss=b.status(0)["last_mean_loss"])
+
+
+def run_precision(iters=5, K=16):
+    """Jittered 256x256 lattice (+-1e-7) and 524,288 circle targets: FP32 (FP64 clip and loss sum)
+    and all-FP64 GPU runs against the FP64 CPU oracle; clipped-cell topology mismatches."""
+    import oracle as O
+    import vmhip
+    from vmhip import workloads as wl
+    sites = wl.lattice(256, 1e-7, seed=3)
+    tg = wl.circle(524288, radius=0.3, sigma=1e-4, seed=3)
+    out = {}
+    runs = {}
+    for prec in (0, 1):
+        opt = vmhip.Optimizer(1.0, 1.0, 1 / 256, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 * 256,
+                              iterations=iters, precision=prec)
+        opt.set_targets(tg)
+        opt.set_generators(sites, 256, 256, 1 / 256)
+        grad = opt.loss_pass(len(tg), want_terms=False)["grad"]
+        mesh0 = opt.extract()
+        opt.step(iters)
+        runs[prec] = (grad, mesh0, opt.generators().astype(np.float64))
+    s64 = sites.astype(np.float64)
+    r64 = O.loss_grad(s64, tg.astype(np.float64), K, 1.0, 1.0, 1 / 128, real=1)
+    lr = float(np.float32(np.float32(1e-3 * 256) * np.float32(1 / 256)))
+    sc = O.make_sched(lr, 1.0, 1.0, milestones=O.scale_milestones((80, 150, 200, 250), iters, iters))
+    box = (-2 / 256, -2 / 256, 1 + 2 / 256, 1 + 2 / 256)
+    p64, _, _, _, _ = O.fit(s64, tg.astype(np.float64), K, 1 / 128, sc, iters, box=box, real=1)
+    ref_clip = O.clip(s64, 1.0, 1.0, 1 / 128, real=1)
+    gnorm = np.linalg.norm(r64["grad"])
+    for prec, name in ((0, "fp32"), (1, "fp64")):
+        grad, mesh0, pos = runs[prec]
+        ge = np.abs(grad - r64["grad"])
+        pe = np.abs(pos - p64)
+        cnt = mesh0["cell_start"][1:] - mesh0["cell_start"][:-1]
+        mism = 0
+        for i in range(len(sites)):
+            a = mesh0["cell_labels"][mesh0["cell_start"][i]:mesh0["cell_start"][i + 1]]
+            b = ref_clip["labels"][i, :ref_clip["count"][i]]
+            mism += int(cnt[i] != ref_clip["count"][i] or not np.array_equal(a, b))
+        out[name] = dict(grad_rel_norm_err=float(np.linalg.norm(grad - r64["grad"]) / gnorm),
+                         grad_max_abs_err=float(ge.max()), grad_mean_abs_err=float(ge.mean()),
+                         pos_max_abs_err=float(pe.max()), pos_mean_abs_err=float(pe.mean()),
+                         topology_mismatches=mism, unsafe_cells=mesh0["unsafe_cells"],
+                         overflow_cells=mesh0["overflow_cells"])
+    out.update(sites=len(sites), targets=len(tg), iterations=iters)
+    return out
+
+
+def run_small(iters=200, K=16):
+    """Plain single fit: Poisson r=0.025 seed 0, 8,192 star targets (sigma 1e-3), 200 iterations, clip;
+    also through the CPU oracle in full."""
+    import oracle as O
+    import vmhip
+    from vmhip import workloads as wl
+    r = 0.025
+    grid = O.poisson(1.0, 1.0, r, 30, 0)
+    gw, gh, cs = O.poisson_dims(1.0, 1.0, r)
+    tg = wl.star(8192, sigma=1e-3, seed=0)
+    sites = wl.live_sites(grid)
+    opt = vmhip.Optimizer(1.0, 1.0, r, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 / r, iterations=iters,
+                          grid_cell=2.0 / math.sqrt(len(sites)))
+    opt.set_targets(tg)
+    opt.set_generators(grid, gw, gh, cs)
+    t0 = time.perf_counter()
+    gpu_ms = opt.optimize()
+    mesh = opt.extract()
+    wall = time.perf_counter() - t0
+    lr = float(np.float32(np.float32(1e-3 / r) * np.float32(r)))
+    sc = O.make_sched(lr, 1.0, 1.0)
+    m = np.float32(2) * np.float32(r)
+    box = (float(-m), float(-m), float(np.float32(gw) * np.float32(cs) + m), float(np.float32(gh) * np.float32(cs) + m))
+    t0 = time.perf_counter()
+    ref, _, _, _, _ = O.fit(sites, tg, K, 2.0 / math.sqrt(len(sites)), sc, iters, box=box)
+    O.clip(ref, 1.0, 1.0, 2.0 / math.sqrt(len(sites)))
+    cpu_s = time.perf_counter() - t0
+    gpu_pos = opt.generators()[grid[:, 0] >= 0]
+    return dict(sites=len(sites), targets=len(tg), iterations=iters, gpu_ms=gpu_ms,
+                gpu_site_iters_per_s=len(sites) * iters / (gpu_ms / 1e3), extract_gpu_ms=mesh["gpu_ms"],
+                extract_host_ms=mesh["host_ms"], wall_s=wall, cpu_s=cpu_s, cpu_threads=O.num_threads(),
+                cpu_site_iters_per_s=len(sites) * iters / cpu_s, positions_bitexact_vs_oracle=bool(
+                    np.array_equal(gpu_pos, ref)), watertight=mesh["watertight"],
+                final_mean_loss=opt.evaluate_loss())
+
+
 def cpu_baseline(insts, K, iters=10, budget_sites=500_000, lr=1e-3):
     import oracle as O
     threads = O.num_threads()
@@ -169,6 +282,7 @@ def main():
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--serve-seconds", type=float, default=0.25, help="arrival window of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the large/small/precision sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is a design "
@@ -217,6 +331,7 @@ def main():
                    "instances": len(insts), "sites": S, "targets": T, "hash_cells": C, "k": args.k},
         "plain_site_iters_per_s": plain_rate,
         "batched_over_plain": value / plain_rate,
+        "loss_paths": b.paths,
         "phase_ms": {"scan": float(phase[0]), "scatter": float(phase[1]), "loss": float(phase[2]),
                      "adam_bin": float(phase[3])},
         "iteration_roofline": {"bytes": iter_bytes(S, T, C),
@@ -225,6 +340,18 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "lossKernel<16,float>",
                      "algorithmic_bytes": lb, "kernel_ms": loss_ms},
     }
+    if not args.no_extra:
+        log("config 1: large single instance ...")
+        out["large_instance"] = run_large()
+        li = out["large_instance"]
+        log(f"  {li['ms_per_iter']:.3f} ms/iter, {li['site_iters_per_s']:.3e} site-iters/s, loss kernel "
+            f"{li['loss_kernel_GBs']:.1f} GB/s")
+        log("config 0: small plain fit (GPU and CPU oracle) ...")
+        out["small"] = run_small()
+        log(f"  {out['small']}")
+        log("config 3: precision stress ...")
+        out["precision"] = run_precision()
+        log(f"  {out['precision']}")
     if args.serve_seconds > 0:
         log(f"serving: Poisson arrivals {args.serve_rate}/s for {args.serve_seconds} s ...")
         sv = run_serving(args.serve_seconds, args.serve_rate)
@@ -236,7 +363,7 @@ def main():
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(insts, args.k)
         log(f"  cpu {out['cpu_baseline']['value']:.3e} site-iters/s")
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
 
 
 if __name__ == "__main__":

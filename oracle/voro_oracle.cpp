@@ -596,9 +596,13 @@ static int clipImpl(const void *sitesV, int S, double W, double H, double cell, 
                     int tilesX, int tilesY, float voxel, float *outVerts, int32_t *outLabels, int32_t *outCount,
                     uint8_t *outInside, int32_t *flags)
 {
-    auto   *sites = static_cast<const V2<R> *>(sitesV);
-    int     gw = std::max(1, int(std::ceil(W / cell))), gh = std::max(1, int(std::ceil(H / cell)));
-    Grid<R> g = buildGrid(sites, S, gw, gh, R(cell), R(std::max(W, H)));
+    auto *sites = static_cast<const V2<R> *>(sitesV);
+    // Neighbours are ordered by FP64 distance in both modes (the clip is FP64
+    // throughout), so FP32 and FP64 sites with equal values clip identically.
+    std::vector<V2<double>> sd(S);
+    for (int i = 0; i < S; ++i) sd[i] = V2<double>{double(sites[i].x), double(sites[i].y)};
+    int          gw = std::max(1, int(std::ceil(W / cell))), gh = std::max(1, int(std::ceil(H / cell)));
+    Grid<double> g  = buildGrid(sd.data(), S, gw, gh, cell, std::max(W, H));
     float   invVoxel = voxel > 0 ? 1.0f / voxel : 0.0f;
     int     unsafeN = 0, overflowN = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : unsafeN, overflowN)
@@ -609,8 +613,8 @@ static int clipImpl(const void *sitesV, int S, double W, double H, double cell, 
             outInside[i] = 0;
             continue;
         }
-        TopK<R> top(Kc);
-        knnGrid(g, ci, top);
+        TopK<double> top(Kc);
+        knnGrid(g, sd[i], top);
         CellPoly poly;
         double   x0 = -double(ci.x), y0 = -double(ci.y), x1 = W - double(ci.x), y1 = H - double(ci.y);
         poly.vx[0] = x0, poly.vy[0] = y0, poly.e[0] = -1;

@@ -304,3 +304,25 @@ def test_batch_instances_match_plain():
         st = b.status(i)
         assert st["iteration"] == iters
         assert st["last_mean_loss"] == pytest.approx(lh[-1] / ch[-1], rel=1e-12)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_clip_lattice_topology_zero_mismatches(precision):
+    """Precision stress: jittered 128x128 lattice; clipped cells (labels and vertices) equal the FP64
+    oracle's in both the FP32 and the all-FP64 mode."""
+    sites = wl.lattice(128, 1e-7, seed=3)
+    opt = _optimizer(1 / 128, 16, precision=precision)
+    opt.set_targets(wl.circle(4096, radius=0.3, sigma=1e-4, seed=3))
+    opt.set_generators(sites, 128, 128, 1 / 128)
+    mesh = opt.extract()
+    ref = O.clip(sites.astype(np.float64), W, H, 1 / 64, real=1)
+    cnt = mesh["cell_start"][1:] - mesh["cell_start"][:-1]
+    assert np.array_equal(cnt, ref["count"])
+    mism = 0
+    for i in range(len(sites)):
+        b = mesh["cell_start"][i]
+        n = cnt[i]
+        mism += not (np.array_equal(mesh["cell_labels"][b:b + n], ref["labels"][i, :n]) and
+                     np.array_equal(mesh["cell_verts"][b:b + n], ref["verts"][i, :n]))
+    assert mism == 0
+    assert mesh["overflow_cells"] == 0

@@ -30,7 +30,8 @@ def _requests(n, seed=0):
 
 def test_serve_result_same_alone_and_in_mix():
     reqs = _requests(12)
-    arrivals = np.linspace(0.0, 0.05, len(reqs))
+    # half arrive together, the rest while the first ones are still running
+    arrivals = np.concatenate([np.zeros(6), np.linspace(0.001, 0.003, 6)])
     mix, st = vmhip.serve(reqs, arrivals, max_iterations=60, tol=0.0, poll_every=5, want_sites=True)
     assert st["completed"] == len(reqs)
     assert st["max_resident_seen"] > 1  # requests overlapped on the device

@@ -307,3 +307,17 @@ def test_poisson_yield():
         g = O.poisson(1, 1, 0.01, 30, seed)
         ys.append((g[:, 0] >= 0).sum() * 0.01 ** 2)
     assert 0.66 < np.mean(ys) < 0.70
+
+
+def test_clip_lattice_fp32_equals_fp64():
+    """Jittered lattice (+-1e-7): FP32-stored sites clip exactly like the same values in FP64
+    (the clip orders neighbours by FP64 distance), so the topology cannot depend on the mode."""
+    rng = np.random.default_rng(8)
+    g = (np.arange(24) + 0.5) / 24
+    sites = np.stack(np.meshgrid(g, g), -1).reshape(-1, 2)
+    sites = (sites + rng.uniform(-1e-7, 1e-7, sites.shape)).astype(np.float32)
+    a = O.clip(sites, 1.0, 1.0, 1 / 12, real=0)
+    b = O.clip(sites.astype(np.float64), 1.0, 1.0, 1 / 12, real=1)
+    assert np.array_equal(a["count"], b["count"]) and np.array_equal(a["labels"], b["labels"])
+    assert np.array_equal(a["verts"], b["verts"])
+    assert a["overflow"] == 0
