@@ -361,6 +361,31 @@ int vm_batch_status(vm_batch *b, int id, int *iteration, int *done, double *last
     });
 }
 
+int vm_copy_probe(int64_t bytes, float *gpuMs)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(bytes > 0 && bytes % 4 == 0, "bytes must be a positive multiple of 4");
+        DBuf<int> src, dst;
+        src.reserve(size_t(bytes / 4)), dst.reserve(size_t(bytes / 4));
+        hipStream_t s;
+        VM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        VM_HIP(hipEventCreate(&e0));
+        VM_HIP(hipEventCreate(&e1));
+        VM_HIP(vmk::launchFillInt(s, src.p, bytes / 4, 1));
+        VM_HIP(hipEventRecord(e0, s));
+        VM_HIP(hipMemcpyAsync(dst.p, src.p, size_t(bytes), hipMemcpyDeviceToDevice, s));
+        VM_HIP(hipEventRecord(e1, s));
+        VM_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        VM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (gpuMs) *gpuMs = ms;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    });
+}
+
 int vm_batch_loss_stats(vm_batch *b, uint64_t *counters)
 {
     return vmrt::guardedCall([&] {

@@ -506,8 +506,8 @@ public:
     {
         if (dirty) commit();
         DBuf<unsigned long long> dS;
-        dS.reserve(6);
-        VM_HIP(hipMemsetAsync(dS.p, 0, 6 * sizeof(unsigned long long), stream));
+        dS.reserve(64 * 8);
+        VM_HIP(hipMemsetAsync(dS.p, 0, 64 * 8 * sizeof(unsigned long long), stream));
         gridBuild(vmk::kModeEvaluate);
         for (size_t g = 0; g < L.groups.size(); ++g) {
             auto a  = lossArgs(int(g), vmk::kModeEvaluate, false);
@@ -515,10 +515,13 @@ public:
             VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, a));
         }
         bin();
-        unsigned long long h[6];
-        VM_HIP(hipMemcpyAsync(h, dS.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+        std::vector<unsigned long long> h(64 * 8);
+        VM_HIP(hipMemcpyAsync(h.data(), dS.p, h.size() * sizeof(h[0]), hipMemcpyDeviceToHost, stream));
         VM_HIP(hipStreamSynchronize(stream));
-        for (int k = 0; k < 6; ++k) out[k] = h[k];
+        for (int k = 0; k < 6; ++k) {
+            out[k] = 0;
+            for (int l = 0; l < 64; ++l) out[k] += h[8 * l + k];
+        }
     }
 
     void knnQuery(int id, const double *q, int nq, int K, int *idx, double *d2) override

@@ -44,8 +44,8 @@ template <class R> struct LossArgs {
     int               mode;   // kModeIterate / kModeEvaluate
     int              *dbgKnn; // optional [T*K] k-NN lists (instance-local indices)
     TermDbg          *dbgTerm;
-    unsigned long long *stats; // optional path counters: fast, ring (staged), ring (global), idle,
-                               // candidates, staged sites
+    unsigned long long *stats; // optional [64 x 8] path counters: fast, ring (staged), ring (global),
+                               // idle, candidates, staged sites (summed over the 64 lines)
 
 };
 

@@ -519,7 +519,7 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                 take = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(iter), in.seed) < in.fraction;
         }
         if (__ballot(take) == 0) {
-            if (a.stats && lane == 0) atomicAdd(&a.stats[3], 1ull);
+            if (a.stats && lane == 0) atomicAdd(&a.stats[8 * (blockIdx.x & 63) + 3], 1ull);
             waveSync();
             continue;
         }
@@ -602,9 +602,10 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
             }
         }
         if (a.stats && lane == 0) {
-            atomicAdd(&a.stats[fast ? 0 : (staged ? 1 : 2)], 1ull);
-            atomicAdd(&a.stats[4], (unsigned long long)m);
-            atomicAdd(&a.stats[5], (unsigned long long)total);
+            unsigned long long *sp = a.stats + 8 * (blockIdx.x & 63); // 64 lines, summed on the host
+            atomicAdd(&sp[fast ? 0 : (staged ? 1 : 2)], 1ull);
+            atomicAdd(&sp[4], (unsigned long long)m);
+            atomicAdd(&sp[5], (unsigned long long)total);
         }
         if (!fast) {
             // ---- fallback: ring search (the ring geometry is wave-uniform) ---

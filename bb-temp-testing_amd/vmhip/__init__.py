@@ -72,7 +72,7 @@ class ServeStats(C.Structure):
 
 # Every symbol include/voromesh_hip.h declares (checked by the CPU test suite).
 EXPORTS = [
-    "vm_default_params", "vm_last_error", "vm_version", "vm_device_count",
+    "vm_default_params", "vm_last_error", "vm_version", "vm_device_count", "vm_copy_probe",
     "vm_create", "vm_destroy", "vm_set_targets", "vm_update_targets", "vm_set_occupancy", "vm_update_occupancy",
     "vm_set_generators", "vm_update_generators", "vm_begin_schedule", "vm_step", "vm_get_state",
     "vm_evaluate_loss", "vm_extract", "vm_get_generators", "vm_knn", "vm_loss_pass",
@@ -101,6 +101,7 @@ def lib():
     L.vm_default_params.argtypes = [P(Params)]
     L.vm_device_count.restype = i32
     sig = {
+        "vm_copy_probe": [C.c_int64, P(f32)],
         "vm_create": [f32, f32, f32, P(Params), P(vp)],
         "vm_destroy": [vp],
         "vm_set_targets": [vp, vp, i32],
@@ -160,6 +161,13 @@ def _pts(a):
 
 def device_count() -> int:
     return int(lib().vm_device_count())
+
+
+def copy_probe(nbytes: int) -> float:
+    """Device-to-device copy of nbytes (traffic-counter calibration); returns GPU ms."""
+    ms = C.c_float()
+    _check(lib().vm_copy_probe(int(nbytes), C.byref(ms)))
+    return ms.value
 
 
 def default_params(**kw) -> Params:

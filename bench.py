@@ -60,6 +60,17 @@ def make_batched(n_inst, seed, lo=256, hi=65536, per_site=8):
     return insts, ms
 
 
+def measured_traffic(key):
+    """HBM bytes per loss-kernel launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    (profiles/loss_kernel_traffic.json, calibrated on a 256 MiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "loss_kernel_traffic.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)[key]["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def loss_bytes(S, T, C):
     """Algorithmic HBM bytes of one loss-kernel launch: targets read (8 B), sorted sites
     read (pos 8 B + index 4 B), cell starts read (4 B), gradient accumulated (8 B)."""
@@ -160,7 +171,8 @@ def run_large(iters=500, K=16, n_sites=262144, n_targets=4194304):
                 loss_kernel_GBs=lb / (phase[2] / 1e3) / 1e9, loss_kernel_frac=lb / (phase[2] / 1e3) / 1e9 / HBM_PEAK_GBS,
                 iteration_GBs=ib / (ms / iters / 1e3) / 1e9,
                 iteration_frac=ib / (ms / iters / 1e3) / 1e9 / HBM_PEAK_GBS, grid_cell=ch["grid_cell"],
-                loss_paths_start=stats0, loss_paths_end=stats1,
+                loss_paths_start=stats0, loss_paths_end=stats1, loss_kernel_traffic_bytes=measured_traffic("large262k"),
+                loss_kernel_algorithmic_bytes=lb,
                 final_mean_loss=b.status(0)["last_mean_loss"])
 
 
@@ -283,6 +295,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=0.25, help="arrival window of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the large/small/precision sections")
+    ap.add_argument("--only", choices=["large", "small", "precision"], default=None,
+                    help="run one section alone and print its JSON (profiling)")
+    ap.add_argument("--large-iters", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is a design "
@@ -291,6 +306,13 @@ def main():
 
     import vmhip
     assert vmhip.device_count() > 0, "no HIP device"
+    # A 256 MiB device copy: the known-size kernel the HBM traffic counters are calibrated on.
+    copy_ms = vmhip.copy_probe(256 << 20)
+    log(f"copy probe: 256 MiB in {copy_ms:.3f} ms ({2 * 256 * 1.048576 / copy_ms:.0f} GB/s read+write)")
+    if args.only:
+        fn = {"large": lambda: run_large(iters=args.large_iters), "small": run_small, "precision": run_precision}
+        print(json.dumps(fn[args.only](), default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
+        return
     log(f"generating {args.instances} instances ...")
     t0 = time.time()
     insts, poisson_ms = make_batched(args.instances, args.seed)
@@ -337,12 +359,13 @@ def main():
         "iteration_roofline": {"bytes": iter_bytes(S, T, C),
                                "achieved_GBs": iter_bytes(S, T, C) / (ms_per_step / 1e3) / 1e9},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "lossKernel<16,float>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("batch512") if
+                     args.instances == 512 and args.seed == 2 else None, "kernel": "lossKernel<16,float>",
                      "algorithmic_bytes": lb, "kernel_ms": loss_ms},
     }
     if not args.no_extra:
         log("config 1: large single instance ...")
-        out["large_instance"] = run_large()
+        out["large_instance"] = run_large(iters=args.large_iters)
         li = out["large_instance"]
         log(f"  {li['ms_per_iter']:.3f} ms/iter, {li['site_iters_per_s']:.3e} site-iters/s, loss kernel "
             f"{li['loss_kernel_GBs']:.1f} GB/s")

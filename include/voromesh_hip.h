@@ -89,6 +89,10 @@ VM_API const char *vm_last_error(void);
 VM_API const char *vm_version(void);
 /* Number of HIP devices visible (0 without a GPU); never fails. */
 VM_API int         vm_device_count(void);
+/* Device-to-device copy of `bytes` (multiple of 4) between two fresh buffers: the
+ * known-size reference for calibrating HBM traffic counters.  No counterpart in the
+ * reference. */
+VM_API int         vm_copy_probe(int64_t bytes, float *gpu_ms);
 
 /* ---- voromesh::VoroOptimizer --------------------------------------------- */
 /* VoroOptimizer(width, height, spacing, params)  voro_optimizer.h:75, .cu:512-532 */
