@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -169,11 +170,11 @@ public:
         std::vector<vmk::Inst>       inst;
         std::vector<vmk::SiteWork>   siteWork;
         std::vector<vmk::TargetWork> targetWork; // grouped by K
-        struct Group { int K, begin, count; };
+        struct Group { int K, begin, count; int64_t qOff; }; // qOff: the group's exact-selection queue
         std::vector<Group>            groups;
         std::vector<int>              instList;
         std::vector<std::vector<int>> perm; // per instance: sorted position -> input target index
-        int64_t sites = 0, cells = 0, targets = 0;
+        int64_t sites = 0, cells = 0, targets = 0, queue = 0;
     } L;
 
     DBuf<V>              dPos, dM, dV, dSpos, dTpos;
@@ -185,6 +186,8 @@ public:
     DBuf<vmk::LossPartial> dPartial;
     DBuf<vmk::SiteWork>  dSiteWork;
     DBuf<vmk::TargetWork> dTargetWork;
+    DBuf<vmk::ExactItem> dExactQ;
+    DBuf<int>            dExactLen; // one per group
     hipGraphExec_t       graph = nullptr;
     hipEvent_t           ev[8] = {};
     int                  gridScan = 1, gridSite = 1;
@@ -212,6 +215,7 @@ public:
         b += dFrozen.n + dInst.n * sizeof(vmk::Inst) + dState.n * sizeof(vmk::InstState) +
              dPartial.n * sizeof(vmk::LossPartial);
         b += dSiteWork.n * sizeof(vmk::SiteWork) + dTargetWork.n * sizeof(vmk::TargetWork);
+        b += dExactQ.n * sizeof(vmk::ExactItem) + dExactLen.n * sizeof(int);
         return b;
     }
 
@@ -237,9 +241,12 @@ public:
         std::sort(ks.begin(), ks.end());
         ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
         for (int K : ks) {
-            typename Layout::Group g{K, int(L.targetWork.size()), 0};
+            typename Layout::Group g{K, int(L.targetWork.size()), 0, L.queue};
             for (int i = 0; i < n; ++i)
-                if (specs[i].K == K) L.targetWork.insert(L.targetWork.end(), twByInst[i].begin(), twByInst[i].end());
+                if (specs[i].K == K) {
+                    L.targetWork.insert(L.targetWork.end(), twByInst[i].begin(), twByInst[i].end());
+                    L.queue += L.inst[i].T; // every target of the group may be queued once per launch
+                }
             g.count = int(L.targetWork.size()) - g.begin;
             L.groups.push_back(g);
         }
@@ -258,6 +265,7 @@ public:
         dInst.reserve(n), dState.reserve(n), dInstList.reserve(n), dPartial.reserve(size_t(n) * vmk::kLossPartials);
         dSiteWork.reserve(L.siteWork.size()), dTargetWork.reserve(L.targetWork.size());
         dCnt.reserve(2 + L.groups.size());
+        dExactQ.reserve(std::max<int64_t>(L.queue, 1)), dExactLen.reserve(L.groups.size());
 
         std::vector<V>       hp(L.sites), hm(L.sites), hv(L.sites), ht(L.targets);
         std::vector<int>     hts(L.targets, 0);
@@ -310,6 +318,10 @@ public:
         a.insts = dInst.p, a.state = dState.p, a.work = dTargetWork.p + L.groups[g].begin, a.nWork = dCnt.p + 2 + g;
         a.pos = dPos.p, a.spos = dSpos.p, a.sidx = dSidx.p, a.start = dStart.p, a.tpos = dTpos.p, a.tslot = dTslot.p;
         a.grad = wantGrad ? dGrad.p : nullptr, a.partial = dPartial.p, a.mode = mode;
+        // VM_LOSS_EXACT=1 sends every lane through the exact k-NN selection (parity checks of that path).
+        const char *ex = std::getenv("VM_LOSS_EXACT");
+        a.exactOnly    = ex && ex[0] == '1';
+        a.exactQ = dExactQ.p + L.groups[g].qOff, a.exactLen = dExactLen.p + g;
         return a;
     }
 
@@ -326,7 +338,8 @@ public:
 
     void gridBuild(int mode)
     {
-        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dInstList.p, dCnt.p, dCounts.p, dStart.p, mode));
+        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
+                               mode, dExactLen.p, int(L.groups.size())));
         VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p, dCellOf.p,
                                      dRank.p, dStart.p, dSpos.p, dSidx.p, mode));
     }
@@ -335,7 +348,7 @@ public:
     {
         if (marks) VM_HIP(hipEventRecord(marks[0], stream));
         VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
-                               vmk::kModeIterate));
+                               vmk::kModeIterate, dExactLen.p, int(L.groups.size())));
         if (marks) VM_HIP(hipEventRecord(marks[1], stream));
         VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p, dCellOf.p,
                                      dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate));
@@ -502,7 +515,7 @@ public:
         }
     }
 
-    void lossStats(uint64_t out[6]) override
+    void lossStats(uint64_t out[8]) override
     {
         if (dirty) commit();
         DBuf<unsigned long long> dS;
@@ -518,7 +531,7 @@ public:
         std::vector<unsigned long long> h(64 * 8);
         VM_HIP(hipMemcpyAsync(h.data(), dS.p, h.size() * sizeof(h[0]), hipMemcpyDeviceToHost, stream));
         VM_HIP(hipStreamSynchronize(stream));
-        for (int k = 0; k < 6; ++k) {
+        for (int k = 0; k < 8; ++k) {
             out[k] = 0;
             for (int l = 0; l < 64; ++l) out[k] += h[8 * l + k];
         }

@@ -116,7 +116,7 @@ public:
     virtual void  clip(int id, const uint64_t *dMasks, int tilesX, int tilesY, float voxel, ClipOut &out) = 0;
     virtual size_t deviceBytes() const = 0;
     // One evaluation-mode loss pass with the loss kernel's path counters.
-    virtual void lossStats(uint64_t out[6]) = 0;
+    virtual void lossStats(uint64_t out[8]) = 0;
 
     bool        dirty = true;
     bool        deviceNewer = false;

@@ -45,8 +45,11 @@ template <class R> struct LossArgs {
     int              *dbgKnn; // optional [T*K] k-NN lists (instance-local indices)
     TermDbg          *dbgTerm;
     unsigned long long *stats; // optional [64 x 8] path counters: fast, ring (staged), ring (global),
-                               // idle, candidates, staged sites (summed over the 64 lines)
-
+                               // idle, candidates, staged sites, fast waves with an unscreened lane,
+                               // unscreened targets in those waves (summed over the 64 lines)
+    int               exactOnly; // 1: every lane takes the exact selection (parity checks of that path)
+    ExactItem        *exactQ;   // targets queued for the exact selection; capacity >= the group's targets
+    int              *exactLen; // queue length (zeroed by the scan kernel)
 };
 
 template <class R> struct AdamArgs {
@@ -78,7 +81,8 @@ template <class R> struct ClipArgs {
 };
 
 hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, LossPartial *partial,
-                      const int *instList, const int *nInst, int *counts, int *start, int mode);
+                      const int *instList, const int *nInst, int *counts, int *start, int mode, int *exactLen,
+                      int nQueues);
 template <class R>
 hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstState *state, const SiteWork *work,
                          const int *nWork, const void *pos, const int *cellOf, const int *rank, const int *start,

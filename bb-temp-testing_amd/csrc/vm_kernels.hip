@@ -212,11 +212,14 @@ constexpr int kScanThreads = 1024;
 
 __global__ __launch_bounds__(kScanThreads) void scanKernel(const Inst *__restrict__ insts, InstState *state,
                                                           LossPartial *partial, const int *__restrict__ instList,
-                                                          const int *nInstPtr, int *counts, int *start, int mode)
+                                                          const int *nInstPtr, int *counts, int *start, int mode,
+                                                          int *exactLen, int nQueues)
 {
     __shared__ int waveSum[kScanThreads / 64];
     __shared__ int skip;
     const int nInst = *nInstPtr;
+    // Empty the exact-selection queues the following loss launches fill.
+    if (blockIdx.x == 0 && int(threadIdx.x) < nQueues) exactLen[threadIdx.x] = 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (int w = blockIdx.x; w < nInst; w += gridDim.x) {
         const int   id = instList[w];
@@ -334,25 +337,161 @@ __global__ __launch_bounds__(kSiteChunk) void scatterKernel(const Inst *__restri
 // Fast path.  With D >= the distance from q to its K-th nearest staged site,
 // every target p of the chunk has D_K(p) <= D + rho, so its K nearest sites lie
 // within D + 2 rho of q.  When the staged block provably contains that disc,
-// the candidates (q's own nearest first) are compacted in LDS and each lane
-// selects its exact top-K under the (d2, index) order as an unsorted set with a
-// tracked maximum.  When c0 is common to the wave, the (c0, cj) bisector terms
-// are computed once per wave.  Otherwise the lanes fall back to a ring search.
-// Wave-wide reductions use DPP (no LDS round trips).
+// the candidates (q's own nearest first) are compacted in LDS.  In FP32 each
+// lane then resolves its term by screening (screenTerm below) and evaluates the
+// exact formula once; lanes the screen cannot prove select their exact top-K
+// under the (d2, index) order as an unsorted set with a tracked maximum.
+// Otherwise the lanes fall back to a ring search.  Wave-wide reductions use DPP
+// (no LDS round trips).
 // ---------------------------------------------------------------------------
 constexpr int kLossWaves     = 4;
 constexpr int kLossThreads   = 64 * kLossWaves;
-constexpr int kStageMaxCells = 25;  // (2*2+1)^2
-constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane
+constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane (block of <= 5 x 5 cells)
 
 template <class R> struct LossLds {
     V2<R>    pos[kLossStage];
     int      idx[kLossStage];
+    uint32_t klo[kLossStage]; // key low word (index << kLdsSlotBits | slot) of each compacted candidate
     GradT<R> grad[2 * kLossStage];
-    R        ex[kLossStage], ey[kLossStage], el[kLossStage]; // cj - c0 and |cj - c0| for the wave's common c0
-    int      cellBeg[kStageMaxCells];
-    int      cellEnd[kStageMaxCells];
 };
+
+// Outcome of one lane's VoroLoss term: nearest site c0 (instance index i0, LDS
+// slot sl0 or kLdsNone), bisector partner j (bj, slj), d = signed distance to
+// their bisector, ev = cj - c0 and l = |ev|.
+template <class R> struct LaneTerm {
+    int   i0 = -1, sl0 = kLdsNone, bj = -1, slj = kLdsNone, used = 0;
+    V2<R> c0{R(0), R(0)}, ev{R(0), R(0)};
+    R     d = R(0), l = R(0);
+};
+
+// Screening walks the compacted candidates in groups of kScreenGroup; the list
+// is padded to a whole group with sentinel sites far outside any instance.
+constexpr int   kScreenGroup = 4;
+constexpr float kFarSite     = 1e30f;
+
+// FP32 screening of one lane's VoroLoss term over the wave's m compacted
+// candidates (a superset of the lane's K nearest sites, padded to mp).
+//   1. The nearest candidate, c0, under the (d2, index) order.
+//   2. An approximation a_i of |distance to the bisector (c0, c_i)| with the
+//      hardware reciprocal square root; a_i is packed with i in its low 7 bits
+//      and the two smallest are kept (min / median-of-three).  c0 itself, sites
+//      on top of it and the sentinels give NaN and never win.
+//   3. The winner w must be one of the K nearest: at most K candidates with
+//      d2 <= d2(w).
+// Error bound: with rsq accurate to 2 ulp, |a_i - |d_i|| <= 2^-21 (|u| + l_i)
+// + 2^-23 |d_i| (u = p - c0, l_i = |c_i - c0|) and packing truncates a_i by at
+// most 2^-16 relative.  lam >= |u| + l_i for every candidate, so
+//   a_2 (1 - 2^-14) > a_1 + 2^-18 lam
+// implies |d_j| > |d_w| (1 + 2^-22) for every other candidate j: RN(d_j^2) >
+// RN(d_w^2) strictly, and the reference's loop over the K nearest (smallest
+// d^2, ties to the earlier key) picks w.  The exact term is then computed for w
+// alone with the reference's formula.  Returns false when any condition fails;
+// the caller runs the exact selection for that lane.
+template <int K>
+__device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, int mp, V2<float> p, float lam, LaneTerm<float> &o)
+{
+    constexpr uint32_t kIdxMask = uint32_t(kLossStage - 1);
+    static_assert((kLossStage & (kLossStage - 1)) == 0, "slot mask");
+    Key<float> kmin = Key<float>::none();
+#pragma unroll 1
+    for (int g = 0; g < mp; g += kScreenGroup) {
+#pragma unroll
+        for (int k = 0; k < kScreenGroup; ++k) {
+            const Key<float> key = Key<float>::make(dist2(lds.pos[g + k], p), lds.klo[g + k]);
+            if (key.lt(kmin)) kmin = key;
+        }
+    }
+    const int       s0 = keySlot(kmin.low());
+    const V2<float> c0 = lds.pos[s0];
+    const V2<float> u{p.x - c0.x, p.y - c0.y};
+    uint32_t        a1 = ~0u, a2 = ~0u;
+#pragma unroll 1
+    for (int g = 0; g < mp; g += kScreenGroup) {
+#pragma unroll
+        for (int k = 0; k < kScreenGroup; ++k) {
+            const V2<float> c  = lds.pos[g + k];
+            const float     ex = c.x - c0.x, ey = c.y - c0.y;
+            const float     ee = __builtin_fmaf(ey, ey, ex * ex);
+            const float     rs = __builtin_amdgcn_rsqf(ee);
+            const float     ue = __builtin_fmaf(u.y, ey, u.x * ex);
+            const float     d  = __builtin_fmaf(-0.5f, ee * rs, ue * rs);
+            const uint32_t  av = (__float_as_uint(d) & (0x7FFFFFFFu & ~kIdxMask)) | uint32_t(g + k);
+            a2                 = max(min(a1, av), min(max(a1, av), a2));
+            a1                 = min(a1, av);
+        }
+    }
+    const int   w  = int(a1 & kIdxMask);
+    const float v1 = __uint_as_float(a1 & ~kIdxMask);
+    const float v2 = __uint_as_float(a2 & ~kIdxMask);
+    bool        ok = a1 < 0x7F800000u &&
+              (a2 >= 0x7F800000u || __builtin_fmaf(v2, -0x1p-14f, v2) > __builtin_fmaf(lam, 0x1p-18f, v1));
+    const V2<float> cw  = lds.pos[w];
+    const float     d2w = dist2(cw, p);
+    int             cnt = 0;
+#pragma unroll 1
+    for (int g = 0; g < mp; g += kScreenGroup) {
+#pragma unroll
+        for (int k = 0; k < kScreenGroup; ++k) cnt += dist2(lds.pos[g + k], p) <= d2w ? 1 : 0;
+    }
+    ok = ok && cnt <= K;
+    // The reference's term for (c0, w).
+    o.ev  = V2<float>{cw.x - c0.x, cw.y - c0.y};
+    o.l   = sqrtR(dotR(o.ev, o.ev));
+    ok    = ok && !(o.l < 1e-12f);
+    o.d   = fmaR(-0.5f, o.l, dotR(u, o.ev) / o.l);
+    o.i0  = keySite(kmin.low()), o.sl0 = s0, o.c0 = c0;
+    o.bj  = keySite(lds.klo[w]), o.slj = w, o.used = 1;
+    return ok;
+}
+
+// Gradient of the term d^2 for the pair (c0, cj) (voro_optimizer.cu:244-254).
+// With u = p - c0, e = cj - c0, l = |e|, n = e / l:
+//   dd/dcj = (u - <u, n> n) / l - n / 2,   dd/dc0 = -n - dd/dcj,
+// scaled by 2d and added to the accumulators of cj (dj) and c0 (d0).
+template <class R>
+__device__ __forceinline__ void lossGrad(V2<R> u, V2<R> ev, R l, R d, GradT<R> *dj, GradT<R> *d0, double gscale)
+{
+    const R     rl = R(1) / l;
+    const V2<R> n{rl * ev.x, rl * ev.y};
+    const R     un = dotR(u, n);
+    const R     tx = fmaR(-un, n.x, u.x), ty = fmaR(-un, n.y, u.y);
+    const V2<R> dcj{fmaR(rl, tx, R(-0.5) * n.x), fmaR(rl, ty, R(-0.5) * n.y)};
+    const R     s = R(2) * d;
+    GradOps<R>::add(dj, s * dcj.x, gscale);
+    GradOps<R>::add(dj + 1, s * dcj.y, gscale);
+    GradOps<R>::add(d0, -(s * (n.x + dcj.x)), gscale);
+    GradOps<R>::add(d0 + 1, -(s * (n.y + dcj.y)), gscale);
+}
+
+// The reference's VoroLoss term (voro_optimizer.cu:225-242) for a target p
+// whose K nearest sites are in ascending (d2, index) order: c0 = the nearest;
+// among the others, the smallest d^2 (strict <, so ties go to the earlier
+// neighbour), skipping sites within 1e-12 of c0.
+template <int K, class R>
+__device__ __forceinline__ void exactTerm(const Key<R> (&keys)[K], const V2<R> *pos, V2<R> p, LaneTerm<R> &o)
+{
+    o = LaneTerm<R>{};
+    if (!keys[0].valid()) return;
+    o.i0 = keySite(keys[0].low());
+    o.c0 = pos[o.i0];
+    const V2<R> u{p.x - o.c0.x, p.y - o.c0.y};
+    R           best = sizeof(R) == 4 ? R(FLT_MAX) : R(DBL_MAX);
+    const R     tiny = sizeof(R) == 4 ? R(1e-12f) : R(1e-12);
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+        if (!keys[k].valid()) break;
+        const int   j  = keySite(keys[k].low());
+        const V2<R> cj = pos[j];
+        const V2<R> ev{cj.x - o.c0.x, cj.y - o.c0.y};
+        const R     l = sqrtR(dotR(ev, ev));
+        if (l < tiny) continue;
+        const R d = fmaR(R(-0.5), l, dotR(u, ev) / l);
+        if (d * d < best) {
+            best = d * d;
+            o.d = d, o.ev = ev, o.l = l, o.bj = j, o.used = 1;
+        }
+    }
+}
 
 template <class R> __device__ __forceinline__ R relMargin();
 template <> __device__ __forceinline__ float  relMargin<float>() { return 4e-6f; }
@@ -454,57 +593,7 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         const Inst      &in = a.insts[wk.inst];
         InstState       *st = a.state + wk.inst;
         if (a.mode == kModeIterate && st->done) continue;
-        const int      iter = st->iter;
-        const int64_t  so   = in.siteOff;
-        const V2<R>   *spos = a.spos + so;
-        const int     *sidx = a.sidx + so;
-        const int     *strt = a.start + in.cellOff;
-        const int      gw = in.gw, gh = in.gh;
-        const int      cx = wk.cell % gw, cy = wk.cell / gw;
-        const int      rs = in.stageR;
-        const int      sx0 = max(cx - rs, 0), sx1 = min(cx + rs, gw - 1);
-        const int      sy0 = max(cy - rs, 0), sy1 = min(cy + rs, gh - 1);
-        const int      snx = sx1 - sx0 + 1, sny = sy1 - sy0 + 1;
-        const int      ncell = snx * sny;
-        const R        cell  = R(in.cell);
-        const double   gscale = __builtin_ldexp(1.0, in.gradShift);
-
-        // ---- stage the cell block [sx0,sx1] x [sy0,sy1] into LDS ----------
-        // Rows of the block are contiguous ranges of the sorted arrays.
-        int b = 0, e = 0;
-        if (lane < ncell) {
-            int c = (sy0 + lane / snx) * gw + sx0 + lane % snx;
-            b     = strt[c];
-            e     = strt[c + 1];
-        }
-        const int myRow = lane < ncell ? lane / snx : -1;
-        int       total = 0, myOff = 0;
-        for (int r = 0; r < sny; ++r) {
-            const int rb = __builtin_amdgcn_readlane(b, r * snx);
-            const int re = __builtin_amdgcn_readlane(e, r * snx + snx - 1);
-            if (myRow == r) myOff = total + b - rb;
-            total += re - rb;
-        }
-        const bool staged = total <= kLossStage;
-        if (staged) {
-            if (lane < ncell) {
-                lds.cellBeg[lane] = myOff;
-                lds.cellEnd[lane] = myOff + e - b;
-            }
-            int roff = 0;
-            for (int r = 0; r < sny; ++r) {
-                const int rb = __builtin_amdgcn_readlane(b, r * snx);
-                const int re = __builtin_amdgcn_readlane(e, r * snx + snx - 1);
-                for (int s = lane; s < re - rb; s += 64) {
-                    lds.pos[roff + s]          = spos[rb + s];
-                    lds.idx[roff + s]          = sidx[rb + s];
-                    lds.grad[2 * (roff + s)]     = GradT<R>(0);
-                    lds.grad[2 * (roff + s) + 1] = GradT<R>(0);
-                }
-                roff += re - rb;
-            }
-        }
-        waveSync();
+        const int64_t so = in.siteOff;
 
         // ---- this lane's target -----------------------------------------
         const int t    = wk.begin + lane;
@@ -516,265 +605,170 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
             if (take && in.useBox && a.mode == kModeIterate)
                 take = p.x >= R(in.bx0) && p.x <= R(in.bx1) && p.y >= R(in.by0) && p.y <= R(in.by1);
             if (take && a.mode == kModeIterate && in.fraction < 1.0f)
-                take = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(iter), in.seed) < in.fraction;
+                take = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(st->iter), in.seed) < in.fraction;
         }
         if (__ballot(take) == 0) {
             if (a.stats && lane == 0) atomicAdd(&a.stats[8 * (blockIdx.x & 63) + 3], 1ull);
-            waveSync();
             continue;
         }
 
-        Key<R> keys[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) keys[k] = Key<R>::none();
-        const R ext  = R(in.ext);
-        bool    fast = staged && total >= K;
-        int     m    = 0; // compacted candidates (fast path)
-        if (fast) {
-            const V2<R> q{R(wk.qx), R(wk.qy)};
-            const R     mg  = relMargin<R>() * fmax(ext, fmax(fabs(q.x), fabs(q.y)) + R(wk.rho));
-            const R     rho = R(wk.rho) + mg;
-            // Squared distance from q to each staged site (two per lane).
-            const int e0 = lane, e1 = lane + 64;
-            R d0 = e0 < total ? dist2(lds.pos[e0], q) : rInf<R>();
-            R d1 = e1 < total ? dist2(lds.pos[e1], q) : rInf<R>();
-            // D^2 >= q's K-th smallest by bisection on [0, max].
-            R lo = R(0);
-            R hi = waveMaxR<R>(fmax(e0 < total ? d0 : R(0), e1 < total ? d1 : R(0)));
-            for (int it = 0; it < 14; ++it) {
-                R   mid = lo + R(0.5) * (hi - lo);
-                int c   = __popcll(__ballot(d0 <= mid)) + __popcll(__ballot(d1 <= mid));
-                if (c >= K) hi = mid;
-                else lo = mid;
+        LaneTerm<R> tr;
+        bool        open  = take; // still to be resolved by the exact selection
+        bool        fast  = false;
+        int         m     = 0; // compacted candidates (fast path)
+        int         total = 0; // sites of the staged block
+        if constexpr (sizeof(R) == 4) {
+            const int *strt = a.start + in.cellOff;
+            const int  gw = in.gw, gh = in.gh;
+            const int  cx = wk.cell % gw, cy = wk.cell / gw;
+            const int  rs = in.stageR;
+            const int  sx0 = max(cx - rs, 0), sx1 = min(cx + rs, gw - 1);
+            const int  sy0 = max(cy - rs, 0), sy1 = min(cy + rs, gh - 1);
+            const int  snx = sx1 - sx0 + 1, sny = sy1 - sy0 + 1;
+            const R    cell = R(in.cell);
+
+            // ---- stage the cell block [sx0,sx1] x [sy0,sy1] into LDS --------
+            // Rows of the block are contiguous ranges of the sorted arrays.
+            int b = 0, e = 0;
+            if (lane < snx * sny) {
+                const int c = (sy0 + lane / snx) * gw + sx0 + lane % snx;
+                b           = strt[c];
+                e           = strt[c + 1];
             }
-            const R D    = sqrtR(hi) + mg;
-            const R thr  = D + R(2) * rho + mg;
-            const R thr2 = thr * thr * (R(1) + relMargin<R>());
-            // The block must contain the disc of radius thr around q.
-            R edge = rInf<R>();
-            if (sx0 > 0) edge = fmin(edge, q.x - R(sx0) * cell);
-            if (sx1 < gw - 1) edge = fmin(edge, R(sx1 + 1) * cell - q.x);
-            if (sy0 > 0) edge = fmin(edge, q.y - R(sy0) * cell);
-            if (sy1 < gh - 1) edge = fmin(edge, R(sy1 + 1) * cell - q.y);
-            fast = edge - R(2) * mg > thr;
+            for (int r = 0; r < sny; ++r)
+                total += __builtin_amdgcn_readlane(e, r * snx + snx - 1) - __builtin_amdgcn_readlane(b, r * snx);
+            fast = total <= kLossStage && total >= K;
             if (fast) {
-                // Compact the candidates into LDS: q's nearest (d2 <= D^2) first.
-                const uint64_t in0 = __ballot(d0 <= hi), in1 = __ballot(d1 <= hi);
-                const uint64_t c0m = __ballot(d0 <= thr2 && !(d0 <= hi)), c1m = __ballot(d1 <= thr2 && !(d1 <= hi));
-                const int      nIn = __popcll(in0) + __popcll(in1);
-                m                  = nIn + __popcll(c0m) + __popcll(c1m);
-                int dst0 = -1, dst1 = -1;
-                if (d0 <= hi) dst0 = laneRank(in0);
-                else if (d0 <= thr2) dst0 = nIn + laneRank(c0m);
-                if (d1 <= hi) dst1 = __popcll(in0) + laneRank(in1);
-                else if (d1 <= thr2) dst1 = nIn + __popcll(c0m) + laneRank(c1m);
-                V2<R> p0{}, p1{};
-                int   i0v = 0, i1v = 0;
-                if (dst0 >= 0) p0 = lds.pos[e0], i0v = lds.idx[e0];
-                if (dst1 >= 0) p1 = lds.pos[e1], i1v = lds.idx[e1];
+                int roff = 0;
+                for (int r = 0; r < sny; ++r) {
+                    const int rb = __builtin_amdgcn_readlane(b, r * snx);
+                    const int re = __builtin_amdgcn_readlane(e, r * snx + snx - 1);
+#pragma unroll 1
+                    for (int s = lane; s < re - rb; s += 64) {
+                        lds.pos[roff + s]            = a.spos[so + rb + s];
+                        lds.idx[roff + s]            = a.sidx[so + rb + s];
+                        lds.grad[2 * (roff + s)]     = GradT<R>(0);
+                        lds.grad[2 * (roff + s) + 1] = GradT<R>(0);
+                    }
+                    roff += re - rb;
+                }
                 waveSync();
-                if (dst0 >= 0) lds.pos[dst0] = p0, lds.idx[dst0] = i0v;
-                if (dst1 >= 0) lds.pos[dst1] = p1, lds.idx[dst1] = i1v;
-                waveSync();
-                // Exact top-K of each lane as an unsorted set with a tracked maximum.
-                if (take) {
-#pragma unroll
-                    for (int k = 0; k < K; ++k)
-                        keys[k] = Key<R>::make(dist2(lds.pos[k], p), (uint32_t(lds.idx[k]) << kLdsSlotBits) | uint32_t(k));
+
+                const R           ext = R(in.ext);
+                const V2<R>       q{R(wk.qx), R(wk.qy)};
+                const R           mg  = relMargin<R>() * fmax(ext, fmax(fabs(q.x), fabs(q.y)) + R(wk.rho));
+                const R           rho = R(wk.rho) + mg;
+                // Squared distance from q to each staged site (two per lane).
+                const int e0 = lane, e1 = lane + 64;
+                R d0 = e0 < total ? dist2(lds.pos[e0], q) : rInf<R>();
+                R d1 = e1 < total ? dist2(lds.pos[e1], q) : rInf<R>();
+                // D^2 >= q's K-th smallest by bisection on [0, max].
+                R lo = R(0);
+                R hi = waveMaxR<R>(fmax(e0 < total ? d0 : R(0), e1 < total ? d1 : R(0)));
+                for (int it = 0; it < 14; ++it) {
+                    R   mid = lo + R(0.5) * (hi - lo);
+                    int c   = __popcll(__ballot(d0 <= mid)) + __popcll(__ballot(d1 <= mid));
+                    if (c >= K) hi = mid;
+                    else lo = mid;
                 }
-                Key<R> kmax = keys[0];
-                int    amax = 0;
-#pragma unroll
-                for (int k = 1; k < K; ++k)
-                    if (kmax.lt(keys[k])) kmax = keys[k], amax = k;
-                for (int j = K; j < m; ++j) {
-                    Key<R> nk = Key<R>::make(dist2(lds.pos[j], p), (uint32_t(lds.idx[j]) << kLdsSlotBits) | uint32_t(j));
-                    if (take && nk.lt(kmax)) {
-#pragma unroll
-                        for (int k = 0; k < K; ++k)
-                            if (k == amax) keys[k] = nk;
-                        kmax = keys[0], amax = 0;
-#pragma unroll
-                        for (int k = 1; k < K; ++k)
-                            if (kmax.lt(keys[k])) kmax = keys[k], amax = k;
+                const R D    = sqrtR(hi) + mg;
+                const R thr  = D + R(2) * rho + mg;
+                const R thr2 = thr * thr * (R(1) + relMargin<R>());
+                // The block must contain the disc of radius thr around q: then it
+                // holds the K nearest sites of every target of the chunk.
+                R edge = rInf<R>();
+                if (sx0 > 0) edge = fmin(edge, q.x - R(sx0) * cell);
+                if (sx1 < gw - 1) edge = fmin(edge, R(sx1 + 1) * cell - q.x);
+                if (sy0 > 0) edge = fmin(edge, q.y - R(sy0) * cell);
+                if (sy1 < gh - 1) edge = fmin(edge, R(sy1 + 1) * cell - q.y);
+                fast = edge - R(2) * mg > thr;
+                if (fast) {
+                    // Compact the candidates into LDS: q's nearest (d2 <= D^2) first.
+                    const uint64_t in0 = __ballot(d0 <= hi), in1 = __ballot(d1 <= hi);
+                    const uint64_t c0m = __ballot(d0 <= thr2 && !(d0 <= hi)), c1m = __ballot(d1 <= thr2 && !(d1 <= hi));
+                    const int      nIn = __popcll(in0) + __popcll(in1);
+                    m                  = nIn + __popcll(c0m) + __popcll(c1m);
+                    int dst0 = -1, dst1 = -1;
+                    if (d0 <= hi) dst0 = laneRank(in0);
+                    else if (d0 <= thr2) dst0 = nIn + laneRank(c0m);
+                    if (d1 <= hi) dst1 = __popcll(in0) + laneRank(in1);
+                    else if (d1 <= thr2) dst1 = nIn + __popcll(c0m) + laneRank(c1m);
+                    V2<R> p0{}, p1{};
+                    int   i0v = 0, i1v = 0;
+                    if (dst0 >= 0) p0 = lds.pos[e0], i0v = lds.idx[e0];
+                    if (dst1 >= 0) p1 = lds.pos[e1], i1v = lds.idx[e1];
+                    waveSync();
+                    if (dst0 >= 0) {
+                        lds.pos[dst0] = p0, lds.idx[dst0] = i0v;
+                        lds.klo[dst0] = (uint32_t(i0v) << kLdsSlotBits) | uint32_t(dst0);
                     }
-                }
-            }
-        }
-        if (a.stats && lane == 0) {
-            unsigned long long *sp = a.stats + 8 * (blockIdx.x & 63); // 64 lines, summed on the host
-            atomicAdd(&sp[fast ? 0 : (staged ? 1 : 2)], 1ull);
-            atomicAdd(&sp[4], (unsigned long long)m);
-            atomicAdd(&sp[5], (unsigned long long)total);
-        }
-        if (!fast) {
-            // ---- fallback: ring search (the ring geometry is wave-uniform) ---
-            const R rel    = ringRel<R>();
-            const R margin = rel * fmax(ext, fmax(fabs(p.x), fabs(p.y)));
-            bool    active = take;
-            for (int ring = 0;; ++ring) {
-                if (!__any(active)) break;
-                const int x0 = cx - ring, x1 = cx + ring, y0 = cy - ring, y1 = cy + ring;
-                for (int y = max(y0, 0); y <= min(y1, gh - 1); ++y) {
-                    const int stride = (y == y0 || y == y1) ? 1 : max(x1 - x0, 1);
-                    for (int x = x0; x <= x1; x += stride) {
-                        if (x < 0 || x >= gw) continue;
-                        if (staged && x >= sx0 && x <= sx1 && y >= sy0 && y <= sy1) {
-                            const int li = (y - sy0) * snx + (x - sx0);
-                            const int lb = lds.cellBeg[li], le = lds.cellEnd[li];
-                            for (int s = lb; s < le; ++s) {
-                                if (active)
-                                    topkInsert<K>(keys, Key<R>::make(dist2(lds.pos[s], p),
-                                                                     (uint32_t(lds.idx[s]) << kLdsSlotBits) | uint32_t(s)));
-                            }
-                        } else {
-                            const int c  = y * gw + x;
-                            const int gb = strt[c], ge = strt[c + 1];
-                            for (int s = gb; s < ge; ++s) {
-                                if (active)
-                                    topkInsert<K>(keys, Key<R>::make(dist2(spos[s], p),
-                                                                     (uint32_t(sidx[s]) << kLdsSlotBits) | uint32_t(kLdsNone)));
-                            }
-                        }
+                    if (dst1 >= 0) {
+                        lds.pos[dst1] = p1, lds.idx[dst1] = i1v;
+                        lds.klo[dst1] = (uint32_t(i1v) << kLdsSlotBits) | uint32_t(dst1);
                     }
-                }
-                if (x0 <= 0 && y0 <= 0 && x1 >= gw - 1 && y1 >= gh - 1) break;
-                if (active) {
-                    R safe = fmin(fmin(p.x - R(x0) * cell, R(x1 + 1) * cell - p.x),
-                                  fmin(p.y - R(y0) * cell, R(y1 + 1) * cell - p.y));
-                    safe   = fmax(safe - margin, R(0));
-                    if (keys[K - 1].valid() && keys[K - 1].d2() * (R(1) + rel) < safe * safe) active = false;
+                    // Pad to whole screening groups with far sentinels.
+                    const int mp = (m + kScreenGroup - 1) & ~(kScreenGroup - 1);
+                    if (lane < mp - m) {
+                        lds.pos[m + lane] = V2<R>{R(kFarSite), R(kFarSite)};
+                        lds.klo[m + lane] = ~0u;
+                    }
+                    waveSync();
+                    if (!a.exactOnly) {
+                        // Screened term (see screenTerm for the bound).
+                        const R lam = R(4) * (thr + rho);
+                        if (screenTerm<K>(lds, __builtin_amdgcn_readfirstlane(mp), p, lam, tr)) open = false;
+                        tr.used = take && !open ? tr.used : 0;
+                    }
                 }
             }
         }
 
-        // ---- nearest site c0 -------------------------------------------------
-        Key<R> kmin = keys[0];
-#pragma unroll
-        for (int k = 1; k < K; ++k)
-            if (keys[k].lt(kmin)) kmin = keys[k];
-        const bool have = take && kmin.valid();
-        const int  i0   = have ? keySite(kmin.low()) : -1;
-        const int  sl0  = have ? keySlot(kmin.low()) : kLdsNone;
-        V2<R>      c0   = V2<R>{R(0), R(0)};
-        if (have) c0 = sl0 != kLdsNone ? lds.pos[sl0] : a.pos[so + i0];
-        // Wave-common c0: tabulate cj - c0 and |cj - c0| once for the candidates.
-        bool useTab = false;
-        if (fast) {
-            const uint64_t haveMask = __ballot(have);
-            const int      first    = haveMask ? __builtin_ctzll(haveMask) : 0;
-            const int      f0       = __builtin_amdgcn_readlane(i0, first);
-            useTab                  = haveMask != 0 && __ballot(have && i0 != f0) == 0;
-            if (useTab) {
-                const int   csl = __builtin_amdgcn_readlane(sl0, first);
-                const V2<R> cc  = lds.pos[csl];
-                for (int j = lane; j < m; j += 64) {
-                    V2<R> ev{lds.pos[j].x - cc.x, lds.pos[j].y - cc.y};
-                    lds.ex[j] = ev.x;
-                    lds.ey[j] = ev.y;
-                    lds.el[j] = sqrtR(dotR(ev, ev));
+        // ---- queue the open targets for the exact selection (and, for the k-NN
+        //      list diagnostics, every taken one: bit 31 of the instance marks a
+        //      term already settled here) ---------------------------------------
+        const bool     push = open || (a.dbgKnn && take);
+        const uint64_t pm   = __ballot(push);
+        if (pm) {
+            int base = 0;
+            if (lane == 0) base = atomicAdd(a.exactLen, __popcll(pm));
+            base = __builtin_amdgcn_readfirstlane(base);
+            if (push)
+                a.exactQ[base + laneRank(pm)] =
+                    ExactItem{open ? wk.inst : int(uint32_t(wk.inst) | 0x80000000u), int(in.targetOff + t)};
+        }
+        if (a.stats) {
+            const uint64_t om = __ballot(open);
+            if (lane == 0) {
+                unsigned long long *sp = a.stats + 8 * (blockIdx.x & 63); // 64 lines, summed on the host
+                atomicAdd(&sp[fast ? 0 : (total <= kLossStage ? 1 : 2)], 1ull);
+                atomicAdd(&sp[4], (unsigned long long)m);
+                atomicAdd(&sp[5], (unsigned long long)total);
+                if (fast && om) {
+                    atomicAdd(&sp[6], 1ull);
+                    atomicAdd(&sp[7], (unsigned long long)__popcll(om));
                 }
             }
-            waveSync();
         }
 
-        if (a.dbgKnn && lane < wk.count) {
-            // Lists in (d2, index) order for the parity tests.
-            Key<R> sk[K];
+        // ---- settled terms: gradient into LDS, diagnostics ------------------
+        const R lossT = tr.used ? tr.d * tr.d : R(0);
+        if (tr.used && a.grad)
+            lossGrad(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, tr.ev, tr.l, tr.d, &lds.grad[2 * tr.slj], &lds.grad[2 * tr.sl0],
+                     __builtin_ldexp(1.0, in.gradShift));
+        if (a.dbgKnn && lane < wk.count && !take) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) sk[k] = keys[k];
-#pragma unroll
-            for (int i = 0; i < K; ++i)
-#pragma unroll
-                for (int j = 0; j + 1 < K - i; ++j)
-                    if (sk[j + 1].lt(sk[j])) {
-                        Key<R> tmp = sk[j];
-                        sk[j]      = sk[j + 1];
-                        sk[j + 1]  = tmp;
-                    }
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-                a.dbgKnn[(in.targetOff + t) * int64_t(K) + k] = (take && sk[k].valid()) ? keySite(sk[k].low()) : -1;
+            for (int k = 0; k < K; ++k) a.dbgKnn[(in.targetOff + t) * int64_t(K) + k] = -1;
         }
-
-        // ---- VoroLoss term: closest bisector among the other K-1 ------------
-        R   lossT = R(0);
-        int used  = 0, bj = -1;
-        if (have) {
-            V2<R>   u{p.x - c0.x, p.y - c0.y};
-            R       best    = sizeof(R) == 4 ? R(FLT_MAX) : R(DBL_MAX);
-            R       bestD   = R(0);
-            Key<R>  bestKey = Key<R>::none();
-            const R tiny    = sizeof(R) == 4 ? R(1e-12f) : R(1e-12);
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (!keys[k].valid() || !kmin.lt(keys[k])) continue; // skip c0 and missing entries
-                const uint32_t lo = keys[k].low();
-                const int      sl = keySlot(lo);
-                V2<R>          ev;
-                R              l;
-                if (useTab) {
-                    ev = V2<R>{lds.ex[sl], lds.ey[sl]};
-                    l  = lds.el[sl];
-                } else {
-                    V2<R> cj = sl != kLdsNone ? lds.pos[sl] : a.pos[so + keySite(lo)];
-                    ev       = V2<R>{cj.x - c0.x, cj.y - c0.y};
-                    l        = sqrtR(dotR(ev, ev));
-                }
-                if (l < tiny) continue;
-                R q  = dotR(u, ev) / l;
-                R d  = fmaR(R(-0.5), l, q);
-                R dd = d * d;
-                if (dd < best || (dd == best && keys[k].lt(bestKey))) {
-                    best    = dd;
-                    bestD   = d;
-                    bestKey = keys[k];
-                }
-            }
-            if (bestKey.valid()) {
-                const uint32_t lob = bestKey.low();
-                bj    = keySite(lob);
-                lossT = best;
-                used  = 1;
-                if (a.grad) {
-                    const int slj = keySlot(lob);
-                    V2<R>     ev;
-                    R         l;
-                    if (useTab) {
-                        ev = V2<R>{lds.ex[slj], lds.ey[slj]};
-                        l  = lds.el[slj];
-                    } else {
-                        V2<R> cj = slj != kLdsNone ? lds.pos[slj] : a.pos[so + bj];
-                        ev       = V2<R>{cj.x - c0.x, cj.y - c0.y};
-                        l        = sqrtR(dotR(ev, ev));
-                    }
-                    R     rl = R(1) / l;
-                    V2<R> n{rl * ev.x, rl * ev.y};
-                    R     un = dotR(u, n);
-                    R     tx = fmaR(-un, n.x, u.x), ty = fmaR(-un, n.y, u.y);
-                    V2<R> dcj{fmaR(rl, tx, R(-0.5) * n.x), fmaR(rl, ty, R(-0.5) * n.y)};
-                    R     s = R(2) * bestD;
-                    V2<R> gj{s * dcj.x, s * dcj.y};
-                    V2<R> g0{-(s * (n.x + dcj.x)), -(s * (n.y + dcj.y))};
-                    GradT<R> *dj = slj != kLdsNone ? &lds.grad[2 * slj] : &a.grad[2 * (so + bj)];
-                    GradT<R> *d0 = sl0 != kLdsNone ? &lds.grad[2 * sl0] : &a.grad[2 * (so + i0)];
-                    GradOps<R>::add(dj, gj.x, gscale);
-                    GradOps<R>::add(dj + 1, gj.y, gscale);
-                    GradOps<R>::add(d0, g0.x, gscale);
-                    GradOps<R>::add(d0 + 1, g0.y, gscale);
-                }
-            }
-        }
-        if (a.dbgTerm && lane < wk.count) {
-            int64_t o = in.targetOff + t;
-            a.dbgTerm[o].i0   = used ? i0 : -1;
-            a.dbgTerm[o].j    = used ? bj : -1;
+        if (a.dbgTerm && lane < wk.count && !open) {
+            const int64_t o   = in.targetOff + t;
+            a.dbgTerm[o].i0   = tr.used ? tr.i0 : -1;
+            a.dbgTerm[o].j    = tr.used ? tr.bj : -1;
             a.dbgTerm[o].loss = double(lossT);
         }
 
         // ---- FP64 loss reduction: one atomic pair per wavefront ----------
         const double ls = waveSumDouble(double(lossT));
-        const int    uc = __popcll(__ballot(used != 0));
+        const int    uc = __popcll(__ballot(tr.used != 0));
         if (lane == 0 && uc > 0) {
             if (a.mode == kModeIterate) {
                 LossPartial *pp = a.partial + int64_t(wk.inst) * kLossPartials + (blockIdx.x & (kLossPartials - 1));
@@ -787,19 +781,71 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         }
 
         // ---- flush the LDS gradient: one atomic per touched site ----------
-        waveSync();
-        const int nflush = fast ? m : (staged ? total : 0);
-        if (a.grad) {
-            for (int s = lane; s < nflush; s += 64) {
-                GradT<R> gx = lds.grad[2 * s], gy = lds.grad[2 * s + 1];
+        if (fast && a.grad) {
+            waveSync();
+            for (int s = lane; s < m; s += 64) {
+                const GradT<R> gx = lds.grad[2 * s], gy = lds.grad[2 * s + 1];
                 if (gx != GradT<R>(0) || gy != GradT<R>(0)) {
-                    int i = lds.idx[s];
+                    const int i = lds.idx[s];
                     GradOps<R>::addRaw(&a.grad[2 * (so + i)], gx);
                     GradOps<R>::addRaw(&a.grad[2 * (so + i) + 1], gy);
                 }
             }
         }
         waveSync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3b. Exact selection for the targets the loss kernel queued: one thread per
+//     target, k-NN by ring search over the instance's grid, the reference's
+//     term and its gradient (global fixed-point atomics, so the sums equal the
+//     staged ones).  The scan kernel empties the queue before each step.
+// ---------------------------------------------------------------------------
+constexpr int kExactThreads = 128;
+constexpr int kExactGrid    = 256;
+
+template <int K, class R>
+__global__ __launch_bounds__(kExactThreads) void exactKernel(LossArgs<R> a)
+{
+    const int n = *a.exactLen;
+    // Entry e goes to workgroup e % grid, so a short queue spreads over the chip.
+    for (int e = threadIdx.x * gridDim.x + blockIdx.x; e < n; e += gridDim.x * kExactThreads) {
+        const ExactItem it       = a.exactQ[e];
+        const bool      listOnly = it.inst < 0;
+        const int       inst     = it.inst & 0x7FFFFFFF;
+        const Inst     &in       = a.insts[inst];
+        const int64_t   so       = in.siteOff;
+        const V2<R>     p        = a.tpos[it.tg];
+        Key<R>          keys[K];
+        knnGlobal<K>(in, a.spos + so, a.sidx + so, a.start + in.cellOff, p, keys);
+        if (a.dbgKnn) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) a.dbgKnn[int64_t(it.tg) * K + k] = keys[k].valid() ? keySite(keys[k].low()) : -1;
+        }
+        if (listOnly) continue;
+        LaneTerm<R> tr;
+        exactTerm<K, R>(keys, a.pos + so, p, tr);
+        const R lossT = tr.used ? tr.d * tr.d : R(0);
+        if (tr.used) {
+            if (a.grad)
+                lossGrad(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, tr.ev, tr.l, tr.d, &a.grad[2 * (so + tr.bj)],
+                         &a.grad[2 * (so + tr.i0)], __builtin_ldexp(1.0, in.gradShift));
+            if (a.mode == kModeIterate) {
+                LossPartial *pp = a.partial + int64_t(inst) * kLossPartials + (e & (kLossPartials - 1));
+                unsafeAtomicAdd(&pp->loss, double(lossT));
+                atomicAdd(&pp->count, 1ull);
+            } else {
+                InstState *st = a.state + inst;
+                unsafeAtomicAdd(&st->evalLoss, double(lossT));
+                atomicAdd((unsigned long long *)&st->evalCount, 1ull);
+            }
+        }
+        if (a.dbgTerm) {
+            a.dbgTerm[it.tg].i0   = tr.used ? tr.i0 : -1;
+            a.dbgTerm[it.tg].j    = tr.used ? tr.bj : -1;
+            a.dbgTerm[it.tg].loss = double(lossT);
+        }
     }
 }
 
@@ -1097,10 +1143,12 @@ __global__ void fillIntKernel(int *p, int64_t n, int v)
 // Host-side launchers
 // ===========================================================================
 hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, LossPartial *partial,
-                      const int *instList, const int *nInst, int *counts, int *start, int mode)
+                      const int *instList, const int *nInst, int *counts, int *start, int mode, int *exactLen,
+                      int nQueues)
 {
+    if (nQueues > kScanThreads) return hipErrorInvalidValue;
     hipLaunchKernelGGL(scanKernel, dim3(grid), dim3(kScanThreads), 0, s, insts, state, partial, instList, nInst,
-                       counts, start, mode);
+                       counts, start, mode, exactLen, nQueues);
     return hipGetLastError();
 }
 
@@ -1118,7 +1166,10 @@ template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const L
 {
     switch (K) {
 #define VM_K(k)                                                                                                   \
-    case k: hipLaunchKernelGGL((lossKernel<k, R>), dim3((grid + kLossWaves - 1) / kLossWaves), dim3(kLossThreads), 0, s, a); break;
+    case k:                                                                                                        \
+        hipLaunchKernelGGL((lossKernel<k, R>), dim3((grid + kLossWaves - 1) / kLossWaves), dim3(kLossThreads), 0, s, a); \
+        hipLaunchKernelGGL((exactKernel<k, R>), dim3(kExactGrid), dim3(kExactThreads), 0, s, a);                         \
+        break;
         VM_K(2) VM_K(4) VM_K(6) VM_K(8) VM_K(12) VM_K(16) VM_K(24) VM_K(32)
 #undef VM_K
     default: return hipErrorInvalidValue;

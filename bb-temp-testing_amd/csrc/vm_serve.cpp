@@ -18,6 +18,7 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -107,7 +108,7 @@ public:
         // Pool capacities: what the budget allows, but no more than the whole
         // request stream could ever hold at once.
         const double perSite = 4.0 * sizeof(V) + 2.0 * sizeof(vmk::GradT<R>) + 3 * sizeof(int) + 1;
-        const double perTgt = sizeof(V) + sizeof(int), perCell = 2 * sizeof(int);
+        const double perTgt = sizeof(V) + sizeof(int) + sizeof(vmk::ExactItem), perCell = 2 * sizeof(int);
         const double need = perSite * needSites + perTgt * needTargets + perCell * needCells;
         const double scale = std::min(1.0, cfg.hbm_budget_bytes / std::max(need, 1.0));
         capSites_   = RangeAlloc::round(int64_t(needSites * scale) + 64 * slots_);
@@ -124,6 +125,7 @@ public:
         dPartial.reserve(size_t(slots_) * vmk::kLossPartials);
         dSiteWork.reserve(capSites_ / vmk::kSiteChunk + slots_), dNewWork.reserve(capSites_ / vmk::kSiteChunk + slots_);
         dCnt.reserve(4), dNewCnt.reserve(1);
+        dExactQ.reserve(capTargets_), dExactLen.reserve(1);
         twCap_ = 0;
         VM_HIP(hipEventCreate(&ev0_));
         VM_HIP(hipEventCreate(&ev1_));
@@ -293,7 +295,7 @@ public:
     double bytesFor(int S, int T, int64_t C) const
     {
         return double(RangeAlloc::round(S)) * (4.0 * sizeof(V) + 2.0 * sizeof(vmk::GradT<R>) + 12 + 1) +
-               double(RangeAlloc::round(T)) * (sizeof(V) + 4) + double(RangeAlloc::round(C)) * 8;
+               double(RangeAlloc::round(T)) * (sizeof(V) + 4 + sizeof(vmk::ExactItem)) + double(RangeAlloc::round(C)) * 8;
     }
 
 private:
@@ -322,13 +324,16 @@ private:
         VM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         try {
             VM_HIP(vmk::launchScan(gScan, stream_, dInst.p, dState.p, dPartial.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
-                                   vmk::kModeIterate));
+                                   vmk::kModeIterate, dExactLen.p, 1));
             VM_HIP(vmk::launchScatter<R>(gSite, stream_, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
                                          dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate));
             vmk::LossArgs<R> la{};
             la.insts = dInst.p, la.state = dState.p, la.work = dTargetWork.p, la.nWork = dCnt.p + 2;
             la.pos = dPos.p, la.spos = dSpos.p, la.sidx = dSidx.p, la.start = dStart.p, la.tpos = dTpos.p;
             la.tslot = dTslot.p, la.grad = dGrad.p, la.partial = dPartial.p, la.mode = vmk::kModeIterate;
+            la.exactQ = dExactQ.p, la.exactLen = dExactLen.p;
+            const char *ex = std::getenv("VM_LOSS_EXACT");
+            la.exactOnly   = ex && ex[0] == '1';
             VM_HIP(vmk::launchLoss<R>(K, gLoss, stream_, la));
             VM_HIP(vmk::launchAdamBin<R>(gSite, stream_, adamArgs(1)));
         } catch (...) {
@@ -358,7 +363,8 @@ private:
 
     DBuf<V>              dPos, dM, dV, dSpos, dTpos;
     DBuf<vmk::GradT<R>>  dGrad;
-    DBuf<int>            dCellOf, dRank, dSidx, dCounts, dStart, dTslot, dInstList, dCnt, dNewCnt;
+    DBuf<int>            dCellOf, dRank, dSidx, dCounts, dStart, dTslot, dInstList, dCnt, dNewCnt, dExactLen;
+    DBuf<vmk::ExactItem> dExactQ;
     DBuf<uint8_t>        dFrozen;
     DBuf<vmk::Inst>      dInst;
     DBuf<vmk::InstState> dState;

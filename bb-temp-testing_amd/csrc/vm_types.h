@@ -80,6 +80,12 @@ struct TargetWork {
     float qx, qy, rho, pad;
 };
 
+// A target queued by the loss kernel for the exact k-NN selection: instance
+// (bit 31 set: only its k-NN list is wanted) and packed target index.
+struct ExactItem {
+    int inst, tg;
+};
+
 // Poisson dart thrower: one record per instance grid being (re)sampled.
 struct PoiInst {
     float    W, H, radius, cellSize;

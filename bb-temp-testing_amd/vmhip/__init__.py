@@ -446,9 +446,10 @@ class Batch:
         return dict(iteration=it.value, done=bool(done.value), last_mean_loss=loss.value)
 
     def loss_stats(self):
-        c = np.zeros(6, np.uint64)
+        c = np.zeros(8, np.uint64)
         _check(lib().vm_batch_loss_stats(self._b, _ptr(c)))
-        keys = ["fast_waves", "ring_staged_waves", "ring_global_waves", "idle_waves", "candidates", "staged_sites"]
+        keys = ["fast_waves", "ring_staged_waves", "ring_global_waves", "idle_waves", "candidates", "staged_sites",
+                "exact_fallback_waves", "exact_fallback_targets"]
         return {k: int(v) for k, v in zip(keys, c)}
 
     def choice(self, i):

@@ -193,7 +193,9 @@ VM_API int vm_batch_choice(vm_batch *b, int id, int *loss_k, float *grid_cell, i
 /* One evaluation pass of the loss kernel over every instance, counting its wavefronts by
  * path: [0] bounding-box fast path, [1] ring search over the staged block, [2] ring
  * search in global memory (staging overflow), [3] no active target; [4] candidates
- * compacted by the fast path, [5] sites staged. */
+ * compacted by the fast path, [5] sites staged; [6] fast-path wavefronts in which at
+ * least one target needed the exact k-NN selection after screening, [7] the number of
+ * such targets. */
 VM_API int vm_batch_loss_stats(vm_batch *b, uint64_t *counters);
 /* Mean of the kernel durations (ms) of the last vm_batch_step, measured with HIP events
  * around each kernel on the batch's stream; phase 0 scan, 1 scatter, 2 loss, 3 adam. */

@@ -72,12 +72,17 @@ def test_knn_query_bitexact_config0(K):
     assert np.array_equal(d2, rd2)
 
 
+@pytest.mark.parametrize("exact", [False, True], ids=["screened", "exact"])
 @pytest.mark.parametrize("cell_factor,rings,K,r", [(2.0, 2, 16, 0.025), (1.0, 1, 16, 0.025), (3.0, 1, 16, 0.025),
                                                    (0.5, 2, 16, 0.025), (0.0, 0, 16, 0.025), (0.0, 0, 12, 0.025),
-                                                   (0.0, 0, 8, 0.01), (0.0, 0, 24, 0.01), (0.0, 0, 16, 0.006)])
-def test_loss_kernel_terms_bitexact(cell_factor, rings, K, r):
+                                                   (0.0, 0, 8, 0.01), (0.0, 0, 24, 0.01), (0.0, 0, 16, 0.006),
+                                                   (0.0, 0, 2, 0.025), (0.0, 0, 32, 0.01)])
+def test_loss_kernel_terms_bitexact(cell_factor, rings, K, r, exact, monkeypatch):
     """k-NN inside the loss kernel, nearest site, partner and loss term, for several grid shapes
-    (cell_factor 0: the tuner's choice, which exercises the wavefront bounding-box path)."""
+    (cell_factor 0: the tuner's choice, which exercises the wavefront bounding-box path), through
+    the screened FP32 term and through the exact selection alone (VM_LOSS_EXACT=1)."""
+    if exact:
+        monkeypatch.setenv("VM_LOSS_EXACT", "1")
     grid, (gw, gh, cs), targets = _config0(r, n_targets=8192 if r > 0.01 else 65536)
     sites = wl.live_sites(grid)
     spacing = (1.0 / len(sites)) ** 0.5
@@ -94,6 +99,47 @@ def test_loss_kernel_terms_bitexact(cell_factor, rings, K, r):
     assert np.array_equal(d["loss_terms"], ref["loss_t"].astype(np.float64))
     assert d["count"] == ref["count"]
     assert d["loss_sum"] == pytest.approx(ref["loss_sum"], rel=1e-12)
+
+
+def test_loss_terms_coincident_sites():
+    """Sites placed exactly on top of others (zero-length bisector normals, skipped by the reference's
+    l < 1e-12 test) and a target exactly on a site: terms, partners and gradients stay bit-exact."""
+    r, K = 0.025, 16
+    grid, (gw, gh, cs), targets = _config0(r)
+    so = _slots(grid)
+    g = grid.copy()
+    for a, b in [(10, 11), (200, 201), (500, 520), (700, 650)]:
+        g[so[b]] = g[so[a]]
+    targets = targets.copy()
+    targets[:64] = g[so[300:364]]
+    sites = wl.live_sites(g)
+    opt = _optimizer(r, K)
+    opt.set_targets(targets)
+    opt.set_generators(g, gw, gh, cs)
+    d = opt.loss_pass(len(targets))
+    ref = O.loss_grad(sites, targets, K, W, H, 0.1, per_target=True)
+    assert np.array_equal(d["nearest"], _map(ref["c0"], so))
+    assert np.array_equal(d["partner"], _map(ref["bestj"], so))
+    assert np.array_equal(d["loss_terms"], ref["loss_t"].astype(np.float64))
+    assert np.array_equal(d["grad"][so], ref["grad"])
+
+
+def test_batch_screening_resolves_most_targets():
+    """On random-polygon fits the screened FP32 term settles nearly every target; the exact selection
+    runs for the rest (counted by the kernel), and the trajectories stay bit-exact (tested elsewhere)."""
+    b = vmhip.Batch()
+    n_t = 0
+    for i in range(6):
+        grid = O.poisson(W, H, 0.012 * (1 + 0.3 * i), 30, 40 + i)
+        sites = wl.live_sites(grid)
+        tg = wl.polygon(8 * len(sites), seed=60 + i)
+        n_t += len(tg)
+        b.add(W, H, sites, tg, 1e-3, loss_k=16)
+    b.commit()
+    st = b.loss_stats()
+    print(st)
+    assert st["fast_waves"] > 0.95 * (st["fast_waves"] + st["ring_staged_waves"] + st["ring_global_waves"])
+    assert st["exact_fallback_targets"] < 0.01 * n_t
 
 
 def test_knn_lattice_ties_bitexact():
