@@ -170,7 +170,7 @@ public:
         std::vector<vmk::Inst>       inst;
         std::vector<vmk::SiteWork>   siteWork;
         std::vector<vmk::TargetWork> targetWork; // grouped by K
-        struct Group { int K, begin, count; int64_t qOff; }; // qOff: the group's exact-selection queue
+        struct Group { int K, begin, count; int64_t qOff, qLen; }; // the group's exact-selection queue
         std::vector<Group>            groups;
         std::vector<int>              instList;
         std::vector<std::vector<int>> perm; // per instance: sorted position -> input target index
@@ -241,13 +241,14 @@ public:
         std::sort(ks.begin(), ks.end());
         ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
         for (int K : ks) {
-            typename Layout::Group g{K, int(L.targetWork.size()), 0, L.queue};
+            typename Layout::Group g{K, int(L.targetWork.size()), 0, L.queue, 0};
             for (int i = 0; i < n; ++i)
                 if (specs[i].K == K) {
                     L.targetWork.insert(L.targetWork.end(), twByInst[i].begin(), twByInst[i].end());
                     L.queue += L.inst[i].T; // every target of the group may be queued once per launch
                 }
             g.count = int(L.targetWork.size()) - g.begin;
+            g.qLen  = L.queue - g.qOff;
             L.groups.push_back(g);
         }
     }
@@ -322,6 +323,7 @@ public:
         const char *ex = std::getenv("VM_LOSS_EXACT");
         a.exactOnly    = ex && ex[0] == '1';
         a.exactQ = dExactQ.p + L.groups[g].qOff, a.exactLen = dExactLen.p + g;
+        a.exactGrid = vmk::exactGridFor(L.groups[g].qLen, sizeof(R) == 8 || a.exactOnly);
         return a;
     }
 

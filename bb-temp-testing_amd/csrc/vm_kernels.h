@@ -50,6 +50,7 @@ template <class R> struct LossArgs {
     int               exactOnly; // 1: every lane takes the exact selection (parity checks of that path)
     ExactItem        *exactQ;   // targets queued for the exact selection; capacity >= the group's targets
     int              *exactLen; // queue length (zeroed by the scan kernel)
+    int               exactGrid; // workgroups of the exact-selection launch (4 wavefronts each)
 };
 
 template <class R> struct AdamArgs {
@@ -97,6 +98,15 @@ hipError_t launchPoissonClear(int grid, hipStream_t s, const PoiInst *insts, con
 hipError_t launchPoissonPhase(int grid, hipStream_t s, const PoiInst *insts, const PoiWork *work, int nWork, float *g,
                               int px, int py, int round);
 hipError_t launchFillInt(hipStream_t s, int *p, int64_t n, int v);
+
+// Workgroups for the exact-selection launch of a group with `targets` targets:
+// about one wavefront per 512 targets, so the few the screen leaves open spread
+// widely; every target when all take the exact path (FP64, VM_LOSS_EXACT).
+inline int exactGridFor(int64_t targets, bool allExact)
+{
+    const int64_t g = allExact ? (targets + 3) / 4 : (targets + 2047) / 2048;
+    return int(g < 16 ? 16 : (g > 4096 ? 4096 : g));
+}
 
 inline bool lossKSupported(int K) { return K == 2 || K == 4 || K == 6 || K == 8 || K == 12 || K == 16 || K == 24 || K == 32; }
 

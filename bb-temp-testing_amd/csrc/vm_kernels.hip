@@ -209,6 +209,20 @@ __device__ void knnGlobal(const Inst &in, const V2<R> *spos, const int *sidx, co
 // 1. Scan: one workgroup per instance (also begins the step)
 // ---------------------------------------------------------------------------
 constexpr int kScanThreads = 1024;
+constexpr int kScanItems   = 16;
+
+// Inclusive prefix sum over the 64 lanes of a wavefront (DPP row shifts, then
+// row broadcasts); every lane must be active.
+__device__ __forceinline__ int waveInclusiveSum(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true); // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true); // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true); // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true); // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false); // row_bcast:15 into rows 1, 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false); // row_bcast:31 into rows 2, 3
+    return v;
+}
 
 __global__ __launch_bounds__(kScanThreads) void scanKernel(const Inst *__restrict__ insts, InstState *state,
                                                           LossPartial *partial, const int *__restrict__ instList,
@@ -258,39 +272,57 @@ __global__ __launch_bounds__(kScanThreads) void scanKernel(const Inst *__restric
             partial[int64_t(id) * kLossPartials + threadIdx.x].loss  = 0.0;
             partial[int64_t(id) * kLossPartials + threadIdx.x].count = 0;
         }
+        // Tiles of kScanThreads x kScanItems cells; each thread owns kScanItems
+        // consecutive cells (16-byte vector loads and stores: cell offsets are
+        // multiples of 64).
         const int n   = in.gw * in.gh + 1;
         int      *cnt = counts + in.cellOff;
         int      *st  = start + in.cellOff;
         int       running = 0;
-        for (int base = 0; base < n; base += kScanThreads * 4) {
-            int i0 = base + threadIdx.x * 4;
-            int v[4];
+        for (int base = 0; base < n; base += kScanThreads * kScanItems) {
+            const int  i0   = base + int(threadIdx.x) * kScanItems;
+            const bool full = i0 + kScanItems <= n;
+            int        v[kScanItems];
+            if (full) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) v[j] = (i0 + j < n) ? cnt[i0 + j] : 0;
-            int tot = v[0] + v[1] + v[2] + v[3];
-            int inc = tot;
+                for (int j = 0; j < kScanItems / 4; ++j) {
+                    const int4 q = reinterpret_cast<const int4 *>(cnt + i0)[j];
+                    v[4 * j] = q.x, v[4 * j + 1] = q.y, v[4 * j + 2] = q.z, v[4 * j + 3] = q.w;
+                }
+            } else {
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                int o = __shfl_up(inc, off, 64);
-                if (lane >= off) inc += o;
+                for (int j = 0; j < kScanItems; ++j) v[j] = (i0 + j < n) ? cnt[i0 + j] : 0;
             }
+            int tot = 0;
+#pragma unroll
+            for (int j = 0; j < kScanItems; ++j) tot += v[j];
+            const int inc = waveInclusiveSum(tot);
             if (lane == 63) waveSum[wave] = inc;
             __syncthreads();
             int wpre = 0, btot = 0;
 #pragma unroll
             for (int k = 0; k < kScanThreads / 64; ++k) {
-                int s = waveSum[k];
+                const int s = waveSum[k];
                 wpre += (k < wave) ? s : 0;
                 btot += s;
             }
             int ex = running + wpre + inc - tot;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (i0 + j < n) {
-                    st[i0 + j]  = ex;
-                    cnt[i0 + j] = 0;
+            for (int j = 0; j < kScanItems; ++j) {
+                const int c = v[j];
+                v[j]        = ex;
+                ex += c;
+            }
+            if (full) {
+#pragma unroll
+                for (int j = 0; j < kScanItems / 4; ++j) {
+                    reinterpret_cast<int4 *>(st + i0)[j]  = int4{v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]};
+                    reinterpret_cast<int4 *>(cnt + i0)[j] = int4{0, 0, 0, 0};
                 }
-                ex += v[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < kScanItems; ++j)
+                    if (i0 + j < n) st[i0 + j] = v[j], cnt[i0 + j] = 0;
             }
             running += btot;
             __syncthreads();
@@ -461,36 +493,6 @@ __device__ __forceinline__ void lossGrad(V2<R> u, V2<R> ev, R l, R d, GradT<R> *
     GradOps<R>::add(dj + 1, s * dcj.y, gscale);
     GradOps<R>::add(d0, -(s * (n.x + dcj.x)), gscale);
     GradOps<R>::add(d0 + 1, -(s * (n.y + dcj.y)), gscale);
-}
-
-// The reference's VoroLoss term (voro_optimizer.cu:225-242) for a target p
-// whose K nearest sites are in ascending (d2, index) order: c0 = the nearest;
-// among the others, the smallest d^2 (strict <, so ties go to the earlier
-// neighbour), skipping sites within 1e-12 of c0.
-template <int K, class R>
-__device__ __forceinline__ void exactTerm(const Key<R> (&keys)[K], const V2<R> *pos, V2<R> p, LaneTerm<R> &o)
-{
-    o = LaneTerm<R>{};
-    if (!keys[0].valid()) return;
-    o.i0 = keySite(keys[0].low());
-    o.c0 = pos[o.i0];
-    const V2<R> u{p.x - o.c0.x, p.y - o.c0.y};
-    R           best = sizeof(R) == 4 ? R(FLT_MAX) : R(DBL_MAX);
-    const R     tiny = sizeof(R) == 4 ? R(1e-12f) : R(1e-12);
-#pragma unroll
-    for (int k = 1; k < K; ++k) {
-        if (!keys[k].valid()) break;
-        const int   j  = keySite(keys[k].low());
-        const V2<R> cj = pos[j];
-        const V2<R> ev{cj.x - o.c0.x, cj.y - o.c0.y};
-        const R     l = sqrtR(dotR(ev, ev));
-        if (l < tiny) continue;
-        const R d = fmaR(R(-0.5), l, dotR(u, ev) / l);
-        if (d * d < best) {
-            best = d * d;
-            o.d = d, o.ev = ev, o.l = l, o.bj = j, o.used = 1;
-        }
-    }
 }
 
 template <class R> __device__ __forceinline__ R relMargin();
@@ -797,37 +799,170 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
 }
 
 // ---------------------------------------------------------------------------
-// 3b. Exact selection for the targets the loss kernel queued: one thread per
-//     target, k-NN by ring search over the instance's grid, the reference's
-//     term and its gradient (global fixed-point atomics, so the sums equal the
-//     staged ones).  The scan kernel empties the queue before each step.
+// 3b. Exact selection for the targets the loss kernel queued, one wavefront
+//     per target.  The wavefront gathers the sites of successive rings of cells
+//     around the target's cell into LDS and keeps the K smallest (d2, index)
+//     keys by rank counting; it stops with the same conservative rule as the
+//     per-thread ring search (knnGlobal), so the list is the exact one.  Lane k
+//     then evaluates the reference's term for neighbour k and the smallest
+//     (d^2, k) wins (voro_optimizer.cu:225-242).  Gradients go to global memory
+//     as fixed-point atomics, so the sums equal the staged ones.  The scan
+//     kernel empties the queue before each step.
 // ---------------------------------------------------------------------------
-constexpr int kExactThreads = 128;
-constexpr int kExactGrid    = 256;
+constexpr int kExactWaves      = 4;
+constexpr int kExactCand       = 256; // LDS keys per wavefront: the current K nearest + a batch of new sites
+constexpr int kExactFirstRings = 2;   // the first round gathers rings 0..2 (25 cells <= 64 lanes)
+
+template <class R> struct ExactLds {
+    Key<R> cand[kExactCand];
+    Key<R> top[32];
+    int    cb[64];   // a round of ring cells: first sorted entry of each
+    int    co[65];   // and the prefix of their site counts
+};
+
+template <class R> __device__ __forceinline__ R waveMinR(R v)
+{
+    return -waveMaxR<R>(-v);
+}
+
+// Sorts the K smallest of the distinct keys cand[0, nc) into cand[0, min(K, nc)).
+template <int K, class R> __device__ __forceinline__ int keepNearest(ExactLds<R> &lds, int nc, int lane)
+{
+    for (int j = lane; j < nc; j += 64) {
+        const Key<R> kj   = lds.cand[j];
+        int          rank = 0;
+        for (int i = 0; i < nc; ++i) rank += lds.cand[i].lt(kj) ? 1 : 0;
+        if (rank < K) lds.top[rank] = kj;
+    }
+    waveSync();
+    const int nk = min(nc, K);
+    if (lane < nk) lds.cand[lane] = lds.top[lane];
+    waveSync();
+    return nk;
+}
+
+// Cell q (0 <= q < 8r) of the ring at Chebyshev distance r >= 1 around (cx, cy).
+__device__ __forceinline__ void ringCell(int r, int q, int cx, int cy, int &x, int &y)
+{
+    const int side = 2 * r + 1;
+    if (q < side) x = cx - r + q, y = cy - r;
+    else if (q < 2 * side) x = cx - r + (q - side), y = cy + r;
+    else if (q < 2 * side + side - 2) x = cx - r, y = cy - r + 1 + (q - 2 * side);
+    else x = cx + r, y = cy - r + 1 + (q - 3 * side + 2);
+}
 
 template <int K, class R>
-__global__ __launch_bounds__(kExactThreads) void exactKernel(LossArgs<R> a)
+__global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
 {
-    const int n = *a.exactLen;
-    // Entry e goes to workgroup e % grid, so a short queue spreads over the chip.
-    for (int e = threadIdx.x * gridDim.x + blockIdx.x; e < n; e += gridDim.x * kExactThreads) {
+    __shared__ ExactLds<R> ldsAll[kExactWaves];
+    ExactLds<R> &lds  = ldsAll[threadIdx.x >> 6];
+    const int    lane = threadIdx.x & 63;
+    const int    n    = *a.exactLen;
+    const int    nw   = gridDim.x * kExactWaves;
+    // Entry e goes to wavefront e % nw, numbered across workgroups first, so a
+    // short queue spreads over the chip.
+    for (int e = blockIdx.x + gridDim.x * int(threadIdx.x >> 6); e < n; e += nw) {
         const ExactItem it       = a.exactQ[e];
         const bool      listOnly = it.inst < 0;
         const int       inst     = it.inst & 0x7FFFFFFF;
         const Inst     &in       = a.insts[inst];
         const int64_t   so       = in.siteOff;
+        const V2<R>    *spos     = a.spos + so;
+        const int      *sidx     = a.sidx + so;
+        const int      *strt     = a.start + in.cellOff;
         const V2<R>     p        = a.tpos[it.tg];
-        Key<R>          keys[K];
-        knnGlobal<K>(in, a.spos + so, a.sidx + so, a.start + in.cellOff, p, keys);
-        if (a.dbgKnn) {
+        const int       gw = in.gw, gh = in.gh;
+        const R         cell   = R(in.cell);
+        const R         inv    = R(1) / cell;
+        const R         rel    = ringRel<R>();
+        const R         margin = rel * fmax(R(in.ext), fmax(fabs(p.x), fabs(p.y)));
+        const int       cx = cellCoord(p.x, inv, gw), cy = cellCoord(p.y, inv, gh);
+
+        // ---- K nearest by rings: first the 5 x 5 block (rings 0-2) in one
+        //      round, then ring by ring in rounds of 64 cells ---------------------
+        int nc = 0;
+        for (int r = kExactFirstRings;; ++r) {
+            const int first     = r == kExactFirstRings;
+            const int roundCells = first ? (2 * r + 1) * (2 * r + 1) : 8 * r;
+            for (int q0 = 0; q0 < roundCells; q0 += 64) {
+                int b = 0, cnt = 0;
+                if (q0 + lane < roundCells) {
+                    int x, y;
+                    if (first) x = cx - r + lane % (2 * r + 1), y = cy - r + lane / (2 * r + 1);
+                    else ringCell(r, q0 + lane, cx, cy, x, y);
+                    if (x >= 0 && x < gw && y >= 0 && y < gh) {
+                        b   = strt[y * gw + x];
+                        cnt = strt[y * gw + x + 1] - b;
+                    }
+                }
+                const int incl = waveInclusiveSum(cnt);
+                lds.cb[lane]     = b;
+                lds.co[lane + 1] = incl;
+                if (lane == 0) lds.co[0] = 0;
+                const int total = __builtin_amdgcn_readlane(incl, 63);
+                waveSync();
+                for (int s0 = 0; s0 < total;) {
+                    const int slice = min(total - s0, kExactCand - nc);
+                    for (int s = lane; s < slice; s += 64) {
+                        // Cell of entry s0 + s: the last c with co[c] <= s0 + s.
+                        const int g  = s0 + s;
+                        int       lo = 0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) a.dbgKnn[int64_t(it.tg) * K + k] = keys[k].valid() ? keySite(keys[k].low()) : -1;
+                        for (int step = 32; step > 0; step >>= 1)
+                            if (lds.co[lo + step] <= g) lo += step;
+                        const int site = lds.cb[lo] + (g - lds.co[lo]);
+                        lds.cand[nc + s] = Key<R>::make(dist2(spos[site], p),
+                                                        (uint32_t(sidx[site]) << kLdsSlotBits) | uint32_t(kLdsNone));
+                    }
+                    waveSync();
+                    nc += slice;
+                    s0 += slice;
+                    if (nc == kExactCand) nc = keepNearest<K>(lds, nc, lane);
+                }
+            }
+            nc = keepNearest<K>(lds, nc, lane);
+            const int x0 = cx - r, x1 = cx + r, y0 = cy - r, y1 = cy + r;
+            if (x0 <= 0 && y0 <= 0 && x1 >= gw - 1 && y1 >= gh - 1) break;
+            if (nc == K) {
+                R safe = fmin(fmin(p.x - R(x0) * cell, R(x1 + 1) * cell - p.x),
+                              fmin(p.y - R(y0) * cell, R(y1 + 1) * cell - p.y));
+                safe   = fmax(safe - margin, R(0));
+                if (lds.cand[K - 1].d2() * (R(1) + rel) < safe * safe) break;
+            }
         }
-        if (listOnly) continue;
+        if (a.dbgKnn && lane < K)
+            a.dbgKnn[int64_t(it.tg) * K + lane] = lane < nc ? keySite(lds.cand[lane].low()) : -1;
+        if (listOnly) {
+            waveSync();
+            continue;
+        }
+
+        // ---- the term: lane k evaluates neighbour k ------------------------
         LaneTerm<R> tr;
-        exactTerm<K, R>(keys, a.pos + so, p, tr);
-        const R lossT = tr.used ? tr.d * tr.d : R(0);
-        if (tr.used) {
+        bool        ok = false;
+        R           dd = rInf<R>();
+        if (nc > 0) {
+            tr.i0 = keySite(lds.cand[0].low());
+            tr.c0 = a.pos[so + tr.i0];
+            if (lane >= 1 && lane < nc) {
+                const int   j  = keySite(lds.cand[lane].low());
+                const V2<R> cj = a.pos[so + j];
+                const V2<R> ev{cj.x - tr.c0.x, cj.y - tr.c0.y};
+                const R     l = sqrtR(dotR(ev, ev));
+                if (!(l < (sizeof(R) == 4 ? R(1e-12f) : R(1e-12)))) {
+                    tr.d = fmaR(R(-0.5), l, dotR(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, ev) / l);
+                    tr.ev = ev, tr.l = l, tr.bj = j;
+                    dd    = tr.d * tr.d;
+                    ok    = true;
+                }
+            }
+        }
+        waveSync();
+        const R        best = waveMinR<R>(dd);
+        const uint64_t win  = __ballot(ok && dd == best);
+        const int      wl   = win ? __builtin_ctzll(win) : -1;
+        const R        lossT = win ? best : R(0);
+        if (lane == wl) {
             if (a.grad)
                 lossGrad(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, tr.ev, tr.l, tr.d, &a.grad[2 * (so + tr.bj)],
                          &a.grad[2 * (so + tr.i0)], __builtin_ldexp(1.0, in.gradShift));
@@ -841,14 +976,13 @@ __global__ __launch_bounds__(kExactThreads) void exactKernel(LossArgs<R> a)
                 atomicAdd((unsigned long long *)&st->evalCount, 1ull);
             }
         }
-        if (a.dbgTerm) {
-            a.dbgTerm[it.tg].i0   = tr.used ? tr.i0 : -1;
-            a.dbgTerm[it.tg].j    = tr.used ? tr.bj : -1;
+        if (a.dbgTerm && lane == (win ? wl : 0)) {
+            a.dbgTerm[it.tg].i0   = win ? tr.i0 : -1;
+            a.dbgTerm[it.tg].j    = win ? tr.bj : -1;
             a.dbgTerm[it.tg].loss = double(lossT);
         }
     }
 }
-
 // ---------------------------------------------------------------------------
 // 4. Adam + MultiStepLR + clamp (voro_optimizer.cu:268-301), then bin the new
 //    position for the next step (voro_optimizer.cu:168-177).
@@ -1168,7 +1302,7 @@ template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const L
 #define VM_K(k)                                                                                                   \
     case k:                                                                                                        \
         hipLaunchKernelGGL((lossKernel<k, R>), dim3((grid + kLossWaves - 1) / kLossWaves), dim3(kLossThreads), 0, s, a); \
-        hipLaunchKernelGGL((exactKernel<k, R>), dim3(kExactGrid), dim3(kExactThreads), 0, s, a);                         \
+        hipLaunchKernelGGL((exactKernel<k, R>), dim3(a.exactGrid), dim3(64 * kExactWaves), 0, s, a);                  \
         break;
         VM_K(2) VM_K(4) VM_K(6) VM_K(8) VM_K(12) VM_K(16) VM_K(24) VM_K(32)
 #undef VM_K

@@ -334,6 +334,7 @@ private:
             la.exactQ = dExactQ.p, la.exactLen = dExactLen.p;
             const char *ex = std::getenv("VM_LOSS_EXACT");
             la.exactOnly   = ex && ex[0] == '1';
+            la.exactGrid   = vmk::exactGridFor(capTargets_, sizeof(R) == 8 || la.exactOnly);
             VM_HIP(vmk::launchLoss<R>(K, gLoss, stream_, la));
             VM_HIP(vmk::launchAdamBin<R>(gSite, stream_, adamArgs(1)));
         } catch (...) {
