@@ -172,14 +172,18 @@ public:
         std::vector<vmk::TargetWork> targetWork; // grouped by K
         struct Group { int K, begin, count; int64_t qOff, qLen; }; // the group's exact-selection queue
         std::vector<Group>            groups;
-        std::vector<int>              instList;
+        std::vector<vmk::ScanTile>    tiles; // scan tiles, instance by instance
         std::vector<std::vector<int>> perm; // per instance: sorted position -> input target index
         int64_t sites = 0, cells = 0, targets = 0, queue = 0;
+        int     dynTiles = 0;
     } L;
 
     DBuf<V>              dPos, dM, dV, dSpos, dTpos;
     DBuf<vmk::GradT<R>>  dGrad; // two per site
-    DBuf<int>            dCellOf, dRank, dSidx, dCounts, dStart, dTslot, dInstList, dCnt;
+    DBuf<int>            dCellOf, dRank, dSidx, dCounts, dStart, dTslot, dCnt;
+    DBuf<vmk::ScanTile>  dTiles;
+    DBuf<vmk::ScanCtl>   dScanCtl;
+    DBuf<unsigned long long> dFlags; // one look-back flag per scan tile
     DBuf<uint8_t>        dFrozen;
     DBuf<vmk::Inst>      dInst;
     DBuf<vmk::InstState> dState;
@@ -211,7 +215,8 @@ public:
     {
         size_t b = 0;
         b += (dPos.n + dM.n + dV.n + dSpos.n + dTpos.n) * sizeof(V) + dGrad.n * sizeof(vmk::GradT<R>);
-        b += (dCellOf.n + dRank.n + dSidx.n + dCounts.n + dStart.n + dTslot.n + dInstList.n + dCnt.n) * sizeof(int);
+        b += (dCellOf.n + dRank.n + dSidx.n + dCounts.n + dStart.n + dTslot.n + dCnt.n) * sizeof(int);
+        b += dTiles.n * sizeof(vmk::ScanTile) + dScanCtl.n * sizeof(vmk::ScanCtl) + dFlags.n * sizeof(unsigned long long);
         b += dFrozen.n + dInst.n * sizeof(vmk::Inst) + dState.n * sizeof(vmk::InstState) +
              dPartial.n * sizeof(vmk::LossPartial);
         b += dSiteWork.n * sizeof(vmk::SiteWork) + dTargetWork.n * sizeof(vmk::TargetWork);
@@ -233,8 +238,21 @@ public:
             L.sites += align64(in.S);
             L.cells += align64(int64_t(in.gw) * in.gh + 1);
             L.targets += align64(in.T);
-            L.instList.push_back(i);
+
         }
+        // Scan tiles: the tiles of multi-tile instances first (look-back), then
+        // the single-tile instances.
+        for (int pass = 0; pass < 2; ++pass)
+            for (int i = 0; i < n; ++i) {
+                const int64_t cells = int64_t(L.inst[i].gw) * L.inst[i].gh + 1;
+                const int     nt    = vmk::scanTilesFor(cells);
+                if ((nt > 1) != (pass == 0)) continue;
+                for (int k = 0; k < nt; ++k) {
+                    const int b = k * vmk::kScanTileCells;
+                    L.tiles.push_back({i, b, int(std::min<int64_t>(vmk::kScanTileCells, cells - b)), k == 0});
+                }
+                if (pass == 0) L.dynTiles += nt;
+            }
         // Group target work by K (one loss launch per distinct K).
         std::vector<int> ks;
         for (auto &s : specs) ks.push_back(s.K);
@@ -263,7 +281,8 @@ public:
         dFrozen.reserve(L.sites);
         dCounts.reserve(L.cells), dStart.reserve(L.cells);
         dTpos.reserve(L.targets), dTslot.reserve(L.targets);
-        dInst.reserve(n), dState.reserve(n), dInstList.reserve(n), dPartial.reserve(size_t(n) * vmk::kLossPartials);
+        dInst.reserve(n), dState.reserve(n), dPartial.reserve(size_t(n) * vmk::kLossPartials);
+        dTiles.reserve(L.tiles.size()), dScanCtl.reserve(1), dFlags.reserve(L.tiles.size());
         dSiteWork.reserve(L.siteWork.size()), dTargetWork.reserve(L.targetWork.size());
         dCnt.reserve(2 + L.groups.size());
         dExactQ.reserve(std::max<int64_t>(L.queue, 1)), dExactLen.reserve(L.groups.size());
@@ -297,12 +316,15 @@ public:
                 VM_HIP(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice, stream));
         };
         up(dPos.p, hp), up(dM.p, hm), up(dV.p, hv), up(dTpos.p, ht), up(dTslot.p, hts), up(dFrozen.p, hf);
-        up(dInst.p, L.inst), up(dState.p, hs), up(dInstList.p, L.instList), up(dSiteWork.p, L.siteWork);
+        up(dInst.p, L.inst), up(dState.p, hs), up(dTiles.p, L.tiles), up(dSiteWork.p, L.siteWork);
+        const std::vector<vmk::ScanCtl> ctl{{0, 0u, int(L.tiles.size()), L.dynTiles}};
+        up(dScanCtl.p, ctl);
+        VM_HIP(hipMemsetAsync(dFlags.p, 0, dFlags.n * sizeof(unsigned long long), stream));
         up(dTargetWork.p, L.targetWork), up(dCnt.p, cnt);
         VM_HIP(hipMemsetAsync(dGrad.p, 0, dGrad.n * sizeof(vmk::GradT<R>), stream));
         VM_HIP(hipMemsetAsync(dPartial.p, 0, dPartial.n * sizeof(vmk::LossPartial), stream));
         VM_HIP(vmk::launchFillInt(stream, dCounts.p, int64_t(dCounts.n), 0));
-        gridScan = std::max(1, std::min(n, kMaxScanGrid));
+        gridScan = std::max(1, std::min(int(L.tiles.size()), kMaxScanGrid));
         gridSite = std::max(1, std::min(int(L.siteWork.size()), kMaxSiteGrid));
         gridLoss.clear();
         for (auto &g : L.groups) gridLoss.push_back(std::max(1, std::min(g.count, kMaxLossGrid)));
@@ -340,20 +362,20 @@ public:
 
     void gridBuild(int mode)
     {
-        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
-                               mode, dExactLen.p, int(L.groups.size())));
+        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dTiles.p, dScanCtl.p, dFlags.p,
+                               dCounts.p, dStart.p, mode, dExactLen.p, int(L.groups.size())));
         VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p, dCellOf.p,
-                                     dRank.p, dStart.p, dSpos.p, dSidx.p, mode));
+                                     dRank.p, dStart.p, dSpos.p, dSidx.p, mode, dScanCtl.p));
     }
 
     void enqueueIteration(hipEvent_t *marks)
     {
         if (marks) VM_HIP(hipEventRecord(marks[0], stream));
-        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
-                               vmk::kModeIterate, dExactLen.p, int(L.groups.size())));
+        VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dTiles.p, dScanCtl.p, dFlags.p,
+                               dCounts.p, dStart.p, vmk::kModeIterate, dExactLen.p, int(L.groups.size())));
         if (marks) VM_HIP(hipEventRecord(marks[1], stream));
         VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p, dCellOf.p,
-                                     dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate));
+                                     dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate, dScanCtl.p));
         if (marks) VM_HIP(hipEventRecord(marks[2], stream));
         for (size_t g = 0; g < L.groups.size(); ++g)
             VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, lossArgs(int(g), vmk::kModeIterate, true)));

@@ -81,13 +81,17 @@ template <class R> struct ClipArgs {
     int            *flags;   // {unsafe, overflow}
 };
 
+// Cells per scan tile, and the tiles of an instance with `cells` cells (gw*gh+1).
+constexpr int kScanTileCells = 16384;
+inline int scanTilesFor(int64_t cells) { return int((cells + kScanTileCells - 1) / kScanTileCells); }
+
 hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, LossPartial *partial,
-                      const int *instList, const int *nInst, int *counts, int *start, int mode, int *exactLen,
-                      int nQueues);
+                      const ScanTile *tiles, ScanCtl *ctl, unsigned long long *flags, int *counts, int *start,
+                      int mode, int *exactLen, int nQueues);
 template <class R>
 hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstState *state, const SiteWork *work,
                          const int *nWork, const void *pos, const int *cellOf, const int *rank, const int *start,
-                         void *spos, int *sidx, int mode);
+                         void *spos, int *sidx, int mode, ScanCtl *ctl);
 template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const LossArgs<R> &a);
 template <class R> hipError_t launchAdamBin(int grid, hipStream_t s, const AdamArgs<R> &a);
 template <class R>

@@ -206,10 +206,20 @@ __device__ void knnGlobal(const Inst &in, const V2<R> *spos, const int *sidx, co
 }
 
 // ---------------------------------------------------------------------------
-// 1. Scan: one workgroup per instance (also begins the step)
+// 1. Scan: exclusive scan of the per-cell site counts into cell starts (and
+//    zeroes the counts), over tiles of kScanTile cells.  Single pass with
+//    decoupled look-back: a workgroup takes the next tile from a counter (so
+//    every tile it waits on belongs to a workgroup that is already running),
+//    publishes the tile's sum, then walks back over the earlier tiles of the
+//    same instance until one with an inclusive prefix.  The first tile of an
+//    instance begins its step (iter += 1, clears the loss sums, convergence
+//    test); a finished instance's tiles publish "skip" and leave its grid alone.
+//    Flag words carry an epoch that the scatter kernel advances after every
+//    scan, so nothing needs clearing between launches.
 // ---------------------------------------------------------------------------
 constexpr int kScanThreads = 1024;
 constexpr int kScanItems   = 16;
+static_assert(kScanThreads * kScanItems == kScanTileCells, "one scan tile per workgroup pass");
 
 // Inclusive prefix sum over the 64 lanes of a wavefront (DPP row shifts, then
 // row broadcasts); every lane must be active.
@@ -224,111 +234,171 @@ __device__ __forceinline__ int waveInclusiveSum(int v)
     return v;
 }
 
+// Look-back flag: epoch (30 bits) | status (2 bits) | value (32 bits).
+constexpr uint32_t kTileAggregate = 1, kTilePrefix = 2, kTileSkip = 3;
+
+__device__ __forceinline__ void publishTile(unsigned long long *f, uint32_t epoch, uint32_t status, int value)
+{
+    const unsigned long long w = (uint64_t(epoch & 0x3FFFFFFFu) << 34) | (uint64_t(status) << 32) | uint32_t(value);
+    __hip_atomic_store(f, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Begins the step of instance `id` (first tile); returns 1 when it is skipped.
+__device__ __forceinline__ int beginStep(const Inst &in, InstState *s, int mode)
+{
+    int d = s->done;
+    if (mode == kModeIterate) {
+        if (!d) {
+            if (s->iter > 0 && s->count > 0 && in.tol > 0.0f) {
+                double mean = s->loss / double(s->count);
+                if (s->iter > 1 && fabs(mean - s->prevMean) < double(in.tol) * fabs(s->prevMean)) d = 1;
+                s->prevMean = mean;
+            }
+            if (in.maxIter > 0 && s->iter >= in.maxIter) d = 1;
+            if (!d) {
+                s->iter += 1;
+                s->count = 0;
+                s->loss  = 0.0;
+            }
+            s->done = d;
+        }
+        return d;
+    }
+    s->evalLoss  = 0.0;
+    s->evalCount = 0;
+    return 0;
+}
+
+// One scan tile: exclusive scan of counts into starts (and zero the counts).
+// Tiles of multi-tile instances find their prefix by look-back (`dyn`); a
+// single-tile instance's tile starts at zero and never waits.
+__device__ __forceinline__ void scanTile(const ScanTile &tl, int tile, bool dyn, uint32_t epoch, const Inst *insts,
+                                         InstState *state, LossPartial *partial, unsigned long long *flags,
+                                         int *counts, int *start, int mode, int *waveSum, int *sShared)
+{
+    const int   lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const Inst &in   = insts[tl.inst];
+    int        *cnt  = counts + in.cellOff + tl.begin;
+    int        *st   = start + in.cellOff + tl.begin;
+    const int   n    = tl.count;
+
+    // Each thread owns kScanItems consecutive cells (16-byte vector loads and
+    // stores: cell offsets and tile starts are multiples of 64).
+    const int  i0   = int(threadIdx.x) * kScanItems;
+    const bool full = i0 + kScanItems <= n;
+    int        v[kScanItems];
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < kScanItems / 4; ++j) {
+            const int4 q = reinterpret_cast<const int4 *>(cnt + i0)[j];
+            v[4 * j] = q.x, v[4 * j + 1] = q.y, v[4 * j + 2] = q.z, v[4 * j + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) v[j] = (i0 + j < n) ? cnt[i0 + j] : 0;
+    }
+    int tot = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) tot += v[j];
+    const int inc = waveInclusiveSum(tot);
+    if (lane == 63) waveSum[wave] = inc;
+    __syncthreads();
+    int wpre = 0, btot = 0;
+#pragma unroll
+    for (int k = 0; k < kScanThreads / 64; ++k) {
+        const int s = waveSum[k];
+        wpre += (k < wave) ? s : 0;
+        btot += s;
+    }
+
+    // ---- tile prefix (look-back over the instance's earlier tiles) ---------
+    if (threadIdx.x == 0) {
+        int skip = 0, prefix = 0;
+        if (tl.first) {
+            skip = beginStep(in, state + tl.inst, mode);
+            if (dyn) publishTile(&flags[tile], epoch, skip ? kTileSkip : kTilePrefix, btot);
+        } else {
+            publishTile(&flags[tile], epoch, kTileAggregate, btot);
+            for (int j = tile - 1;; --j) {
+                unsigned long long f;
+                do {
+                    f = __hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } while (uint32_t(f >> 34) != (epoch & 0x3FFFFFFFu) || ((f >> 32) & 3u) == 0);
+                const uint32_t status = uint32_t(f >> 32) & 3u;
+                if (status == kTileSkip) {
+                    skip = 1;
+                    break;
+                }
+                prefix += int(uint32_t(f));
+                if (status == kTilePrefix) break;
+            }
+            publishTile(&flags[tile], epoch, skip ? kTileSkip : kTilePrefix, prefix + btot);
+        }
+        sShared[0] = skip, sShared[1] = prefix;
+    }
+    __syncthreads();
+    if (!sShared[0]) {
+        if (tl.first && mode == kModeIterate && threadIdx.x < kLossPartials) {
+            partial[int64_t(tl.inst) * kLossPartials + threadIdx.x].loss  = 0.0;
+            partial[int64_t(tl.inst) * kLossPartials + threadIdx.x].count = 0;
+        }
+        int ex = sShared[1] + wpre + inc - tot;
+#pragma unroll
+        for (int j = 0; j < kScanItems; ++j) {
+            const int c = v[j];
+            v[j]        = ex;
+            ex += c;
+        }
+        if (full) {
+#pragma unroll
+            for (int j = 0; j < kScanItems / 4; ++j) {
+                reinterpret_cast<int4 *>(st + i0)[j]  = int4{v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]};
+                reinterpret_cast<int4 *>(cnt + i0)[j] = int4{0, 0, 0, 0};
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kScanItems; ++j)
+                if (i0 + j < n) st[i0 + j] = v[j], cnt[i0 + j] = 0;
+        }
+    }
+    __syncthreads();
+}
+
+// Tiles [0, nDyn) belong to multi-tile instances and are taken in order from
+// the counter (a workgroup only ever waits on tiles that running workgroups
+// took before it); tiles [nDyn, nTiles) are single-tile instances, spread
+// statically.  The scatter kernel, which follows every scan, rearms the
+// counter and advances the flag epoch.
 __global__ __launch_bounds__(kScanThreads) void scanKernel(const Inst *__restrict__ insts, InstState *state,
-                                                          LossPartial *partial, const int *__restrict__ instList,
-                                                          const int *nInstPtr, int *counts, int *start, int mode,
-                                                          int *exactLen, int nQueues)
+                                                          LossPartial *partial, const ScanTile *__restrict__ tiles,
+                                                          ScanCtl *ctl, unsigned long long *flags, int *counts,
+                                                          int *start, int mode, int *exactLen, int nQueues)
 {
     __shared__ int waveSum[kScanThreads / 64];
-    __shared__ int skip;
-    const int nInst = *nInstPtr;
+    __shared__ int sShared[3];
     // Empty the exact-selection queues the following loss launches fill.
     if (blockIdx.x == 0 && int(threadIdx.x) < nQueues) exactLen[threadIdx.x] = 0;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int w = blockIdx.x; w < nInst; w += gridDim.x) {
-        const int   id = instList[w];
-        const Inst &in = insts[id];
-        if (threadIdx.x == 0) {
-            InstState *s = state + id;
-            int        d = s->done;
-            if (mode == kModeIterate) {
-                if (!d) {
-                    if (s->iter > 0 && s->count > 0 && in.tol > 0.0f) {
-                        double mean = s->loss / double(s->count);
-                        if (s->iter > 1 && fabs(mean - s->prevMean) < double(in.tol) * fabs(s->prevMean)) d = 1;
-                        s->prevMean = mean;
-                    }
-                    if (in.maxIter > 0 && s->iter >= in.maxIter) d = 1;
-                    if (!d) {
-                        s->iter += 1;
-                        s->count = 0;
-                        s->loss  = 0.0;
-                    }
-                    s->done = d;
-                }
-            } else {
-                s->evalLoss  = 0.0;
-                s->evalCount = 0;
-                d            = 0;
+    const int      nTiles = ctl->nTiles, nDyn = ctl->nDyn;
+    const uint32_t epoch  = ctl->epoch;
+    if (nDyn > 0) {
+        for (;;) {
+            if (threadIdx.x == 0) {
+                int t = nDyn;
+                if (__hip_atomic_load(&ctl->next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nDyn)
+                    t = atomicAdd(&ctl->next, 1);
+                sShared[2] = t;
             }
-            skip = d;
-        }
-        __syncthreads();
-        if (skip) {
             __syncthreads();
-            continue;
-        }
-        if (mode == kModeIterate && threadIdx.x < kLossPartials) {
-            partial[int64_t(id) * kLossPartials + threadIdx.x].loss  = 0.0;
-            partial[int64_t(id) * kLossPartials + threadIdx.x].count = 0;
-        }
-        // Tiles of kScanThreads x kScanItems cells; each thread owns kScanItems
-        // consecutive cells (16-byte vector loads and stores: cell offsets are
-        // multiples of 64).
-        const int n   = in.gw * in.gh + 1;
-        int      *cnt = counts + in.cellOff;
-        int      *st  = start + in.cellOff;
-        int       running = 0;
-        for (int base = 0; base < n; base += kScanThreads * kScanItems) {
-            const int  i0   = base + int(threadIdx.x) * kScanItems;
-            const bool full = i0 + kScanItems <= n;
-            int        v[kScanItems];
-            if (full) {
-#pragma unroll
-                for (int j = 0; j < kScanItems / 4; ++j) {
-                    const int4 q = reinterpret_cast<const int4 *>(cnt + i0)[j];
-                    v[4 * j] = q.x, v[4 * j + 1] = q.y, v[4 * j + 2] = q.z, v[4 * j + 3] = q.w;
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < kScanItems; ++j) v[j] = (i0 + j < n) ? cnt[i0 + j] : 0;
-            }
-            int tot = 0;
-#pragma unroll
-            for (int j = 0; j < kScanItems; ++j) tot += v[j];
-            const int inc = waveInclusiveSum(tot);
-            if (lane == 63) waveSum[wave] = inc;
+            const int tile = sShared[2];
             __syncthreads();
-            int wpre = 0, btot = 0;
-#pragma unroll
-            for (int k = 0; k < kScanThreads / 64; ++k) {
-                const int s = waveSum[k];
-                wpre += (k < wave) ? s : 0;
-                btot += s;
-            }
-            int ex = running + wpre + inc - tot;
-#pragma unroll
-            for (int j = 0; j < kScanItems; ++j) {
-                const int c = v[j];
-                v[j]        = ex;
-                ex += c;
-            }
-            if (full) {
-#pragma unroll
-                for (int j = 0; j < kScanItems / 4; ++j) {
-                    reinterpret_cast<int4 *>(st + i0)[j]  = int4{v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]};
-                    reinterpret_cast<int4 *>(cnt + i0)[j] = int4{0, 0, 0, 0};
-                }
-            } else {
-#pragma unroll
-                for (int j = 0; j < kScanItems; ++j)
-                    if (i0 + j < n) st[i0 + j] = v[j], cnt[i0 + j] = 0;
-            }
-            running += btot;
-            __syncthreads();
+            if (tile >= nDyn) break;
+            scanTile(tiles[tile], tile, true, epoch, insts, state, partial, flags, counts, start, mode, waveSum,
+                     sShared);
         }
-        __syncthreads();
     }
+    for (int tile = nDyn + blockIdx.x; tile < nTiles; tile += gridDim.x)
+        scanTile(tiles[tile], tile, false, epoch, insts, state, partial, flags, counts, start, mode, waveSum,
+                 sShared);
 }
 
 // ---------------------------------------------------------------------------
@@ -342,8 +412,14 @@ __global__ __launch_bounds__(kSiteChunk) void scatterKernel(const Inst *__restri
                                                             const int *__restrict__ cellOf,
                                                             const int *__restrict__ rank,
                                                             const int *__restrict__ start, V2<R> *spos, int *sidx,
-                                                            int mode)
+                                                            int mode, ScanCtl *ctl)
 {
+    // The scan before this launch is complete: rearm its tile counter and move
+    // its look-back flags to a new epoch.
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->next  = 0;
+        ctl->epoch = ctl->epoch + 1;
+    }
     const int nWork = *nWorkPtr;
     for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
         const SiteWork wk = work[w];
@@ -1277,11 +1353,11 @@ __global__ void fillIntKernel(int *p, int64_t n, int v)
 // Host-side launchers
 // ===========================================================================
 hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, LossPartial *partial,
-                      const int *instList, const int *nInst, int *counts, int *start, int mode, int *exactLen,
-                      int nQueues)
+                      const ScanTile *tiles, ScanCtl *ctl, unsigned long long *flags, int *counts, int *start,
+                      int mode, int *exactLen, int nQueues)
 {
     if (nQueues > kScanThreads) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(scanKernel, dim3(grid), dim3(kScanThreads), 0, s, insts, state, partial, instList, nInst,
+    hipLaunchKernelGGL(scanKernel, dim3(grid), dim3(kScanThreads), 0, s, insts, state, partial, tiles, ctl, flags,
                        counts, start, mode, exactLen, nQueues);
     return hipGetLastError();
 }
@@ -1289,10 +1365,11 @@ hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *sta
 template <class R>
 hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstState *state, const SiteWork *work,
                          const int *nWork, const void *pos, const int *cellOf, const int *rank, const int *start,
-                         void *spos, int *sidx, int mode)
+                         void *spos, int *sidx, int mode, ScanCtl *ctl)
 {
     hipLaunchKernelGGL(scatterKernel<R>, dim3(grid), dim3(kSiteChunk), 0, s, insts, state, work, nWork,
-                       static_cast<const V2<R> *>(pos), cellOf, rank, start, static_cast<V2<R> *>(spos), sidx, mode);
+                       static_cast<const V2<R> *>(pos), cellOf, rank, start, static_cast<V2<R> *>(spos), sidx, mode,
+                       ctl);
     return hipGetLastError();
 }
 
@@ -1365,10 +1442,10 @@ hipError_t launchFillInt(hipStream_t s, int *p, int64_t n, int v)
 
 template hipError_t launchScatter<float>(int, hipStream_t, const Inst *, const InstState *, const SiteWork *,
                                          const int *, const void *, const int *, const int *, const int *, void *,
-                                         int *, int);
+                                         int *, int, ScanCtl *);
 template hipError_t launchScatter<double>(int, hipStream_t, const Inst *, const InstState *, const SiteWork *,
                                           const int *, const void *, const int *, const int *, const int *, void *,
-                                          int *, int);
+                                          int *, int, ScanCtl *);
 template hipError_t launchLoss<float>(int, int, hipStream_t, const LossArgs<float> &);
 template hipError_t launchLoss<double>(int, int, hipStream_t, const LossArgs<double> &);
 template hipError_t launchAdamBin<float>(int, hipStream_t, const AdamArgs<float> &);

@@ -121,7 +121,9 @@ public:
         dFrozen.reserve(1);
         dTpos.reserve(capTargets_), dTslot.reserve(capTargets_);
         dCounts.reserve(capCells_), dStart.reserve(capCells_);
-        dInst.reserve(slots_), dState.reserve(slots_), dInstList.reserve(slots_);
+        dInst.reserve(slots_), dState.reserve(slots_);
+        maxTiles_ = int(capCells_ / vmk::kScanTileCells) + slots_ + 1;
+        dTiles.reserve(maxTiles_), dScanCtl.reserve(1), dFlags.reserve(maxTiles_);
         dPartial.reserve(size_t(slots_) * vmk::kLossPartials);
         dSiteWork.reserve(capSites_ / vmk::kSiteChunk + slots_), dNewWork.reserve(capSites_ / vmk::kSiteChunk + slots_);
         dCnt.reserve(4), dNewCnt.reserve(1);
@@ -132,6 +134,8 @@ public:
         slotReq_.assign(slots_, -1);
         slotInfo_.resize(slots_);
         VM_HIP(hipMemsetAsync(dState.p, 0, slots_ * sizeof(vmk::InstState), stream_));
+        VM_HIP(hipMemsetAsync(dScanCtl.p, 0, sizeof(vmk::ScanCtl), stream_));
+        VM_HIP(hipMemsetAsync(dFlags.p, 0, dFlags.n * sizeof(unsigned long long), stream_));
         VM_HIP(hipMemsetAsync(dPartial.p, 0, size_t(slots_) * vmk::kLossPartials * sizeof(vmk::LossPartial), stream_));
         VM_HIP(hipStreamSynchronize(stream_));
     }
@@ -213,10 +217,22 @@ public:
         if (!changed_) return;
         std::vector<vmk::SiteWork>   sw;
         std::vector<vmk::TargetWork> tw;
-        std::vector<int>             il;
+        std::vector<vmk::ScanTile>   tiles;
+        int                          nDyn = 0;
+        // Scan tiles: multi-tile instances first (look-back), then single-tile ones.
+        for (int pass = 0; pass < 2; ++pass)
+            for (int s = 0; s < slots_; ++s) {
+                if (slotReq_[s] < 0) continue;
+                const int nt = vmk::scanTilesFor(slotInfo_[s].C);
+                if ((nt > 1) != (pass == 0)) continue;
+                for (int k = 0; k < nt; ++k) {
+                    const int b = k * vmk::kScanTileCells;
+                    tiles.push_back({s, b, int(std::min<int64_t>(vmk::kScanTileCells, slotInfo_[s].C - b)), k == 0});
+                }
+                if (pass == 0) nDyn += nt;
+            }
         for (int s = 0; s < slots_; ++s) {
             if (slotReq_[s] < 0) continue;
-            il.push_back(s);
             sw.insert(sw.end(), slotInfo_[s].sw.begin(), slotInfo_[s].sw.end());
             tw.insert(tw.end(), slotInfo_[s].tw.begin(), slotInfo_[s].tw.end());
         }
@@ -226,8 +242,12 @@ public:
             dTargetWork.reserve(twCap_);
             if (graph_) (void)hipGraphExecDestroy(graph_), graph_ = nullptr;
         }
-        int cnt[3] = {int(il.size()), int(sw.size()), int(tw.size())};
-        if (!il.empty()) VM_HIP(hipMemcpyAsync(dInstList.p, il.data(), il.size() * 4, hipMemcpyHostToDevice, stream_));
+        int cnt[3] = {0, int(sw.size()), int(tw.size())};
+        const int nTiles = int(tiles.size());
+        if (nTiles > maxTiles_) throw VmError(VM_ESTATE, "scan tile table overflow");
+        if (nTiles) VM_HIP(hipMemcpyAsync(dTiles.p, tiles.data(), tiles.size() * sizeof(tiles[0]), hipMemcpyHostToDevice, stream_));
+        const int tileCounts[2] = {nTiles, nDyn};
+        VM_HIP(hipMemcpyAsync(&dScanCtl.p->nTiles, tileCounts, sizeof(tileCounts), hipMemcpyHostToDevice, stream_));
         if (!sw.empty())
             VM_HIP(hipMemcpyAsync(dSiteWork.p, sw.data(), sw.size() * sizeof(sw[0]), hipMemcpyHostToDevice, stream_));
         if (!tw.empty())
@@ -319,14 +339,15 @@ private:
 
     void buildGraph(int K)
     {
-        const int gScan = std::min(slots_, 4096), gSite = 8192, gLoss = 16384;
+        const int gScan = std::min(maxTiles_, 1024), gSite = 8192, gLoss = 16384;
         hipGraph_t g;
         VM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         try {
-            VM_HIP(vmk::launchScan(gScan, stream_, dInst.p, dState.p, dPartial.p, dInstList.p, dCnt.p, dCounts.p, dStart.p,
-                                   vmk::kModeIterate, dExactLen.p, 1));
+            VM_HIP(vmk::launchScan(gScan, stream_, dInst.p, dState.p, dPartial.p, dTiles.p, dScanCtl.p, dFlags.p,
+                                   dCounts.p, dStart.p, vmk::kModeIterate, dExactLen.p, 1));
             VM_HIP(vmk::launchScatter<R>(gSite, stream_, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
-                                         dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate));
+                                         dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate,
+                                         dScanCtl.p));
             vmk::LossArgs<R> la{};
             la.insts = dInst.p, la.state = dState.p, la.work = dTargetWork.p, la.nWork = dCnt.p + 2;
             la.pos = dPos.p, la.spos = dSpos.p, la.sidx = dSidx.p, la.start = dStart.p, la.tpos = dTpos.p;
@@ -364,7 +385,11 @@ private:
 
     DBuf<V>              dPos, dM, dV, dSpos, dTpos;
     DBuf<vmk::GradT<R>>  dGrad;
-    DBuf<int>            dCellOf, dRank, dSidx, dCounts, dStart, dTslot, dInstList, dCnt, dNewCnt, dExactLen;
+    DBuf<int>            dCellOf, dRank, dSidx, dCounts, dStart, dTslot, dCnt, dNewCnt, dExactLen;
+    DBuf<vmk::ScanTile>  dTiles;
+    DBuf<vmk::ScanCtl>   dScanCtl;
+    DBuf<unsigned long long> dFlags;
+    int                  maxTiles_ = 0;
     DBuf<vmk::ExactItem> dExactQ;
     DBuf<uint8_t>        dFrozen;
     DBuf<vmk::Inst>      dInst;

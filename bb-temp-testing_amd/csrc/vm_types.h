@@ -80,6 +80,22 @@ struct TargetWork {
     float qx, qy, rho, pad;
 };
 
+// Scan tile: cells [begin, begin+count) of instance `inst` (first: begin == 0).
+// The tiles of one instance are consecutive in the table.
+struct ScanTile {
+    int inst, begin, count, first;
+};
+
+// Scan control block: counter over the look-back tiles and flag epoch (the
+// scatter kernel after every scan rearms them), the number of tiles and of
+// look-back tiles (the tiles of multi-tile instances, first in the table).
+struct ScanCtl {
+    int      next;
+    uint32_t epoch;
+    int      nTiles;
+    int      nDyn;
+};
+
 // A target queued by the loss kernel for the exact k-NN selection: instance
 // (bit 31 set: only its k-NN list is wanted) and packed target index.
 struct ExactItem {
