@@ -455,6 +455,7 @@ __global__ __launch_bounds__(kSiteChunk) void scatterKernel(const Inst *__restri
 constexpr int kLossWaves     = 4;
 constexpr int kLossThreads   = 64 * kLossWaves;
 constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane (block of <= 5 x 5 cells)
+constexpr int kBisectSteps   = 9;   // brackets q's K-th distance^2 to 1/512 of the block's largest
 
 template <class R> struct LossLds {
     V2<R>    pos[kLossStage];
@@ -477,17 +478,29 @@ template <class R> struct LaneTerm {
 constexpr int   kScreenGroup = 4;
 constexpr float kFarSite     = 1e30f;
 
-// FP32 screening of one lane's VoroLoss term over the wave's m compacted
-// candidates (a superset of the lane's K nearest sites, padded to mp).
-//   1. The nearest candidate, c0, under the (d2, index) order.
+// Wave-uniform bounds for the screen (computed from the chunk centre q).
+struct ScreenBounds {
+    int   mp1;  // candidates [0, mp1) hold every site that can be a target's nearest
+    int   mp;   // all candidates, padded to whole groups
+    float lam;  // >= |p - c0| + |c_i - c0| for every target p and candidate c_i
+    float t2;   // d2(p, w) below t2 proves w is one of p's K nearest
+};
+
+// FP32 screening of one lane's VoroLoss term over the wave's compacted
+// candidates (a superset of the lane's K nearest sites).
+//   1. The nearest candidate, c0, under the (d2, index) order, searched among
+//      the first band (sites within D1 + 2 rho of q, D1 = q's nearest distance:
+//      a target's nearest site is no farther from q).
 //   2. An approximation a_i of |distance to the bisector (c0, c_i)| with the
-//      hardware reciprocal square root; a_i is packed with i in its low 7 bits
-//      and the two smallest are kept (min / median-of-three).  c0 itself, sites
-//      on top of it and the sentinels give NaN and never win.
-//   3. The winner w must be one of the K nearest: at most K candidates with
-//      d2 <= d2(w).
+//      hardware reciprocal square root, a_i = rsq(|e|^2) (<u, e> - |e|^2 / 2);
+//      a_i is packed with i in its low 7 bits and the two smallest are kept
+//      (min / median-of-three).  c0 itself, sites on top of it and the
+//      sentinels give NaN and never win.
+//   3. The winner w must be one of the K nearest: d2(w) < t2 proves it (fewer
+//      than K sites lie within sqrt(t2) + |p - q| of q), else at most K
+//      candidates with d2 <= d2(w).
 // Error bound: with rsq accurate to 2 ulp, |a_i - |d_i|| <= 2^-21 (|u| + l_i)
-// + 2^-23 |d_i| (u = p - c0, l_i = |c_i - c0|) and packing truncates a_i by at
+// + 2^-22 |d_i| (u = p - c0, l_i = |c_i - c0|) and packing truncates a_i by at
 // most 2^-16 relative.  lam >= |u| + l_i for every candidate, so
 //   a_2 (1 - 2^-14) > a_1 + 2^-18 lam
 // implies |d_j| > |d_w| (1 + 2^-22) for every other candidate j: RN(d_j^2) >
@@ -496,13 +509,14 @@ constexpr float kFarSite     = 1e30f;
 // alone with the reference's formula.  Returns false when any condition fails;
 // the caller runs the exact selection for that lane.
 template <int K>
-__device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, int mp, V2<float> p, float lam, LaneTerm<float> &o)
+__device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const ScreenBounds &sb, V2<float> p,
+                                           LaneTerm<float> &o)
 {
     constexpr uint32_t kIdxMask = uint32_t(kLossStage - 1);
     static_assert((kLossStage & (kLossStage - 1)) == 0, "slot mask");
     Key<float> kmin = Key<float>::none();
 #pragma unroll 1
-    for (int g = 0; g < mp; g += kScreenGroup) {
+    for (int g = 0; g < sb.mp1; g += kScreenGroup) {
 #pragma unroll
         for (int k = 0; k < kScreenGroup; ++k) {
             const Key<float> key = Key<float>::make(dist2(lds.pos[g + k], p), lds.klo[g + k]);
@@ -514,15 +528,14 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, int mp, V2
     const V2<float> u{p.x - c0.x, p.y - c0.y};
     uint32_t        a1 = ~0u, a2 = ~0u;
 #pragma unroll 1
-    for (int g = 0; g < mp; g += kScreenGroup) {
+    for (int g = 0; g < sb.mp; g += kScreenGroup) {
 #pragma unroll
         for (int k = 0; k < kScreenGroup; ++k) {
             const V2<float> c  = lds.pos[g + k];
             const float     ex = c.x - c0.x, ey = c.y - c0.y;
             const float     ee = __builtin_fmaf(ey, ey, ex * ex);
-            const float     rs = __builtin_amdgcn_rsqf(ee);
             const float     ue = __builtin_fmaf(u.y, ey, u.x * ex);
-            const float     d  = __builtin_fmaf(-0.5f, ee * rs, ue * rs);
+            const float     d  = __builtin_amdgcn_rsqf(ee) * __builtin_fmaf(-0.5f, ee, ue);
             const uint32_t  av = (__float_as_uint(d) & (0x7FFFFFFFu & ~kIdxMask)) | uint32_t(g + k);
             a2                 = max(min(a1, av), min(max(a1, av), a2));
             a1                 = min(a1, av);
@@ -532,16 +545,20 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, int mp, V2
     const float v1 = __uint_as_float(a1 & ~kIdxMask);
     const float v2 = __uint_as_float(a2 & ~kIdxMask);
     bool        ok = a1 < 0x7F800000u &&
-              (a2 >= 0x7F800000u || __builtin_fmaf(v2, -0x1p-14f, v2) > __builtin_fmaf(lam, 0x1p-18f, v1));
+              (a2 >= 0x7F800000u || __builtin_fmaf(v2, -0x1p-14f, v2) > __builtin_fmaf(sb.lam, 0x1p-18f, v1));
     const V2<float> cw  = lds.pos[w];
     const float     d2w = dist2(cw, p);
-    int             cnt = 0;
+    bool            member = d2w * (1.0f + 4e-6f) < sb.t2;
+    if (__ballot(ok && !member)) {
+        int cnt = 0;
 #pragma unroll 1
-    for (int g = 0; g < mp; g += kScreenGroup) {
+        for (int g = 0; g < sb.mp; g += kScreenGroup) {
 #pragma unroll
-        for (int k = 0; k < kScreenGroup; ++k) cnt += dist2(lds.pos[g + k], p) <= d2w ? 1 : 0;
+            for (int k = 0; k < kScreenGroup; ++k) cnt += dist2(lds.pos[g + k], p) <= d2w ? 1 : 0;
+        }
+        member = cnt <= K;
     }
-    ok = ok && cnt <= K;
+    ok = ok && member;
     // The reference's term for (c0, w).
     o.ev  = V2<float>{cw.x - c0.x, cw.y - c0.y};
     o.l   = sqrtR(dotR(o.ev, o.ev));
@@ -653,6 +670,11 @@ template <> __device__ __forceinline__ double waveMaxR<double>(double v)
     return __builtin_bit_cast(double, (uint64_t(hi) << 32) | lo);
 }
 
+template <class R> __device__ __forceinline__ R waveMinR(R v)
+{
+    return -waveMaxR<R>(-v);
+}
+
 // Number of set bits of `mask` below this lane.
 __device__ __forceinline__ int laneRank(uint64_t mask)
 {
@@ -740,10 +762,11 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                 const int e0 = lane, e1 = lane + 64;
                 R d0 = e0 < total ? dist2(lds.pos[e0], q) : rInf<R>();
                 R d1 = e1 < total ? dist2(lds.pos[e1], q) : rInf<R>();
-                // D^2 >= q's K-th smallest by bisection on [0, max].
+                // Bisection: lo < D^2 <= hi for D^2 = q's K-th smallest, i.e.
+                // count(d2 <= lo) < K <= count(d2 <= hi) over the staged sites.
                 R lo = R(0);
                 R hi = waveMaxR<R>(fmax(e0 < total ? d0 : R(0), e1 < total ? d1 : R(0)));
-                for (int it = 0; it < 14; ++it) {
+                for (int it = 0; it < kBisectSteps; ++it) {
                     R   mid = lo + R(0.5) * (hi - lo);
                     int c   = __popcll(__ballot(d0 <= mid)) + __popcll(__ballot(d1 <= mid));
                     if (c >= K) hi = mid;
@@ -761,16 +784,20 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                 if (sy1 < gh - 1) edge = fmin(edge, R(sy1 + 1) * cell - q.y);
                 fast = edge - R(2) * mg > thr;
                 if (fast) {
-                    // Compact the candidates into LDS: q's nearest (d2 <= D^2) first.
-                    const uint64_t in0 = __ballot(d0 <= hi), in1 = __ballot(d1 <= hi);
-                    const uint64_t c0m = __ballot(d0 <= thr2 && !(d0 <= hi)), c1m = __ballot(d1 <= thr2 && !(d1 <= hi));
-                    const int      nIn = __popcll(in0) + __popcll(in1);
-                    m                  = nIn + __popcll(c0m) + __popcll(c1m);
+                    // Compact the candidates (d2 <= thr^2) into LDS, the first band
+                    // (d2 <= r1^2, r1 = q's nearest distance + 2 rho: it holds every
+                    // target's nearest site) first.
+                    const R        r1   = sqrtR(waveMinR<R>(fmin(d0, d1))) + R(2) * rho + R(2) * mg;
+                    const R        r1sq = fmin(r1 * r1 * (R(1) + relMargin<R>()), thr2);
+                    const uint64_t a0 = __ballot(d0 <= r1sq), a1 = __ballot(d1 <= r1sq);
+                    const uint64_t b0 = __ballot(d0 <= thr2 && !(d0 <= r1sq)), b1 = __ballot(d1 <= thr2 && !(d1 <= r1sq));
+                    const int      nA = __popcll(a0) + __popcll(a1);
+                    m                 = nA + __popcll(b0) + __popcll(b1);
                     int dst0 = -1, dst1 = -1;
-                    if (d0 <= hi) dst0 = laneRank(in0);
-                    else if (d0 <= thr2) dst0 = nIn + laneRank(c0m);
-                    if (d1 <= hi) dst1 = __popcll(in0) + laneRank(in1);
-                    else if (d1 <= thr2) dst1 = nIn + __popcll(c0m) + laneRank(c1m);
+                    if (d0 <= r1sq) dst0 = laneRank(a0);
+                    else if (d0 <= thr2) dst0 = nA + laneRank(b0);
+                    if (d1 <= r1sq) dst1 = __popcll(a0) + laneRank(a1);
+                    else if (d1 <= thr2) dst1 = nA + __popcll(b0) + laneRank(b1);
                     V2<R> p0{}, p1{};
                     int   i0v = 0, i1v = 0;
                     if (dst0 >= 0) p0 = lds.pos[e0], i0v = lds.idx[e0];
@@ -792,9 +819,15 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                     }
                     waveSync();
                     if (!a.exactOnly) {
-                        // Screened term (see screenTerm for the bound).
-                        const R lam = R(4) * (thr + rho);
-                        if (screenTerm<K>(lds, __builtin_amdgcn_readfirstlane(mp), p, lam, tr)) open = false;
+                        // Screened term (see screenTerm for the bounds).  Fewer
+                        // than K staged sites lie within sqrt(lo) of q.
+                        ScreenBounds sb;
+                        sb.mp1        = __builtin_amdgcn_readfirstlane((nA + kScreenGroup - 1) & ~(kScreenGroup - 1));
+                        sb.mp         = __builtin_amdgcn_readfirstlane(mp);
+                        sb.lam        = R(4) * (thr + rho);
+                        const R tq    = sqrtR(lo) * R(1 - 4e-6) - rho - mg;
+                        sb.t2         = tq > R(0) ? tq * tq : R(0);
+                        if (screenTerm<K>(lds, sb, p, tr)) open = false;
                         tr.used = take && !open ? tr.used : 0;
                     }
                 }
@@ -895,11 +928,6 @@ template <class R> struct ExactLds {
     int    cb[64];   // a round of ring cells: first sorted entry of each
     int    co[65];   // and the prefix of their site counts
 };
-
-template <class R> __device__ __forceinline__ R waveMinR(R v)
-{
-    return -waveMaxR<R>(-v);
-}
 
 // Sorts the K smallest of the distinct keys cand[0, nc) into cand[0, min(K, nc)).
 template <int K, class R> __device__ __forceinline__ int keepNearest(ExactLds<R> &lds, int nc, int lane)
