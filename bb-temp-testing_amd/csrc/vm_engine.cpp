@@ -147,7 +147,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         for (int k = t; k < e; ++k)
             rho = std::max(rho, std::hypot(s.tgt[2 * perm[k]] - double(qx), s.tgt[2 * perm[k] + 1] - double(qy)));
         float rhof = std::nextafter(std::nextafter(float(rho), 1e30f), 1e30f);
-        targetWork.push_back({id, t, e - t, c, qx, qy, rhof, 0.0f});
+        targetWork.push_back({id, t, e - t, c % in.gw, qx, qy, rhof, c / in.gw});
         t = e;
     }
 }

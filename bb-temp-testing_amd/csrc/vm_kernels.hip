@@ -695,32 +695,37 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         if (a.mode == kModeIterate && st->done) continue;
         const int64_t so = in.siteOff;
 
-        // ---- this lane's target -----------------------------------------
-        const int t    = wk.begin + lane;
-        V2<R>     p    = V2<R>{R(-1), R(-1)};
-        bool      take = false;
-        if (lane < wk.count) {
-            p    = a.tpos[in.targetOff + t];
-            take = p.x >= R(0);
-            if (take && in.useBox && a.mode == kModeIterate)
-                take = p.x >= R(in.bx0) && p.x <= R(in.bx1) && p.y >= R(in.by0) && p.y <= R(in.by1);
-            if (take && a.mode == kModeIterate && in.fraction < 1.0f)
-                take = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(st->iter), in.seed) < in.fraction;
+        // ---- this lane's targets: lane + 64 h, h < kTargetsPerLane -------
+        V2<R> p[kTargetsPerLane];
+        bool  take[kTargetsPerLane];
+        bool  anyTake = false;
+#pragma unroll
+        for (int h = 0; h < kTargetsPerLane; ++h) {
+            const int t = wk.begin + lane + 64 * h;
+            p[h] = V2<R>{R(-1), R(-1)}, take[h] = false;
+            if (lane + 64 * h < wk.count) {
+                p[h]    = a.tpos[in.targetOff + t];
+                take[h] = p[h].x >= R(0);
+                if (take[h] && in.useBox && a.mode == kModeIterate)
+                    take[h] = p[h].x >= R(in.bx0) && p[h].x <= R(in.bx1) && p[h].y >= R(in.by0) && p[h].y <= R(in.by1);
+                if (take[h] && a.mode == kModeIterate && in.fraction < 1.0f)
+                    take[h] = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(st->iter), in.seed) < in.fraction;
+            }
+            anyTake = anyTake || take[h];
         }
-        if (__ballot(take) == 0) {
+        if (__ballot(anyTake) == 0) {
             if (a.stats && lane == 0) atomicAdd(&a.stats[8 * (blockIdx.x & 63) + 3], 1ull);
             continue;
         }
 
-        LaneTerm<R> tr;
-        bool        open  = take; // still to be resolved by the exact selection
-        bool        fast  = false;
-        int         m     = 0; // compacted candidates (fast path)
-        int         total = 0; // sites of the staged block
+        bool         fast  = false;
+        int          m     = 0; // compacted candidates (fast path)
+        int          total = 0; // sites of the staged block
+        ScreenBounds sb{};
         if constexpr (sizeof(R) == 4) {
             const int *strt = a.start + in.cellOff;
             const int  gw = in.gw, gh = in.gh;
-            const int  cx = wk.cell % gw, cy = wk.cell / gw;
+            const int  cx = wk.cx, cy = wk.cy;
             const int  rs = in.stageR;
             const int  sx0 = max(cx - rs, 0), sx1 = min(cx + rs, gw - 1);
             const int  sy0 = max(cy - rs, 0), sy1 = min(cy + rs, gh - 1);
@@ -728,29 +733,42 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
             const R    cell = R(in.cell);
 
             // ---- stage the cell block [sx0,sx1] x [sy0,sy1] into LDS --------
-            // Rows of the block are contiguous ranges of the sorted arrays.
-            int b = 0, e = 0;
+            // Lane r*snx + i reads the range of cell (sx0 + i, sy0 + r); lane / snx
+            // is a multiply and shift (exact for lane < 25, snx <= 5).  Rows of
+            // the block are contiguous ranges of the sorted arrays: row r covers
+            // entries [rb[r], rb[r] + len) and LDS slots [roff[r], roff[r+1]).
+            const int rowMul = snx == 1 ? 256 : snx == 2 ? 128 : snx == 3 ? 86 : snx == 4 ? 64 : 52;
+            int       b = 0, e = 0;
             if (lane < snx * sny) {
-                const int c = (sy0 + lane / snx) * gw + sx0 + lane % snx;
+                const int r = (lane * rowMul) >> 8;
+                const int c = (sy0 + r) * gw + sx0 + (lane - r * snx);
                 b           = strt[c];
                 e           = strt[c + 1];
             }
-            for (int r = 0; r < sny; ++r)
-                total += __builtin_amdgcn_readlane(e, r * snx + snx - 1) - __builtin_amdgcn_readlane(b, r * snx);
-            fast = total <= kLossStage && total >= K;
+            int rb[5], roff[6];
+            roff[0] = 0;
+#pragma unroll
+            for (int r = 0; r < 5; ++r) {
+                const int rr = min(r, sny - 1);
+                rb[r]        = __builtin_amdgcn_readlane(b, rr * snx);
+                roff[r + 1]  = roff[r] + (r < sny ? __builtin_amdgcn_readlane(e, rr * snx + snx - 1) - rb[r] : 0);
+            }
+            total = roff[5];
+            fast  = total <= kLossStage && total >= K;
             if (fast) {
-                int roff = 0;
-                for (int r = 0; r < sny; ++r) {
-                    const int rb = __builtin_amdgcn_readlane(b, r * snx);
-                    const int re = __builtin_amdgcn_readlane(e, r * snx + snx - 1);
-#pragma unroll 1
-                    for (int s = lane; s < re - rb; s += 64) {
-                        lds.pos[roff + s]            = a.spos[so + rb + s];
-                        lds.idx[roff + s]            = a.sidx[so + rb + s];
-                        lds.grad[2 * (roff + s)]     = GradT<R>(0);
-                        lds.grad[2 * (roff + s) + 1] = GradT<R>(0);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int s = lane + 64 * h;
+                    if (s < total) {
+                        int src = rb[0] + s; // the row of slot s: the last r with roff[r] <= s
+#pragma unroll
+                        for (int r = 1; r < 5; ++r)
+                            if (s >= roff[r]) src = rb[r] + (s - roff[r]);
+                        lds.pos[s]          = a.spos[so + src];
+                        lds.idx[s]          = a.sidx[so + src];
+                        lds.grad[2 * s]     = GradT<R>(0);
+                        lds.grad[2 * s + 1] = GradT<R>(0);
                     }
-                    roff += re - rb;
                 }
                 waveSync();
 
@@ -818,68 +836,77 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                         lds.klo[m + lane] = ~0u;
                     }
                     waveSync();
-                    if (!a.exactOnly) {
-                        // Screened term (see screenTerm for the bounds).  Fewer
-                        // than K staged sites lie within sqrt(lo) of q.
-                        ScreenBounds sb;
-                        sb.mp1        = __builtin_amdgcn_readfirstlane((nA + kScreenGroup - 1) & ~(kScreenGroup - 1));
-                        sb.mp         = __builtin_amdgcn_readfirstlane(mp);
-                        sb.lam        = R(4) * (thr + rho);
-                        const R tq    = sqrtR(lo) * R(1 - 4e-6) - rho - mg;
-                        sb.t2         = tq > R(0) ? tq * tq : R(0);
-                        if (screenTerm<K>(lds, sb, p, tr)) open = false;
-                        tr.used = take && !open ? tr.used : 0;
-                    }
+                    // Screen bounds (see screenTerm).  Fewer than K staged sites
+                    // lie within sqrt(lo) of q.
+                    sb.mp1     = __builtin_amdgcn_readfirstlane((nA + kScreenGroup - 1) & ~(kScreenGroup - 1));
+                    sb.mp      = __builtin_amdgcn_readfirstlane(mp);
+                    sb.lam     = R(4) * (thr + rho);
+                    const R tq = sqrtR(lo) * R(1 - 4e-6) - rho - mg;
+                    sb.t2      = tq > R(0) ? tq * tq : R(0);
                 }
             }
         }
 
-        // ---- queue the open targets for the exact selection (and, for the k-NN
-        //      list diagnostics, every taken one: bit 31 of the instance marks a
-        //      term already settled here) ---------------------------------------
-        const bool     push = open || (a.dbgKnn && take);
-        const uint64_t pm   = __ballot(push);
-        if (pm) {
-            int base = 0;
-            if (lane == 0) base = atomicAdd(a.exactLen, __popcll(pm));
-            base = __builtin_amdgcn_readfirstlane(base);
-            if (push)
-                a.exactQ[base + laneRank(pm)] =
-                    ExactItem{open ? wk.inst : int(uint32_t(wk.inst) | 0x80000000u), int(in.targetOff + t)};
-        }
-        if (a.stats) {
-            const uint64_t om = __ballot(open);
-            if (lane == 0) {
-                unsigned long long *sp = a.stats + 8 * (blockIdx.x & 63); // 64 lines, summed on the host
-                atomicAdd(&sp[fast ? 0 : (total <= kLossStage ? 1 : 2)], 1ull);
-                atomicAdd(&sp[4], (unsigned long long)m);
-                atomicAdd(&sp[5], (unsigned long long)total);
-                if (fast && om) {
-                    atomicAdd(&sp[6], 1ull);
-                    atomicAdd(&sp[7], (unsigned long long)__popcll(om));
-                }
-            }
-        }
-
-        // ---- settled terms: gradient into LDS, diagnostics ------------------
-        const R lossT = tr.used ? tr.d * tr.d : R(0);
-        if (tr.used && a.grad)
-            lossGrad(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, tr.ev, tr.l, tr.d, &lds.grad[2 * tr.slj], &lds.grad[2 * tr.sl0],
-                     __builtin_ldexp(1.0, in.gradShift));
-        if (a.dbgKnn && lane < wk.count && !take) {
+        // ---- each target: screened term, or queued for the exact selection
+        //      (for the k-NN list diagnostics every taken target is queued; bit
+        //      31 of the instance marks a term already settled here) ------------
+        double   lossSum = 0.0;
+        int      used    = 0;
+        int      openCnt = 0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) a.dbgKnn[(in.targetOff + t) * int64_t(K) + k] = -1;
+        for (int h = 0; h < kTargetsPerLane; ++h) {
+            if (__ballot(take[h] || (a.dbgKnn && lane + 64 * h < wk.count)) == 0) continue;
+            const int   t    = wk.begin + lane + 64 * h;
+            LaneTerm<R> tr;
+            bool        open = take[h];
+            if constexpr (sizeof(R) == 4) {
+                if (fast && !a.exactOnly) {
+                    if (screenTerm<K>(lds, sb, p[h], tr)) open = false;
+                    tr.used = take[h] && !open ? tr.used : 0;
+                }
+            }
+            const bool     push = open || (a.dbgKnn && take[h]);
+            const uint64_t pm   = __ballot(push);
+            if (pm) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(a.exactLen, __popcll(pm));
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (push)
+                    a.exactQ[base + laneRank(pm)] =
+                        ExactItem{open ? wk.inst : int(uint32_t(wk.inst) | 0x80000000u), int(in.targetOff + t)};
+            }
+            openCnt += __popcll(__ballot(open));
+            const R lossT = tr.used ? tr.d * tr.d : R(0);
+            if (tr.used && a.grad)
+                lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.l, tr.d, &lds.grad[2 * tr.slj],
+                         &lds.grad[2 * tr.sl0], __builtin_ldexp(1.0, in.gradShift));
+            if (a.dbgKnn && lane + 64 * h < wk.count && !take[h]) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) a.dbgKnn[(in.targetOff + t) * int64_t(K) + k] = -1;
+            }
+            if (a.dbgTerm && lane + 64 * h < wk.count && !open) {
+                const int64_t o   = in.targetOff + t;
+                a.dbgTerm[o].i0   = tr.used ? tr.i0 : -1;
+                a.dbgTerm[o].j    = tr.used ? tr.bj : -1;
+                a.dbgTerm[o].loss = double(lossT);
+            }
+            lossSum += double(lossT);
+            used += tr.used;
         }
-        if (a.dbgTerm && lane < wk.count && !open) {
-            const int64_t o   = in.targetOff + t;
-            a.dbgTerm[o].i0   = tr.used ? tr.i0 : -1;
-            a.dbgTerm[o].j    = tr.used ? tr.bj : -1;
-            a.dbgTerm[o].loss = double(lossT);
+        if (a.stats && lane == 0) {
+            unsigned long long *sp = a.stats + 8 * (blockIdx.x & 63); // 64 lines, summed on the host
+            atomicAdd(&sp[fast ? 0 : (total <= kLossStage ? 1 : 2)], 1ull);
+            atomicAdd(&sp[4], (unsigned long long)m);
+            atomicAdd(&sp[5], (unsigned long long)total);
+            if (fast && openCnt) {
+                atomicAdd(&sp[6], 1ull);
+                atomicAdd(&sp[7], (unsigned long long)openCnt);
+            }
         }
 
         // ---- FP64 loss reduction: one atomic pair per wavefront ----------
-        const double ls = waveSumDouble(double(lossT));
-        const int    uc = __popcll(__ballot(tr.used != 0));
+        const double ls = waveSumDouble(lossSum);
+        const int    uc = waveReduce32<int>(used, 0, [](int x, int y) { return x + y; });
         if (lane == 0 && uc > 0) {
             if (a.mode == kModeIterate) {
                 LossPartial *pp = a.partial + int64_t(wk.inst) * kLossPartials + (blockIdx.x & (kLossPartials - 1));

@@ -13,7 +13,8 @@ constexpr int kMaxCellVerts = 24; // clipped cell vertex budget (voro_optimizer.
 constexpr int kCellK        = 32; // clip neighbours before brute force (voro_optimizer.cu:55)
 constexpr int kBoxLabels    = 4;  // edge labels -1..-4 are the box sides (voro_optimizer.cu:58-59)
 constexpr int kSiteChunk    = 256;
-constexpr int kTargetChunk  = 64;  // one wavefront of targets, all in one hash cell
+constexpr int kTargetsPerLane = 2;                      // loss kernel: targets per lane
+constexpr int kTargetChunk    = 64 * kTargetsPerLane;  // targets of one wavefront, all in one hash cell
 constexpr int kLdsSlotBits  = 10;  // low bits of a k-NN key: slot of the site in the staged LDS block
 constexpr int kLdsNone      = (1 << kLdsSlotBits) - 1;
 constexpr int kStageCap     = kLdsNone - 1; // staged sites per target chunk
@@ -74,10 +75,11 @@ struct SiteWork {
 };
 
 // Targets [begin, begin+count) of `inst` (sorted order), all in hash cell
-// `cell`; every one lies within `rho` of (qx, qy) (static: targets do not move).
+// (cx, cy); every one lies within `rho` of (qx, qy) (static: targets do not move).
 struct TargetWork {
-    int   inst, begin, count, cell;
-    float qx, qy, rho, pad;
+    int   inst, begin, count, cx;
+    float qx, qy, rho;
+    int   cy;
 };
 
 // Scan tile: cells [begin, begin+count) of instance `inst` (first: begin == 0).
