@@ -481,9 +481,12 @@ constexpr float kFarSite     = 1e30f;
 // Wave-uniform bounds for the screen (computed from the chunk centre q).
 struct ScreenBounds {
     int   mp1;  // candidates [0, mp1) hold every site that can be a target's nearest
+    int   mp2;  // candidates [mp2, mp) lie farther than r2 from q (outer band)
     int   mp;   // all candidates, padded to whole groups
     float lam;  // >= |p - c0| + |c_i - c0| for every target p and candidate c_i
     float t2;   // d2(p, w) below t2 proves w is one of p's K nearest
+    float r2;   // inner radius of the outer band (lower bound)
+    float rho;  // >= |p - q| for every target of the chunk
 };
 
 // FP32 screening of one lane's VoroLoss term over the wave's compacted
@@ -496,6 +499,9 @@ struct ScreenBounds {
 //      a_i is packed with i in its low 7 bits and the two smallest are kept
 //      (min / median-of-three).  c0 itself, sites on top of it and the
 //      sentinels give NaN and never win.
+//      Candidates of the outer band (farther than r2 from q) are skipped when
+//      their lower bound (r2 - rho - |p - c0|) / 2 exceeds every active lane's
+//      runner-up.
 //   3. The winner w must be one of the K nearest: d2(w) < t2 proves it (fewer
 //      than K sites lie within sqrt(t2) + |p - q| of q), else at most K
 //      candidates with d2 <= d2(w).
@@ -509,7 +515,7 @@ struct ScreenBounds {
 // alone with the reference's formula.  Returns false when any condition fails;
 // the caller runs the exact selection for that lane.
 template <int K>
-__device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const ScreenBounds &sb, V2<float> p,
+__device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const ScreenBounds &sb, V2<float> p, bool active,
                                            LaneTerm<float> &o)
 {
     constexpr uint32_t kIdxMask = uint32_t(kLossStage - 1);
@@ -527,19 +533,31 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     const V2<float> c0 = lds.pos[s0];
     const V2<float> u{p.x - c0.x, p.y - c0.y};
     uint32_t        a1 = ~0u, a2 = ~0u;
+    auto            screenGroups = [&](int g0, int g1) {
 #pragma unroll 1
-    for (int g = 0; g < sb.mp; g += kScreenGroup) {
+        for (int g = g0; g < g1; g += kScreenGroup) {
 #pragma unroll
-        for (int k = 0; k < kScreenGroup; ++k) {
-            const V2<float> c  = lds.pos[g + k];
-            const float     ex = c.x - c0.x, ey = c.y - c0.y;
-            const float     ee = __builtin_fmaf(ey, ey, ex * ex);
-            const float     ue = __builtin_fmaf(u.y, ey, u.x * ex);
-            const float     d  = __builtin_amdgcn_rsqf(ee) * __builtin_fmaf(-0.5f, ee, ue);
-            const uint32_t  av = (__float_as_uint(d) & (0x7FFFFFFFu & ~kIdxMask)) | uint32_t(g + k);
-            a2                 = max(min(a1, av), min(max(a1, av), a2));
-            a1                 = min(a1, av);
+            for (int k = 0; k < kScreenGroup; ++k) {
+                const V2<float> c  = lds.pos[g + k];
+                const float     ex = c.x - c0.x, ey = c.y - c0.y;
+                const float     ee = __builtin_fmaf(ey, ey, ex * ex);
+                const float     ue = __builtin_fmaf(u.y, ey, u.x * ex);
+                const float     d  = __builtin_amdgcn_rsqf(ee) * __builtin_fmaf(-0.5f, ee, ue);
+                const uint32_t  av = (__float_as_uint(d) & (0x7FFFFFFFu & ~kIdxMask)) | uint32_t(g + k);
+                a2                 = max(min(a1, av), min(max(a1, av), a2));
+                a1                 = min(a1, av);
+            }
         }
+    };
+    screenGroups(0, sb.mp2);
+    // Outer band: |d_i| >= (|p - c_i| - |p - c0|) / 2 >= (r2 - rho - |u|) / 2.
+    // When that exceeds the runner-up (with room for the screen's error) in
+    // every lane, no outer candidate can be the winner or the runner-up.
+    {
+        const float v2  = __uint_as_float(a2 & ~kIdxMask);
+        const float lb  = 0.5f * (sb.r2 * (1.0f - 1e-5f) - sb.rho - sqrtf(kmin.d2()) * (1.0f + 1e-5f));
+        const bool  out = v2 * (1.0f + 0x1p-10f) + sb.lam * 0x1p-16f < lb; // false when v2 is NaN (no runner-up)
+        if (__ballot(active && !out)) screenGroups(sb.mp2, sb.mp);
     }
     const int   w  = int(a1 & kIdxMask);
     const float v1 = __uint_as_float(a1 & ~kIdxMask);
@@ -807,15 +825,20 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                     // target's nearest site) first.
                     const R        r1   = sqrtR(waveMinR<R>(fmin(d0, d1))) + R(2) * rho + R(2) * mg;
                     const R        r1sq = fmin(r1 * r1 * (R(1) + relMargin<R>()), thr2);
+                    const R        r2sq = fmin(fmax(lo, r1sq), thr2); // outer band: r2sq < d2 <= thr2
                     const uint64_t a0 = __ballot(d0 <= r1sq), a1 = __ballot(d1 <= r1sq);
-                    const uint64_t b0 = __ballot(d0 <= thr2 && !(d0 <= r1sq)), b1 = __ballot(d1 <= thr2 && !(d1 <= r1sq));
+                    const uint64_t b0 = __ballot(d0 <= r2sq && !(d0 <= r1sq)), b1 = __ballot(d1 <= r2sq && !(d1 <= r1sq));
+                    const uint64_t c0b = __ballot(d0 <= thr2 && !(d0 <= r2sq)), c1b = __ballot(d1 <= thr2 && !(d1 <= r2sq));
                     const int      nA = __popcll(a0) + __popcll(a1);
-                    m                 = nA + __popcll(b0) + __popcll(b1);
+                    const int      nB = nA + __popcll(b0) + __popcll(b1);
+                    m                 = nB + __popcll(c0b) + __popcll(c1b);
                     int dst0 = -1, dst1 = -1;
                     if (d0 <= r1sq) dst0 = laneRank(a0);
-                    else if (d0 <= thr2) dst0 = nA + laneRank(b0);
+                    else if (d0 <= r2sq) dst0 = nA + laneRank(b0);
+                    else if (d0 <= thr2) dst0 = nB + laneRank(c0b);
                     if (d1 <= r1sq) dst1 = __popcll(a0) + laneRank(a1);
-                    else if (d1 <= thr2) dst1 = nA + __popcll(b0) + laneRank(b1);
+                    else if (d1 <= r2sq) dst1 = nA + __popcll(b0) + laneRank(b1);
+                    else if (d1 <= thr2) dst1 = nB + __popcll(c0b) + laneRank(c1b);
                     V2<R> p0{}, p1{};
                     int   i0v = 0, i1v = 0;
                     if (dst0 >= 0) p0 = lds.pos[e0], i0v = lds.idx[e0];
@@ -839,7 +862,10 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                     // Screen bounds (see screenTerm).  Fewer than K staged sites
                     // lie within sqrt(lo) of q.
                     sb.mp1     = __builtin_amdgcn_readfirstlane((nA + kScreenGroup - 1) & ~(kScreenGroup - 1));
+                    sb.mp2     = __builtin_amdgcn_readfirstlane((nB + kScreenGroup - 1) & ~(kScreenGroup - 1));
                     sb.mp      = __builtin_amdgcn_readfirstlane(mp);
+                    sb.r2      = sqrtR(r2sq);
+                    sb.rho     = rho;
                     sb.lam     = R(4) * (thr + rho);
                     const R tq = sqrtR(lo) * R(1 - 4e-6) - rho - mg;
                     sb.t2      = tq > R(0) ? tq * tq : R(0);
@@ -861,7 +887,7 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
             bool        open = take[h];
             if constexpr (sizeof(R) == 4) {
                 if (fast && !a.exactOnly) {
-                    if (screenTerm<K>(lds, sb, p[h], tr)) open = false;
+                    if (screenTerm<K>(lds, sb, p[h], take[h], tr)) open = false;
                     tr.used = take[h] && !open ? tr.used : 0;
                 }
             }
