@@ -380,8 +380,11 @@ public:
         for (size_t g = 0; g < L.groups.size(); ++g)
             VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, lossArgs(int(g), vmk::kModeIterate, true)));
         if (marks) VM_HIP(hipEventRecord(marks[3], stream));
-        VM_HIP(vmk::launchAdamBin<R>(gridSite, stream, adamArgs(1)));
+        for (size_t g = 0; g < L.groups.size(); ++g)
+            VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, lossArgs(int(g), vmk::kModeIterate, true)));
         if (marks) VM_HIP(hipEventRecord(marks[4], stream));
+        VM_HIP(vmk::launchAdamBin<R>(gridSite, stream, adamArgs(1)));
+        if (marks) VM_HIP(hipEventRecord(marks[5], stream));
     }
 
     void buildGraph()
@@ -421,17 +424,17 @@ public:
     {
         if (dirty) commit();
         ensureEvents();
-        double acc[4] = {0, 0, 0, 0};
+        double acc[kPhases] = {};
         for (int k = 0; k < iterations; ++k) {
             enqueueIteration(ev);
-            VM_HIP(hipEventSynchronize(ev[4]));
-            for (int p = 0; p < 4; ++p) {
+            VM_HIP(hipEventSynchronize(ev[kPhases]));
+            for (int p = 0; p < kPhases; ++p) {
                 float ms = 0;
                 VM_HIP(hipEventElapsedTime(&ms, ev[p], ev[p + 1]));
                 acc[p] += ms;
             }
         }
-        for (int p = 0; p < 4; ++p) phaseMs[p] = float(acc[p] / std::max(iterations, 1));
+        for (int p = 0; p < kPhases; ++p) phaseMs[p] = float(acc[p] / std::max(iterations, 1));
         deviceNewer = true;
     }
 
@@ -474,7 +477,10 @@ public:
         if (dirty) commit();
         gridBuild(vmk::kModeEvaluate);
         for (size_t g = 0; g < L.groups.size(); ++g)
+        {
             VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, lossArgs(int(g), vmk::kModeEvaluate, false)));
+            VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, lossArgs(int(g), vmk::kModeEvaluate, false)));
+        }
         bin();
         vmk::InstState st;
         VM_HIP(hipMemcpyAsync(&st, dState.p + id, sizeof(st), hipMemcpyDeviceToHost, stream));
@@ -499,6 +505,7 @@ public:
             auto a = lossArgs(int(g), vmk::kModeEvaluate, wantGrad);
             if (wantDbg && L.groups[g].K == K) a.dbgKnn = dK.p, a.dbgTerm = dT.p;
             VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, a));
+            VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, a));
         }
         std::vector<vmk::GradT<R>> hg;
         if (wantGrad) {
@@ -550,6 +557,7 @@ public:
             auto a  = lossArgs(int(g), vmk::kModeEvaluate, false);
             a.stats = dS.p;
             VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, a));
+            VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, a));
         }
         bin();
         std::vector<unsigned long long> h(64 * 8);

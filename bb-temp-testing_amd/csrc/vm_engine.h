@@ -108,7 +108,9 @@ public:
     // Lays out, uploads and bins every instance; rebuilds the iteration graph.
     virtual void commit() = 0;
     virtual float step(int iterations) = 0;                 // GPU ms
-    virtual void  profile(int iterations, float *phaseMs) = 0; // mean ms per phase
+    // Mean ms per phase over `iterations` iterations: scan, scatter, loss, exact selection, adam.
+    static constexpr int kPhases = 5;
+    virtual void  profile(int iterations, float *phaseMs) = 0;
     virtual void  readStates() = 0;                          // device -> specs[i].state
     virtual double evaluate(int id, int64_t *count) = 0;     // loss sum over all live targets of `id`
     virtual void  diag(int id, bool wantGrad, bool wantDbg, DiagOut &out) = 0;

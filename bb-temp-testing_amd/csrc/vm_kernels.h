@@ -93,6 +93,8 @@ hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstS
                          const int *nWork, const void *pos, const int *cellOf, const int *rank, const int *start,
                          void *spos, int *sidx, int mode, ScanCtl *ctl);
 template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const LossArgs<R> &a);
+// The exact selection for the targets the loss launch queued; always follows it.
+template <class R> hipError_t launchExact(int K, hipStream_t s, const LossArgs<R> &a);
 template <class R> hipError_t launchAdamBin(int grid, hipStream_t s, const AdamArgs<R> &a);
 template <class R>
 hipError_t launchKnnQuery(int K, hipStream_t s, const Inst *insts, int inst, const void *spos, const int *sidx,

@@ -455,7 +455,8 @@ __global__ __launch_bounds__(kSiteChunk) void scatterKernel(const Inst *__restri
 constexpr int kLossWaves     = 4;
 constexpr int kLossThreads   = 64 * kLossWaves;
 constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane (block of <= 5 x 5 cells)
-constexpr int kBisectSteps   = 9;   // brackets q's K-th distance^2 to 1/512 of the block's largest
+constexpr int kBisectSteps   = 9;
+constexpr int kStageRetry    = 3;   // rings of the retried block (7 x 7 cells <= 64 lanes)   // brackets q's K-th distance^2 to 1/512 of the block's largest
 
 template <class R> struct LossLds {
     V2<R>    pos[kLossStage];
@@ -483,7 +484,6 @@ struct ScreenBounds {
     int   mp1;  // candidates [0, mp1) hold every site that can be a target's nearest
     int   mp2;  // candidates [mp2, mp) lie farther than r2 from q (outer band)
     int   mp;   // all candidates, padded to whole groups
-    float lam;  // >= |p - c0| + |c_i - c0| for every target p and candidate c_i
     float t2;   // d2(p, w) below t2 proves w is one of p's K nearest
     float r2;   // inner radius of the outer band (lower bound)
     float rho;  // >= |p - q| for every target of the chunk
@@ -505,10 +505,15 @@ struct ScreenBounds {
 //   3. The winner w must be one of the K nearest: d2(w) < t2 proves it (fewer
 //      than K sites lie within sqrt(t2) + |p - q| of q), else at most K
 //      candidates with d2 <= d2(w).
-// Error bound: with rsq accurate to 2 ulp, |a_i - |d_i|| <= 2^-21 (|u| + l_i)
-// + 2^-22 |d_i| (u = p - c0, l_i = |c_i - c0|) and packing truncates a_i by at
-// most 2^-16 relative.  lam >= |u| + l_i for every candidate, so
-//   a_2 (1 - 2^-14) > a_1 + 2^-18 lam
+// Error bound (u = p - c0, e = c_i - c0, ee and ue the rounded |e|^2 and
+// <u, e> both computations share, delta = ue / sqrt(ee) - sqrt(ee) / 2): the
+// fma forms ue - ee / 2 with one rounding, so with rsq accurate to 2 ulp
+// |a_i - delta| <= 1.5 2^-22 |delta|; the reference's d_i = RN(RN(ue / l) -
+// l / 2) has |d_i - delta| <= 2^-22.9 |u| + 2^-25 l + 2^-24 |d_i|, and l <= 4
+// max(|u|, |d_i|) (if l > 4 |u| then d_i >= (l - 2|u|) / 2 >= l / 4).  Hence
+// |a_i - |d_i|| <= 2^-20 (|u| + |d_i|); packing truncates a_i by at most 2^-16
+// relative.  So
+//   a_2 (1 - 2^-13) > a_1 (1 + 2^-13) + 2^-17 |u|
 // implies |d_j| > |d_w| (1 + 2^-22) for every other candidate j: RN(d_j^2) >
 // RN(d_w^2) strictly, and the reference's loop over the K nearest (smallest
 // d^2, ties to the earlier key) picks w.  The exact term is then computed for w
@@ -532,6 +537,7 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     const int       s0 = keySlot(kmin.low());
     const V2<float> c0 = lds.pos[s0];
     const V2<float> u{p.x - c0.x, p.y - c0.y};
+    const float     uB = sqrtf(kmin.d2()) * (1.0f + 0x1p-20f); // >= |u|
     uint32_t        a1 = ~0u, a2 = ~0u;
     auto            screenGroups = [&](int g0, int g1) {
 #pragma unroll 1
@@ -555,15 +561,15 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     // every lane, no outer candidate can be the winner or the runner-up.
     {
         const float v2  = __uint_as_float(a2 & ~kIdxMask);
-        const float lb  = 0.5f * (sb.r2 * (1.0f - 1e-5f) - sb.rho - sqrtf(kmin.d2()) * (1.0f + 1e-5f));
-        const bool  out = v2 * (1.0f + 0x1p-10f) + sb.lam * 0x1p-16f < lb; // false when v2 is NaN (no runner-up)
+        const float lb  = 0.5f * (sb.r2 * (1.0f - 1e-5f) - sb.rho - uB * (1.0f + 1e-5f));
+        const bool  out = v2 * (1.0f + 0x1p-10f) + uB * 0x1p-14f < lb; // false when v2 is NaN (no runner-up)
         if (__ballot(active && !out)) screenGroups(sb.mp2, sb.mp);
     }
     const int   w  = int(a1 & kIdxMask);
     const float v1 = __uint_as_float(a1 & ~kIdxMask);
     const float v2 = __uint_as_float(a2 & ~kIdxMask);
     bool        ok = a1 < 0x7F800000u &&
-              (a2 >= 0x7F800000u || __builtin_fmaf(v2, -0x1p-14f, v2) > __builtin_fmaf(sb.lam, 0x1p-18f, v1));
+              (a2 >= 0x7F800000u || __builtin_fmaf(v2, -0x1p-13f, v2) > __builtin_fmaf(v1, 0x1p-13f, v1) + uB * 0x1p-17f);
     const V2<float> cw  = lds.pos[w];
     const float     d2w = dist2(cw, p);
     bool            member = d2w * (1.0f + 4e-6f) < sb.t2;
@@ -741,135 +747,142 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
         int          total = 0; // sites of the staged block
         ScreenBounds sb{};
         if constexpr (sizeof(R) == 4) {
-            const int *strt = a.start + in.cellOff;
-            const int  gw = in.gw, gh = in.gh;
-            const int  cx = wk.cx, cy = wk.cy;
-            const int  rs = in.stageR;
-            const int  sx0 = max(cx - rs, 0), sx1 = min(cx + rs, gw - 1);
-            const int  sy0 = max(cy - rs, 0), sy1 = min(cy + rs, gh - 1);
-            const int  snx = sx1 - sx0 + 1, sny = sy1 - sy0 + 1;
-            const R    cell = R(in.cell);
+            const int  *strt = a.start + in.cellOff;
+            const int   gw = in.gw, gh = in.gh;
+            const int   cx = wk.cx, cy = wk.cy;
+            const R     cell = R(in.cell);
+            const R     ext  = R(in.ext);
+            const V2<R> q{R(wk.qx), R(wk.qy)};
+            const R     mg  = relMargin<R>() * fmax(ext, fmax(fabs(q.x), fabs(q.y)) + R(wk.rho));
+            const R     rho = R(wk.rho) + mg;
+            const int   e0 = lane, e1 = lane + 64;
+            R           d0 = rInf<R>(), d1 = rInf<R>(), lo = R(0), hi = R(0), thr = R(0), thr2 = R(0);
+            // The block of rings 0..rs around the chunk's cell; a block that
+            // misses the candidate disc (next to the domain's edge) is retried
+            // once with kStageRetry rings.
+            for (int rs = in.stageR;; rs = kStageRetry) {
+                const int sx0 = max(cx - rs, 0), sx1 = min(cx + rs, gw - 1);
+                const int sy0 = max(cy - rs, 0), sy1 = min(cy + rs, gh - 1);
+                const int snx = sx1 - sx0 + 1, sny = sy1 - sy0 + 1;
 
-            // ---- stage the cell block [sx0,sx1] x [sy0,sy1] into LDS --------
-            // Lane r*snx + i reads the range of cell (sx0 + i, sy0 + r); lane / snx
-            // is a multiply and shift (exact for lane < 25, snx <= 5).  Rows of
-            // the block are contiguous ranges of the sorted arrays: row r covers
-            // entries [rb[r], rb[r] + len) and LDS slots [roff[r], roff[r+1]).
-            const int rowMul = snx == 1 ? 256 : snx == 2 ? 128 : snx == 3 ? 86 : snx == 4 ? 64 : 52;
-            int       b = 0, e = 0;
-            if (lane < snx * sny) {
-                const int r = (lane * rowMul) >> 8;
-                const int c = (sy0 + r) * gw + sx0 + (lane - r * snx);
-                b           = strt[c];
-                e           = strt[c + 1];
-            }
-            int rb[5], roff[6];
-            roff[0] = 0;
+                // ---- stage the cell block [sx0,sx1] x [sy0,sy1] into LDS ----
+                // Lane r*snx + i reads the range of cell (sx0 + i, sy0 + r);
+                // lane / snx is a multiply and shift (exact for lane < 7 snx,
+                // snx <= 7).  Rows of the block are contiguous ranges of the
+                // sorted arrays: row r covers entries [rb[r], rb[r] + len) and
+                // LDS slots [roff[r], roff[r+1]).
+                const int rowMul = (256 + snx - 1) / snx;
+                int       b = 0, e = 0;
+                if (lane < snx * sny) {
+                    const int r = (lane * rowMul) >> 8;
+                    const int c = (sy0 + r) * gw + sx0 + (lane - r * snx);
+                    b           = strt[c];
+                    e           = strt[c + 1];
+                }
+                constexpr int kRows = 2 * kStageRetry + 1;
+                int           rb[kRows], roff[kRows + 1];
+                roff[0] = 0;
 #pragma unroll
-            for (int r = 0; r < 5; ++r) {
-                const int rr = min(r, sny - 1);
-                rb[r]        = __builtin_amdgcn_readlane(b, rr * snx);
-                roff[r + 1]  = roff[r] + (r < sny ? __builtin_amdgcn_readlane(e, rr * snx + snx - 1) - rb[r] : 0);
-            }
-            total = roff[5];
-            fast  = total <= kLossStage && total >= K;
-            if (fast) {
+                for (int r = 0; r < kRows; ++r) {
+                    const int rr = min(r, sny - 1);
+                    rb[r]        = __builtin_amdgcn_readlane(b, rr * snx);
+                    roff[r + 1]  = roff[r] + (r < sny ? __builtin_amdgcn_readlane(e, rr * snx + snx - 1) - rb[r] : 0);
+                }
+                total = roff[kRows];
+                fast  = total <= kLossStage && total >= K;
+                if (fast) {
 #pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int s = lane + 64 * h;
-                    if (s < total) {
-                        int src = rb[0] + s; // the row of slot s: the last r with roff[r] <= s
+                    for (int h = 0; h < 2; ++h) {
+                        const int s = lane + 64 * h;
+                        if (s < total) {
+                            int src = rb[0] + s; // the row of slot s: the last r with roff[r] <= s
 #pragma unroll
-                        for (int r = 1; r < 5; ++r)
-                            if (s >= roff[r]) src = rb[r] + (s - roff[r]);
-                        lds.pos[s]          = a.spos[so + src];
-                        lds.idx[s]          = a.sidx[so + src];
-                        lds.grad[2 * s]     = GradT<R>(0);
-                        lds.grad[2 * s + 1] = GradT<R>(0);
+                            for (int r = 1; r < kRows; ++r)
+                                if (s >= roff[r]) src = rb[r] + (s - roff[r]);
+                            lds.pos[s]          = a.spos[so + src];
+                            lds.idx[s]          = a.sidx[so + src];
+                            lds.grad[2 * s]     = GradT<R>(0);
+                            lds.grad[2 * s + 1] = GradT<R>(0);
+                        }
                     }
+                    waveSync();
+                    // Squared distance from q to each staged site (two per lane).
+                    d0 = e0 < total ? dist2(lds.pos[e0], q) : rInf<R>();
+                    d1 = e1 < total ? dist2(lds.pos[e1], q) : rInf<R>();
+                    // Bisection: lo < D^2 <= hi for D^2 = q's K-th smallest, i.e.
+                    // count(d2 <= lo) < K <= count(d2 <= hi) over the staged sites.
+                    lo = R(0);
+                    hi = waveMaxR<R>(fmax(e0 < total ? d0 : R(0), e1 < total ? d1 : R(0)));
+                    for (int it = 0; it < kBisectSteps; ++it) {
+                        R   mid = lo + R(0.5) * (hi - lo);
+                        int c   = __popcll(__ballot(d0 <= mid)) + __popcll(__ballot(d1 <= mid));
+                        if (c >= K) hi = mid;
+                        else lo = mid;
+                    }
+                    const R D = sqrtR(hi) + mg;
+                    thr       = D + R(2) * rho + mg;
+                    thr2      = thr * thr * (R(1) + relMargin<R>());
+                    // The block must contain the disc of radius thr around q: then
+                    // it holds the K nearest sites of every target of the chunk.
+                    R edge = rInf<R>();
+                    if (sx0 > 0) edge = fmin(edge, q.x - R(sx0) * cell);
+                    if (sx1 < gw - 1) edge = fmin(edge, R(sx1 + 1) * cell - q.x);
+                    if (sy0 > 0) edge = fmin(edge, q.y - R(sy0) * cell);
+                    if (sy1 < gh - 1) edge = fmin(edge, R(sy1 + 1) * cell - q.y);
+                    fast = edge - R(2) * mg > thr;
+                }
+                if (fast || rs >= kStageRetry || total > kLossStage) break;
+                waveSync();
+            }
+            if (fast) {
+                // Compact the candidates (d2 <= thr^2) into LDS, the first band
+                // (d2 <= r1^2, r1 = q's nearest distance + 2 rho: it holds every
+                // target's nearest site) first.
+                const R        r1   = sqrtR(waveMinR<R>(fmin(d0, d1))) + R(2) * rho + R(2) * mg;
+                const R        r1sq = fmin(r1 * r1 * (R(1) + relMargin<R>()), thr2);
+                const R        r2sq = fmin(fmax(lo, r1sq), thr2); // outer band: r2sq < d2 <= thr2
+                const uint64_t a0 = __ballot(d0 <= r1sq), a1 = __ballot(d1 <= r1sq);
+                const uint64_t b0 = __ballot(d0 <= r2sq && !(d0 <= r1sq)), b1 = __ballot(d1 <= r2sq && !(d1 <= r1sq));
+                const uint64_t c0b = __ballot(d0 <= thr2 && !(d0 <= r2sq)), c1b = __ballot(d1 <= thr2 && !(d1 <= r2sq));
+                const int      nA = __popcll(a0) + __popcll(a1);
+                const int      nB = nA + __popcll(b0) + __popcll(b1);
+                m                 = nB + __popcll(c0b) + __popcll(c1b);
+                int dst0 = -1, dst1 = -1;
+                if (d0 <= r1sq) dst0 = laneRank(a0);
+                else if (d0 <= r2sq) dst0 = nA + laneRank(b0);
+                else if (d0 <= thr2) dst0 = nB + laneRank(c0b);
+                if (d1 <= r1sq) dst1 = __popcll(a0) + laneRank(a1);
+                else if (d1 <= r2sq) dst1 = nA + __popcll(b0) + laneRank(b1);
+                else if (d1 <= thr2) dst1 = nB + __popcll(c0b) + laneRank(c1b);
+                V2<R> p0{}, p1{};
+                int   i0v = 0, i1v = 0;
+                if (dst0 >= 0) p0 = lds.pos[e0], i0v = lds.idx[e0];
+                if (dst1 >= 0) p1 = lds.pos[e1], i1v = lds.idx[e1];
+                waveSync();
+                if (dst0 >= 0) {
+                    lds.pos[dst0] = p0, lds.idx[dst0] = i0v;
+                    lds.klo[dst0] = (uint32_t(i0v) << kLdsSlotBits) | uint32_t(dst0);
+                }
+                if (dst1 >= 0) {
+                    lds.pos[dst1] = p1, lds.idx[dst1] = i1v;
+                    lds.klo[dst1] = (uint32_t(i1v) << kLdsSlotBits) | uint32_t(dst1);
+                }
+                // Pad to whole screening groups with far sentinels.
+                const int mp = (m + kScreenGroup - 1) & ~(kScreenGroup - 1);
+                if (lane < mp - m) {
+                    lds.pos[m + lane] = V2<R>{R(kFarSite), R(kFarSite)};
+                    lds.klo[m + lane] = ~0u;
                 }
                 waveSync();
-
-                const R           ext = R(in.ext);
-                const V2<R>       q{R(wk.qx), R(wk.qy)};
-                const R           mg  = relMargin<R>() * fmax(ext, fmax(fabs(q.x), fabs(q.y)) + R(wk.rho));
-                const R           rho = R(wk.rho) + mg;
-                // Squared distance from q to each staged site (two per lane).
-                const int e0 = lane, e1 = lane + 64;
-                R d0 = e0 < total ? dist2(lds.pos[e0], q) : rInf<R>();
-                R d1 = e1 < total ? dist2(lds.pos[e1], q) : rInf<R>();
-                // Bisection: lo < D^2 <= hi for D^2 = q's K-th smallest, i.e.
-                // count(d2 <= lo) < K <= count(d2 <= hi) over the staged sites.
-                R lo = R(0);
-                R hi = waveMaxR<R>(fmax(e0 < total ? d0 : R(0), e1 < total ? d1 : R(0)));
-                for (int it = 0; it < kBisectSteps; ++it) {
-                    R   mid = lo + R(0.5) * (hi - lo);
-                    int c   = __popcll(__ballot(d0 <= mid)) + __popcll(__ballot(d1 <= mid));
-                    if (c >= K) hi = mid;
-                    else lo = mid;
-                }
-                const R D    = sqrtR(hi) + mg;
-                const R thr  = D + R(2) * rho + mg;
-                const R thr2 = thr * thr * (R(1) + relMargin<R>());
-                // The block must contain the disc of radius thr around q: then it
-                // holds the K nearest sites of every target of the chunk.
-                R edge = rInf<R>();
-                if (sx0 > 0) edge = fmin(edge, q.x - R(sx0) * cell);
-                if (sx1 < gw - 1) edge = fmin(edge, R(sx1 + 1) * cell - q.x);
-                if (sy0 > 0) edge = fmin(edge, q.y - R(sy0) * cell);
-                if (sy1 < gh - 1) edge = fmin(edge, R(sy1 + 1) * cell - q.y);
-                fast = edge - R(2) * mg > thr;
-                if (fast) {
-                    // Compact the candidates (d2 <= thr^2) into LDS, the first band
-                    // (d2 <= r1^2, r1 = q's nearest distance + 2 rho: it holds every
-                    // target's nearest site) first.
-                    const R        r1   = sqrtR(waveMinR<R>(fmin(d0, d1))) + R(2) * rho + R(2) * mg;
-                    const R        r1sq = fmin(r1 * r1 * (R(1) + relMargin<R>()), thr2);
-                    const R        r2sq = fmin(fmax(lo, r1sq), thr2); // outer band: r2sq < d2 <= thr2
-                    const uint64_t a0 = __ballot(d0 <= r1sq), a1 = __ballot(d1 <= r1sq);
-                    const uint64_t b0 = __ballot(d0 <= r2sq && !(d0 <= r1sq)), b1 = __ballot(d1 <= r2sq && !(d1 <= r1sq));
-                    const uint64_t c0b = __ballot(d0 <= thr2 && !(d0 <= r2sq)), c1b = __ballot(d1 <= thr2 && !(d1 <= r2sq));
-                    const int      nA = __popcll(a0) + __popcll(a1);
-                    const int      nB = nA + __popcll(b0) + __popcll(b1);
-                    m                 = nB + __popcll(c0b) + __popcll(c1b);
-                    int dst0 = -1, dst1 = -1;
-                    if (d0 <= r1sq) dst0 = laneRank(a0);
-                    else if (d0 <= r2sq) dst0 = nA + laneRank(b0);
-                    else if (d0 <= thr2) dst0 = nB + laneRank(c0b);
-                    if (d1 <= r1sq) dst1 = __popcll(a0) + laneRank(a1);
-                    else if (d1 <= r2sq) dst1 = nA + __popcll(b0) + laneRank(b1);
-                    else if (d1 <= thr2) dst1 = nB + __popcll(c0b) + laneRank(c1b);
-                    V2<R> p0{}, p1{};
-                    int   i0v = 0, i1v = 0;
-                    if (dst0 >= 0) p0 = lds.pos[e0], i0v = lds.idx[e0];
-                    if (dst1 >= 0) p1 = lds.pos[e1], i1v = lds.idx[e1];
-                    waveSync();
-                    if (dst0 >= 0) {
-                        lds.pos[dst0] = p0, lds.idx[dst0] = i0v;
-                        lds.klo[dst0] = (uint32_t(i0v) << kLdsSlotBits) | uint32_t(dst0);
-                    }
-                    if (dst1 >= 0) {
-                        lds.pos[dst1] = p1, lds.idx[dst1] = i1v;
-                        lds.klo[dst1] = (uint32_t(i1v) << kLdsSlotBits) | uint32_t(dst1);
-                    }
-                    // Pad to whole screening groups with far sentinels.
-                    const int mp = (m + kScreenGroup - 1) & ~(kScreenGroup - 1);
-                    if (lane < mp - m) {
-                        lds.pos[m + lane] = V2<R>{R(kFarSite), R(kFarSite)};
-                        lds.klo[m + lane] = ~0u;
-                    }
-                    waveSync();
-                    // Screen bounds (see screenTerm).  Fewer than K staged sites
-                    // lie within sqrt(lo) of q.
-                    sb.mp1     = __builtin_amdgcn_readfirstlane((nA + kScreenGroup - 1) & ~(kScreenGroup - 1));
-                    sb.mp2     = __builtin_amdgcn_readfirstlane((nB + kScreenGroup - 1) & ~(kScreenGroup - 1));
-                    sb.mp      = __builtin_amdgcn_readfirstlane(mp);
-                    sb.r2      = sqrtR(r2sq);
-                    sb.rho     = rho;
-                    sb.lam     = R(4) * (thr + rho);
-                    const R tq = sqrtR(lo) * R(1 - 4e-6) - rho - mg;
-                    sb.t2      = tq > R(0) ? tq * tq : R(0);
-                }
+                // Screen bounds (see screenTerm).  Fewer than K staged sites
+                // lie within sqrt(lo) of q.
+                sb.mp1     = __builtin_amdgcn_readfirstlane((nA + kScreenGroup - 1) & ~(kScreenGroup - 1));
+                sb.mp2     = __builtin_amdgcn_readfirstlane((nB + kScreenGroup - 1) & ~(kScreenGroup - 1));
+                sb.mp      = __builtin_amdgcn_readfirstlane(mp);
+                sb.r2      = sqrtR(r2sq);
+                sb.rho     = rho;
+                const R tq = sqrtR(lo) * R(1 - 4e-6) - rho - mg;
+                sb.t2      = tq > R(0) ? tq * tq : R(0);
             }
         }
 
@@ -1458,10 +1471,19 @@ template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const L
 {
     switch (K) {
 #define VM_K(k)                                                                                                   \
-    case k:                                                                                                        \
-        hipLaunchKernelGGL((lossKernel<k, R>), dim3((grid + kLossWaves - 1) / kLossWaves), dim3(kLossThreads), 0, s, a); \
-        hipLaunchKernelGGL((exactKernel<k, R>), dim3(a.exactGrid), dim3(64 * kExactWaves), 0, s, a);                  \
-        break;
+    case k: hipLaunchKernelGGL((lossKernel<k, R>), dim3((grid + kLossWaves - 1) / kLossWaves), dim3(kLossThreads), 0, s, a); break;
+        VM_K(2) VM_K(4) VM_K(6) VM_K(8) VM_K(12) VM_K(16) VM_K(24) VM_K(32)
+#undef VM_K
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <class R> hipError_t launchExact(int K, hipStream_t s, const LossArgs<R> &a)
+{
+    switch (K) {
+#define VM_K(k)                                                                                                   \
+    case k: hipLaunchKernelGGL((exactKernel<k, R>), dim3(a.exactGrid), dim3(64 * kExactWaves), 0, s, a); break;
         VM_K(2) VM_K(4) VM_K(6) VM_K(8) VM_K(12) VM_K(16) VM_K(24) VM_K(32)
 #undef VM_K
     default: return hipErrorInvalidValue;
@@ -1529,6 +1551,8 @@ template hipError_t launchScatter<double>(int, hipStream_t, const Inst *, const 
                                           int *, int, ScanCtl *);
 template hipError_t launchLoss<float>(int, int, hipStream_t, const LossArgs<float> &);
 template hipError_t launchLoss<double>(int, int, hipStream_t, const LossArgs<double> &);
+template hipError_t launchExact<float>(int, hipStream_t, const LossArgs<float> &);
+template hipError_t launchExact<double>(int, hipStream_t, const LossArgs<double> &);
 template hipError_t launchAdamBin<float>(int, hipStream_t, const AdamArgs<float> &);
 template hipError_t launchAdamBin<double>(int, hipStream_t, const AdamArgs<double> &);
 template hipError_t launchKnnQuery<float>(int, hipStream_t, const Inst *, int, const void *, const int *, const int *,

@@ -357,6 +357,7 @@ private:
             la.exactOnly   = ex && ex[0] == '1';
             la.exactGrid   = vmk::exactGridFor(capTargets_, sizeof(R) == 8 || la.exactOnly);
             VM_HIP(vmk::launchLoss<R>(K, gLoss, stream_, la));
+            VM_HIP(vmk::launchExact<R>(K, stream_, la));
             VM_HIP(vmk::launchAdamBin<R>(gSite, stream_, adamArgs(1)));
         } catch (...) {
             hipGraph_t dummy;

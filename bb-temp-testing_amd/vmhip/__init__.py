@@ -431,7 +431,8 @@ class Batch:
         return ms.value
 
     def profile(self, iterations):
-        out = np.zeros(4, np.float32)
+        """Mean ms per phase: scan, scatter, loss, exact selection, adam."""
+        out = np.zeros(5, np.float32)
         _check(lib().vm_batch_profile(self._b, iterations, _ptr(out)))
         return out
 

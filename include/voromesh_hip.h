@@ -197,8 +197,9 @@ VM_API int vm_batch_choice(vm_batch *b, int id, int *loss_k, float *grid_cell, i
  * least one target needed the exact k-NN selection after screening, [7] the number of
  * such targets. */
 VM_API int vm_batch_loss_stats(vm_batch *b, uint64_t *counters);
-/* Mean of the kernel durations (ms) of the last vm_batch_step, measured with HIP events
- * around each kernel on the batch's stream; phase 0 scan, 1 scatter, 2 loss, 3 adam. */
+/* Runs `iterations` optimizer iterations with HIP events around each phase on the batch's
+ * stream and returns the mean ms per phase in phase_ms[5]: 0 scan, 1 scatter, 2 loss,
+ * 3 exact k-NN selection for the targets the loss kernel could not settle, 4 adam. */
 VM_API int vm_batch_profile(vm_batch *b, int iterations, float *phase_ms);
 
 /* ---- continuous-batching request loop ---------------------------------------- */
