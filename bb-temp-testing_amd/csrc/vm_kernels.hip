@@ -710,9 +710,13 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
 {
     __shared__ LossLds<R> ldsAll[kLossWaves];
     const int  lane  = threadIdx.x & 63;
-    LossLds<R> &lds  = ldsAll[threadIdx.x >> 6];
+    // The wavefront's index is uniform: read it into a scalar register so that
+    // everything derived from the work item (chunk, instance, cell block,
+    // bounds) is scalar arithmetic.
+    const int  wave  = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    LossLds<R> &lds  = ldsAll[wave];
     const int  nWork = *a.nWork;
-    for (int w = blockIdx.x * kLossWaves + (threadIdx.x >> 6); w < nWork; w += gridDim.x * kLossWaves) {
+    for (int w = blockIdx.x * kLossWaves + wave; w < nWork; w += gridDim.x * kLossWaves) {
         const TargetWork wk = a.work[w];
         const Inst      &in = a.insts[wk.inst];
         InstState       *st = a.state + wk.inst;
@@ -771,7 +775,8 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                 // snx <= 7).  Rows of the block are contiguous ranges of the
                 // sorted arrays: row r covers entries [rb[r], rb[r] + len) and
                 // LDS slots [roff[r], roff[r+1]).
-                const int rowMul = (256 + snx - 1) / snx;
+                const int rowMul = snx == 1 ? 256 : snx == 2 ? 128 : snx == 3 ? 86 : snx == 4 ? 64 : snx == 5 ? 52
+                                   : snx == 6 ? 43 : 37; // ceil(256 / snx)
                 int       b = 0, e = 0;
                 if (lane < snx * sny) {
                     const int r = (lane * rowMul) >> 8;
@@ -791,18 +796,17 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                 total = roff[kRows];
                 fast  = total <= kLossStage && total >= K;
                 if (fast) {
+                    // Row by row (the bounds are scalar): entry rb[r] + s goes to
+                    // slot roff[r] + s.
 #pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const int s = lane + 64 * h;
-                        if (s < total) {
-                            int src = rb[0] + s; // the row of slot s: the last r with roff[r] <= s
-#pragma unroll
-                            for (int r = 1; r < kRows; ++r)
-                                if (s >= roff[r]) src = rb[r] + (s - roff[r]);
-                            lds.pos[s]          = a.spos[so + src];
-                            lds.idx[s]          = a.sidx[so + src];
-                            lds.grad[2 * s]     = GradT<R>(0);
-                            lds.grad[2 * s + 1] = GradT<R>(0);
+                    for (int r = 0; r < kRows; ++r) {
+                        if (r >= sny) break;
+                        for (int s = lane; s < roff[r + 1] - roff[r]; s += 64) {
+                            const int d         = roff[r] + s;
+                            lds.pos[d]          = a.spos[so + rb[r] + s];
+                            lds.idx[d]          = a.sidx[so + rb[r] + s];
+                            lds.grad[2 * d]     = GradT<R>(0);
+                            lds.grad[2 * d + 1] = GradT<R>(0);
                         }
                     }
                     waveSync();
@@ -1025,13 +1029,14 @@ template <int K, class R>
 __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
 {
     __shared__ ExactLds<R> ldsAll[kExactWaves];
-    ExactLds<R> &lds  = ldsAll[threadIdx.x >> 6];
+    const int    wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    ExactLds<R> &lds  = ldsAll[wave];
     const int    lane = threadIdx.x & 63;
     const int    n    = *a.exactLen;
     const int    nw   = gridDim.x * kExactWaves;
     // Entry e goes to wavefront e % nw, numbered across workgroups first, so a
     // short queue spreads over the chip.
-    for (int e = blockIdx.x + gridDim.x * int(threadIdx.x >> 6); e < n; e += nw) {
+    for (int e = blockIdx.x + gridDim.x * wave; e < n; e += nw) {
         const ExactItem it       = a.exactQ[e];
         const bool      listOnly = it.inst < 0;
         const int       inst     = it.inst & 0x7FFFFFFF;
@@ -1167,6 +1172,17 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
         const Inst    &in = a.insts[wk.inst];
         InstState     *st = a.state + wk.inst;
         if (a.doAdam && st->done) continue;
+        const int     i    = wk.begin + int(threadIdx.x);
+        const bool    mine = i < wk.begin + wk.count;
+        const int64_t g    = in.siteOff + i;
+        // The site's state is loaded before the loss partials are folded, so
+        // the two latencies overlap.
+        V2<R>    p{}, mo{}, vo{};
+        GradT<R> gx{}, gy{};
+        if (mine) {
+            p = a.pos[g];
+            if (a.doAdam) gx = a.grad[2 * g], gy = a.grad[2 * g + 1], mo = a.m[g], vo = a.v[g];
+        }
         if (a.doAdam) {
             // Fold the step's loss partials: every chunk needs the target count,
             // the instance's first chunk also publishes the totals.
@@ -1182,47 +1198,43 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
             }
             __syncthreads();
         }
-        const long long cnt = a.doAdam ? sCount : 0;
-        int i = wk.begin + int(threadIdx.x);
-        if (i < wk.begin + wk.count) {
-        const int64_t g = in.siteOff + i;
-        V2<R>         p = a.pos[g];
-        if (a.doAdam) {
-            const double ginv = __builtin_ldexp(1.0, -in.gradShift);
-            V2<R>        gr{GradOps<R>::toReal(a.grad[2 * g], ginv), GradOps<R>::toReal(a.grad[2 * g + 1], ginv)};
-            a.grad[2 * g]     = GradT<R>(0);
-            a.grad[2 * g + 1] = GradT<R>(0);
-            bool movable = in.allMovable || !a.frozen[g];
-            if (movable) {
-                const int t  = st->iter;
-                R         lr = R(in.lr);
+        if (mine) {
+            if (a.doAdam) {
+                const long long cnt  = sCount;
+                const double    ginv = __builtin_ldexp(1.0, -in.gradShift);
+                V2<R>           gr{GradOps<R>::toReal(gx, ginv), GradOps<R>::toReal(gy, ginv)};
+                a.grad[2 * g]     = GradT<R>(0);
+                a.grad[2 * g + 1] = GradT<R>(0);
+                bool movable      = in.allMovable || !a.frozen[g];
+                if (movable) {
+                    const int t  = st->iter;
+                    R         lr = R(in.lr);
 #pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (t - 1 >= in.milestones[k]) lr = lr * R(in.gamma);
-                const R b1 = R(in.beta1), b2 = R(in.beta2), eps = R(in.eps);
-                const R bc1 = R(1.0 - powi(double(b1), t));
-                const R bc2 = R(1.0 - powi(double(b2), t));
-                const R inv = R(1) / R(cnt > 1 ? cnt : 1);
-                const R ob1 = R(1) - b1, ob2 = R(1) - b2;
-                V2<R>   gg{inv * gr.x, inv * gr.y};
-                V2<R>   mo = a.m[g], vo = a.v[g];
-                V2<R>   mi{b1 * mo.x + ob1 * gg.x, b1 * mo.y + ob1 * gg.y};
-                V2<R>   vi{b2 * vo.x + ob2 * (gg.x * gg.x), b2 * vo.y + ob2 * (gg.y * gg.y)};
-                a.m[g] = mi;
-                a.v[g] = vi;
-                p.x    = p.x - (lr * (mi.x / bc1)) / (sqrtR(vi.x / bc2) + eps);
-                p.y    = p.y - (lr * (mi.y / bc1)) / (sqrtR(vi.y / bc2) + eps);
-                p.x    = fmin(fmax(p.x, R(0)), R(in.W));
-                p.y    = fmin(fmax(p.y, R(0)), R(in.H));
-                a.pos[g] = p;
+                    for (int k = 0; k < 4; ++k)
+                        if (t - 1 >= in.milestones[k]) lr = lr * R(in.gamma);
+                    const R b1 = R(in.beta1), b2 = R(in.beta2), eps = R(in.eps);
+                    const R bc1 = R(1.0 - powi(double(b1), t));
+                    const R bc2 = R(1.0 - powi(double(b2), t));
+                    const R inv = R(1) / R(cnt > 1 ? cnt : 1);
+                    const R ob1 = R(1) - b1, ob2 = R(1) - b2;
+                    V2<R>   gg{inv * gr.x, inv * gr.y};
+                    V2<R>   mi{b1 * mo.x + ob1 * gg.x, b1 * mo.y + ob1 * gg.y};
+                    V2<R>   vi{b2 * vo.x + ob2 * (gg.x * gg.x), b2 * vo.y + ob2 * (gg.y * gg.y)};
+                    a.m[g] = mi;
+                    a.v[g] = vi;
+                    p.x    = p.x - (lr * (mi.x / bc1)) / (sqrtR(vi.x / bc2) + eps);
+                    p.y    = p.y - (lr * (mi.y / bc1)) / (sqrtR(vi.y / bc2) + eps);
+                    p.x    = fmin(fmax(p.x, R(0)), R(in.W));
+                    p.y    = fmin(fmax(p.y, R(0)), R(in.H));
+                    a.pos[g] = p;
+                }
             }
+            const R   inv = R(1) / R(in.cell);
+            const int c   = cellCoord(p.y, inv, in.gh) * in.gw + cellCoord(p.x, inv, in.gw);
+            a.cellOf[g]   = c;
+            a.rank[g]     = atomicAdd(&a.counts[in.cellOff + c], 1);
         }
-        const R   inv = R(1) / R(in.cell);
-        const int c   = cellCoord(p.y, inv, in.gh) * in.gw + cellCoord(p.x, inv, in.gw);
-        a.cellOf[g]   = c;
-        a.rank[g]     = atomicAdd(&a.counts[in.cellOff + c], 1);
-        }
-        __syncthreads();
+        if (a.doAdam) __syncthreads(); // sCount is rewritten by the next chunk
     }
 }
 
