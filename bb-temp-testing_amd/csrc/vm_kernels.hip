@@ -1005,6 +1005,7 @@ template <int K, class R> __device__ __forceinline__ int keepNearest(ExactLds<R>
     for (int j = lane; j < nc; j += 64) {
         const Key<R> kj   = lds.cand[j];
         int          rank = 0;
+#pragma unroll 8
         for (int i = 0; i < nc; ++i) rank += lds.cand[i].lt(kj) ? 1 : 0;
         if (rank < K) lds.top[rank] = kj;
     }
