@@ -154,7 +154,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
 
 namespace {
 
-constexpr int kMaxLossGrid = 16384;
+constexpr int kMaxLossGrid = 65536; // wavefronts (measured: beats both the resident count and one per chunk)
 constexpr int kMaxSiteGrid = 65536; // one site chunk per workgroup up to 16.7 M sites
 constexpr int kMaxScanGrid = 4096;
 

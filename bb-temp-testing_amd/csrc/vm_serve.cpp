@@ -339,7 +339,7 @@ private:
 
     void buildGraph(int K)
     {
-        const int gScan = std::min(maxTiles_, 1024), gLoss = 16384;
+        const int gScan = std::min(maxTiles_, 1024), gLoss = 65536;
         const int gSite = int(std::min<int64_t>(capSites_ / vmk::kSiteChunk + slots_, 65536));
         hipGraph_t g;
         VM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
