@@ -131,12 +131,13 @@ struct vm_handle {
             s.m.push_back(slotM[sl].x), s.m.push_back(slotM[sl].y);
             s.v.push_back(slotV[sl].x), s.v.push_back(slotV[sl].y);
         }
-        if (prm.grid_cell <= 0.0f || prm.stage_rings <= 0) {
-            float cell;
+        {
+            float cell, ce;
             int   rs;
-            vmrt::tunedChoice(std::max(int(slotOfSite.size()), 1), W, H, s.K, &cell, &rs);
-            s.cell   = prm.grid_cell > 0.0f ? prm.grid_cell : cell;
-            s.stageR = prm.stage_rings > 0 ? prm.stage_rings : rs;
+            vmrt::tunedChoice(std::max(int(slotOfSite.size()), 1), W, H, s.K, &cell, &rs, &ce);
+            s.cell     = prm.grid_cell > 0.0f ? prm.grid_cell : cell;
+            s.stageR   = prm.stage_rings > 0 ? prm.stage_rings : rs;
+            s.chunkExt = ce;
         }
         eng->dirty = true;
     }
@@ -237,11 +238,12 @@ int vm_create(float width, float height, float spacing, const vm_params *params,
         s.K        = h->prm.loss_k;
         s.fraction = h->prm.sample_fraction;
         s.seed     = h->prm.seed;
-        float cell;
+        float cell, ce;
         int   rs;
-        vmrt::tunedChoice(1, width, height, s.K, &cell, &rs);
-        s.cell   = h->prm.grid_cell > 0.0f ? h->prm.grid_cell : 2.0f * spacing;
-        s.stageR = h->prm.stage_rings > 0 ? h->prm.stage_rings : rs;
+        vmrt::tunedChoice(1, width, height, s.K, &cell, &rs, &ce);
+        s.cell     = h->prm.grid_cell > 0.0f ? h->prm.grid_cell : 2.0f * spacing;
+        s.stageR   = h->prm.stage_rings > 0 ? h->prm.stage_rings : rs;
+        s.chunkExt = ce;
         h->eng->add(std::move(s));
         *out = h.release();
     });

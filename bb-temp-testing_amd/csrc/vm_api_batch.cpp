@@ -272,11 +272,12 @@ InstSpec specFromDesc(const vm_fit_desc &d)
     s.fraction = d.sample_fraction, s.seed = d.seed;
     s.K        = d.loss_k > 0 ? d.loss_k : 16;
     VM_REQUIRE(vmk::lossKSupported(s.K), "unsupported loss_k");
-    float cell;
+    float cell, ce;
     int   rs;
-    tunedChoice(std::max(int(s.pos.size() / 2), 1), d.width, d.height, s.K, &cell, &rs);
-    s.cell    = d.grid_cell > 0 ? d.grid_cell : cell;
-    s.stageR  = d.stage_rings > 0 ? d.stage_rings : rs;
+    tunedChoice(std::max(int(s.pos.size() / 2), 1), d.width, d.height, s.K, &cell, &rs, &ce);
+    s.cell     = d.grid_cell > 0 ? d.grid_cell : cell;
+    s.stageR   = d.stage_rings > 0 ? d.stage_rings : rs;
+    s.chunkExt = d.chunk_extent > 0 ? d.chunk_extent : ce;
     s.maxIter = d.max_iterations, s.tol = d.tol;
     return s;
 }
@@ -394,7 +395,7 @@ int vm_batch_loss_stats(vm_batch *b, uint64_t *counters)
     });
 }
 
-int vm_batch_choice(vm_batch *b, int id, int *k, float *cell, int *rings)
+int vm_batch_choice(vm_batch *b, int id, int *k, float *cell, int *rings, float *chunkExtent)
 {
     return vmrt::guardedCall([&] {
         VM_REQUIRE(b && id >= 0 && id < int(b->eng->specs.size()), "invalid arguments");
@@ -402,6 +403,7 @@ int vm_batch_choice(vm_batch *b, int id, int *k, float *cell, int *rings)
         if (k) *k = s.K;
         if (cell) *cell = s.cell;
         if (rings) *rings = s.stageR;
+        if (chunkExtent) *chunkExtent = s.chunkExt;
     });
 }
 

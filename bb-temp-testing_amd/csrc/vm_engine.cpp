@@ -1,6 +1,7 @@
 // vm_engine.cpp -- pooled layout, work tables and launch sequencing for a batch
 // of VoroMesh instances (see vm_engine.h).
 #include "vm_engine.h"
+#include "vm_tune_table.h"
 
 #include <algorithm>
 #include <cmath>
@@ -40,15 +41,19 @@ int guardedCall(const std::function<void()> &f)
 
 const char *lastError() { return t_lastError.c_str(); }
 
-void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR)
+void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR, float *chunkExt)
 {
-    // Mean site spacing a = sqrt(area / S).
-    // The loss kernel's fast path needs two staged rings to cover the K-NN
-    // radius (~a*sqrt(K/pi)) plus the chunk extent (<= 0.7a diagonal).
-    double a = std::sqrt(double(W) * double(H) / double(std::max(numSites, 1)));
-    double f = std::max(1.5, 0.5 * (std::sqrt(double(K) / 3.141592653589793) + 0.8));
-    *cell    = float(f * a);
-    *stageR  = 2;
+    // Mean site spacing a = sqrt(area / S).  The loss kernel's fast path needs
+    // two staged rings to cover the K-NN radius (~a sqrt(K/pi)) plus the chunk
+    // extent: the base cell is max(1.5, (sqrt(K/pi) + 0.8) / 2) spacings, scaled
+    // by the measured factor of the size class.
+    const double a   = std::sqrt(double(W) * double(H) / double(std::max(numSites, 1)));
+    const double f   = std::max(1.5, 0.5 * (std::sqrt(double(K) / 3.141592653589793) + 0.8));
+    const int    lg  = int(std::floor(std::log2(double(std::max(numSites, 1)))));
+    const auto  &row = kTuneTable[std::min(std::max(lg, kTuneLog2Min), kTuneLog2Max) - kTuneLog2Min];
+    *cell            = float(f * row.cellScale * a);
+    *stageR          = 2;
+    *chunkExt        = row.chunkExtent;
 }
 
 namespace {
@@ -130,7 +135,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         return cellOfT[a] != cellOfT[b] ? cellOfT[a] < cellOfT[b] : keyT[a] < keyT[b];
     });
     const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(S, 1)));
-    const double maxDiag = 0.7 * spacing;
+    const double maxDiag = double(s.chunkExt > 0.0f ? s.chunkExt : 1.6f) * spacing;
     for (int t = 0; t < T;) {
         int    c = cellOfT[perm[t]], e = t;
         double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;

@@ -66,6 +66,7 @@ struct InstSpec {
     int    K = 16;
     float  cell = 0.0f;      // hash-grid cell
     int    stageR = 2;
+    float  chunkExt = 1.6f;  // target chunk diagonal, in mean site spacings
     int    useBox = 0;
     float  box[4] = {0, 0, 0, 0};
     int    maxIter = 0;
@@ -137,6 +138,8 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
 int gradShiftFor(int numTargets, float ext);
 
 // Default per-size-class choices (hash cell in mean spacings, staged rings).
-void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR);
+// The tuner's per-size-class choice (vm_tune_table.h): hash-grid cell, staged rings
+// and target chunk extent (mean spacings) for a fit of numSites sites with k = K.
+void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR, float *chunkExt);
 
 } // namespace vmrt

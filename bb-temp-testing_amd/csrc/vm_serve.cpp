@@ -415,9 +415,10 @@ void serve(const vm_fit_desc *reqs, const double *arrival, int n, const vm_serve
         if (i > 0 && arrival[i] < arrival[i - 1]) throw VmError(VM_EINVAL, "arrival times must be non-decreasing");
         needS += RangeAlloc::round(reqs[i].num_sites);
         needT += RangeAlloc::round(reqs[i].num_targets);
-        float cell = reqs[i].grid_cell;
+        float cell = reqs[i].grid_cell, ce;
         int   rs;
-        if (!(cell > 0.0f)) vmrt::tunedChoice(std::max(reqs[i].num_sites, 1), reqs[i].width, reqs[i].height, k, &cell, &rs);
+        if (!(cell > 0.0f))
+            vmrt::tunedChoice(std::max(reqs[i].num_sites, 1), reqs[i].width, reqs[i].height, k, &cell, &rs, &ce);
         needC += RangeAlloc::round(int64_t(std::ceil(reqs[i].width / cell)) * int64_t(std::ceil(reqs[i].height / cell)) + 1);
     }
     Server<R> srv(cfg, n, needS, needT, needC);

@@ -52,7 +52,7 @@ class FitDesc(C.Structure):
                 ("targets", C.c_void_p), ("num_targets", C.c_int), ("lr", C.c_float), ("gamma", C.c_float),
                 ("milestones", C.c_int * 4), ("sample_fraction", C.c_float), ("seed", C.c_uint),
                 ("loss_k", C.c_int), ("grid_cell", C.c_float), ("stage_rings", C.c_int),
-                ("max_iterations", C.c_int), ("tol", C.c_float)]
+                ("max_iterations", C.c_int), ("tol", C.c_float), ("chunk_extent", C.c_float)]
 
 
 class ServeConfig(C.Structure):
@@ -133,7 +133,7 @@ def lib():
         "vm_batch_step": [vp, i32, P(f32)],
         "vm_batch_get_sites": [vp, i32, vp, i32],
         "vm_batch_status": [vp, i32, P(i32), P(i32), P(f64)],
-        "vm_batch_choice": [vp, i32, P(i32), P(f32), P(i32)],
+        "vm_batch_choice": [vp, i32, P(i32), P(f32), P(i32), P(f32)],
         "vm_batch_profile": [vp, i32, vp],
         "vm_batch_loss_stats": [vp, vp],
         "vm_serve": [P(FitDesc), vp, i32, P(ServeConfig), vp, P(ServeStats)],
@@ -378,7 +378,7 @@ def poisson_batch(widths, heights, radii, seeds, trials=30):
 
 
 def fit_desc(width, height, sites, targets, lr, milestones=(80, 150, 200, 250), gamma=0.5, sample_fraction=1.0,
-             seed=1, loss_k=16, grid_cell=0.0, stage_rings=0, max_iterations=0, tol=0.0):
+             seed=1, loss_k=16, grid_cell=0.0, stage_rings=0, max_iterations=0, tol=0.0, chunk_extent=0.0):
     """A FitDesc plus the arrays it points to (keep the pair alive together)."""
     s, t = _pts(sites), _pts(targets)
     d = FitDesc()
@@ -390,6 +390,7 @@ def fit_desc(width, height, sites, targets, lr, milestones=(80, 150, 200, 250), 
         d.milestones[i] = int(milestones[i])
     d.sample_fraction, d.seed, d.loss_k = sample_fraction, seed, loss_k
     d.grid_cell, d.stage_rings, d.max_iterations, d.tol = grid_cell, stage_rings, max_iterations, tol
+    d.chunk_extent = chunk_extent
     return d, (s, t)
 
 
@@ -454,9 +455,9 @@ class Batch:
         return {k: int(v) for k, v in zip(keys, c)}
 
     def choice(self, i):
-        k, cell, rings = C.c_int(), C.c_float(), C.c_int()
-        _check(lib().vm_batch_choice(self._b, i, C.byref(k), C.byref(cell), C.byref(rings)))
-        return dict(loss_k=k.value, grid_cell=cell.value, stage_rings=rings.value)
+        k, cell, rings, ext = C.c_int(), C.c_float(), C.c_int(), C.c_float()
+        _check(lib().vm_batch_choice(self._b, i, C.byref(k), C.byref(cell), C.byref(rings), C.byref(ext)))
+        return dict(loss_k=k.value, grid_cell=cell.value, stage_rings=rings.value, chunk_extent=ext.value)
 
 
 def serve(requests, arrivals, hbm_budget_bytes=200e9, max_resident=0, max_iterations=300, tol=1e-6, poll_every=10,

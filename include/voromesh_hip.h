@@ -174,6 +174,7 @@ typedef struct vm_fit_desc {
     int            stage_rings;     /* 0: tuner choice */
     int            max_iterations;  /* 0: no limit */
     float          tol;             /* stop when |dL/L| < tol between steps (0: off) */
+    float          chunk_extent;    /* target chunk diagonal in mean site spacings (0: tuner choice) */
 } vm_fit_desc;
 
 typedef struct vm_batch vm_batch;
@@ -188,8 +189,9 @@ VM_API int vm_batch_step(vm_batch *b, int iterations, float *gpu_ms);
 VM_API int vm_batch_get_sites(vm_batch *b, int id, vm_vec2 *out, int n);
 /* Per instance: completed iterations, done flag and the mean loss of its last step. */
 VM_API int vm_batch_status(vm_batch *b, int id, int *iteration, int *done, double *last_mean_loss);
-/* Resolved per-instance choices (after commit): k, grid cell, staged rings. */
-VM_API int vm_batch_choice(vm_batch *b, int id, int *loss_k, float *grid_cell, int *stage_rings);
+/* Resolved per-instance choices: k, grid cell, staged rings, target chunk extent (mean spacings). */
+VM_API int vm_batch_choice(vm_batch *b, int id, int *loss_k, float *grid_cell, int *stage_rings,
+                           float *chunk_extent);
 /* One evaluation pass of the loss kernel over every instance, counting its wavefronts by
  * path: [0] bounding-box fast path, [1] ring search over the staged block, [2] ring
  * search in global memory (staging overflow), [3] no active target; [4] candidates
