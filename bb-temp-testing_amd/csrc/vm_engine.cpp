@@ -87,7 +87,7 @@ int gradShiftFor(int numTargets, float ext)
 
 void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff, int64_t targetOff, vmk::Inst &in,
                     std::vector<vmk::SiteWork> &siteWork, std::vector<vmk::TargetWork> &targetWork,
-                    std::vector<int> &perm)
+                    std::vector<int> &perm, float extentCap)
 {
     std::memset(&in, 0, sizeof(in));
     const int S = int(s.pos.size() / 2), T = int(s.tgt.size() / 2);
@@ -135,7 +135,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         return cellOfT[a] != cellOfT[b] ? cellOfT[a] < cellOfT[b] : keyT[a] < keyT[b];
     });
     const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(S, 1)));
-    const double maxDiag = double(s.chunkExt > 0.0f ? s.chunkExt : 1.6f) * spacing;
+    const double maxDiag = double(std::min(s.chunkExt > 0.0f ? s.chunkExt : 1.6f, extentCap)) * spacing;
     for (int t = 0; t < T;) {
         int    c = cellOfT[perm[t]], e = t;
         double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
@@ -236,14 +236,21 @@ public:
         L.inst.resize(n);
         L.perm.resize(n);
         std::vector<std::vector<vmk::TargetWork>> twByInst(n);
-        for (int i = 0; i < n; ++i) {
-            InstSpec &s  = specs[i];
-            vmk::Inst  &in = L.inst[i];
-            layoutInstance(s, i, L.sites, L.cells, L.targets, in, L.siteWork, twByInst[i], L.perm[i]);
-            L.sites += align64(in.S);
-            L.cells += align64(int64_t(in.gw) * in.gh + 1);
-            L.targets += align64(in.T);
-
+        for (float cap : {1e30f, kSmallLayoutExtent}) {
+            L.sites = L.cells = L.targets = 0;
+            L.siteWork.clear();
+            size_t chunks = 0;
+            for (int i = 0; i < n; ++i) {
+                InstSpec  &s  = specs[i];
+                vmk::Inst &in = L.inst[i];
+                twByInst[i].clear();
+                layoutInstance(s, i, L.sites, L.cells, L.targets, in, L.siteWork, twByInst[i], L.perm[i], cap);
+                L.sites += align64(in.S);
+                L.cells += align64(int64_t(in.gw) * in.gh + 1);
+                L.targets += align64(in.T);
+                chunks += twByInst[i].size();
+            }
+            if (chunks >= size_t(kMinLayoutChunks)) break;
         }
         // Scan tiles: the tiles of multi-tile instances first (look-back), then
         // the single-tile instances.

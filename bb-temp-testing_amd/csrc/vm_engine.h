@@ -129,9 +129,16 @@ public:
 // Lays out one instance at the given pooled offsets: its Inst record, site
 // chunks, target chunks (one wavefront each) and the sort permutation of its
 // targets (sorted position -> input index).
+// extentCap bounds the chunk extent (mean spacings) below the spec's choice.
 void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff, int64_t targetOff, vmk::Inst &in,
                     std::vector<vmk::SiteWork> &siteWork, std::vector<vmk::TargetWork> &targetWork,
-                    std::vector<int> &perm);
+                    std::vector<int> &perm, float extentCap = 1e30f);
+
+// A layout with fewer target chunks than this (one wavefront each) is re-chunked
+// with extents of at most kSmallLayoutExtent spacings, so small fits still spread
+// over the chip.
+constexpr int   kMinLayoutChunks   = 8192;
+constexpr float kSmallLayoutExtent = 0.7f;
 
 // Fixed-point shift of the FP32 gradient sums of an instance with T targets in
 // a domain of extent `ext`.
