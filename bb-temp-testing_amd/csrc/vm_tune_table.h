@@ -13,21 +13,21 @@ struct TuneRow {
 constexpr int     kTuneLog2Min = 8;
 constexpr int     kTuneLog2Max = 22;
 constexpr TuneRow kTuneTable[kTuneLog2Max - kTuneLog2Min + 1] = {
-    {1.00f, 2.20f}, // 2^8: measured 1.546e+09 site-iters/s
-    {1.00f, 2.20f}, // 2^9: measured 2.077e+09 site-iters/s
-    {1.00f, 2.20f}, // 2^10: measured 2.641e+09 site-iters/s
-    {1.00f, 2.20f}, // 2^11: measured 3.070e+09 site-iters/s
-    {1.00f, 2.20f}, // 2^12: measured 3.374e+09 site-iters/s
-    {1.00f, 1.60f}, // 2^13: measured 3.525e+09 site-iters/s
-    {1.00f, 1.60f}, // 2^14: measured 3.485e+09 site-iters/s
-    {1.00f, 2.20f}, // 2^15: measured 3.611e+09 site-iters/s
-    {1.00f, 1.60f}, // 2^16: measured 3.672e+09 site-iters/s
-    {1.00f, 2.20f}, // 2^17: measured 3.793e+09 site-iters/s
-    {0.90f, 2.20f}, // 2^18: measured 3.645e+09 site-iters/s
-    {0.90f, 2.20f}, // 2^19: largest measured class
-    {0.90f, 2.20f}, // 2^20: largest measured class
-    {0.90f, 2.20f}, // 2^21: largest measured class
-    {0.90f, 2.20f}, // 2^22: largest measured class
+    {1.00f, 2.20f}, // 2^8: measured 1.515e+09 site-iters/s
+    {1.00f, 2.20f}, // 2^9: measured 2.062e+09 site-iters/s
+    {1.00f, 2.20f}, // 2^10: measured 2.613e+09 site-iters/s
+    {1.00f, 3.00f}, // 2^11: measured 3.030e+09 site-iters/s
+    {1.00f, 2.20f}, // 2^12: measured 3.355e+09 site-iters/s
+    {1.00f, 2.20f}, // 2^13: measured 3.588e+09 site-iters/s
+    {1.00f, 3.00f}, // 2^14: measured 3.569e+09 site-iters/s
+    {1.00f, 2.20f}, // 2^15: measured 3.764e+09 site-iters/s
+    {1.00f, 2.20f}, // 2^16: measured 3.852e+09 site-iters/s
+    {1.00f, 3.00f}, // 2^17: measured 3.902e+09 site-iters/s
+    {1.00f, 1.60f}, // 2^18: measured 3.817e+09 site-iters/s
+    {1.00f, 1.60f}, // 2^19: largest measured class
+    {1.00f, 1.60f}, // 2^20: largest measured class
+    {1.00f, 1.60f}, // 2^21: largest measured class
+    {1.00f, 1.60f}, // 2^22: largest measured class
 };
 
 } // namespace vmrt

@@ -18,7 +18,7 @@ import vmhip
 from vmhip import workloads as wl
 
 CELL_SCALES = (0.9, 1.0, 1.15)
-CHUNK_EXTENTS = (0.7, 1.0, 1.6, 2.2)
+CHUNK_EXTENTS = (1.0, 1.6, 2.2, 3.0)
 LOG2_MIN, LOG2_MAX = 8, 22
 
 
