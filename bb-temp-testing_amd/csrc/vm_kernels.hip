@@ -1163,10 +1163,18 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
 // 4. Adam + MultiStepLR + clamp (voro_optimizer.cu:268-301), then bin the new
 //    position for the next step (voro_optimizer.cu:168-177).
 // ---------------------------------------------------------------------------
+// Cells a site chunk bins through LDS: when the chunk's new cells span at most
+// this many consecutive cell indices (sites given in a spatially coherent
+// order, as grid slots are), the chunk counts its sites per cell in LDS and
+// takes one global atomic per occupied cell; otherwise one per site.
+constexpr int kBinSpan = 2048;
+
 template <class R>
 __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
 {
     __shared__ long long sCount;
+    __shared__ int       sBin[kBinSpan];
+    __shared__ int       sLo[kSiteChunk / 64], sHi[kSiteChunk / 64];
     const int nWork = *a.nWork;
     for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
         const SiteWork wk = a.work[w];
@@ -1230,12 +1238,39 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
                     a.pos[g] = p;
                 }
             }
-            const R   inv = R(1) / R(in.cell);
-            const int c   = cellCoord(p.y, inv, in.gh) * in.gw + cellCoord(p.x, inv, in.gw);
-            a.cellOf[g]   = c;
-            a.rank[g]     = atomicAdd(&a.counts[in.cellOff + c], 1);
         }
-        if (a.doAdam) __syncthreads(); // sCount is rewritten by the next chunk
+        int c = 0;
+        if (mine) {
+            const R inv = R(1) / R(in.cell);
+            c           = cellCoord(p.y, inv, in.gh) * in.gw + cellCoord(p.x, inv, in.gw);
+            a.cellOf[g] = c;
+        }
+        // Rank of the site within its cell for the counting sort (any order
+        // within a cell gives the same results: the k-NN order is (d2, index)).
+        const int wv = int(threadIdx.x) >> 6;
+        const int lo = waveReduce32<int>(mine ? c : INT_MAX, INT_MAX, [](int x, int y) { return min(x, y); });
+        const int hi = waveReduce32<int>(mine ? c : -1, -1, [](int x, int y) { return max(x, y); });
+        if ((threadIdx.x & 63) == 0) sLo[wv] = lo, sHi[wv] = hi;
+        __syncthreads();
+        int cLo = sLo[0], cHi = sHi[0];
+#pragma unroll
+        for (int k = 1; k < kSiteChunk / 64; ++k) cLo = min(cLo, sLo[k]), cHi = max(cHi, sHi[k]);
+        if (cHi >= cLo && cHi - cLo < kBinSpan) {
+            const int span = cHi - cLo + 1;
+            for (int b = int(threadIdx.x); b < span; b += kSiteChunk) sBin[b] = 0;
+            __syncthreads();
+            const int local = mine ? atomicAdd(&sBin[c - cLo], 1) : 0;
+            __syncthreads();
+            for (int b = int(threadIdx.x); b < span; b += kSiteChunk) {
+                const int n = sBin[b];
+                if (n) sBin[b] = atomicAdd(&a.counts[in.cellOff + cLo + b], n);
+            }
+            __syncthreads();
+            if (mine) a.rank[g] = sBin[c - cLo] + local;
+        } else if (mine) {
+            a.rank[g] = atomicAdd(&a.counts[in.cellOff + c], 1);
+        }
+        __syncthreads(); // sCount, sBin and the bounds are rewritten by the next chunk
     }
 }
 
