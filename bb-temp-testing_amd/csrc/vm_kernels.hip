@@ -52,6 +52,20 @@ __device__ __forceinline__ float rand01_poisson(uint32_t cell, uint32_t round, u
     return float(h & 0x00FFFFFFu) * (1.0f / 16777216.0f);
 }
 
+// Wave-uniform records that no kernel writes while it runs (instances and work
+// tables, written by the host; queue entries and instance states written by an
+// earlier launch) are read through the constant address space: the loads are
+// scalar, their fields stay in scalar registers, and stores elsewhere in the
+// kernel do not force re-reads.  The address must be wave-uniform.
+template <class T> __device__ __forceinline__ T loadUniform(const T *p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return *(const __attribute__((address_space(4))) T *)p;
+#else
+    return *p; // host pass only parses device code
+#endif
+}
+
 __device__ __forceinline__ float  fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float  sqrtR(float a) { return sqrtf(a); }
@@ -705,8 +719,11 @@ __device__ __forceinline__ int laneRank(uint64_t mask)
     return int(__builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u)));
 }
 
+// At most 72 VGPRs, so 7 wavefronts share a SIMD: the kernel is bound by issue
+// and by the latency of its per-chunk loads, and occupancy hides the latter
+// (K = 4 and 6 would spill at 72 and take 6).
 template <int K, class R>
-__global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
+__global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K == 4 || K == 6 ? 6 : 7))) void lossKernel(LossArgs<R> a)
 {
     __shared__ LossLds<R> ldsAll[kLossWaves];
     const int  lane  = threadIdx.x & 63;
@@ -717,10 +734,10 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
     LossLds<R> &lds  = ldsAll[wave];
     const int  nWork = *a.nWork;
     for (int w = blockIdx.x * kLossWaves + wave; w < nWork; w += gridDim.x * kLossWaves) {
-        const TargetWork wk = a.work[w];
-        const Inst      &in = a.insts[wk.inst];
+        const TargetWork wk = loadUniform(a.work + w);
+        const Inst       in = loadUniform(a.insts + wk.inst);
         InstState       *st = a.state + wk.inst;
-        if (a.mode == kModeIterate && st->done) continue;
+        if (a.mode == kModeIterate && loadUniform(&st->done)) continue;
         const int64_t so = in.siteOff;
 
         // ---- this lane's targets: lane + 64 h, h < kTargetsPerLane -------
@@ -737,7 +754,7 @@ __global__ __launch_bounds__(kLossThreads) void lossKernel(LossArgs<R> a)
                 if (take[h] && in.useBox && a.mode == kModeIterate)
                     take[h] = p[h].x >= R(in.bx0) && p[h].x <= R(in.bx1) && p[h].y >= R(in.by0) && p[h].y <= R(in.by1);
                 if (take[h] && a.mode == kModeIterate && in.fraction < 1.0f)
-                    take[h] = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(st->iter), in.seed) < in.fraction;
+                    take[h] = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(loadUniform(&st->iter)), in.seed) < in.fraction;
             }
             anyTake = anyTake || take[h];
         }
@@ -1038,10 +1055,10 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
     // Entry e goes to wavefront e % nw, numbered across workgroups first, so a
     // short queue spreads over the chip.
     for (int e = blockIdx.x + gridDim.x * wave; e < n; e += nw) {
-        const ExactItem it       = a.exactQ[e];
+        const ExactItem it       = loadUniform(a.exactQ + e);
         const bool      listOnly = it.inst < 0;
         const int       inst     = it.inst & 0x7FFFFFFF;
-        const Inst     &in       = a.insts[inst];
+        const Inst      in       = loadUniform(a.insts + inst);
         const int64_t   so       = in.siteOff;
         const V2<R>    *spos     = a.spos + so;
         const int      *sidx     = a.sidx + so;
@@ -1177,10 +1194,10 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
     __shared__ int       sLo[kSiteChunk / 64], sHi[kSiteChunk / 64];
     const int nWork = *a.nWork;
     for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
-        const SiteWork wk = a.work[w];
-        const Inst    &in = a.insts[wk.inst];
+        const SiteWork wk = loadUniform(a.work + w);
+        const Inst     in = loadUniform(a.insts + wk.inst);
         InstState     *st = a.state + wk.inst;
-        if (a.doAdam && st->done) continue;
+        if (a.doAdam && loadUniform(&st->done)) continue;
         const int     i    = wk.begin + int(threadIdx.x);
         const bool    mine = i < wk.begin + wk.count;
         const int64_t g    = in.siteOff + i;
@@ -1216,7 +1233,7 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
                 a.grad[2 * g + 1] = GradT<R>(0);
                 bool movable      = in.allMovable || !a.frozen[g];
                 if (movable) {
-                    const int t  = st->iter;
+                    const int t  = loadUniform(&st->iter);
                     R         lr = R(in.lr);
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
