@@ -503,16 +503,43 @@ struct ScreenBounds {
     float rho;  // >= |p - q| for every target of the chunk
 };
 
+// Packed screening keys: |v| (a 32-bit float) with its low 7 bits replaced by
+// the candidate's slot, so unsigned min / median-of-three order them by
+// magnitude and carry the slot.  With the mask in a vector register
+// (packMask()) the packing is one v_and_or_b32 per key.
+constexpr uint32_t kPackMask = 0x7FFFFFFFu & ~uint32_t(kLossStage - 1);
+constexpr uint32_t kPackInf  = 0x7F800000u; // packed keys below this hold finite values
+
+__device__ __forceinline__ uint32_t packMask()
+{
+    uint32_t m;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(m) : "i"(kPackMask));
+    return m;
+}
+__device__ __forceinline__ uint32_t packKey(uint32_t mask, float v, int slot)
+{
+    return (__float_as_uint(v) & mask) | uint32_t(slot);
+}
+// Two smallest keys: a2 = median(a1, k, a2) then a1 = min(a1, k).
+__device__ __forceinline__ void keepTwo(uint32_t &a1, uint32_t &a2, uint32_t k)
+{
+    a2 = max(min(a1, k), min(max(a1, k), a2));
+    a1 = min(a1, k);
+}
+
 // FP32 screening of one lane's VoroLoss term over the wave's compacted
 // candidates (a superset of the lane's K nearest sites).
-//   1. The nearest candidate, c0, under the (d2, index) order, searched among
-//      the first band (sites within D1 + 2 rho of q, D1 = q's nearest distance:
-//      a target's nearest site is no farther from q).
+//   1. The nearest candidate, c0, searched among the first band (sites within
+//      D1 + 2 rho of q, D1 = q's nearest distance: a target's nearest site is no
+//      farther from q) with packed keys of d2.  The packing truncates d2 by at
+//      most 2^-16 relative: when the two smallest keys differ above the slot
+//      bits, the smallest is strictly the nearest (d2 below the other's
+//      truncated value), which is the (d2, index) order's first; otherwise the
+//      lane takes the exact selection.
 //   2. An approximation a_i of |distance to the bisector (c0, c_i)| with the
-//      hardware reciprocal square root, a_i = rsq(|e|^2) (<u, e> - |e|^2 / 2);
-//      a_i is packed with i in its low 7 bits and the two smallest are kept
-//      (min / median-of-three).  c0 itself, sites on top of it and the
-//      sentinels give NaN and never win.
+//      hardware reciprocal square root, a_i = rsq(|e|^2) (<u, e> - |e|^2 / 2),
+//      packed; the two smallest are kept.  c0 itself, sites on top of it and
+//      the sentinels give NaN and never win.
 //      Candidates of the outer band (farther than r2 from q) are skipped when
 //      their lower bound (r2 - rho - |p - c0|) / 2 exceeds every active lane's
 //      runner-up.
@@ -535,23 +562,21 @@ struct ScreenBounds {
 // the caller runs the exact selection for that lane.
 template <int K>
 __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const ScreenBounds &sb, V2<float> p, bool active,
-                                           LaneTerm<float> &o)
+                                           uint32_t mask, LaneTerm<float> &o)
 {
     constexpr uint32_t kIdxMask = uint32_t(kLossStage - 1);
     static_assert((kLossStage & (kLossStage - 1)) == 0, "slot mask");
-    Key<float> kmin = Key<float>::none();
+    uint32_t b1 = ~0u, b2 = ~0u;
 #pragma unroll 1
     for (int g = 0; g < sb.mp1; g += kScreenGroup) {
 #pragma unroll
-        for (int k = 0; k < kScreenGroup; ++k) {
-            const Key<float> key = Key<float>::make(dist2(lds.pos[g + k], p), lds.klo[g + k]);
-            if (key.lt(kmin)) kmin = key;
-        }
+        for (int k = 0; k < kScreenGroup; ++k) keepTwo(b1, b2, packKey(mask, dist2(lds.pos[g + k], p), g + k));
     }
-    const int       s0 = keySlot(kmin.low());
-    const V2<float> c0 = lds.pos[s0];
+    const bool      nearOk = (b1 ^ b2) > kIdxMask;
+    const int       s0     = int(b1 & kIdxMask);
+    const V2<float> c0     = lds.pos[s0];
     const V2<float> u{p.x - c0.x, p.y - c0.y};
-    const float     uB = sqrtf(kmin.d2()) * (1.0f + 0x1p-20f); // >= |u|
+    const float     uB = sqrtf(__uint_as_float(b1 | kIdxMask)) * (1.0f + 0x1p-20f); // >= |u|
     uint32_t        a1 = ~0u, a2 = ~0u;
     auto            screenGroups = [&](int g0, int g1) {
 #pragma unroll 1
@@ -562,10 +587,7 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
                 const float     ex = c.x - c0.x, ey = c.y - c0.y;
                 const float     ee = __builtin_fmaf(ey, ey, ex * ex);
                 const float     ue = __builtin_fmaf(u.y, ey, u.x * ex);
-                const float     d  = __builtin_amdgcn_rsqf(ee) * __builtin_fmaf(-0.5f, ee, ue);
-                const uint32_t  av = (__float_as_uint(d) & (0x7FFFFFFFu & ~kIdxMask)) | uint32_t(g + k);
-                a2                 = max(min(a1, av), min(max(a1, av), a2));
-                a1                 = min(a1, av);
+                keepTwo(a1, a2, packKey(mask, __builtin_amdgcn_rsqf(ee) * __builtin_fmaf(-0.5f, ee, ue), g + k));
             }
         }
     };
@@ -582,8 +604,8 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     const int   w  = int(a1 & kIdxMask);
     const float v1 = __uint_as_float(a1 & ~kIdxMask);
     const float v2 = __uint_as_float(a2 & ~kIdxMask);
-    bool        ok = a1 < 0x7F800000u &&
-              (a2 >= 0x7F800000u || __builtin_fmaf(v2, -0x1p-13f, v2) > __builtin_fmaf(v1, 0x1p-13f, v1) + uB * 0x1p-17f);
+    bool        ok = nearOk && a1 < kPackInf &&
+              (a2 >= kPackInf || __builtin_fmaf(v2, -0x1p-13f, v2) > __builtin_fmaf(v1, 0x1p-13f, v1) + uB * 0x1p-17f);
     const V2<float> cw  = lds.pos[w];
     const float     d2w = dist2(cw, p);
     bool            member = d2w * (1.0f + 4e-6f) < sb.t2;
@@ -602,7 +624,7 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     o.l   = sqrtR(dotR(o.ev, o.ev));
     ok    = ok && !(o.l < 1e-12f);
     o.d   = fmaR(-0.5f, o.l, dotR(u, o.ev) / o.l);
-    o.i0  = keySite(kmin.low()), o.sl0 = s0, o.c0 = c0;
+    o.i0  = keySite(lds.klo[s0]), o.sl0 = s0, o.c0 = c0;
     o.bj  = keySite(lds.klo[w]), o.slj = w, o.used = 1;
     return ok;
 }
@@ -733,6 +755,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
     const int  wave  = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
     LossLds<R> &lds  = ldsAll[wave];
     const int  nWork = *a.nWork;
+    const uint32_t mask = sizeof(R) == 4 ? packMask() : 0u;
     for (int w = blockIdx.x * kLossWaves + wave; w < nWork; w += gridDim.x * kLossWaves) {
         const TargetWork wk = loadUniform(a.work + w);
         const Inst       in = loadUniform(a.insts + wk.inst);
@@ -921,7 +944,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             bool        open = take[h];
             if constexpr (sizeof(R) == 4) {
                 if (fast && !a.exactOnly) {
-                    if (screenTerm<K>(lds, sb, p[h], take[h], tr)) open = false;
+                    if (screenTerm<K>(lds, sb, p[h], take[h], mask, tr)) open = false;
                     tr.used = take[h] && !open ? tr.used : 0;
                 }
             }
