@@ -114,7 +114,8 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
 
     // Targets sorted by hash cell, then by a 4x4-bit Morton key inside the cell,
     // then by input order; a chunk (one wavefront) stays in one cell, holds at
-    // most 64 targets and its bounding-box diagonal stays within 0.7 mean spacings.
+    // most kTargetChunk targets and its bounding-box diagonal stays within the
+    // chunk extent (in mean spacings).
     std::vector<int>      cellOfT(T);
     std::vector<uint32_t> keyT(T);
     for (int t = 0; t < T; ++t) {
@@ -153,6 +154,17 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
             rho = std::max(rho, std::hypot(s.tgt[2 * perm[k]] - double(qx), s.tgt[2 * perm[k] + 1] - double(qy)));
         float rhof = std::nextafter(std::nextafter(float(rho), 1e30f), 1e30f);
         targetWork.push_back({id, t, e - t, c % in.gw, qx, qy, rhof, c / in.gw});
+        // Lane l takes the chunk's targets l, l + 64, l + 128.  Interleave the
+        // Morton order with stride H = ceil(n / 64), so the 64 targets of one
+        // pass are spread over the chunk and fewer lanes of a pass add their
+        // gradient to the same site (LDS atomics on one address serialize).
+        const int n = e - t, H = (n + 63) / 64;
+        if (H > 1) {
+            std::vector<int> run(perm.begin() + t, perm.begin() + e);
+            int              j = t;
+            for (int r0 = 0; r0 < H; ++r0)
+                for (int r = r0; r < n; r += H) perm[j++] = run[r];
+        }
         t = e;
     }
 }
