@@ -735,6 +735,16 @@ template <class R> __device__ __forceinline__ R waveMinR(R v)
     return -waveMaxR<R>(-v);
 }
 
+// The LDS gradient of a wavefront holds `copies` (1, 2 or 4) copies of
+// kLossStage / copies slots; lane l adds into copy l % copies, so lanes whose
+// terms touch the same site collide less.  Copy c is rotated by c slots, which
+// puts the copies of one slot in different LDS banks.  The flush sums them.
+__device__ __forceinline__ int gradSlot(int s, int c, int copies)
+{
+    const int M = kLossStage / copies;
+    return c * M + ((s + c) & (M - 1));
+}
+
 // Number of set bits of `mask` below this lane.
 __device__ __forceinline__ int laneRank(uint64_t mask)
 {
@@ -788,6 +798,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
 
         bool         fast  = false;
         int          m     = 0; // compacted candidates (fast path)
+        int          gcopies = 1;
         int          total = 0; // sites of the staged block
         ScreenBounds sb{};
         if constexpr (sizeof(R) == 4) {
@@ -843,10 +854,8 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                         if (r >= sny) break;
                         for (int s = lane; s < roff[r + 1] - roff[r]; s += 64) {
                             const int d         = roff[r] + s;
-                            lds.pos[d]          = a.spos[so + rb[r] + s];
-                            lds.idx[d]          = a.sidx[so + rb[r] + s];
-                            lds.grad[2 * d]     = GradT<R>(0);
-                            lds.grad[2 * d + 1] = GradT<R>(0);
+                            lds.pos[d] = a.spos[so + rb[r] + s];
+                            lds.idx[d] = a.sidx[so + rb[r] + s];
                         }
                     }
                     waveSync();
@@ -917,6 +926,12 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                     lds.pos[m + lane] = V2<R>{R(kFarSite), R(kFarSite)};
                     lds.klo[m + lane] = ~0u;
                 }
+                // Clear the gradient copies.
+                gcopies = m <= kLossStage / 4 ? 4 : (m <= kLossStage / 2 ? 2 : 1);
+                if (a.grad) {
+#pragma unroll
+                    for (int i = lane; i < 2 * kLossStage; i += 64) lds.grad[i] = GradT<R>(0);
+                }
                 waveSync();
                 // Screen bounds (see screenTerm).  Fewer than K staged sites
                 // lie within sqrt(lo) of q.
@@ -961,8 +976,9 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             openCnt += __popcll(__ballot(open));
             const R lossT = tr.used ? tr.d * tr.d : R(0);
             if (tr.used && a.grad)
-                lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.l, tr.d, &lds.grad[2 * tr.slj],
-                         &lds.grad[2 * tr.sl0], __builtin_ldexp(1.0, in.gradShift));
+                lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.l, tr.d,
+                         &lds.grad[2 * gradSlot(tr.slj, lane & (gcopies - 1), gcopies)],
+                         &lds.grad[2 * gradSlot(tr.sl0, lane & (gcopies - 1), gcopies)], __builtin_ldexp(1.0, in.gradShift));
             if (a.dbgKnn && lane + 64 * h < wk.count && !take[h]) {
 #pragma unroll
                 for (int k = 0; k < K; ++k) a.dbgKnn[(in.targetOff + t) * int64_t(K) + k] = -1;
@@ -1005,7 +1021,11 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
         if (fast && a.grad) {
             waveSync();
             for (int s = lane; s < m; s += 64) {
-                const GradT<R> gx = lds.grad[2 * s], gy = lds.grad[2 * s + 1];
+                GradT<R> gx = GradT<R>(0), gy = GradT<R>(0);
+                for (int c = 0; c < gcopies; ++c) {
+                    const int g = gradSlot(s, c, gcopies);
+                    gx += lds.grad[2 * g], gy += lds.grad[2 * g + 1];
+                }
                 if (gx != GradT<R>(0) || gy != GradT<R>(0)) {
                     const int i = lds.idx[s];
                     GradOps<R>::addRaw(&a.grad[2 * (so + i)], gx);
