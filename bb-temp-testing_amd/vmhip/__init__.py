@@ -81,9 +81,17 @@ EXPORTS = [
     "vm_batch_create", "vm_batch_destroy", "vm_batch_add", "vm_batch_commit", "vm_batch_step",
     "vm_batch_get_sites", "vm_batch_status", "vm_batch_choice", "vm_batch_profile", "vm_batch_loss_stats",
     "vm_serve",
+    "vm_canvas_create", "vm_canvas_destroy", "vm_canvas_rasterize", "vm_canvas_stamp_disc", "vm_canvas_dirty",
+    "vm_canvas_dims", "vm_canvas_counts", "vm_canvas_slots", "vm_canvas_masks", "vm_canvas_voxel",
 ]
 
 _lib = None
+
+
+class CanvasEdit(C.Structure):
+    """vm_canvas_edit: occupancy tiles rewritten, samples removed/added, voxels flipped, dirty slots."""
+
+    _fields_ = [(n, C.c_int32) for n in ("tx0", "ty0", "tx1", "ty1", "removed", "added", "flipped", "num_dirty")]
 
 
 def lib():
@@ -137,6 +145,16 @@ def lib():
         "vm_batch_profile": [vp, i32, vp],
         "vm_batch_loss_stats": [vp, vp],
         "vm_serve": [P(FitDesc), vp, i32, P(ServeConfig), vp, P(ServeStats)],
+        "vm_canvas_create": [f32, f32, f32, P(vp)],
+        "vm_canvas_destroy": [vp],
+        "vm_canvas_rasterize": [vp, vp, i32],
+        "vm_canvas_stamp_disc": [vp, Vec2, f32, i32, P(CanvasEdit)],
+        "vm_canvas_dirty": [vp, vp, i32],
+        "vm_canvas_dims": [vp, P(i32), P(i32), P(i32), P(i32), P(f32)],
+        "vm_canvas_counts": [vp, P(C.c_int64), P(C.c_int64), P(C.c_int64)],
+        "vm_canvas_slots": [vp, vp, C.c_int64],
+        "vm_canvas_masks": [vp, vp, C.c_int64],
+        "vm_canvas_voxel": [vp, i32, i32, P(i32), P(i32)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -307,6 +325,63 @@ class Optimizer:
             edges=arr(m.edges, 2 * m.num_edges, np.int32, (m.num_edges, 2)),
             watertight=bool(m.watertight), loops=m.loops, unsafe_cells=m.unsafe_cells,
             overflow_cells=m.overflow_cells, gpu_ms=m.gpu_ms, host_ms=m.host_ms)
+
+
+class Canvas:
+    """voromesh::Canvas (host voxel canvas): target slots and occupancy masks from a
+    polygon and disc edits (include/voromesh/canvas.h:27-89 of the reference)."""
+
+    def __init__(self, width, height, voxel_size):
+        self._c = C.c_void_p()
+        _check(lib().vm_canvas_create(width, height, voxel_size, C.byref(self._c)))
+        vx, vy, tx, ty, v = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_float()
+        _check(lib().vm_canvas_dims(self._c, C.byref(vx), C.byref(vy), C.byref(tx), C.byref(ty), C.byref(v)))
+        self.voxels_x, self.voxels_y, self.tiles_x, self.tiles_y = vx.value, vy.value, tx.value, ty.value
+        self.voxel_size = v.value
+
+    def close(self):
+        if self._c:
+            lib().vm_canvas_destroy(self._c)
+            self._c = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rasterize(self, polygon):
+        poly = _pts(polygon)
+        _check(lib().vm_canvas_rasterize(self._c, _ptr(poly), len(poly)))
+
+    def stamp_disc(self, center, radius, inside):
+        """Returns (edit dict, dirty slots int32 array)."""
+        e = CanvasEdit()
+        _check(lib().vm_canvas_stamp_disc(self._c, Vec2(float(center[0]), float(center[1])), float(radius),
+                                          1 if inside else 0, C.byref(e)))
+        dirty = np.zeros(max(e.num_dirty, 0), np.int32)
+        _check(lib().vm_canvas_dirty(self._c, _ptr(dirty), len(dirty)))
+        return {n: getattr(e, n) for n, _ in CanvasEdit._fields_}, dirty
+
+    def counts(self):
+        n, s, t = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().vm_canvas_counts(self._c, C.byref(n), C.byref(s), C.byref(t)))
+        return n.value, s.value, t.value
+
+    def target_slots(self):
+        out = np.zeros((self.counts()[0], 2), np.float32)
+        _check(lib().vm_canvas_slots(self._c, _ptr(out), len(out)))
+        return out
+
+    def inside_masks(self):
+        out = np.zeros(self.tiles_x * self.tiles_y, np.uint64)
+        _check(lib().vm_canvas_masks(self._c, _ptr(out), len(out)))
+        return out
+
+    def voxel(self, i, j):
+        a, b = C.c_int(), C.c_int()
+        _check(lib().vm_canvas_voxel(self._c, i, j, C.byref(a), C.byref(b)))
+        return bool(a.value), bool(b.value)
 
 
 class Sampler:

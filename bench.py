@@ -223,6 +223,20 @@ def run_precision(iters=5, K=16):
     return out
 
 
+def run_local_edit(timeout=120):
+    """The demo's scripted edit sequence, headless (tests/cpp/shot_demo, apps/voromesh_demo.cpp:526-570):
+    per-stamp latency of the local pipeline (canvas, upload, Poisson resample, 100 local iterations,
+    extract) on a 1024^2 canvas with Poisson r = 24 generators.  Run as a child process."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "shot_demo")
+    if not os.path.exists(exe):
+        return {"skipped": "tests/cpp/shot_demo not built"}
+    r = subprocess.run([exe, "24", "1024", "100", "--json"], capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"shot_demo failed ({r.returncode}): {r.stderr[-500:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def run_small(iters=200, K=16):
     """Plain single fit: Poisson r=0.025 seed 0, 8,192 star targets (sigma 1e-3), 200 iterations, clip;
     also through the CPU oracle in full."""
@@ -296,7 +310,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=0.25, help="arrival window of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the large/small/precision sections")
-    ap.add_argument("--only", choices=["large", "small", "precision"], default=None,
+    ap.add_argument("--only", choices=["large", "small", "precision", "local_edit"], default=None,
                     help="run one section alone and print its JSON (profiling)")
     ap.add_argument("--large-iters", type=int, default=500)
     args = ap.parse_args()
@@ -311,7 +325,8 @@ def main():
     copy_ms = vmhip.copy_probe(256 << 20)
     log(f"copy probe: 256 MiB in {copy_ms:.3f} ms ({2 * 256 * 1.048576 / copy_ms:.0f} GB/s read+write)")
     if args.only:
-        fn = {"large": lambda: run_large(iters=args.large_iters), "small": run_small, "precision": run_precision}
+        fn = {"large": lambda: run_large(iters=args.large_iters), "small": run_small, "precision": run_precision,
+              "local_edit": run_local_edit}
         print(json.dumps(fn[args.only](), default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
         return
     log(f"generating {args.instances} instances ...")
@@ -376,6 +391,13 @@ def main():
         log("config 3: precision stress ...")
         out["precision"] = run_precision()
         log(f"  {out['precision']}")
+        log("local edits: headless scripted demo sequence ...")
+        out["local_edit"] = run_local_edit()
+        le = out["local_edit"]
+        if "total_ms" in le:
+            log(f"  {le['stamps']} stamps, total {le['total_ms']['mean']:.3f} ms mean per edit "
+                f"(optimize {le['optimize_ms']['mean']:.3f}, poisson {le['poisson_ms']['mean']:.3f}, "
+                f"extract {le['extract_ms']['mean']:.3f}); watertight {le['after']['watertight']}")
     if args.serve_seconds > 0:
         log(f"serving: Poisson arrivals {args.serve_rate}/s for {args.serve_seconds} s ...")
         sv = run_serving(args.serve_seconds, args.serve_rate)

@@ -157,6 +157,38 @@ VM_API int vm_poisson_grid_dims(float width, float height, float radius, int *gr
 VM_API int vm_poisson_batch(int n, const float *w, const float *h, const float *r, const unsigned *seed, int trials,
                             const int64_t *offsets, vm_vec2 *cells, float *gpu_ms);
 
+/* ---- voxel canvas: target slots and occupancy from shapes and disc edits -------
+ * voromesh::Canvas (include/voromesh/canvas.h:27-89, src/voromesh/canvas.cpp of
+ * the reference).  Host-only: no GPU call.  Voxel (i, j) covers [i, i+1) x
+ * [j, j+1) voxels; tiles are 8x8 voxels, voxel (i, j) is bit (j % 8) * 8 + i % 8
+ * of its row-major tile word.  Free target slots hold (-1, -1). */
+typedef struct vm_canvas vm_canvas;
+typedef struct vm_canvas_edit {
+    int32_t tx0, ty0, tx1, ty1;      /* occupancy tiles [tx0,tx1) x [ty0,ty1) rewritten (Canvas::Edit, canvas.h:38-45) */
+    int32_t removed, added, flipped; /* surface samples removed / added, occupancy voxels flipped */
+    int32_t num_dirty;               /* target slots whose value changed: vm_canvas_dirty */
+} vm_canvas_edit;
+/* Canvas(width, height, voxelSize) (canvas.h:47) */
+VM_API int vm_canvas_create(float width, float height, float voxel_size, vm_canvas **out);
+VM_API int vm_canvas_destroy(vm_canvas *c);
+/* RasterizeShape(polygon) (canvas.h:51): closed polygon, last vertex not repeated */
+VM_API int vm_canvas_rasterize(vm_canvas *c, const vm_vec2 *polygon, int n);
+/* StampDisc(center, radius, inside) (canvas.h:57) */
+VM_API int vm_canvas_stamp_disc(vm_canvas *c, vm_vec2 center, float radius, int inside, vm_canvas_edit *edit);
+/* Edit::dirtySlots of the last stamp, in the reference's order (n >= num_dirty) */
+VM_API int vm_canvas_dirty(const vm_canvas *c, int32_t *out, int n);
+/* TilesX/TilesY/VoxelSize (canvas.h:67-69) and the voxel counts */
+VM_API int vm_canvas_dims(const vm_canvas *c, int *voxels_x, int *voxels_y, int *tiles_x, int *tiles_y,
+                          float *voxel_size);
+/* TargetSlots().size(), SurfaceCount(), SurfaceTileCount() (canvas.h:63,70-71) */
+VM_API int vm_canvas_counts(const vm_canvas *c, int64_t *num_slots, int64_t *surface_count, int64_t *surface_tiles);
+/* TargetSlots() (canvas.h:63): n >= num_slots */
+VM_API int vm_canvas_slots(const vm_canvas *c, vm_vec2 *out, int64_t n);
+/* InsideMasks() (canvas.h:64): n >= tiles_x * tiles_y */
+VM_API int vm_canvas_masks(const vm_canvas *c, uint64_t *out, int64_t n);
+/* IsInside / IsSurface (canvas.h:59-60) */
+VM_API int vm_canvas_voxel(const vm_canvas *c, int i, int j, int *inside, int *surface);
+
 /* ---- batched fits: one set of launches per iteration for many instances ------ */
 typedef struct vm_fit_desc {
     float          width, height;   /* domain [0,width] x [0,height] */
