@@ -34,34 +34,6 @@ hipStream_t makeStream()
     return s;
 }
 
-// Boundary mesh from per-slot clipped cells (voro_optimizer.cu:798-854): every
-// edge of an inside cell whose other side is an outside cell or the box;
-// vertices are identified by the sorted generator triple they are
-// equidistant to (box sides are -1..-4), so the mesh is watertight by
-// construction.  Keys are three 32-bit fields, so slot counts are not capped.
-struct TripleKey {
-    uint32_t a, b, c;
-    bool     operator==(const TripleKey &o) const { return a == o.a && b == o.b && c == o.c; }
-};
-struct TripleHash {
-    size_t operator()(const TripleKey &k) const
-    {
-        uint64_t h = uint64_t(k.a) * 0x9E3779B97F4A7C15ull;
-        h ^= (uint64_t(k.b) + 0x632BE59BD9B4E019ull + (h << 6) + (h >> 2));
-        h ^= (uint64_t(k.c) + 0x85EBCA77C2B2AE63ull + (h << 6) + (h >> 2));
-        return size_t(h);
-    }
-};
-
-TripleKey tripleKey(int a, int b, int c)
-{
-    uint32_t v[3] = {uint32_t(a + vmk::kBoxLabels), uint32_t(b + vmk::kBoxLabels), uint32_t(c + vmk::kBoxLabels)};
-    if (v[0] > v[1]) std::swap(v[0], v[1]);
-    if (v[1] > v[2]) std::swap(v[1], v[2]);
-    if (v[0] > v[1]) std::swap(v[0], v[1]);
-    return TripleKey{v[0], v[1], v[2]};
-}
-
 } // namespace
 
 // ===========================================================================
@@ -456,49 +428,15 @@ int vm_extract(vm_handle *h, vm_mesh *out)
             }
         }
         // Boundary assembly in slot space.
-        std::unordered_map<TripleKey, int, TripleHash> vertexOf;
-        auto vertex = [&](TripleKey k, vm_vec2 p) {
-            auto it = vertexOf.find(k);
-            if (it != vertexOf.end()) return it->second;
-            int id = int(h->mVertices.size());
-            vertexOf.emplace(k, id);
-            h->mVertices.push_back(p);
-            return id;
-        };
-        for (int i = 0; i < n; ++i) {
-            if (!h->mInside[i]) continue;
-            int            cc  = cnt[i];
-            const int32_t *lab = h->mLabels.data() + h->mStart[i];
-            const vm_vec2 *pos = h->mVerts.data() + h->mStart[i];
-            for (int k = 0; k < cc; ++k) {
-                int l = lab[k];
-                if (l >= 0 && h->mInside[l]) continue;
-                int prev = lab[(k + cc - 1) % cc], next = lab[(k + 1) % cc];
-                int a = vertex(tripleKey(i, prev, l), pos[k]);
-                int b = vertex(tripleKey(i, l, next), pos[(k + 1) % cc]);
-                h->mEdges.push_back(a), h->mEdges.push_back(b);
-            }
-        }
-        const int        V = int(h->mVertices.size()), E = int(h->mEdges.size() / 2);
-        std::vector<int> nxt(V, -1), inDeg(V, 0);
-        bool             wt = true;
-        for (int e = 0; e < E; ++e) {
-            int a = h->mEdges[2 * e], b = h->mEdges[2 * e + 1];
-            if (nxt[a] != -1) wt = false;
-            nxt[a] = b;
-            ++inDeg[b];
-        }
-        for (int v = 0; v < V; ++v)
-            if (nxt[v] == -1 || inDeg[v] != 1) wt = false;
-        int loops = 0;
-        if (wt) {
-            std::vector<uint8_t> seen(V, 0);
-            for (int v = 0; v < V; ++v) {
-                if (seen[v]) continue;
-                ++loops;
-                for (int w = v; !seen[w]; w = nxt[w]) seen[w] = 1;
-            }
-        }
+        vmrt::Boundary bd;
+        vmrt::assembleBoundary(n, h->mInside.data(), h->mStart.data(), reinterpret_cast<const float *>(h->mVerts.data()),
+                               h->mLabels.data(), bd);
+        const int V = int(bd.vertices.size() / 2), E = int(bd.edges.size() / 2);
+        h->mVertices.resize(size_t(V));
+        for (int v = 0; v < V; ++v) h->mVertices[size_t(v)] = vm_vec2{bd.vertices[2 * v], bd.vertices[2 * v + 1]};
+        h->mEdges.assign(bd.edges.begin(), bd.edges.end());
+        const bool wt    = bd.watertight;
+        const int  loops = bd.loops;
         out->num_slots      = n;
         out->generators     = h->mGen.data();
         out->inside         = h->mInside.data();

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -243,8 +244,16 @@ struct vm_batch {
     hipStream_t                   stream    = nullptr;
     std::unique_ptr<vmrt::Engine> eng;
     bool                          committed = false;
+    // vm_batch_extract: occupancy upload and the returned mesh (valid until the next call).
+    vmrt::DBuf<uint64_t> dMasks;
+    std::vector<vm_vec2> xGen;
+    std::vector<uint8_t> xInside;
+    std::vector<int32_t> xStart, xLabels;
+    std::vector<float>   xVerts;
+    vmrt::Boundary       xBound;
     ~vm_batch()
     {
+        dMasks.release();
         eng.reset();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -347,6 +356,63 @@ int vm_batch_get_sites(vm_batch *b, int id, vm_vec2 *out, int n)
         const vmrt::InstSpec &s = b->eng->specs[id];
         VM_REQUIRE(size_t(n) == s.pos.size() / 2, "n must equal the instance's live site count");
         for (int i = 0; i < n; ++i) out[i] = vm_vec2{float(s.pos[2 * i]), float(s.pos[2 * i + 1])};
+    });
+}
+
+int vm_batch_extract(vm_batch *b, int id, const uint64_t *masks, int tiles_x, int tiles_y, float voxel_size,
+                     vm_mesh *out)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(b && out && id >= 0 && id < int(b->eng->specs.size()), "invalid arguments");
+        VM_REQUIRE(!masks || (tiles_x > 0 && tiles_y > 0 && voxel_size > 0.0f),
+                   "occupancy masks need positive tile counts and voxel size");
+        std::memset(out, 0, sizeof(*out));
+        const uint64_t *dm = nullptr;
+        if (masks) {
+            const size_t nt = size_t(tiles_x) * size_t(tiles_y);
+            b->dMasks.reserve(nt);
+            VM_HIP(hipMemcpyAsync(b->dMasks.p, masks, nt * sizeof(uint64_t), hipMemcpyHostToDevice, b->stream));
+            dm = b->dMasks.p;
+        }
+        vmrt::ClipOut c;
+        b->eng->clip(id, dm, tiles_x, tiles_y, voxel_size, c);
+        const auto t0 = std::chrono::steady_clock::now();
+        b->eng->pull();
+        const vmrt::InstSpec &s = b->eng->specs[id];
+        const int             n = int(s.pos.size() / 2);
+        VM_REQUIRE(int(c.count.size()) == n, "clip output does not match the instance");
+        b->xGen.resize(size_t(n));
+        for (int i = 0; i < n; ++i) b->xGen[size_t(i)] = vm_vec2{float(s.pos[2 * i]), float(s.pos[2 * i + 1])};
+        b->xInside.assign(c.inside.begin(), c.inside.end());
+        b->xStart.assign(size_t(n) + 1, 0);
+        for (int i = 0; i < n; ++i) b->xStart[size_t(i) + 1] = b->xStart[size_t(i)] + c.count[size_t(i)];
+        const size_t nv = size_t(b->xStart[size_t(n)]);
+        b->xVerts.resize(2 * nv), b->xLabels.resize(nv);
+        for (int i = 0; i < n; ++i) {
+            const size_t o = size_t(b->xStart[size_t(i)]), src = size_t(i) * vmk::kMaxCellVerts;
+            for (int k = 0; k < c.count[size_t(i)]; ++k) {
+                b->xVerts[2 * (o + k)]     = c.verts[2 * (src + k)];
+                b->xVerts[2 * (o + k) + 1] = c.verts[2 * (src + k) + 1];
+                b->xLabels[o + k]          = c.labels[src + k];
+            }
+        }
+        vmrt::assembleBoundary(n, b->xInside.data(), b->xStart.data(), b->xVerts.data(), b->xLabels.data(), b->xBound);
+        out->num_slots      = n;
+        out->generators     = b->xGen.data();
+        out->inside         = b->xInside.data();
+        out->cell_start     = b->xStart.data();
+        out->cell_verts     = reinterpret_cast<const vm_vec2 *>(b->xVerts.data());
+        out->cell_labels    = b->xLabels.data();
+        out->num_vertices   = int(b->xBound.vertices.size() / 2);
+        out->vertices       = reinterpret_cast<const vm_vec2 *>(b->xBound.vertices.data());
+        out->num_edges      = int(b->xBound.edges.size() / 2);
+        out->edges          = b->xBound.edges.data();
+        out->watertight     = b->xBound.watertight ? 1 : 0;
+        out->loops          = b->xBound.loops;
+        out->unsafe_cells   = c.unsafe;
+        out->overflow_cells = c.overflow;
+        out->gpu_ms         = c.gpuMs;
+        out->host_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     });
 }
 

@@ -69,6 +69,85 @@ inline int64_t align64(int64_t x) { return (x + 63) & ~int64_t(63); }
 
 } // namespace
 
+namespace {
+// Vertex key of the boundary mesh: the sorted triple (cells, box sides shifted
+// by kBoxLabels) as three 32-bit fields, so cell counts are not capped.
+struct TripleKey {
+    uint32_t a, b, c;
+    bool     operator==(const TripleKey &o) const { return a == o.a && b == o.b && c == o.c; }
+};
+struct TripleHash {
+    size_t operator()(const TripleKey &k) const
+    {
+        uint64_t h = uint64_t(k.a) * 0x9E3779B97F4A7C15ull;
+        h ^= (uint64_t(k.b) + 0x632BE59BD9B4E019ull + (h << 6) + (h >> 2));
+        h ^= (uint64_t(k.c) + 0x85EBCA77C2B2AE63ull + (h << 6) + (h >> 2));
+        return size_t(h);
+    }
+};
+
+TripleKey tripleKey(int a, int b, int c)
+{
+    uint32_t v[3] = {uint32_t(a + vmk::kBoxLabels), uint32_t(b + vmk::kBoxLabels), uint32_t(c + vmk::kBoxLabels)};
+    if (v[0] > v[1]) std::swap(v[0], v[1]);
+    if (v[1] > v[2]) std::swap(v[1], v[2]);
+    if (v[0] > v[1]) std::swap(v[0], v[1]);
+    return TripleKey{v[0], v[1], v[2]};
+}
+
+} // namespace
+
+void assembleBoundary(int n, const uint8_t *inside, const int32_t *start, const float *verts, const int32_t *labels,
+                      Boundary &out)
+{
+    out.vertices.clear(), out.edges.clear(), out.watertight = false, out.loops = 0;
+    std::unordered_map<TripleKey, int, TripleHash> vertexOf;
+    auto vertex = [&](TripleKey k, const float *p) {
+        auto it = vertexOf.find(k);
+        if (it != vertexOf.end()) return it->second;
+        const int id = int(out.vertices.size() / 2);
+        vertexOf.emplace(k, id);
+        out.vertices.push_back(p[0]), out.vertices.push_back(p[1]);
+        return id;
+    };
+    for (int i = 0; i < n; ++i) {
+        if (!inside[i]) continue;
+        const int      cc  = start[i + 1] - start[i];
+        const int32_t *lab = labels + start[i];
+        const float   *pos = verts + 2 * size_t(start[i]);
+        for (int k = 0; k < cc; ++k) {
+            const int l = lab[k];
+            if (l >= 0 && inside[l]) continue;
+            const int prev = lab[(k + cc - 1) % cc], next = lab[(k + 1) % cc];
+            const int a = vertex(tripleKey(i, prev, l), pos + 2 * k);
+            const int b = vertex(tripleKey(i, l, next), pos + 2 * ((k + 1) % cc));
+            out.edges.push_back(a), out.edges.push_back(b);
+        }
+    }
+    const int        V = int(out.vertices.size() / 2), E = int(out.edges.size() / 2);
+    std::vector<int> nxt(size_t(V), -1), inDeg(size_t(V), 0);
+    bool             wt = true;
+    for (int e = 0; e < E; ++e) {
+        const int a = out.edges[2 * size_t(e)], b = out.edges[2 * size_t(e) + 1];
+        if (nxt[size_t(a)] != -1) wt = false;
+        nxt[size_t(a)] = b;
+        ++inDeg[size_t(b)];
+    }
+    for (int v = 0; v < V; ++v)
+        if (nxt[size_t(v)] == -1 || inDeg[size_t(v)] != 1) wt = false;
+    int loops = 0;
+    if (wt) {
+        std::vector<uint8_t> seen(size_t(V), 0);
+        for (int v = 0; v < V; ++v) {
+            if (seen[size_t(v)]) continue;
+            ++loops;
+            for (int w = v; !seen[size_t(w)]; w = nxt[size_t(w)]) seen[size_t(w)] = 1;
+        }
+    }
+    out.watertight = wt;
+    out.loops      = loops;
+}
+
 // ceil(log2(x)) for x > 0, exact (frexp gives x = m 2^e with m in [0.5, 1)).
 static int ceilLog2(double x)
 {
@@ -616,15 +695,20 @@ public:
         for (size_t i = 0; i < hd.size(); ++i) d2[i] = double(hd[i]);
     }
 
+    DBuf<float>   clipVerts;
+    DBuf<int>     clipLabels, clipCount, clipFlags;
+    DBuf<uint8_t> clipInside;
+
     void clip(int id, const uint64_t *dMasks, int tilesX, int tilesY, float voxel, ClipOut &out) override
     {
         if (dirty) commit();
         ensureEvents();
         const vmk::Inst &in = L.inst[id];
         const int        S  = in.S;
-        DBuf<float>      dv;
-        DBuf<int>        dl, dc, df;
-        DBuf<uint8_t>    di;
+        // Output buffers are kept between calls (an extract per local edit).
+        DBuf<float>   &dv = clipVerts;
+        DBuf<int>     &dl = clipLabels, &dc = clipCount, &df = clipFlags;
+        DBuf<uint8_t> &di = clipInside;
         dv.reserve(size_t(S) * vmk::kMaxCellVerts * 2), dl.reserve(size_t(S) * vmk::kMaxCellVerts);
         dc.reserve(S), di.reserve(S), df.reserve(2);
         VM_HIP(hipEventRecord(ev[5], stream));

@@ -1,5 +1,5 @@
 // vm_engine.h -- host runtime of the HIP optimizer: a batch of instances in
-// pooled device arrays, driven by the four per-iteration launches of
+// pooled device arrays, driven by the five per-iteration launches of
 // vm_kernels.hip, one iteration captured as a HIP graph.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -125,6 +125,22 @@ public:
     bool        deviceNewer = false;
     hipStream_t stream = nullptr;
 };
+
+// Boundary mesh of clipped cells (voro_optimizer.cu:798-854): every edge of an
+// inside cell whose other side is an outside cell or the box.  Cells are CSR
+// (start[n + 1], verts, labels: >= 0 the neighbour cell, -1..-4 the box sides)
+// in one index space; vertices are identified by the sorted triple of cells /
+// box sides they are equidistant to and numbered in order of first use, so the
+// mesh is watertight by construction.  watertight: every vertex has exactly
+// one outgoing and one incoming edge; loops then counts the cycles.
+struct Boundary {
+    std::vector<float>   vertices; // [V * 2]
+    std::vector<int32_t> edges;    // [E * 2]
+    bool                 watertight = false;
+    int                  loops      = 0;
+};
+void assembleBoundary(int n, const uint8_t *inside, const int32_t *start, const float *verts, const int32_t *labels,
+                      Boundary &out);
 
 // Lays out one instance at the given pooled offsets: its Inst record, site
 // chunks, target chunks (one wavefront each) and the sort permutation of its

@@ -79,7 +79,7 @@ EXPORTS = [
     "vm_sampler_create", "vm_sampler_destroy", "vm_sampler_resample", "vm_sampler_verify", "vm_sampler_dims",
     "vm_sampler_cells", "vm_poisson_grid_dims", "vm_poisson_batch",
     "vm_batch_create", "vm_batch_destroy", "vm_batch_add", "vm_batch_commit", "vm_batch_step",
-    "vm_batch_get_sites", "vm_batch_status", "vm_batch_choice", "vm_batch_profile", "vm_batch_loss_stats",
+    "vm_batch_get_sites", "vm_batch_status", "vm_batch_extract", "vm_batch_choice", "vm_batch_profile", "vm_batch_loss_stats",
     "vm_serve",
     "vm_canvas_create", "vm_canvas_destroy", "vm_canvas_rasterize", "vm_canvas_stamp_disc", "vm_canvas_dirty",
     "vm_canvas_dims", "vm_canvas_counts", "vm_canvas_slots", "vm_canvas_masks", "vm_canvas_voxel",
@@ -141,6 +141,7 @@ def lib():
         "vm_batch_step": [vp, i32, P(f32)],
         "vm_batch_get_sites": [vp, i32, vp, i32],
         "vm_batch_status": [vp, i32, P(i32), P(i32), P(f64)],
+        "vm_batch_extract": [vp, i32, vp, i32, i32, f32, P(Mesh)],
         "vm_batch_choice": [vp, i32, P(i32), P(f32), P(i32), P(f32)],
         "vm_batch_profile": [vp, i32, vp],
         "vm_batch_loss_stats": [vp, vp],
@@ -171,6 +172,31 @@ def _check(rc):
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _mesh_dict(m):
+    """Copies a vm_mesh into numpy arrays."""
+    n = m.num_slots
+
+    def arr(p, count, dt, shape=None):
+        if count == 0 or not p:
+            return np.zeros(shape if shape else (0,), dt)
+        a = np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(count,))
+        a = a.copy()
+        return a.reshape(shape) if shape else a
+
+    start = arr(m.cell_start, n + 1, np.int32)
+    nv = int(start[-1]) if n else 0
+    return dict(
+        generators=arr(C.cast(m.generators, C.POINTER(C.c_float)), 2 * n, np.float32, (n, 2)),
+        inside=arr(m.inside, n, np.uint8), cell_start=start,
+        cell_verts=arr(C.cast(m.cell_verts, C.POINTER(C.c_float)), 2 * nv, np.float32, (nv, 2)),
+        cell_labels=arr(m.cell_labels, nv, np.int32),
+        vertices=arr(C.cast(m.vertices, C.POINTER(C.c_float)), 2 * m.num_vertices, np.float32,
+                     (m.num_vertices, 2)),
+        edges=arr(m.edges, 2 * m.num_edges, np.int32, (m.num_edges, 2)),
+        watertight=bool(m.watertight), loops=m.loops, unsafe_cells=m.unsafe_cells,
+        overflow_cells=m.overflow_cells, gpu_ms=m.gpu_ms, host_ms=m.host_ms)
 
 
 def _pts(a):
@@ -304,27 +330,7 @@ class Optimizer:
     def extract(self):
         m = Mesh()
         _check(lib().vm_extract(self._h, C.byref(m)))
-        n = m.num_slots
-
-        def arr(p, count, dt, shape=None):
-            if count == 0 or not p:
-                return np.zeros(shape if shape else (0,), dt)
-            a = np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(count,))
-            a = a.copy()
-            return a.reshape(shape) if shape else a
-
-        start = arr(m.cell_start, n + 1, np.int32)
-        nv = int(start[-1]) if n else 0
-        return dict(
-            generators=arr(C.cast(m.generators, C.POINTER(C.c_float)), 2 * n, np.float32, (n, 2)),
-            inside=arr(m.inside, n, np.uint8), cell_start=start,
-            cell_verts=arr(C.cast(m.cell_verts, C.POINTER(C.c_float)), 2 * nv, np.float32, (nv, 2)),
-            cell_labels=arr(m.cell_labels, nv, np.int32),
-            vertices=arr(C.cast(m.vertices, C.POINTER(C.c_float)), 2 * m.num_vertices, np.float32,
-                         (m.num_vertices, 2)),
-            edges=arr(m.edges, 2 * m.num_edges, np.int32, (m.num_edges, 2)),
-            watertight=bool(m.watertight), loops=m.loops, unsafe_cells=m.unsafe_cells,
-            overflow_cells=m.overflow_cells, gpu_ms=m.gpu_ms, host_ms=m.host_ms)
+        return _mesh_dict(m)
 
 
 class Canvas:
@@ -528,6 +534,13 @@ class Batch:
         keys = ["fast_waves", "ring_staged_waves", "ring_global_waves", "idle_waves", "candidates", "staged_sites",
                 "exact_fallback_waves", "exact_fallback_targets"]
         return {k: int(v) for k, v in zip(keys, c)}
+
+    def extract(self, i, masks=None, tiles=(0, 0), voxel=0.0):
+        """Clipped Voronoi cells of fit i (CSR over its live sites) and, with occupancy masks, the boundary."""
+        m = Mesh()
+        mk = None if masks is None else np.ascontiguousarray(masks, np.uint64)
+        _check(lib().vm_batch_extract(self._b, i, _ptr(mk), int(tiles[0]), int(tiles[1]), float(voxel), C.byref(m)))
+        return _mesh_dict(m)
 
     def choice(self, i):
         k, cell, rings, ext = C.c_int(), C.c_float(), C.c_int(), C.c_float()

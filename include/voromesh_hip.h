@@ -221,6 +221,12 @@ VM_API int vm_batch_step(vm_batch *b, int iterations, float *gpu_ms);
 VM_API int vm_batch_get_sites(vm_batch *b, int id, vm_vec2 *out, int n);
 /* Per instance: completed iterations, done flag and the mean loss of its last step. */
 VM_API int vm_batch_status(vm_batch *b, int id, int *iteration, int *done, double *last_mean_loss);
+/* Voronoi cells of fit `id` clipped to its box (FP64, as vm_extract) as a CSR mesh in the order of the fit's live
+ * sites (cell_labels >= 0 are site indices), and with occupancy masks (8x8-voxel tile words, as vm_set_occupancy;
+ * null: every cell outside) the watertight boundary of the inside cells.  The arrays stay valid until the next
+ * vm_batch_extract on `b`.  No reference counterpart: the batched form of Extract (voro_optimizer.h:116). */
+VM_API int vm_batch_extract(vm_batch *b, int id, const uint64_t *masks, int tiles_x, int tiles_y, float voxel_size,
+                            vm_mesh *out);
 /* Resolved per-instance choices: k, grid cell, staged rings, target chunk extent (mean spacings). */
 VM_API int vm_batch_choice(vm_batch *b, int id, int *loss_k, float *grid_cell, int *stage_rings,
                            float *chunk_extent);

@@ -372,3 +372,37 @@ def test_clip_lattice_topology_zero_mismatches(precision):
                      np.array_equal(mesh["cell_verts"][b:b + n], ref["verts"][i, :n]))
     assert mism == 0
     assert mesh["overflow_cells"] == 0
+
+
+def test_batch_extract_matches_oracle_clip_and_boundary():
+    """vm_batch_extract: every fit of a batch clipped (FP64) and its boundary assembled, bit-exact against the
+    oracle's clip and boundary assembly in the fit's site order; without occupancy no cell is inside."""
+    r, K, iters = 0.04, 16, 15
+    masks, tx, ty, vox = _star_masks(64)
+    b = vmhip.Batch()
+    for i in range(3):
+        grid = O.poisson(W, H, r * (1 + 0.25 * i), 30, 30 + i)
+        sites = wl.live_sites(grid)
+        b.add(W, H, sites, wl.star(8 * len(sites), seed=40 + i), 1e-3, loss_k=K)
+    b.commit()
+    b.step(iters)
+    for i in range(3):
+        print(f"case: batch extract fit {i}", flush=True)
+        live = b.sites(i)
+        m = b.extract(i, masks, (tx, ty), vox)
+        ref = O.clip(live, W, H, 0.1, masks=masks, tiles=(tx, ty), voxel=vox)
+        assert np.array_equal(m["generators"], live)
+        cnt = m["cell_start"][1:] - m["cell_start"][:-1]
+        assert np.array_equal(cnt, ref["count"])
+        for k in range(len(live)):
+            s0, n = m["cell_start"][k], ref["count"][k]
+            assert np.array_equal(m["cell_verts"][s0:s0 + n], ref["verts"][k, :n])
+            assert np.array_equal(m["cell_labels"][s0:s0 + n], ref["labels"][k, :n])
+        assert np.array_equal(m["inside"], ref["inside"])
+        assert m["unsafe_cells"] == ref["unsafe"] and m["overflow_cells"] == ref["overflow"] == 0
+        bd = O.boundary(ref["verts"], ref["labels"], ref["count"], ref["inside"])
+        assert np.array_equal(m["vertices"], bd["vertices"]) and np.array_equal(m["edges"], bd["edges"])
+        assert m["watertight"] and bd["watertight"] and m["loops"] == bd["loops"] >= 1
+        bare = b.extract(i)
+        assert not bare["inside"].any() and len(bare["edges"]) == 0
+        assert np.array_equal(bare["cell_verts"], m["cell_verts"])
