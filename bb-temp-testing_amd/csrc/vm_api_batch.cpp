@@ -109,7 +109,8 @@ int vm_sampler_resample(vm_sampler *s, int cx0, int cy0, int cx1, int cy1, float
         p.W = s->W, p.H = s->H, p.radius = s->r, p.cellSize = s->cs, p.gw = s->gw, p.gh = s->gh;
         p.x0 = x0, p.y0 = y0, p.x1 = x1, p.y1 = y1, p.seed = s->seed, p.roundBase = s->roundBase, p.gridOff = 0;
         VM_HIP(hipEventRecord(s->e0, s->stream));
-        s->runner.run(s->stream, {p}, s->dGrid.p, s->trials);
+        if (vmk::poissonRegionFits(p)) VM_HIP(vmk::launchPoissonRegion(s->stream, p, s->dGrid.p, s->trials));
+        else s->runner.run(s->stream, {p}, s->dGrid.p, s->trials);
         VM_HIP(hipEventRecord(s->e1, s->stream));
         s->roundBase += unsigned(s->trials);
         size_t pitch = size_t(s->gw) * sizeof(vm_vec2), off = size_t(y0) * s->gw + x0;

@@ -104,6 +104,11 @@ hipError_t launchPoissonClear(int grid, hipStream_t s, const PoiInst *insts, con
 hipError_t launchPoissonPhase(int grid, hipStream_t s, const PoiInst *insts, const PoiWork *work, int nWork, float *g,
                               int px, int py, int round);
 hipError_t launchFillInt(hipStream_t s, int *p, int64_t n, int v);
+// One-launch resample of a region whose 2-cell halo fits kPoiRegionCells cells of LDS.
+constexpr int kPoiRegionCells   = 8000; // 64,000 B of float2
+constexpr int kPoiRegionThreads = 1024;
+bool          poissonRegionFits(const PoiInst &pi);
+hipError_t    launchPoissonRegion(hipStream_t s, const PoiInst &pi, float *grid, int trials);
 
 // Workgroups for the exact-selection launch of a group with `targets` targets:
 // about one wavefront per 512 targets, so the few the screen leaves open spread

@@ -22,6 +22,8 @@
 // correctly rounded division and square root, so every per-target term is
 // bit-identical to the oracle's.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <float.h>
 #include <limits.h>
 #include <stdint.h>
@@ -1545,6 +1547,65 @@ __global__ __launch_bounds__(256) void poissonPhaseKernel(const PoiInst *__restr
     }
 }
 
+// The whole resample of one region in one workgroup: the region (cleared) and
+// its 2-cell halo (kept: the dart test reads cells within +-2) are staged in
+// LDS, every (round, phase) step of the launch sequence above runs between
+// block barriers in the same order and with the same arithmetic, and the region
+// is written back.  One launch instead of 9 x trials; the grids are identical.
+__global__ __launch_bounds__(kPoiRegionThreads) void poissonRegionKernel(PoiInst pi, float *grid, int trials)
+{
+    __shared__ float2 cells[kPoiRegionCells];
+    const int hx0 = max(pi.x0 - 2, 0), hy0 = max(pi.y0 - 2, 0);
+    const int hx1 = min(pi.x1 + 2, pi.gw), hy1 = min(pi.y1 + 2, pi.gh);
+    const int hw = hx1 - hx0, hh = hy1 - hy0;
+    float2   *g  = reinterpret_cast<float2 *>(grid) + pi.gridOff;
+    for (int c = int(threadIdx.x); c < hw * hh; c += kPoiRegionThreads) {
+        const int  cx = hx0 + c % hw, cy = hy0 + c / hw;
+        const bool in = cx >= pi.x0 && cx < pi.x1 && cy >= pi.y0 && cy < pi.y1;
+        cells[c]      = in ? make_float2(-1.0f, -1.0f) : g[int64_t(cy) * pi.gw + cx];
+    }
+    __syncthreads();
+    const int   rw = pi.x1 - pi.x0, rh = pi.y1 - pi.y0;
+    const float r2 = pi.radius * pi.radius;
+    for (int round = 0; round < trials; ++round) {
+        const uint32_t rnd = pi.roundBase + uint32_t(round);
+        for (int py = 0; py < 3; ++py) {
+            for (int px = 0; px < 3; ++px) {
+                const int nx = (rw - px + 2) / 3, ny = (rh - py + 2) / 3;
+                for (int k = int(threadIdx.x); k < nx * ny; k += kPoiRegionThreads) {
+                    const int cx = pi.x0 + 3 * (k % nx) + px, cy = pi.y0 + 3 * (k / nx) + py;
+                    const int lc = (cy - hy0) * hw + (cx - hx0);
+                    if (cells[lc].x >= 0.0f) continue;
+                    const int   cellId = cy * pi.gw + cx;
+                    const float x      = (float(cx) + rand01_poisson(uint32_t(cellId), rnd, pi.seed, 0)) * pi.cellSize;
+                    const float y      = (float(cy) + rand01_poisson(uint32_t(cellId), rnd, pi.seed, 1)) * pi.cellSize;
+                    if (x >= pi.W || y >= pi.H) continue;
+                    // The 5x5 neighbourhood (clipped to the grid) read at once:
+                    // a conflict with any occupied cell rejects the dart.
+                    bool ok = true;
+#pragma unroll
+                    for (int dyc = -2; dyc <= 2; ++dyc) {
+#pragma unroll
+                        for (int dxc = -2; dxc <= 2; ++dxc) {
+                            const int    xx = cx + dxc, yy = cy + dyc;
+                            const bool   in = xx >= 0 && xx < pi.gw && yy >= 0 && yy < pi.gh;
+                            const float2 q  = cells[in ? (yy - hy0) * hw + (xx - hx0) : lc];
+                            const float  dx = q.x - x, dy = q.y - y;
+                            ok = ok && !(in && q.x >= 0.0f && __builtin_fmaf(dy, dy, dx * dx) < r2);
+                        }
+                    }
+                    if (ok) cells[lc] = make_float2(x, y);
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (int c = int(threadIdx.x); c < rw * rh; c += kPoiRegionThreads) {
+        const int cx = pi.x0 + c % rw, cy = pi.y0 + c / rw;
+        g[int64_t(cy) * pi.gw + cx] = cells[(cy - hy0) * hw + (cx - hx0)];
+    }
+}
+
 __global__ void fillIntKernel(int *p, int64_t n, int v)
 {
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
@@ -1632,6 +1693,20 @@ hipError_t launchPoissonClear(int grid, hipStream_t s, const PoiInst *insts, con
 {
     if (nWork <= 0) return hipSuccess;
     hipLaunchKernelGGL(poissonClearKernel, dim3(grid), dim3(256), 0, s, insts, work, nWork, g);
+    return hipGetLastError();
+}
+
+bool poissonRegionFits(const PoiInst &pi)
+{
+    const int64_t hw = int64_t(std::min(pi.x1 + 2, pi.gw) - std::max(pi.x0 - 2, 0));
+    const int64_t hh = int64_t(std::min(pi.y1 + 2, pi.gh) - std::max(pi.y0 - 2, 0));
+    return pi.x1 > pi.x0 && pi.y1 > pi.y0 && hw * hh <= kPoiRegionCells;
+}
+
+hipError_t launchPoissonRegion(hipStream_t s, const PoiInst &pi, float *grid, int trials)
+{
+    if (!poissonRegionFits(pi)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(poissonRegionKernel, dim3(1), dim3(kPoiRegionThreads), 0, s, pi, grid, trials);
     return hipGetLastError();
 }
 
