@@ -94,6 +94,10 @@ struct Prepared {
     std::vector<vmk::TargetWork>  tw;
     std::vector<int>              perm;
     int64_t                       cells = 0;
+    // Upload images in the device element type, built by the preparation
+    // worker: positions, targets in sorted order and their slot numbers.
+    std::vector<unsigned char>    pos, tgt;
+    std::vector<int>              tslot;
 };
 
 template <class R> class Server {
@@ -150,9 +154,26 @@ public:
     double peakBytes() const { return peakBytes_; }
     int    maxResident() const { return maxResident_; }
 
-    // Tries to place a prepared request; returns false when no slot or pool room.
-    bool admit(int req, Prepared &p)
+    // Upload images of a prepared request in this server's element type.
+    static void buildImages(Prepared &p)
     {
+        const int S = p.in.S, T = p.in.T;
+        p.pos.resize(size_t(S) * sizeof(V)), p.tgt.resize(size_t(T) * sizeof(V)), p.tslot.resize(size_t(T));
+        V *hp = reinterpret_cast<V *>(p.pos.data()), *ht = reinterpret_cast<V *>(p.tgt.data());
+        for (int k = 0; k < S; ++k) hp[k] = V{R(p.spec.pos[2 * k]), R(p.spec.pos[2 * k + 1])};
+        for (int t = 0; t < T; ++t) {
+            const int o = p.perm[size_t(t)];
+            ht[t]       = V{R(p.spec.tgt[2 * o]), R(p.spec.tgt[2 * o + 1])};
+            p.tslot[size_t(t)] = p.spec.tslot.empty() ? o : p.spec.tslot[size_t(o)];
+        }
+    }
+
+    // Tries to place a prepared request; returns false when no slot or pool room.
+    // The uploads are only enqueued: the request's host images are kept (it is
+    // moved into the pending list) until refresh() has synchronized the stream.
+    bool admit(int req, std::unique_ptr<Prepared> &pp)
+    {
+        Prepared &p = *pp;
         int slot = -1;
         for (int s = 0; s < slots_; ++s)
             if (slotReq_[s] < 0) { slot = s; break; }
@@ -177,30 +198,21 @@ public:
             si.tw.push_back(w);
         }
         // Upload: positions, zero moments/gradient/counts, sorted targets, record, state.
-        std::vector<V>   hp(S), ht(T);
-        std::vector<int> hs(T);
-        for (int k = 0; k < S; ++k) hp[k] = V{R(p.spec.pos[2 * k]), R(p.spec.pos[2 * k + 1])};
-        for (int t = 0; t < T; ++t) {
-            int o = p.perm[t];
-            ht[t] = V{R(p.spec.tgt[2 * o]), R(p.spec.tgt[2 * o + 1])};
-            hs[t] = p.spec.tslot.empty() ? o : p.spec.tslot[o];
-        }
         if (S) {
-            VM_HIP(hipMemcpyAsync(dPos.p + so, hp.data(), S * sizeof(V), hipMemcpyHostToDevice, stream_));
+            VM_HIP(hipMemcpyAsync(dPos.p + so, p.pos.data(), S * sizeof(V), hipMemcpyHostToDevice, stream_));
             VM_HIP(hipMemsetAsync(dM.p + so, 0, S * sizeof(V), stream_));
             VM_HIP(hipMemsetAsync(dV.p + so, 0, S * sizeof(V), stream_));
             VM_HIP(hipMemsetAsync(dGrad.p + 2 * so, 0, 2 * S * sizeof(vmk::GradT<R>), stream_));
         }
         if (T) {
-            VM_HIP(hipMemcpyAsync(dTpos.p + to, ht.data(), T * sizeof(V), hipMemcpyHostToDevice, stream_));
-            VM_HIP(hipMemcpyAsync(dTslot.p + to, hs.data(), T * sizeof(int), hipMemcpyHostToDevice, stream_));
+            VM_HIP(hipMemcpyAsync(dTpos.p + to, p.tgt.data(), T * sizeof(V), hipMemcpyHostToDevice, stream_));
+            VM_HIP(hipMemcpyAsync(dTslot.p + to, p.tslot.data(), T * sizeof(int), hipMemcpyHostToDevice, stream_));
         }
         VM_HIP(hipMemsetAsync(dCounts.p + co, 0, p.cells * sizeof(int), stream_));
-        VM_HIP(hipMemcpyAsync(dInst.p + slot, &in, sizeof(in), hipMemcpyHostToDevice, stream_));
-        vmk::InstState z{};
-        VM_HIP(hipMemcpyAsync(dState.p + slot, &z, sizeof(z), hipMemcpyHostToDevice, stream_));
-        // The host copies above must stay alive until the stream consumed them.
-        VM_HIP(hipStreamSynchronize(stream_));
+        p.in = in; // the record and a zero state are uploaded from the kept request
+        VM_HIP(hipMemcpyAsync(dInst.p + slot, &p.in, sizeof(p.in), hipMemcpyHostToDevice, stream_));
+        VM_HIP(hipMemcpyAsync(dState.p + slot, &zeroState_, sizeof(zeroState_), hipMemcpyHostToDevice, stream_));
+        pending_.push_back(std::move(pp));
         newWork_.insert(newWork_.end(), si.sw.begin(), si.sw.end());
         slotReq_[slot] = req;
         residentBytes_ += bytes;
@@ -263,6 +275,7 @@ public:
             VM_HIP(vmk::launchAdamBin<R>(std::min(nn, 8192), stream_, a));
         }
         VM_HIP(hipStreamSynchronize(stream_));
+        pending_.clear(); // admitted requests' host images have been consumed
         newWork_.clear();
         changed_ = false;
     }
@@ -384,6 +397,8 @@ private:
     std::vector<int>      slotReq_;
     std::vector<SlotInfo> slotInfo_;
     std::vector<vmk::SiteWork> newWork_;
+    std::vector<std::unique_ptr<Prepared>> pending_;
+    const vmk::InstState                   zeroState_{};
 
     DBuf<V>              dPos, dM, dV, dSpos, dTpos;
     DBuf<vmk::GradT<R>>  dGrad;
@@ -450,6 +465,7 @@ void serve(const vm_fit_desc *reqs, const double *arrival, int n, const vm_serve
                     p->spec.tol     = cfg.tol > 0 ? cfg.tol : reqs[i].tol;
                     vmrt::layoutInstance(p->spec, -1, 0, 0, 0, p->in, p->sw, p->tw, p->perm);
                     p->cells = int64_t(p->in.gw) * p->in.gh + 1;
+                    Server<R>::buildImages(*p);
                     prep[i]  = std::move(p);
                     ready[i].store(1);
                 } catch (const std::exception &e) {
@@ -478,12 +494,11 @@ void serve(const vm_fit_desc *reqs, const double *arrival, int n, const vm_serve
         // Admit prepared requests in arrival order.
         while (nextAdmit < n && ready[nextAdmit].load() != 0) {
             if (ready[nextAdmit].load() == 2) throw VmError(VM_EINVAL, "request preparation failed: " + prepError);
-            if (!srv.admit(nextAdmit, *prep[nextAdmit])) {
+            if (!srv.admit(nextAdmit, prep[nextAdmit])) {
                 if (srv.resident() == 0) throw VmError(VM_ENOMEM, "request does not fit the HBM budget");
                 break;
             }
             admitT[nextAdmit] = elapsed();
-            prep[nextAdmit].reset();
             ++nextAdmit;
         }
         srv.refresh();

@@ -40,7 +40,7 @@ double msBetween(Clock::time_point a, Clock::time_point b)
 
 struct EditTiming {
     double canvas = 0, upload = 0, poisson = 0, optimize = 0, extract = 0, total = 0;
-    float  poissonGpu = 0, optimizeGpu = 0;
+    float  poissonGpu = 0, optimizeGpu = 0, clipGpu = 0, assembleHost = 0;
     int    removed = 0, added = 0, slots = 0;
 };
 
@@ -99,6 +99,8 @@ public:
         mMesh         = mOpt.Extract();
         const auto t5 = Clock::now();
 
+        t.clipGpu      = mMesh.gpuMs;
+        t.assembleHost = mMesh.hostMs;
         t.canvas   = msBetween(t0, t1);
         t.upload   = msBetween(t1, t2);
         t.poisson  = msBetween(t2, t3);
@@ -283,10 +285,11 @@ int main(int argc, char **argv)
             if (ed.Stamp(c, d.r, d.add, et)) {
                 edits.push_back(et);
                 std::printf("  %-5s (%4.0f,%4.0f): -%3d +%3d pts, %3d slots | canvas %.3f  upload %.3f  poisson %.3f "
-                            "(%.3f GPU)  opt %.3f (%.3f GPU, %d it)  extract %.3f | total %.3f ms\n",
+                            "(%.3f GPU)  opt %.3f (%.3f GPU, %d it)  extract %.3f (clip %.3f GPU, host %.3f) | total "
+                            "%.3f ms\n",
                             d.add ? "add" : "carve", c.x, c.y, et.removed, et.added, et.slots, et.canvas, et.upload,
                             et.poisson, double(et.poissonGpu), et.optimize, double(et.optimizeGpu),
-                            params.localIterations, et.extract, et.total);
+                            params.localIterations, et.extract, double(et.clipGpu), double(et.assembleHost), et.total);
             }
             if (len == 0.0f) break;
         }
