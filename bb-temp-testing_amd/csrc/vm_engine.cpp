@@ -209,11 +209,36 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         cellOfT[t] = ccy * in.gw + ccx;
         keyT[t]    = mort;
     }
-    perm.resize(T);
-    std::iota(perm.begin(), perm.end(), 0);
-    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
-        return cellOfT[a] != cellOfT[b] ? cellOfT[a] < cellOfT[b] : keyT[a] < keyT[b];
-    });
+    // Stable LSD radix sort on (cell << 8 | Morton key), 12-bit digits: input
+    // order breaks ties, as a stable comparison sort would.
+    {
+        const uint64_t maxKey = (uint64_t(std::max(in.gw * in.gh, 1)) << 8) - 1;
+        int            bits   = 1;
+        while (bits < 64 && (maxKey >> bits)) ++bits;
+        const size_t          nT = static_cast<size_t>(T);
+        std::vector<uint64_t> key(nT), key2(nT);
+        std::vector<int>      idx2(nT);
+        perm.resize(nT);
+        for (int t = 0; t < T; ++t) key[size_t(t)] = (uint64_t(uint32_t(cellOfT[size_t(t)])) << 8) | keyT[size_t(t)];
+        std::iota(perm.begin(), perm.end(), 0);
+        constexpr int kDigit = 12, kBuckets = 1 << kDigit;
+        std::vector<int> count(kBuckets);
+        for (int shift = 0; shift < bits; shift += kDigit) {
+            std::fill(count.begin(), count.end(), 0);
+            for (int t = 0; t < T; ++t) ++count[size_t((key[size_t(t)] >> shift) & (kBuckets - 1))];
+            int sum = 0;
+            for (int d = 0; d < kBuckets; ++d) {
+                const int c = count[size_t(d)];
+                count[size_t(d)] = sum;
+                sum += c;
+            }
+            for (int t = 0; t < T; ++t) {
+                const int d = int((key[size_t(t)] >> shift) & (kBuckets - 1)), o = count[size_t(d)]++;
+                key2[size_t(o)] = key[size_t(t)], idx2[size_t(o)] = perm[size_t(t)];
+            }
+            key.swap(key2), perm.swap(idx2);
+        }
+    }
     const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(S, 1)));
     const double maxDiag = double(std::min(s.chunkExt > 0.0f ? s.chunkExt : 1.6f, extentCap)) * spacing;
     for (int t = 0; t < T;) {
@@ -222,15 +247,18 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         while (e < T && cellOfT[perm[e]] == c && e - t < vmk::kTargetChunk) {
             double px = s.tgt[2 * perm[e]], py = s.tgt[2 * perm[e] + 1];
             double nx0 = std::min(x0, px), ny0 = std::min(y0, py), nx1 = std::max(x1, px), ny1 = std::max(y1, py);
-            if (e > t && std::hypot(nx1 - nx0, ny1 - ny0) > maxDiag) break;
+            if (e > t && (nx1 - nx0) * (nx1 - nx0) + (ny1 - ny0) * (ny1 - ny0) > maxDiag * maxDiag) break;
             x0 = nx0, y0 = ny0, x1 = nx1, y1 = ny1;
             ++e;
         }
         // Chunk centre (FP32) and an upper bound on the distance of its targets to it.
         float  qx = float(0.5 * (x0 + x1)), qy = float(0.5 * (y0 + y1));
         double rho = 0.0;
-        for (int k = t; k < e; ++k)
-            rho = std::max(rho, std::hypot(s.tgt[2 * perm[k]] - double(qx), s.tgt[2 * perm[k] + 1] - double(qy)));
+        for (int k = t; k < e; ++k) {
+            const double dx = s.tgt[2 * perm[k]] - double(qx), dy = s.tgt[2 * perm[k] + 1] - double(qy);
+            rho = std::max(rho, dx * dx + dy * dy);
+        }
+        rho = std::sqrt(rho); // double: within an ulp, and rounded up twice in float below
         float rhof = std::nextafter(std::nextafter(float(rho), 1e30f), 1e30f);
         targetWork.push_back({id, t, e - t, c % in.gw, qx, qy, rhof, c / in.gw});
         // Lane l takes the chunk's targets l, l + 64, l + 128.  Interleave the

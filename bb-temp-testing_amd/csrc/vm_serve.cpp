@@ -100,6 +100,26 @@ struct Prepared {
     std::vector<int>              tslot;
 };
 
+// Grow-only pinned host buffer.
+struct PinnedBuf {
+    unsigned char *p = nullptr;
+    size_t         n = 0;
+    PinnedBuf()                             = default;
+    PinnedBuf(const PinnedBuf &)            = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    void reserve(size_t bytes)
+    {
+        if (bytes <= n && p) return;
+        if (p) (void)hipHostFree(p), p = nullptr;
+        n = std::max(bytes + bytes / 2, size_t(1) << 20);
+        VM_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), n, hipHostMallocDefault));
+    }
+};
+
 template <class R> class Server {
     using V = vmk::V2<R>;
 
@@ -257,14 +277,21 @@ public:
         int cnt[3] = {0, int(sw.size()), int(tw.size())};
         const int nTiles = int(tiles.size());
         if (nTiles > maxTiles_) throw VmError(VM_ESTATE, "scan tile table overflow");
-        if (nTiles) VM_HIP(hipMemcpyAsync(dTiles.p, tiles.data(), tiles.size() * sizeof(tiles[0]), hipMemcpyHostToDevice, stream_));
         const int tileCounts[2] = {nTiles, nDyn};
-        VM_HIP(hipMemcpyAsync(&dScanCtl.p->nTiles, tileCounts, sizeof(tileCounts), hipMemcpyHostToDevice, stream_));
-        if (!sw.empty())
-            VM_HIP(hipMemcpyAsync(dSiteWork.p, sw.data(), sw.size() * sizeof(sw[0]), hipMemcpyHostToDevice, stream_));
-        if (!tw.empty())
-            VM_HIP(hipMemcpyAsync(dTargetWork.p, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice, stream_));
-        VM_HIP(hipMemcpyAsync(dCnt.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, stream_));
+        // The tables go up through one pinned staging buffer (DMA at full PCIe
+        // rate; a pageable copy is staged by the runtime on the host thread).
+        const size_t bT = tiles.size() * sizeof(tiles[0]), bS = sw.size() * sizeof(sw[0]);
+        const size_t bW = tw.size() * sizeof(tw[0]), bytes = bT + bS + bW + sizeof(tileCounts) + sizeof(cnt);
+        stage_.reserve(bytes);
+        unsigned char *h = stage_.p;
+        std::memcpy(h, tiles.data(), bT), std::memcpy(h + bT, sw.data(), bS), std::memcpy(h + bT + bS, tw.data(), bW);
+        std::memcpy(h + bT + bS + bW, tileCounts, sizeof(tileCounts));
+        std::memcpy(h + bT + bS + bW + sizeof(tileCounts), cnt, sizeof(cnt));
+        if (bT) VM_HIP(hipMemcpyAsync(dTiles.p, h, bT, hipMemcpyHostToDevice, stream_));
+        VM_HIP(hipMemcpyAsync(&dScanCtl.p->nTiles, h + bT + bS + bW, sizeof(tileCounts), hipMemcpyHostToDevice, stream_));
+        if (bS) VM_HIP(hipMemcpyAsync(dSiteWork.p, h + bT, bS, hipMemcpyHostToDevice, stream_));
+        if (bW) VM_HIP(hipMemcpyAsync(dTargetWork.p, h + bT + bS, bW, hipMemcpyHostToDevice, stream_));
+        VM_HIP(hipMemcpyAsync(dCnt.p, h + bT + bS + bW + sizeof(tileCounts), sizeof(cnt), hipMemcpyHostToDevice, stream_));
         if (!newWork_.empty()) {
             int nn = int(newWork_.size());
             VM_HIP(hipMemcpyAsync(dNewWork.p, newWork_.data(), newWork_.size() * sizeof(newWork_[0]),
@@ -280,13 +307,17 @@ public:
         changed_ = false;
     }
 
-    // `iters` iterations of every resident instance; returns GPU ms.
-    float iterate(int iters, int K)
+    // Enqueues `iters` iterations of every resident instance; wait() blocks
+    // until they ran and returns their GPU ms.
+    void launch(int iters, int K)
     {
         if (!graph_) buildGraph(K);
         VM_HIP(hipEventRecord(ev0_, stream_));
         for (int k = 0; k < iters; ++k) VM_HIP(hipGraphLaunch(graph_, stream_));
         VM_HIP(hipEventRecord(ev1_, stream_));
+    }
+    float wait()
+    {
         VM_HIP(hipEventSynchronize(ev1_));
         float ms = 0;
         VM_HIP(hipEventElapsedTime(&ms, ev0_, ev1_));
@@ -398,6 +429,7 @@ private:
     std::vector<SlotInfo> slotInfo_;
     std::vector<vmk::SiteWork> newWork_;
     std::vector<std::unique_ptr<Prepared>> pending_;
+    PinnedBuf                              stage_;
     const vmk::InstState                   zeroState_{};
 
     DBuf<V>              dPos, dM, dV, dSpos, dTpos;
@@ -491,6 +523,10 @@ void serve(const vm_fit_desc *reqs, const double *arrival, int n, const vm_serve
     double siteIters = 0;
     const int poll = std::max(1, cfg.poll_every);
     while (completed < n) {
+        // The next iterations of the resident instances run while this thread
+        // admits arrived requests (their uploads queue behind the iterations).
+        const bool running = srv.resident() > 0;
+        if (running) srv.launch(poll, K);
         // Admit prepared requests in arrival order.
         while (nextAdmit < n && ready[nextAdmit].load() != 0) {
             if (ready[nextAdmit].load() == 2) throw VmError(VM_EINVAL, "request preparation failed: " + prepError);
@@ -501,12 +537,12 @@ void serve(const vm_fit_desc *reqs, const double *arrival, int n, const vm_serve
             admitT[nextAdmit] = elapsed();
             ++nextAdmit;
         }
-        srv.refresh();
-        if (srv.resident() == 0) {
-            std::this_thread::sleep_for(std::chrono::microseconds(100));
+        if (!running) {
+            srv.refresh();
+            if (srv.resident() == 0) std::this_thread::sleep_for(std::chrono::microseconds(100));
             continue;
         }
-        srv.iterate(poll, K);
+        srv.wait();
         srv.retire([&](int req, const vmk::InstState &st, int slot) {
             double now = elapsed();
             vm_serve_result &r = res[req];
@@ -518,6 +554,7 @@ void serve(const vm_fit_desc *reqs, const double *arrival, int n, const vm_serve
             if (r.final_sites) srv.downloadSites(slot, r.final_sites);
             ++completed;
         });
+        srv.refresh();
     }
     if (stats) {
         stats->wall_s            = elapsed();
