@@ -1088,6 +1088,75 @@ __device__ __forceinline__ void ringCell(int r, int q, int cx, int cy, int &x, i
     else x = cx + r, y = cy - r + 1 + (q - 3 * side + 2);
 }
 
+// The K nearest sites of p under the (d2, index) order, d2 in the key type KR,
+// gathered by one wavefront into lds.cand[0, nc) (sorted; nc = min(K, sites)):
+// the 5 x 5 block (rings 0-2) in one round, then ring by ring in rounds of 64
+// cells; a ring ends the search when the K-th distance is below the distance to
+// the searched block's edge (less a rounding margin).
+template <int K, class R, class KR>
+__device__ __forceinline__ int gatherKnn(ExactLds<KR> &lds, const Inst &in, const V2<R> *spos, const int *sidx,
+                                         const int *strt, V2<R> pr, int lane)
+{
+    const int    gw = in.gw, gh = in.gh;
+    const R      inv    = R(1) / R(in.cell);
+    const KR     cell   = KR(R(in.cell));
+    const KR     rel    = KR(ringRel<R>());
+    const V2<KR> p{KR(pr.x), KR(pr.y)};
+    const KR     margin = rel * fmax(KR(in.ext), fmax(fabs(p.x), fabs(p.y)));
+    const int    cx = cellCoord(pr.x, inv, gw), cy = cellCoord(pr.y, inv, gh);
+    int          nc = 0;
+    for (int r = kExactFirstRings;; ++r) {
+        const int first      = r == kExactFirstRings;
+        const int roundCells = first ? (2 * r + 1) * (2 * r + 1) : 8 * r;
+        for (int q0 = 0; q0 < roundCells; q0 += 64) {
+            int b = 0, cnt = 0;
+            if (q0 + lane < roundCells) {
+                int x, y;
+                if (first) x = cx - r + lane % (2 * r + 1), y = cy - r + lane / (2 * r + 1);
+                else ringCell(r, q0 + lane, cx, cy, x, y);
+                if (x >= 0 && x < gw && y >= 0 && y < gh) {
+                    b   = strt[y * gw + x];
+                    cnt = strt[y * gw + x + 1] - b;
+                }
+            }
+            const int incl   = waveInclusiveSum(cnt);
+            lds.cb[lane]     = b;
+            lds.co[lane + 1] = incl;
+            if (lane == 0) lds.co[0] = 0;
+            const int total = __builtin_amdgcn_readlane(incl, 63);
+            waveSync();
+            for (int s0 = 0; s0 < total;) {
+                const int slice = min(total - s0, kExactCand - nc);
+                for (int s = lane; s < slice; s += 64) {
+                    // Cell of entry s0 + s: the last c with co[c] <= s0 + s.
+                    const int g  = s0 + s;
+                    int       lo = 0;
+#pragma unroll
+                    for (int step = 32; step > 0; step >>= 1)
+                        if (lds.co[lo + step] <= g) lo += step;
+                    const int    site = lds.cb[lo] + (g - lds.co[lo]);
+                    const V2<KR> q{KR(spos[site].x), KR(spos[site].y)};
+                    lds.cand[nc + s] = Key<KR>::make(dist2(q, p), (uint32_t(sidx[site]) << kLdsSlotBits) | uint32_t(kLdsNone));
+                }
+                waveSync();
+                nc += slice;
+                s0 += slice;
+                if (nc == kExactCand) nc = keepNearest<K>(lds, nc, lane);
+            }
+        }
+        nc = keepNearest<K>(lds, nc, lane);
+        const int x0 = cx - r, x1 = cx + r, y0 = cy - r, y1 = cy + r;
+        if (x0 <= 0 && y0 <= 0 && x1 >= gw - 1 && y1 >= gh - 1) break;
+        if (nc == K) {
+            KR safe = fmin(fmin(p.x - KR(x0) * cell, KR(x1 + 1) * cell - p.x),
+                           fmin(p.y - KR(y0) * cell, KR(y1 + 1) * cell - p.y));
+            safe    = fmax(safe - margin, KR(0));
+            if (lds.cand[K - 1].d2() * (KR(1) + rel) < safe * safe) break;
+        }
+    }
+    return nc;
+}
+
 template <int K, class R>
 __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
 {
@@ -1109,65 +1178,7 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
         const int      *sidx     = a.sidx + so;
         const int      *strt     = a.start + in.cellOff;
         const V2<R>     p        = a.tpos[it.tg];
-        const int       gw = in.gw, gh = in.gh;
-        const R         cell   = R(in.cell);
-        const R         inv    = R(1) / cell;
-        const R         rel    = ringRel<R>();
-        const R         margin = rel * fmax(R(in.ext), fmax(fabs(p.x), fabs(p.y)));
-        const int       cx = cellCoord(p.x, inv, gw), cy = cellCoord(p.y, inv, gh);
-
-        // ---- K nearest by rings: first the 5 x 5 block (rings 0-2) in one
-        //      round, then ring by ring in rounds of 64 cells ---------------------
-        int nc = 0;
-        for (int r = kExactFirstRings;; ++r) {
-            const int first     = r == kExactFirstRings;
-            const int roundCells = first ? (2 * r + 1) * (2 * r + 1) : 8 * r;
-            for (int q0 = 0; q0 < roundCells; q0 += 64) {
-                int b = 0, cnt = 0;
-                if (q0 + lane < roundCells) {
-                    int x, y;
-                    if (first) x = cx - r + lane % (2 * r + 1), y = cy - r + lane / (2 * r + 1);
-                    else ringCell(r, q0 + lane, cx, cy, x, y);
-                    if (x >= 0 && x < gw && y >= 0 && y < gh) {
-                        b   = strt[y * gw + x];
-                        cnt = strt[y * gw + x + 1] - b;
-                    }
-                }
-                const int incl = waveInclusiveSum(cnt);
-                lds.cb[lane]     = b;
-                lds.co[lane + 1] = incl;
-                if (lane == 0) lds.co[0] = 0;
-                const int total = __builtin_amdgcn_readlane(incl, 63);
-                waveSync();
-                for (int s0 = 0; s0 < total;) {
-                    const int slice = min(total - s0, kExactCand - nc);
-                    for (int s = lane; s < slice; s += 64) {
-                        // Cell of entry s0 + s: the last c with co[c] <= s0 + s.
-                        const int g  = s0 + s;
-                        int       lo = 0;
-#pragma unroll
-                        for (int step = 32; step > 0; step >>= 1)
-                            if (lds.co[lo + step] <= g) lo += step;
-                        const int site = lds.cb[lo] + (g - lds.co[lo]);
-                        lds.cand[nc + s] = Key<R>::make(dist2(spos[site], p),
-                                                        (uint32_t(sidx[site]) << kLdsSlotBits) | uint32_t(kLdsNone));
-                    }
-                    waveSync();
-                    nc += slice;
-                    s0 += slice;
-                    if (nc == kExactCand) nc = keepNearest<K>(lds, nc, lane);
-                }
-            }
-            nc = keepNearest<K>(lds, nc, lane);
-            const int x0 = cx - r, x1 = cx + r, y0 = cy - r, y1 = cy + r;
-            if (x0 <= 0 && y0 <= 0 && x1 >= gw - 1 && y1 >= gh - 1) break;
-            if (nc == K) {
-                R safe = fmin(fmin(p.x - R(x0) * cell, R(x1 + 1) * cell - p.x),
-                              fmin(p.y - R(y0) * cell, R(y1 + 1) * cell - p.y));
-                safe   = fmax(safe - margin, R(0));
-                if (lds.cand[K - 1].d2() * (R(1) + rel) < safe * safe) break;
-            }
-        }
+        const int       nc       = gatherKnn<K, R, R>(lds, in, spos, sidx, strt, p, lane);
         if (a.dbgKnn && lane < K)
             a.dbgKnn[int64_t(it.tg) * K + lane] = lane < nc ? keySite(lds.cand[lane].low()) : -1;
         if (listOnly) {
@@ -1362,55 +1373,6 @@ __global__ void knnQueryKernel(const Inst *__restrict__ insts, int inst, const V
 // Cell clipping (voro_optimizer.cu:312-437).  FP64 predicates and
 // intersections; the polygon is rebuilt in place through a small ring buffer.
 // ---------------------------------------------------------------------------
-struct CellPoly {
-    double vx[kMaxCellVerts], vy[kMaxCellVerts];
-    int    e[kMaxCellVerts];
-    int    n;
-};
-
-__device__ bool clipByBisector(CellPoly &poly, double rx, double ry, int label)
-{
-    double c      = 0.5 * (rx * rx + ry * ry);
-    bool   anyOut = false;
-    for (int k = 0; k < poly.n; ++k) anyOut |= poly.vx[k] * rx + poly.vy[k] * ry - c > 0.0;
-    if (!anyOut) return true;
-    CellPoly out;
-    out.n   = 0;
-    bool ok = true;
-    for (int k = 0; k < poly.n; ++k) {
-        int    k1 = (k + 1 == poly.n) ? 0 : k + 1;
-        double ax = poly.vx[k], ay = poly.vy[k], bx = poly.vx[k1], by = poly.vy[k1];
-        double fa = ax * rx + ay * ry - c;
-        double fb = bx * rx + by * ry - c;
-        bool   inA = fa <= 0.0, inB = fb <= 0.0;
-        if (inA) {
-            if (out.n < kMaxCellVerts) {
-                out.vx[out.n] = ax, out.vy[out.n] = ay, out.e[out.n] = poly.e[k];
-                ++out.n;
-            } else
-                ok = false;
-        }
-        if (inA != inB) {
-            double t = fa / (fa - fb);
-            if (out.n < kMaxCellVerts) {
-                out.vx[out.n] = ax + t * (bx - ax), out.vy[out.n] = ay + t * (by - ay);
-                out.e[out.n]  = inA ? label : poly.e[k];
-                ++out.n;
-            } else
-                ok = false;
-        }
-    }
-    poly = out;
-    return ok;
-}
-
-__device__ double maxRadius2(const CellPoly &p)
-{
-    double r2 = 0.0;
-    for (int k = 0; k < p.n; ++k) r2 = fmax(r2, p.vx[k] * p.vx[k] + p.vy[k] * p.vy[k]);
-    return r2;
-}
-
 __device__ bool occupied(const uint64_t *masks, int tilesX, int tilesY, float invVoxel, float px, float py)
 {
     if (!masks) return false;
@@ -1421,74 +1383,146 @@ __device__ bool occupied(const uint64_t *masks, int tilesX, int tilesY, float in
     return (masks[int64_t(ty) * tilesX + tx] >> (((j & 7) << 3) | (i & 7))) & 1ull;
 }
 
-template <class R>
-__global__ __launch_bounds__(64) void clipKernel(ClipArgs<R> a)
-{
-    const Inst &in = a.insts[a.inst];
-    int         i  = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= in.S) return;
-    const V2<R> *pos = a.pos + in.siteOff;
-    V2<R>        ci  = pos[i];
-    Key<double>  keys[kCellK]; // neighbours ordered by FP64 distance: the whole clip is FP64
-    knnGlobal<kCellK, R, double>(in, a.spos + in.siteOff, a.sidx + in.siteOff, a.start + in.cellOff, ci, keys);
+// One wavefront per site (4 per workgroup).  The cell polygon is held one
+// vertex per lane; LDS holds the k-NN gather and one clip's output.
+constexpr int kClipWaves = 4;
 
-    CellPoly poly;
-    double   x0 = -double(ci.x), y0 = -double(ci.y), x1 = double(in.W) - double(ci.x), y1 = double(in.H) - double(ci.y);
-    poly.vx[0] = x0, poly.vy[0] = y0, poly.e[0] = -1;
-    poly.vx[1] = x1, poly.vy[1] = y0, poly.e[1] = -2;
-    poly.vx[2] = x1, poly.vy[2] = y1, poly.e[2] = -3;
-    poly.vx[3] = x0, poly.vy[3] = y1, poly.e[3] = -4;
-    poly.n     = 4;
-    bool   ok = true, certified = false;
-    double r2max = maxRadius2(poly);
-    int    idx[kCellK];
-#pragma unroll
-    for (int k = 0; k < kCellK; ++k) idx[k] = keys[k].valid() ? keySite(keys[k].low()) : -1;
-    for (int k = 0; k < kCellK; ++k) {
-        int j = idx[k];
-        if (j < 0) { certified = true; break; }
-        if (j == i) continue;
-        if (double(keys[k].d2()) > 4.0 * r2max) { certified = true; break; }
-        V2<R> cj = pos[j];
-        ok &= clipByBisector(poly, double(cj.x) - double(ci.x), double(cj.y) - double(ci.y), j);
-        r2max = maxRadius2(poly);
-    }
-    if (!certified && double(keys[kCellK - 1].d2()) > 4.0 * r2max) certified = true;
-    if (!certified) {
-        atomicAdd(&a.flags[0], 1);
-        for (int j = 0; j < in.S; ++j) {
-            if (j == i) continue;
-            bool seen = false;
-            for (int k = 0; k < kCellK; ++k) seen |= idx[k] == j;
-            if (seen) continue;
-            V2<R>  cj = pos[j];
-            double rx = double(cj.x) - double(ci.x), ry = double(cj.y) - double(ci.y);
-            if (rx * rx + ry * ry > 4.0 * r2max) continue;
-            ok &= clipByBisector(poly, rx, ry, j);
-            r2max = maxRadius2(poly);
+struct ClipLds {
+    ExactLds<double> knn;
+    double           px[kMaxCellVerts], py[kMaxCellVerts];
+    int              pe[kMaxCellVerts];
+};
+
+// Clips the polygon (vertex k in lane k < n, relative to the site) to the
+// half-plane <x, r> <= |r|^2 / 2 of the bisector with neighbour `label` at
+// offset r (voro_optimizer.cu:312-347).  The output vertices are placed by a
+// prefix sum in the order a sequential Sutherland-Hodgman pass emits them and
+// computed with the same FP64 expressions, so the polygon is bit-identical; an
+// exit crossing takes the new label, an entry keeps the old edge's.  Vertices
+// beyond kMaxCellVerts are dropped and the call returns false.
+__device__ bool clipWave(ClipLds &lds, double &vx, double &vy, int &ve, int &n, double rx, double ry, int label,
+                         int lane)
+{
+    const double c   = 0.5 * (rx * rx + ry * ry);
+    const bool   act = lane < n;
+    const double f   = act ? vx * rx + vy * ry - c : 0.0;
+    if (!__ballot(act && f > 0.0)) return true;
+    const int    k1  = lane + 1 == n ? 0 : lane + 1;
+    const double bx  = __shfl(vx, k1), by = __shfl(vy, k1), fb = __shfl(f, k1);
+    const bool   inA = f <= 0.0, inB = fb <= 0.0;
+    const int    cnt = act ? int(inA) + int(inA != inB) : 0;
+    const int    inc = waveInclusiveSum(cnt);
+    const int    pos = inc - cnt, total = __builtin_amdgcn_readlane(inc, 63);
+    if (act && inA && pos < kMaxCellVerts) lds.px[pos] = vx, lds.py[pos] = vy, lds.pe[pos] = ve;
+    if (act && inA != inB) {
+        const int q = pos + int(inA);
+        if (q < kMaxCellVerts) {
+            const double t = f / (f - fb);
+            lds.px[q] = vx + t * (bx - vx), lds.py[q] = vy + t * (by - vy), lds.pe[q] = inA ? label : ve;
         }
     }
-    if (!ok) atomicAdd(&a.flags[1], 1);
+    waveSync();
+    n = min(total, kMaxCellVerts);
+    if (lane < n) vx = lds.px[lane], vy = lds.py[lane], ve = lds.pe[lane];
+    waveSync();
+    return total <= kMaxCellVerts;
+}
 
-    double area = 0.0, cx = 0.0, cy = 0.0;
-    for (int k = 0; k < poly.n; ++k) {
-        int    k1 = (k + 1 == poly.n) ? 0 : k + 1;
-        double cr = poly.vx[k] * poly.vy[k1] - poly.vx[k1] * poly.vy[k];
-        area += cr;
-        cx += (poly.vx[k] + poly.vx[k1]) * cr;
-        cy += (poly.vy[k] + poly.vy[k1]) * cr;
-    }
-    float cenx = float(ci.x), ceny = float(ci.y);
-    if (fabs(area) > 1e-12) {
-        cenx = cenx + float(cx / (3.0 * area));
-        ceny = ceny + float(cy / (3.0 * area));
-    }
-    a.inside[i] = occupied(a.masks, a.tilesX, a.tilesY, a.invVoxel, cenx, ceny) ? 1 : 0;
-    a.count[i]  = poly.n;
-    for (int k = 0; k < poly.n; ++k) {
-        a.verts[(int64_t(i) * kMaxCellVerts + k) * 2 + 0] = float(double(ci.x) + poly.vx[k]);
-        a.verts[(int64_t(i) * kMaxCellVerts + k) * 2 + 1] = float(double(ci.y) + poly.vy[k]);
-        a.labels[int64_t(i) * kMaxCellVerts + k]          = poly.e[k];
+// Clipped Voronoi cell of every site of one instance (voro_optimizer.cu:361-437):
+// the box clipped by the bisectors of the 32 nearest sites in FP64 order,
+// certified once the next neighbour is farther than twice the cell's radius,
+// else by brute force over every site within that distance (ascending index);
+// FP64 shoelace centroid -> occupancy.
+template <class R>
+__global__ __launch_bounds__(64 * kClipWaves) void clipKernel(ClipArgs<R> a)
+{
+    __shared__ ClipLds ldsAll[kClipWaves];
+    const int    wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    ClipLds     &lds  = ldsAll[wave];
+    const int    lane = threadIdx.x & 63;
+    const Inst   in   = loadUniform(a.insts + a.inst);
+    const V2<R> *pos  = a.pos + in.siteOff;
+    const V2<R> *spos = a.spos + in.siteOff;
+    const int   *sidx = a.sidx + in.siteOff;
+    const int   *strt = a.start + in.cellOff;
+    for (int i = blockIdx.x * kClipWaves + wave; i < in.S; i += gridDim.x * kClipWaves) {
+        const V2<R>  ci  = pos[i];
+        const int    nc  = gatherKnn<kCellK, R, double>(lds.knn, in, spos, sidx, strt, ci, lane);
+        const double cxd = double(ci.x), cyd = double(ci.y);
+        const double x0 = -cxd, y0 = -cyd, x1 = double(in.W) - cxd, y1 = double(in.H) - cyd;
+        // The box: lanes 0-3 hold (x0,y0), (x1,y0), (x1,y1), (x0,y1), edges -1..-4.
+        double vx = (lane == 0 || lane == 3) ? x0 : x1, vy = lane < 2 ? y0 : y1;
+        int    ve = -1 - lane, n = 4;
+        bool   ok = true, certified = false;
+        double r2max = waveMaxR<double>(lane < n ? vx * vx + vy * vy : 0.0);
+        for (int k = 0; k < kCellK; ++k) {
+            const Key<double> key = lds.knn.cand[k < nc ? k : 0];
+            const int         j   = k < nc ? keySite(key.low()) : -1;
+            if (j < 0) {
+                certified = true;
+                break;
+            }
+            if (j == i) continue;
+            if (key.d2() > 4.0 * r2max) {
+                certified = true;
+                break;
+            }
+            const V2<R> cj = pos[j];
+            ok    = clipWave(lds, vx, vy, ve, n, double(cj.x) - cxd, double(cj.y) - cyd, j, lane) && ok;
+            r2max = waveMaxR<double>(lane < n ? vx * vx + vy * vy : 0.0);
+        }
+        if (!certified && lds.knn.cand[kCellK - 1].d2() > 4.0 * r2max) certified = true;
+        if (!certified) {
+            if (lane == 0) atomicAdd(&a.flags[0], 1);
+            for (int jb = 0; jb < in.S; jb += 64) {
+                const int j    = jb + lane;
+                bool      cand = false;
+                if (j < in.S && j != i) {
+                    bool seen = false;
+                    for (int k = 0; k < kCellK; ++k) seen |= k < nc && keySite(lds.knn.cand[k].low()) == j;
+                    if (!seen) {
+                        const V2<R>  cj = pos[j];
+                        const double rx = double(cj.x) - cxd, ry = double(cj.y) - cyd;
+                        cand            = !(rx * rx + ry * ry > 4.0 * r2max); // r2max only shrinks: rechecked below
+                    }
+                }
+                for (uint64_t m = __ballot(cand); m; m &= m - 1) {
+                    const int    jj = jb + __builtin_ctzll(m);
+                    const V2<R>  cj = pos[jj];
+                    const double rx = double(cj.x) - cxd, ry = double(cj.y) - cyd;
+                    if (rx * rx + ry * ry > 4.0 * r2max) continue;
+                    ok    = clipWave(lds, vx, vy, ve, n, rx, ry, jj, lane) && ok;
+                    r2max = waveMaxR<double>(lane < n ? vx * vx + vy * vy : 0.0);
+                }
+            }
+        }
+        if (!ok && lane == 0) atomicAdd(&a.flags[1], 1);
+        // The polygon in LDS for the sequential centroid sums and the output.
+        if (lane < n) lds.px[lane] = vx, lds.py[lane] = vy, lds.pe[lane] = ve;
+        waveSync();
+        if (lane == 0) {
+            double area = 0.0, sx = 0.0, sy = 0.0;
+            for (int k = 0; k < n; ++k) {
+                const int    k1 = (k + 1 == n) ? 0 : k + 1;
+                const double cr = lds.px[k] * lds.py[k1] - lds.px[k1] * lds.py[k];
+                area += cr;
+                sx += (lds.px[k] + lds.px[k1]) * cr;
+                sy += (lds.py[k] + lds.py[k1]) * cr;
+            }
+            float cenx = float(ci.x), ceny = float(ci.y);
+            if (fabs(area) > 1e-12) {
+                cenx = cenx + float(sx / (3.0 * area));
+                ceny = ceny + float(sy / (3.0 * area));
+            }
+            a.inside[i] = occupied(a.masks, a.tilesX, a.tilesY, a.invVoxel, cenx, ceny) ? 1 : 0;
+            a.count[i]  = n;
+        }
+        if (lane < n) {
+            a.verts[(int64_t(i) * kMaxCellVerts + lane) * 2 + 0] = float(cxd + vx);
+            a.verts[(int64_t(i) * kMaxCellVerts + lane) * 2 + 1] = float(cyd + vy);
+            a.labels[int64_t(i) * kMaxCellVerts + lane]          = ve;
+        }
+        waveSync();
     }
 }
 
@@ -1685,7 +1719,8 @@ hipError_t launchKnnQuery(int K, hipStream_t s, const Inst *insts, int inst, con
 template <class R> hipError_t launchClip(int nSites, hipStream_t s, const ClipArgs<R> &a)
 {
     if (nSites <= 0) return hipSuccess;
-    hipLaunchKernelGGL(clipKernel<R>, dim3((nSites + 63) / 64), dim3(64), 0, s, a);
+    const int grid = std::min((nSites + kClipWaves - 1) / kClipWaves, 65536);
+    hipLaunchKernelGGL(clipKernel<R>, dim3(grid), dim3(64 * kClipWaves), 0, s, a);
     return hipGetLastError();
 }
 
