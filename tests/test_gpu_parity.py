@@ -406,3 +406,76 @@ def test_batch_extract_matches_oracle_clip_and_boundary():
         bare = b.extract(i)
         assert not bare["inside"].any() and len(bare["edges"]) == 0
         assert np.array_equal(bare["cell_verts"], m["cell_verts"])
+
+
+def _star_polygon(tips=5, center=(0.5, 0.5), r_out=0.40, r_in=0.18):
+    k = np.arange(2 * tips)
+    a = np.pi / 2 + k * np.pi / tips
+    rad = np.where(k % 2 == 0, r_out, r_in)
+    return np.stack([center[0] + rad * np.cos(a), center[1] + rad * np.sin(a)], 1).astype(np.float32)
+
+
+def _live_targets(slots):
+    """Canvas slot array -> (live targets in slot order, their slot numbers), as the optimizer compacts them."""
+    live = np.nonzero(slots[:, 0] >= 0)[0]
+    return slots[live], live.astype(np.int32)
+
+
+def test_local_edit_matches_oracle():
+    """The local edit sequence -- Canvas.StampDisc -> UpdateTargets(dirty) -> Sampler.Resample(rect) ->
+    UpdateGenerators(rect) -> BeginSchedule(n, rect) -> n steps -- bit-exact against the oracle replaying it:
+    the canvas's live slots as targets (subsampled by slot number), the resampled rect's generators in place of
+    the fitted ones, Adam moments zeroed and sites movable inside the rect only, milestones scaled to the local
+    schedule, the rect's target box and step numbers restarting at 1.  The first edit grows the canvas past the
+    optimizer's target capacity (full re-upload); the second stays within it (dirty slots only)."""
+    r, K, iters, local = 0.025, 12, 120, 50
+    f = np.float32
+    canvas = vmhip.Canvas(W, H, 1.0 / 1024)
+    canvas.rasterize(_star_polygon())
+    sampler = vmhip.Sampler(W, H, r, 30, 1)
+    gw, gh, cs = sampler.grid_w, sampler.grid_h, sampler.cell_size
+    grid = sampler.cells()
+    slots = canvas.target_slots()
+    opt = _optimizer(r, K, fraction=0.2, iterations=iters)
+    opt.set_targets(slots)
+    opt.set_generators(grid, gw, gh, cs)
+    print("case: local edit, global fit", flush=True)
+    opt.step(iters)
+    tgt, tslot = _live_targets(slots)
+    live = _slots(grid)
+    ref, m, v, _, _ = O.fit(grid[live], tgt, K, 0.1, O.make_sched(_lr(r), W, H), iters, fraction=0.2, seed=1,
+                            box=_box(r, gw, gh, cs), slot_of=tslot)
+    assert np.array_equal(opt.generators()[live], ref)
+    pos, mom, vel = grid.copy(), np.zeros_like(grid), np.zeros_like(grid)
+    pos[live], mom[live], vel[live] = ref, m, v
+    x, y = np.arange(gw * gh) % gw, np.arange(gw * gh) // gw
+    sched = O.make_sched(_lr(r), W, H, milestones=O.scale_milestones((80, 150, 200, 250), local, iters))
+    capacity = len(slots)
+    for e, (c, brush, add) in enumerate((((0.5, 0.9), 0.05, True), ((0.5, 0.32), 0.04, False))):
+        edit, dirty = canvas.stamp_disc(c, brush, add)
+        slots = canvas.target_slots()
+        assert edit["removed"] > 0 and edit["added"] > 0 and len(dirty) > 0
+        assert (len(slots) > capacity) == (e == 0)
+        capacity = max(capacity, len(slots) * 3 // 2, len(slots) + 64)
+        opt.update_targets(dirty, slots)
+        margin = brush + 2 * r
+        rect = (max(int(np.floor((c[0] - margin) / cs)), 0), max(int(np.floor((c[1] - margin) / cs)), 0),
+                min(int(np.ceil((c[0] + margin) / cs)), gw), min(int(np.ceil((c[1] + margin) / cs)), gh))
+        print(f"case: local edit {e}, rect {rect}, {len(dirty)} dirty slots", flush=True)
+        sampler.resample(*rect)
+        cells = sampler.cells()
+        opt.update_generators(cells, *rect)
+        opt.begin_schedule(local, rect)
+        opt.step(local)
+        in_rect = (x >= rect[0]) & (x < rect[2]) & (y >= rect[1]) & (y < rect[3])
+        pos[in_rect], mom[in_rect], vel[in_rect] = cells[in_rect], 0, 0
+        live = np.nonzero(pos[:, 0] >= 0)[0]
+        tgt, tslot = _live_targets(slots)
+        box = tuple(float(f(a) * f(cs) + s * f(2) * f(r)) for a, s in zip(rect, (-1, -1, 1, 1)))
+        ref, m, v, _, ch = O.fit(pos[live], tgt, K, 0.1, sched, local, fraction=0.2, seed=1, box=box,
+                                 movable=in_rect[live], slot_of=tslot, m=mom[live], v=vel[live])
+        got = opt.generators()
+        assert ch.min() > 0
+        assert not np.array_equal(ref[in_rect[live]], pos[live][in_rect[live]])  # the rect's sites moved
+        assert np.array_equal(got[live], ref)
+        pos[live], mom[live], vel[live] = ref, m, v
