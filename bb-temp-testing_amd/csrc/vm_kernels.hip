@@ -61,8 +61,18 @@ __device__ __forceinline__ float rand01_poisson(uint32_t cell, uint32_t round, u
 // kernel do not force re-reads.  The address must be wave-uniform.
 template <class T> __device__ __forceinline__ T loadUniform(const T *p)
 {
+    static_assert(sizeof(T) % 4 == 0 && alignof(T) >= 4, "records of whole dwords");
 #if defined(__HIP_DEVICE_COMPILE__)
-    return *(const __attribute__((address_space(4))) T *)p;
+    // Dword by dword: a whole-record copy from this address space is partly
+    // lowered to vector loads.
+    using CU32 = const __attribute__((address_space(4))) uint32_t;
+    CU32    *q = (CU32 *)p;
+    uint32_t w[sizeof(T) / 4];
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(T) / 4; ++k) w[k] = q[k];
+    T out;
+    __builtin_memcpy(&out, w, sizeof(T));
+    return out;
 #else
     return *p; // host pass only parses device code
 #endif
@@ -72,6 +82,19 @@ __device__ __forceinline__ float  fmaR(float a, float b, float c) { return __bui
 __device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float  sqrtR(float a) { return sqrtf(a); }
 __device__ __forceinline__ double sqrtR(double a) { return sqrt(a); }
+
+// Correctly rounded sqrt for finite x >= 2^-96: v_sqrt_f32 is within 1 ulp,
+// and the residuals x - s' s of its neighbours s' pick the rounded root (the
+// same steps sqrtf takes there, without its rescaling of tiny inputs and its
+// zero / infinity cases).
+__device__ __forceinline__ float sqrtNormal(float x)
+{
+    const float s  = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    float       r  = __builtin_fmaf(-sd, s, x) <= 0.0f ? sd : s;
+    return __builtin_fmaf(-su, s, x) > 0.0f ? su : r;
+}
 
 template <class R> __device__ __forceinline__ R dist2(V2<R> p, V2<R> q)
 {
@@ -106,13 +129,14 @@ __device__ __forceinline__ double powi(double b, int e)
 // Gradient accumulation.  FP32: a term c becomes round(c * 2^shift) (double
 // scaling is exact, rounding to nearest even), clamped to +-2^56, and is summed
 // with 64-bit integer atomics; the sum is order-independent.  FP64: double atomics.
-constexpr double kQuantClamp = 72057594037927936.0; // 2^56
-
+// scale = 2^shift, shift in [8, 52].  The (finite) term is clamped to
+// +-2^(56 - shift) in FP32 before the exact scaling, which is the same as
+// clamping c * scale to +-2^56 after it.
 __device__ __forceinline__ long long quantize(float c, double scale)
 {
-    double x = double(c) * scale;
-    x        = fmin(fmax(x, -kQuantClamp), kQuantClamp);
-    return __double2ll_rn(x);
+    const int   g   = int(uint64_t(__double_as_longlong(scale)) >> 52) - 1023;
+    const float lim = __uint_as_float(uint32_t(127 + 56 - g) << 23); // 2^(56 - g)
+    return __double2ll_rn(double(__builtin_amdgcn_fmed3f(c, -lim, lim)) * scale);
 }
 
 template <class R> struct GradOps;
@@ -578,7 +602,9 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     const int       s0     = int(b1 & kIdxMask);
     const V2<float> c0     = lds.pos[s0];
     const V2<float> u{p.x - c0.x, p.y - c0.y};
-    const float     uB = sqrtf(__uint_as_float(b1 | kIdxMask)) * (1.0f + 0x1p-20f); // >= |u|
+    // uB >= |u|: the key's value bounds d2 from above, and v_sqrt_f32 is within
+    // 1 ulp on normal inputs (smaller ones are raised to 2^-100).
+    const float     uB = __builtin_amdgcn_sqrtf(fmaxf(__uint_as_float(b1 | kIdxMask), 0x1p-100f)) * (1.0f + 0x1p-20f);
     uint32_t        a1 = ~0u, a2 = ~0u;
     auto            screenGroups = [&](int g0, int g1) {
 #pragma unroll 1
@@ -621,10 +647,12 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
         member = cnt <= K;
     }
     ok = ok && member;
-    // The reference's term for (c0, w).
-    o.ev  = V2<float>{cw.x - c0.x, cw.y - c0.y};
-    o.l   = sqrtR(dotR(o.ev, o.ev));
-    ok    = ok && !(o.l < 1e-12f);
+    // The reference's term for (c0, w).  Below 2^-90 the length is under
+    // 2^-45 < 1e-12 and the reference skips the pair.
+    o.ev           = V2<float>{cw.x - c0.x, cw.y - c0.y};
+    const float ee = dotR(o.ev, o.ev);
+    o.l            = sqrtNormal(ee);
+    ok             = ok && ee >= 0x1p-90f && !(o.l < 1e-12f);
     o.d   = fmaR(-0.5f, o.l, dotR(u, o.ev) / o.l);
     o.i0  = keySite(lds.klo[s0]), o.sl0 = s0, o.c0 = c0;
     o.bj  = keySite(lds.klo[w]), o.slj = w, o.used = 1;
@@ -741,9 +769,8 @@ template <class R> __device__ __forceinline__ R waveMinR(R v)
 // kLossStage / copies slots; lane l adds into copy l % copies, so lanes whose
 // terms touch the same site collide less.  Copy c is rotated by c slots, which
 // puts the copies of one slot in different LDS banks.  The flush sums them.
-__device__ __forceinline__ int gradSlot(int s, int c, int copies)
+__device__ __forceinline__ int gradSlot(int s, int c, int M) // M = kLossStage / copies
 {
-    const int M = kLossStage / copies;
     return c * M + ((s + c) & (M - 1));
 }
 
@@ -756,9 +783,15 @@ __device__ __forceinline__ int laneRank(uint64_t mask)
 // At most 72 VGPRs, so 7 wavefronts share a SIMD: the kernel is bound by issue
 // and by the latency of its per-chunk loads, and occupancy hides the latter
 // (K = 4 and 6 would spill at 72 and take 6).
-template <int K, class R>
+template <int K, class R, bool Diag>
 __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K == 4 || K == 6 ? 6 : 7))) void lossKernel(LossArgs<R> a)
 {
+    // Diagnostics (path counters, k-NN lists, per-target terms, exact-only
+    // runs) exist only in the Diag instantiation.
+    unsigned long long *const stats     = Diag ? a.stats : nullptr;
+    int *const                dbgKnn    = Diag ? a.dbgKnn : nullptr;
+    TermDbg *const            dbgTerm   = Diag ? a.dbgTerm : nullptr;
+    const bool                exactOnly = Diag && a.exactOnly;
     __shared__ LossLds<R> ldsAll[kLossWaves];
     const int  lane  = threadIdx.x & 63;
     // The wavefront's index is uniform: read it into a scalar register so that
@@ -794,13 +827,13 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             anyTake = anyTake || take[h];
         }
         if (__ballot(anyTake) == 0) {
-            if (a.stats && lane == 0) atomicAdd(&a.stats[8 * (blockIdx.x & 63) + 3], 1ull);
+            if (stats && lane == 0) atomicAdd(&stats[8 * (blockIdx.x & 63) + 3], 1ull);
             continue;
         }
 
         bool         fast  = false;
         int          m     = 0; // compacted candidates (fast path)
-        int          gcopies = 1;
+        int          gcopies = 1, gM = kLossStage; // gradient copies, slots per copy
         int          total = 0; // sites of the staged block
         ScreenBounds sb{};
         if constexpr (sizeof(R) == 4) {
@@ -930,6 +963,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                 }
                 // Clear the gradient copies.
                 gcopies = m <= kLossStage / 4 ? 4 : (m <= kLossStage / 2 ? 2 : 1);
+                gM      = m <= kLossStage / 4 ? kLossStage / 4 : (m <= kLossStage / 2 ? kLossStage / 2 : kLossStage);
                 if (a.grad) {
 #pragma unroll
                     for (int i = lane; i < 2 * kLossStage; i += 64) lds.grad[i] = GradT<R>(0);
@@ -955,17 +989,17 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
         int      openCnt = 0;
 #pragma unroll
         for (int h = 0; h < kTargetsPerLane; ++h) {
-            if (__ballot(take[h] || (a.dbgKnn && lane + 64 * h < wk.count)) == 0) continue;
+            if (__ballot(take[h] || (dbgKnn && lane + 64 * h < wk.count)) == 0) continue;
             const int   t    = wk.begin + lane + 64 * h;
             LaneTerm<R> tr;
             bool        open = take[h];
             if constexpr (sizeof(R) == 4) {
-                if (fast && !a.exactOnly) {
+                if (fast && !exactOnly) {
                     if (screenTerm<K>(lds, sb, p[h], take[h], mask, tr)) open = false;
                     tr.used = take[h] && !open ? tr.used : 0;
                 }
             }
-            const bool     push = open || (a.dbgKnn && take[h]);
+            const bool     push = open || (dbgKnn && take[h]);
             const uint64_t pm   = __ballot(push);
             if (pm) {
                 int base = 0;
@@ -979,23 +1013,23 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             const R lossT = tr.used ? tr.d * tr.d : R(0);
             if (tr.used && a.grad)
                 lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.l, tr.d,
-                         &lds.grad[2 * gradSlot(tr.slj, lane & (gcopies - 1), gcopies)],
-                         &lds.grad[2 * gradSlot(tr.sl0, lane & (gcopies - 1), gcopies)], __builtin_ldexp(1.0, in.gradShift));
-            if (a.dbgKnn && lane + 64 * h < wk.count && !take[h]) {
+                         &lds.grad[2 * gradSlot(tr.slj, lane & (gcopies - 1), gM)],
+                         &lds.grad[2 * gradSlot(tr.sl0, lane & (gcopies - 1), gM)], __builtin_ldexp(1.0, in.gradShift));
+            if (dbgKnn && lane + 64 * h < wk.count && !take[h]) {
 #pragma unroll
-                for (int k = 0; k < K; ++k) a.dbgKnn[(in.targetOff + t) * int64_t(K) + k] = -1;
+                for (int k = 0; k < K; ++k) dbgKnn[(in.targetOff + t) * int64_t(K) + k] = -1;
             }
-            if (a.dbgTerm && lane + 64 * h < wk.count && !open) {
+            if (dbgTerm && lane + 64 * h < wk.count && !open) {
                 const int64_t o   = in.targetOff + t;
-                a.dbgTerm[o].i0   = tr.used ? tr.i0 : -1;
-                a.dbgTerm[o].j    = tr.used ? tr.bj : -1;
-                a.dbgTerm[o].loss = double(lossT);
+                dbgTerm[o].i0   = tr.used ? tr.i0 : -1;
+                dbgTerm[o].j    = tr.used ? tr.bj : -1;
+                dbgTerm[o].loss = double(lossT);
             }
             lossSum += double(lossT);
             used += tr.used;
         }
-        if (a.stats && lane == 0) {
-            unsigned long long *sp = a.stats + 8 * (blockIdx.x & 63); // 64 lines, summed on the host
+        if (stats && lane == 0) {
+            unsigned long long *sp = stats + 8 * (blockIdx.x & 63); // 64 lines, summed on the host
             atomicAdd(&sp[fast ? 0 : (total <= kLossStage ? 1 : 2)], 1ull);
             atomicAdd(&sp[4], (unsigned long long)m);
             atomicAdd(&sp[5], (unsigned long long)total);
@@ -1025,7 +1059,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             for (int s = lane; s < m; s += 64) {
                 GradT<R> gx = GradT<R>(0), gy = GradT<R>(0);
                 for (int c = 0; c < gcopies; ++c) {
-                    const int g = gradSlot(s, c, gcopies);
+                    const int g = gradSlot(s, c, gM);
                     gx += lds.grad[2 * g], gy += lds.grad[2 * g + 1];
                 }
                 if (gx != GradT<R>(0) || gy != GradT<R>(0)) {
@@ -1672,9 +1706,16 @@ hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstS
 
 template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const LossArgs<R> &a)
 {
+    const bool diag   = a.stats || a.dbgKnn || a.dbgTerm || a.exactOnly;
+    const int  blocks = (grid + kLossWaves - 1) / kLossWaves;
     switch (K) {
 #define VM_K(k)                                                                                                   \
-    case k: hipLaunchKernelGGL((lossKernel<k, R>), dim3((grid + kLossWaves - 1) / kLossWaves), dim3(kLossThreads), 0, s, a); break;
+    case k:                                                                                                       \
+        if (diag)                                                                                                 \
+            hipLaunchKernelGGL((lossKernel<k, R, true>), dim3(blocks), dim3(kLossThreads), 0, s, a);              \
+        else                                                                                                      \
+            hipLaunchKernelGGL((lossKernel<k, R, false>), dim3(blocks), dim3(kLossThreads), 0, s, a);             \
+        break;
         VM_K(2) VM_K(4) VM_K(6) VM_K(8) VM_K(12) VM_K(16) VM_K(24) VM_K(32)
 #undef VM_K
     default: return hipErrorInvalidValue;
