@@ -427,10 +427,9 @@ int vm_extract(vm_handle *h, vm_mesh *out)
                 h->mLabels[o + k] = l >= 0 ? h->slotOfSite[l] : l;
             }
         }
-        // Boundary assembly in slot space.
-        vmrt::Boundary bd;
-        vmrt::assembleBoundary(n, h->mInside.data(), h->mStart.data(), reinterpret_cast<const float *>(h->mVerts.data()),
-                               h->mLabels.data(), bd);
+        // The boundary was assembled on the device in site space; live sites are
+        // in slot order, so its vertex numbering and edges are those of slot space.
+        const vmrt::Boundary &bd = c.bound;
         const int V = int(bd.vertices.size() / 2), E = int(bd.edges.size() / 2);
         h->mVertices.resize(size_t(V));
         for (int v = 0; v < V; ++v) h->mVertices[size_t(v)] = vm_vec2{bd.vertices[2 * v], bd.vertices[2 * v + 1]};

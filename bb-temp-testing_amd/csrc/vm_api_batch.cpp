@@ -397,7 +397,7 @@ int vm_batch_extract(vm_batch *b, int id, const uint64_t *masks, int tiles_x, in
                 b->xLabels[o + k]          = c.labels[src + k];
             }
         }
-        vmrt::assembleBoundary(n, b->xInside.data(), b->xStart.data(), b->xVerts.data(), b->xLabels.data(), b->xBound);
+        b->xBound = std::move(c.bound);
         out->num_slots      = n;
         out->generators     = b->xGen.data();
         out->inside         = b->xInside.data();

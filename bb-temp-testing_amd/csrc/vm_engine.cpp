@@ -69,85 +69,6 @@ inline int64_t align64(int64_t x) { return (x + 63) & ~int64_t(63); }
 
 } // namespace
 
-namespace {
-// Vertex key of the boundary mesh: the sorted triple (cells, box sides shifted
-// by kBoxLabels) as three 32-bit fields, so cell counts are not capped.
-struct TripleKey {
-    uint32_t a, b, c;
-    bool     operator==(const TripleKey &o) const { return a == o.a && b == o.b && c == o.c; }
-};
-struct TripleHash {
-    size_t operator()(const TripleKey &k) const
-    {
-        uint64_t h = uint64_t(k.a) * 0x9E3779B97F4A7C15ull;
-        h ^= (uint64_t(k.b) + 0x632BE59BD9B4E019ull + (h << 6) + (h >> 2));
-        h ^= (uint64_t(k.c) + 0x85EBCA77C2B2AE63ull + (h << 6) + (h >> 2));
-        return size_t(h);
-    }
-};
-
-TripleKey tripleKey(int a, int b, int c)
-{
-    uint32_t v[3] = {uint32_t(a + vmk::kBoxLabels), uint32_t(b + vmk::kBoxLabels), uint32_t(c + vmk::kBoxLabels)};
-    if (v[0] > v[1]) std::swap(v[0], v[1]);
-    if (v[1] > v[2]) std::swap(v[1], v[2]);
-    if (v[0] > v[1]) std::swap(v[0], v[1]);
-    return TripleKey{v[0], v[1], v[2]};
-}
-
-} // namespace
-
-void assembleBoundary(int n, const uint8_t *inside, const int32_t *start, const float *verts, const int32_t *labels,
-                      Boundary &out)
-{
-    out.vertices.clear(), out.edges.clear(), out.watertight = false, out.loops = 0;
-    std::unordered_map<TripleKey, int, TripleHash> vertexOf;
-    auto vertex = [&](TripleKey k, const float *p) {
-        auto it = vertexOf.find(k);
-        if (it != vertexOf.end()) return it->second;
-        const int id = int(out.vertices.size() / 2);
-        vertexOf.emplace(k, id);
-        out.vertices.push_back(p[0]), out.vertices.push_back(p[1]);
-        return id;
-    };
-    for (int i = 0; i < n; ++i) {
-        if (!inside[i]) continue;
-        const int      cc  = start[i + 1] - start[i];
-        const int32_t *lab = labels + start[i];
-        const float   *pos = verts + 2 * size_t(start[i]);
-        for (int k = 0; k < cc; ++k) {
-            const int l = lab[k];
-            if (l >= 0 && inside[l]) continue;
-            const int prev = lab[(k + cc - 1) % cc], next = lab[(k + 1) % cc];
-            const int a = vertex(tripleKey(i, prev, l), pos + 2 * k);
-            const int b = vertex(tripleKey(i, l, next), pos + 2 * ((k + 1) % cc));
-            out.edges.push_back(a), out.edges.push_back(b);
-        }
-    }
-    const int        V = int(out.vertices.size() / 2), E = int(out.edges.size() / 2);
-    std::vector<int> nxt(size_t(V), -1), inDeg(size_t(V), 0);
-    bool             wt = true;
-    for (int e = 0; e < E; ++e) {
-        const int a = out.edges[2 * size_t(e)], b = out.edges[2 * size_t(e) + 1];
-        if (nxt[size_t(a)] != -1) wt = false;
-        nxt[size_t(a)] = b;
-        ++inDeg[size_t(b)];
-    }
-    for (int v = 0; v < V; ++v)
-        if (nxt[size_t(v)] == -1 || inDeg[size_t(v)] != 1) wt = false;
-    int loops = 0;
-    if (wt) {
-        std::vector<uint8_t> seen(size_t(V), 0);
-        for (int v = 0; v < V; ++v) {
-            if (seen[size_t(v)]) continue;
-            ++loops;
-            for (int w = v; !seen[size_t(w)]; w = nxt[size_t(w)]) seen[size_t(w)] = 1;
-        }
-    }
-    out.watertight = wt;
-    out.loops      = loops;
-}
-
 // ceil(log2(x)) for x > 0, exact (frexp gives x = m 2^e with m in [0.5, 1)).
 static int ceilLog2(double x)
 {
@@ -316,7 +237,7 @@ public:
     DBuf<vmk::TargetWork> dTargetWork;
     DBuf<vmk::ExactItem> dExactQ;
     DBuf<int>            dExactLen; // one per group
-    hipGraphExec_t       graph = nullptr;
+    hipGraphExec_t       graph = nullptr, graphN = nullptr; // one iteration, kGraphIters iterations
     hipEvent_t           ev[8] = {};
     int                  gridScan = 1, gridSite = 1;
     std::vector<int>     gridLoss;
@@ -324,7 +245,8 @@ public:
     void destroyGraph()
     {
         if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
+        if (graphN) (void)hipGraphExecDestroy(graphN);
+        graph = graphN = nullptr;
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e), e = nullptr;
     }
@@ -518,19 +440,23 @@ public:
         if (marks) VM_HIP(hipEventRecord(marks[5], stream));
     }
 
-    void buildGraph()
+    // Iterations per replay of the long graph: each graph launch costs about
+    // 5 us on the queue, which dominates small fits.
+    static constexpr int kGraphIters = 8;
+
+    void buildGraph(hipGraphExec_t &out, int iters)
     {
         hipGraph_t g;
         VM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         try {
-            enqueueIteration(nullptr);
+            for (int k = 0; k < iters; ++k) enqueueIteration(nullptr);
         } catch (...) {
             hipGraph_t dummy;
             (void)hipStreamEndCapture(stream, &dummy);
             throw;
         }
         VM_HIP(hipStreamEndCapture(stream, &g));
-        hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        hipError_t e = hipGraphInstantiate(&out, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         VM_HIP(e);
     }
@@ -539,10 +465,13 @@ public:
     {
         if (dirty) commit();
         if (iterations <= 0) return 0.0f;
-        if (!graph) buildGraph();
+        if (!graph) buildGraph(graph, 1);
+        if (iterations >= kGraphIters && !graphN) buildGraph(graphN, kGraphIters);
         ensureEvents();
         VM_HIP(hipEventRecord(ev[5], stream));
-        for (int k = 0; k < iterations; ++k) VM_HIP(hipGraphLaunch(graph, stream));
+        int k = 0;
+        for (; k + kGraphIters <= iterations; k += kGraphIters) VM_HIP(hipGraphLaunch(graphN, stream));
+        for (; k < iterations; ++k) VM_HIP(hipGraphLaunch(graph, stream));
         VM_HIP(hipEventRecord(ev[6], stream));
         VM_HIP(hipEventSynchronize(ev[6]));
         float ms = 0;
@@ -726,6 +655,61 @@ public:
     DBuf<float>   clipVerts;
     DBuf<int>     clipLabels, clipCount, clipFlags;
     DBuf<uint8_t> clipInside;
+    // Boundary assembly buffers, kept between calls like the clip's; their end
+    // point capacity grows when a boundary does not fit.
+    DBuf<int>         bndEcount, bndEoff, bndSlot, bndRep, bndVid, bndEdges, bndOut, bndIn, bndNxt, bndLab0, bndLab1,
+                      bndNxt1, bndResult;
+    DBuf<vmk::BndKey> bndKey;
+    DBuf<float>       bndPos, bndVerts;
+    DBuf<uint32_t>    bndTable;
+    int               bndCap = 8192;
+
+    // Boundary of the inside cells of the last clip, assembled on the device
+    // (site space: labels are instance site indices, which keeps slot order).
+    // The outputs are copied at capacity, so the clip's synchronization serves
+    // both in the usual case.
+    struct BndHost {
+        int                res[5] = {0, 0, 0, 0, 0}; // V, watertight, loops, overflow, E
+        std::vector<float> verts;
+        std::vector<int>   edges;
+    };
+
+    void boundaryEnqueue(int S, BndHost &h)
+    {
+        const size_t cap = size_t(bndCap);
+        size_t       tsz = 64;
+        while (tsz < 2 * cap) tsz *= 2;
+        bndEcount.reserve(size_t(S)), bndEoff.reserve(size_t(S));
+        bndKey.reserve(cap), bndPos.reserve(2 * cap), bndTable.reserve(tsz), bndSlot.reserve(cap);
+        bndRep.reserve(cap), bndVid.reserve(cap), bndVerts.reserve(2 * cap), bndEdges.reserve(cap);
+        bndOut.reserve(cap), bndIn.reserve(cap), bndNxt.reserve(cap), bndLab0.reserve(cap), bndLab1.reserve(cap);
+        bndNxt1.reserve(cap), bndResult.reserve(5);
+        vmk::BoundaryArgs b{};
+        b.S = S, b.count = clipCount.p, b.labels = clipLabels.p, b.verts = clipVerts.p, b.inside = clipInside.p;
+        b.cap = int(cap), b.ecount = bndEcount.p, b.eoff = bndEoff.p, b.key = bndKey.p, b.pos = bndPos.p;
+        b.table = bndTable.p, b.tableMask = uint32_t(tsz - 1), b.slot = bndSlot.p, b.rep = bndRep.p;
+        b.vid = bndVid.p, b.vertices = bndVerts.p, b.edges = bndEdges.p, b.outDeg = bndOut.p, b.inDeg = bndIn.p;
+        b.nxt = bndNxt.p, b.lab0 = bndLab0.p, b.lab1 = bndLab1.p, b.nxt1 = bndNxt1.p, b.result = bndResult.p;
+        VM_HIP(vmk::launchBoundary(stream, b));
+        h.verts.resize(2 * cap), h.edges.resize(cap);
+        VM_HIP(hipMemcpyAsync(h.res, bndResult.p, sizeof(h.res), hipMemcpyDeviceToHost, stream));
+        VM_HIP(hipMemcpyAsync(h.verts.data(), bndVerts.p, h.verts.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+        VM_HIP(hipMemcpyAsync(h.edges.data(), bndEdges.p, cap * sizeof(int), hipMemcpyDeviceToHost, stream));
+    }
+
+    // After the copies completed: false (and a larger capacity) when the
+    // boundary had more end points than the capacity.
+    bool boundaryFinish(BndHost &h, Boundary &out)
+    {
+        if (h.res[3]) {
+            while (bndCap < 2 * h.res[4]) bndCap *= 2;
+            return false;
+        }
+        out.vertices.assign(h.verts.begin(), h.verts.begin() + 2 * size_t(h.res[0]));
+        out.edges.assign(h.edges.begin(), h.edges.begin() + 2 * size_t(h.res[4]));
+        out.watertight = h.res[1] != 0, out.loops = h.res[2];
+        return true;
+    }
 
     void clip(int id, const uint64_t *dMasks, int tilesX, int tilesY, float voxel, ClipOut &out) override
     {
@@ -748,6 +732,10 @@ public:
         a.verts = dv.p, a.labels = dl.p, a.count = dc.p, a.inside = di.p, a.flags = df.p;
         VM_HIP(vmk::launchClip<R>(S, stream, a));
         VM_HIP(hipEventRecord(ev[6], stream));
+        // Without occupancy no cell is inside and the boundary is empty.
+        const bool haveBoundary = dMasks != nullptr && S > 0;
+        BndHost    bh;
+        if (haveBoundary) boundaryEnqueue(S, bh);
         bin();
         out.verts.assign(size_t(S) * vmk::kMaxCellVerts * 2, 0.0f);
         out.labels.assign(size_t(S) * vmk::kMaxCellVerts, 0);
@@ -763,6 +751,13 @@ public:
         VM_HIP(hipStreamSynchronize(stream));
         VM_HIP(hipEventElapsedTime(&out.gpuMs, ev[5], ev[6]));
         out.unsafe = flags[0], out.overflow = flags[1];
+        out.bound  = Boundary{};
+        out.bound.watertight = true;
+        if (haveBoundary && !boundaryFinish(bh, out.bound)) {
+            boundaryEnqueue(S, bh);
+            VM_HIP(hipStreamSynchronize(stream));
+            if (!boundaryFinish(bh, out.bound)) throw VmError(2, "boundary assembly: capacity did not grow");
+        }
     }
 };
 

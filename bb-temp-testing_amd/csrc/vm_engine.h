@@ -82,6 +82,15 @@ struct DiagOut {
     std::vector<vmk::TermDbg> terms; // [T] per live target (input order)
 };
 
+// Boundary mesh of the inside cells: vertices welded by label triple, edges as
+// vertex-id pairs, watertight iff every vertex has one in- and one out-edge.
+struct Boundary {
+    std::vector<float>   vertices; // [V * 2]
+    std::vector<int32_t> edges;    // [E * 2]
+    bool                 watertight = false;
+    int                  loops      = 0;
+};
+
 struct ClipOut {
     std::vector<float>   verts;  // [S*24*2]
     std::vector<int>     labels; // [S*24]
@@ -89,6 +98,7 @@ struct ClipOut {
     std::vector<uint8_t> inside; // [S]
     int                  unsafe = 0, overflow = 0;
     float                gpuMs = 0;
+    Boundary             bound;  // assembled on the device (site-index labels)
 };
 
 class Engine {
@@ -133,14 +143,6 @@ public:
 // box sides they are equidistant to and numbered in order of first use, so the
 // mesh is watertight by construction.  watertight: every vertex has exactly
 // one outgoing and one incoming edge; loops then counts the cycles.
-struct Boundary {
-    std::vector<float>   vertices; // [V * 2]
-    std::vector<int32_t> edges;    // [E * 2]
-    bool                 watertight = false;
-    int                  loops      = 0;
-};
-void assembleBoundary(int n, const uint8_t *inside, const int32_t *start, const float *verts, const int32_t *labels,
-                      Boundary &out);
 
 // Lays out one instance at the given pooled offsets: its Inst record, site
 // chunks, target chunks (one wavefront each) and the sort permutation of its

@@ -81,6 +81,39 @@ template <class R> struct ClipArgs {
     int            *flags;   // {unsafe, overflow}
 };
 
+// Boundary of the inside cells, assembled on the device from the clip outputs
+// (voro_optimizer.cu:798-854): every edge of an inside cell facing an outside
+// cell or the box, its end points welded by their sorted label triple.  Vertex
+// ids follow the first appearance of each triple in the order (cell, edge,
+// start before end), as a sequential pass would number them.
+struct BndKey {
+    uint32_t a, b, c, pad; // sorted label triple, box sides shifted by kBoxLabels
+};
+struct BoundaryArgs {
+    int                  S;      // cells (the instance's live sites)
+    const int           *count;  // clip outputs: [S], [S*24], [S*24*2], [S]
+    const int           *labels;
+    const float         *verts;
+    const unsigned char *inside;
+    int                  cap;    // end point capacity of the arrays below (2 per edge)
+    int                 *ecount; // [S] boundary edges per cell
+    int                 *eoff;   // [S] their exclusive offsets
+    BndKey              *key;    // [cap] end point keys: 2e = start, 2e+1 = end of edge e
+    float               *pos;    // [cap*2] end point positions
+    uint32_t            *table;  // [tableMask+1 >= 2 cap] open-addressing table
+    uint32_t             tableMask;
+    int                 *slot;   // [cap] table slot of each end point
+    int                 *rep;    // [cap] first end point with the same key
+    int                 *vid;    // [cap] vertex id of each representative
+    float               *vertices; // [cap*2] out: V vertices
+    int                 *edges;    // [cap] out: 2E vertex ids
+    int                 *outDeg, *inDeg, *nxt; // [cap] (V <= 2E)
+    int                 *lab0, *lab1, *nxt1;   // [cap] loop-count scratch
+    int                 *result; // [5] out: V, watertight, loops, overflow (more than cap end points), E
+};
+// One workgroup: count, emit, weld, degrees, watertight test and loop count.
+hipError_t launchBoundary(hipStream_t s, const BoundaryArgs &a);
+
 // Cells per scan tile, and the tiles of an instance with `cells` cells (gw*gh+1).
 constexpr int kScanTileCells = 16384;
 inline int scanTilesFor(int64_t cells) { return int((cells + kScanTileCells - 1) / kScanTileCells); }

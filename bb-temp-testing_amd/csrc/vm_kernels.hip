@@ -1561,6 +1561,196 @@ __global__ __launch_bounds__(64 * kClipWaves) void clipKernel(ClipArgs<R> a)
 }
 
 // ---------------------------------------------------------------------------
+// 5b. Boundary assembly (voro_optimizer.cu:798-854) on the device, by one
+//     workgroup in phases separated by barriers (boundaries hold O(sqrt S)
+//     edges, and one launch costs less than a chain of them):
+//   count   boundary edges per inside cell, scanned into offsets;
+//   emit    each edge's two end points in the order (cell, edge): key = the
+//           sorted label triple (cell, previous/next label, edge label);
+//   insert  end points into an open-addressing table keyed by the triple; a
+//           slot keeps the smallest end point index of its key (atomicMin);
+//   flag    an end point is its key's representative when it is that minimum;
+//           vertex ids = exclusive scan of the flags (first-appearance order);
+//   final   vertex positions from the representatives, edge ids, in/out
+//           degrees and each vertex's successor;
+//   loops   watertight iff every vertex has one in- and one out-edge; loops =
+//           cycles of the successor map, by pointer jumping (after round r a
+//           vertex holds the smallest id among its next 2^r successors, so
+//           with 2^r >= V every vertex holds its cycle's minimum; loops =
+//           vertices that are their own minimum).
+// More than `cap` end points: result[3] = 1 and nothing past the count.
+// ---------------------------------------------------------------------------
+constexpr int      kBndThreads  = 1024;
+constexpr int      kBndLdsVerts = 4096;
+constexpr uint32_t kBndEmpty    = ~0u;
+
+__device__ __forceinline__ bool facesOut(int l, const unsigned char *inside) { return l < 0 || !inside[l]; }
+
+__device__ __forceinline__ BndKey bndKey(int x, int y, int z)
+{
+    uint32_t v0 = uint32_t(x + kBoxLabels), v1 = uint32_t(y + kBoxLabels), v2 = uint32_t(z + kBoxLabels), t;
+    if (v0 > v1) t = v0, v0 = v1, v1 = t;
+    if (v1 > v2) t = v1, v1 = v2, v2 = t;
+    if (v0 > v1) t = v0, v0 = v1, v1 = t;
+    return BndKey{v0, v1, v2, 0u};
+}
+
+__device__ __forceinline__ uint32_t bndHash(const BndKey &k)
+{
+    uint32_t h = k.a * 0x9E3779B1u;
+    h ^= k.b * 0x85EBCA77u + (h << 6) + (h >> 2);
+    h ^= k.c * 0xC2B2AE3Du + (h << 6) + (h >> 2);
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    return h ^ (h >> 12);
+}
+
+// Exclusive scan of the values val(0..n-1) by the workgroup (contiguous
+// segments per thread) into out; returns the total to every thread.
+template <class F> __device__ int bndScan(F val, int *out, int n, int *waveTot)
+{
+    const int t = int(threadIdx.x), per = (n + kBndThreads - 1) / kBndThreads;
+    const int b = min(t * per, n), e = min(b + per, n);
+    int       s = 0;
+    for (int i = b; i < e; ++i) s += val(i);
+    const int incl = waveInclusiveSum(s);
+    if ((t & 63) == 63) waveTot[t >> 6] = incl;
+    __syncthreads();
+    int run = incl - s, total = 0;
+    for (int w = 0; w < kBndThreads / 64; ++w) {
+        if (w < (t >> 6)) run += waveTot[w];
+        total += waveTot[w];
+    }
+    for (int i = b; i < e; ++i) {
+        const int v = val(i);
+        out[i]      = run;
+        run += v;
+    }
+    __syncthreads();
+    return total;
+}
+
+// Pointer jumping over the successor map nxt0 of V vertices (in LDS or in
+// global scratch); returns how many of this thread's vertices are the
+// smallest of their cycle.
+template <class P>
+__device__ int bndJump(P lab, P nx, P labN, P nxN, const int *nxt0, int V)
+{
+    const int t = int(threadIdx.x);
+    for (int v = t; v < V; v += kBndThreads) lab[v] = v, nx[v] = nxt0[v];
+    __syncthreads();
+    for (int span = 1; span < V; span *= 2) {
+        for (int v = t; v < V; v += kBndThreads) {
+            const int w = nx[v];
+            labN[v]     = min(lab[v], lab[w]);
+            nxN[v]      = nx[w];
+        }
+        __syncthreads();
+        P tl = lab, tn = nx;
+        lab = labN, nx = nxN, labN = tl, nxN = tn;
+    }
+    int own = 0;
+    for (int v = t; v < V; v += kBndThreads) own += lab[v] == v ? 1 : 0;
+    return own;
+}
+
+__global__ __launch_bounds__(kBndThreads) void boundaryKernel(BoundaryArgs a)
+{
+    __shared__ int waveTot[kBndThreads / 64];
+    __shared__ int sBad, sLoops;
+    __shared__ int sJump[4][kBndLdsVerts]; // pointer-jumping labels and successors, double-buffered
+    const int      t = int(threadIdx.x);
+    if (t == 0) sBad = 0, sLoops = 0;
+    // count
+    for (int i = t; i < a.S; i += kBndThreads) {
+        int n = 0;
+        if (a.inside[i]) {
+            const int *lab = a.labels + size_t(i) * kMaxCellVerts;
+            for (int k = 0; k < a.count[i]; ++k) n += facesOut(lab[k], a.inside) ? 1 : 0;
+        }
+        a.ecount[i] = n;
+    }
+    __syncthreads();
+    const int E = bndScan([&](int i) { return a.ecount[i]; }, a.eoff, a.S, waveTot);
+    const int n = 2 * E;
+    if (n > a.cap) {
+        if (t == 0) a.result[0] = 0, a.result[1] = 0, a.result[2] = 0, a.result[3] = 1, a.result[4] = E;
+        return;
+    }
+    for (uint32_t h = uint32_t(t); h <= a.tableMask; h += kBndThreads) a.table[h] = kBndEmpty;
+    for (int v = t; v < n; v += kBndThreads) a.outDeg[v] = 0, a.inDeg[v] = 0;
+    // emit
+    for (int i = t; i < a.S; i += kBndThreads) {
+        if (!a.inside[i]) continue;
+        const int    cc  = a.count[i];
+        const int   *lab = a.labels + size_t(i) * kMaxCellVerts;
+        const float *vp  = a.verts + size_t(i) * kMaxCellVerts * 2;
+        int          j   = 2 * a.eoff[i];
+        for (int k = 0; k < cc; ++k) {
+            const int l = lab[k];
+            if (!facesOut(l, a.inside)) continue;
+            const int kp = k == 0 ? cc - 1 : k - 1, kn = k + 1 == cc ? 0 : k + 1;
+            a.key[j]         = bndKey(i, lab[kp], l);
+            a.key[j + 1]     = bndKey(i, l, lab[kn]);
+            a.pos[2 * j]     = vp[2 * k];
+            a.pos[2 * j + 1] = vp[2 * k + 1];
+            a.pos[2 * j + 2] = vp[2 * kn];
+            a.pos[2 * j + 3] = vp[2 * kn + 1];
+            j += 2;
+        }
+    }
+    __syncthreads();
+    // insert (the table has at least twice as many slots as end points, so
+    // probing ends)
+    for (int j = t; j < n; j += kBndThreads) {
+        const BndKey k = a.key[j];
+        uint32_t     h = bndHash(k) & a.tableMask;
+        for (;;) {
+            const uint32_t v = atomicCAS(&a.table[h], kBndEmpty, uint32_t(j));
+            if (v == kBndEmpty) break;
+            const BndKey o = a.key[v];
+            if (o.a == k.a && o.b == k.b && o.c == k.c) {
+                atomicMin(&a.table[h], uint32_t(j));
+                break;
+            }
+            h = (h + 1) & a.tableMask;
+        }
+        a.slot[j] = int(h);
+    }
+    __syncthreads();
+    // flag
+    for (int j = t; j < n; j += kBndThreads) a.rep[j] = int(a.table[a.slot[j]]);
+    __syncthreads();
+    const int V = bndScan([&](int j) { return a.rep[j] == j ? 1 : 0; }, a.vid, n, waveTot);
+    // final
+    for (int j = t; j < n; j += kBndThreads) {
+        const int r = a.rep[j], id = a.vid[r];
+        a.edges[j]  = id;
+        if (r == j) a.vertices[2 * id] = a.pos[2 * j], a.vertices[2 * id + 1] = a.pos[2 * j + 1];
+        if ((j & 1) == 0) {
+            atomicAdd(&a.outDeg[id], 1);
+            a.nxt[id] = a.vid[a.rep[j + 1]];
+        } else {
+            atomicAdd(&a.inDeg[id], 1);
+        }
+    }
+    __syncthreads();
+    // loops
+    for (int v = t; v < V; v += kBndThreads)
+        if (a.outDeg[v] != 1 || a.inDeg[v] != 1) atomicOr(&sBad, 1);
+    __syncthreads();
+    const bool wt = sBad == 0;
+    if (wt) {
+        // Up to kBndLdsVerts vertices the rounds run in LDS.
+        const int own = V <= kBndLdsVerts ? bndJump(sJump[0], sJump[1], sJump[2], sJump[3], a.nxt, V)
+                                          : bndJump(a.lab0, a.nxt, a.lab1, a.nxt1, a.nxt, V);
+        if (own) atomicAdd(&sLoops, own);
+        __syncthreads();
+    }
+    if (t == 0) a.result[0] = V, a.result[1] = wt ? 1 : 0, a.result[2] = wt ? sLoops : 0, a.result[3] = 0, a.result[4] = E;
+}
+
+// ---------------------------------------------------------------------------
 // Poisson-disk dart throwing over a table of instance grids
 // (poisson_disk.cu:138-177).  Cells of one phase group are >= 3 apart, so
 // every thread writes only its own cell and the result is deterministic.
@@ -1762,6 +1952,13 @@ template <class R> hipError_t launchClip(int nSites, hipStream_t s, const ClipAr
     if (nSites <= 0) return hipSuccess;
     const int grid = std::min((nSites + kClipWaves - 1) / kClipWaves, 65536);
     hipLaunchKernelGGL(clipKernel<R>, dim3(grid), dim3(64 * kClipWaves), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchBoundary(hipStream_t s, const BoundaryArgs &a)
+{
+    if (a.S <= 0) return hipSuccess;
+    hipLaunchKernelGGL(boundaryKernel, dim3(1), dim3(kBndThreads), 0, s, a);
     return hipGetLastError();
 }
 

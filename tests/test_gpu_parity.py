@@ -479,3 +479,44 @@ def test_local_edit_matches_oracle():
         assert not np.array_equal(ref[in_rect[live]], pos[live][in_rect[live]])  # the rect's sites moved
         assert np.array_equal(got[live], ref)
         pos[live], mom[live], vel[live] = ref, m, v
+
+
+def _disc_grid_masks(vox_n, n_side, radius):
+    """Occupancy of an n_side x n_side grid of discs (voxel centres strictly inside a disc)."""
+    vox = 1.0 / vox_n
+    tx = ty = (vox_n + 7) // 8
+    c = (np.arange(vox_n) + 0.5) * vox
+    X, Y = np.meshgrid(c, c)
+    cx, cy = (np.floor(X * n_side) + 0.5) / n_side, (np.floor(Y * n_side) + 0.5) / n_side
+    j, i = np.nonzero((X - cx) ** 2 + (Y - cy) ** 2 < radius ** 2)
+    masks = np.zeros(tx * ty, np.uint64)
+    np.bitwise_or.at(masks, (j >> 3) * tx + (i >> 3), np.left_shift(np.uint64(1), (((j & 7) << 3) | (i & 7)).astype(np.uint64)))
+    return masks, tx, ty, np.float32(vox)
+
+
+def test_boundary_long_bitexact_and_regrow():
+    """A boundary of 576 discs (about 9,000 edges): more end points than the assembly's first capacity (the
+    buffers grow and the assembly runs again) and more vertices than its LDS pointer jumping holds (the
+    rounds run in global memory).  Vertices, edges, watertightness and loops are bit-exact against the
+    oracle's sequential assembly, and a second extract (grown buffers) returns the same mesh."""
+    n_side = 24
+    masks, tx, ty, vox = _disc_grid_masks(1024, n_side, 0.3 / n_side)
+    grid = O.poisson(W, H, wl.poisson_radius_for(40000), 30, 11)
+    sites = wl.live_sites(grid)
+    b = vmhip.Batch()
+    b.add(W, H, sites, wl.star(8 * len(sites), seed=11), 1e-3, loss_k=16)
+    b.commit()
+    b.step(3)
+    live = b.sites(0)
+    print(f"case: long boundary, {len(live)} sites", flush=True)
+    m = b.extract(0, masks, (tx, ty), vox)
+    ref = O.clip(live, W, H, 0.05, masks=masks, tiles=(tx, ty), voxel=vox)
+    assert np.array_equal(m["inside"], ref["inside"])
+    bd = O.boundary(ref["verts"], ref["labels"], ref["count"], ref["inside"])
+    assert len(bd["edges"]) > 4096 and len(bd["vertices"]) > 4096
+    assert np.array_equal(m["vertices"], bd["vertices"]) and np.array_equal(m["edges"], bd["edges"])
+    assert m["watertight"] == bd["watertight"] and m["loops"] == bd["loops"]
+    assert bd["watertight"] and bd["loops"] >= n_side * n_side // 2
+    again = b.extract(0, masks, (tx, ty), vox)
+    assert np.array_equal(again["vertices"], m["vertices"]) and np.array_equal(again["edges"], m["edges"])
+    assert again["loops"] == m["loops"]
