@@ -413,22 +413,33 @@ public:
 
     void bin() { VM_HIP(vmk::launchAdamBin<R>(gridSite, stream, adamArgs(0))); }
 
+    // Hash-grid build: scan, then scatter (folded into the scan when no instance
+    // has more than one scan tile: one launch less per iteration).
     void gridBuild(int mode)
     {
+        vmk::ScanScatter sc{};
+        if (L.dynTiles == 0)
+            sc = vmk::ScanScatter{dPos.p, dCellOf.p, dRank.p, dSpos.p, dSidx.p, int(sizeof(V))};
         VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dTiles.p, dScanCtl.p, dFlags.p,
-                               dCounts.p, dStart.p, mode, dExactLen.p, int(L.groups.size())));
-        VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p, dCellOf.p,
-                                     dRank.p, dStart.p, dSpos.p, dSidx.p, mode, dScanCtl.p));
+                               dCounts.p, dStart.p, mode, dExactLen.p, int(L.groups.size()), sc));
+        if (L.dynTiles != 0)
+            VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
+                                         dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, mode, dScanCtl.p));
     }
 
     void enqueueIteration(hipEvent_t *marks)
     {
         if (marks) VM_HIP(hipEventRecord(marks[0], stream));
+        vmk::ScanScatter sc{};
+        if (L.dynTiles == 0)
+            sc = vmk::ScanScatter{dPos.p, dCellOf.p, dRank.p, dSpos.p, dSidx.p, int(sizeof(V))};
         VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dTiles.p, dScanCtl.p, dFlags.p,
-                               dCounts.p, dStart.p, vmk::kModeIterate, dExactLen.p, int(L.groups.size())));
+                               dCounts.p, dStart.p, vmk::kModeIterate, dExactLen.p, int(L.groups.size()), sc));
         if (marks) VM_HIP(hipEventRecord(marks[1], stream));
-        VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p, dCellOf.p,
-                                     dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate, dScanCtl.p));
+        if (L.dynTiles != 0)
+            VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
+                                         dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate,
+                                         dScanCtl.p));
         if (marks) VM_HIP(hipEventRecord(marks[2], stream));
         for (size_t g = 0; g < L.groups.size(); ++g)
             VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, lossArgs(int(g), vmk::kModeIterate, true)));

@@ -118,9 +118,18 @@ hipError_t launchBoundary(hipStream_t s, const BoundaryArgs &a);
 constexpr int kScanTileCells = 16384;
 inline int scanTilesFor(int64_t cells) { return int((cells + kScanTileCells - 1) / kScanTileCells); }
 
+// Scatter folded into the scan when every instance has a single tile
+// (posBytes = sizeof(V2<R>); 0: the scatter kernel follows).
+struct ScanScatter {
+    const void *pos;
+    const int  *cellOf, *rank;
+    void       *spos;
+    int        *sidx;
+    int         posBytes;
+};
 hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, LossPartial *partial,
                       const ScanTile *tiles, ScanCtl *ctl, unsigned long long *flags, int *counts, int *start,
-                      int mode, int *exactLen, int nQueues);
+                      int mode, int *exactLen, int nQueues, const ScanScatter &sc);
 template <class R>
 hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstState *state, const SiteWork *work,
                          const int *nWork, const void *pos, const int *cellOf, const int *rank, const int *start,

@@ -404,15 +404,34 @@ __device__ __forceinline__ void scanTile(const ScanTile &tl, int tile, bool dyn,
     __syncthreads();
 }
 
+// Counting-sort scatter of one instance's sites by the workgroup that just
+// scanned its (single) tile: the starts are this workgroup's own writes.
+__device__ void scatterInstance(const Inst &in, const ScanScatter &sc, const int *start)
+{
+    for (int i = int(threadIdx.x); i < in.S; i += kScanThreads) {
+        const int64_t g = in.siteOff + i;
+        const int64_t d = in.siteOff + start[in.cellOff + sc.cellOf[g]] + sc.rank[g];
+        if (sc.posBytes == 8)
+            reinterpret_cast<int2 *>(sc.spos)[d] = reinterpret_cast<const int2 *>(sc.pos)[g];
+        else
+            reinterpret_cast<int4 *>(sc.spos)[d] = reinterpret_cast<const int4 *>(sc.pos)[g];
+        sc.sidx[d] = i;
+    }
+}
+
 // Tiles [0, nDyn) belong to multi-tile instances and are taken in order from
 // the counter (a workgroup only ever waits on tiles that running workgroups
 // took before it); tiles [nDyn, nTiles) are single-tile instances, spread
 // statically.  The scatter kernel, which follows every scan, rearms the
-// counter and advances the flag epoch.
+// counter and advances the flag epoch.  When no instance has more than one
+// tile (sc.posBytes != 0) each workgroup scatters the instances it scanned and
+// there is no scatter launch (nothing to rearm: only multi-tile instances use
+// the counter and the flags).
 __global__ __launch_bounds__(kScanThreads) void scanKernel(const Inst *__restrict__ insts, InstState *state,
                                                           LossPartial *partial, const ScanTile *__restrict__ tiles,
                                                           ScanCtl *ctl, unsigned long long *flags, int *counts,
-                                                          int *start, int mode, int *exactLen, int nQueues)
+                                                          int *start, int mode, int *exactLen, int nQueues,
+                                                          ScanScatter sc)
 {
     __shared__ int waveSum[kScanThreads / 64];
     __shared__ int sShared[3];
@@ -436,9 +455,11 @@ __global__ __launch_bounds__(kScanThreads) void scanKernel(const Inst *__restric
                      sShared);
         }
     }
-    for (int tile = nDyn + blockIdx.x; tile < nTiles; tile += gridDim.x)
-        scanTile(tiles[tile], tile, false, epoch, insts, state, partial, flags, counts, start, mode, waveSum,
-                 sShared);
+    for (int tile = nDyn + blockIdx.x; tile < nTiles; tile += gridDim.x) {
+        const ScanTile tl = tiles[tile];
+        scanTile(tl, tile, false, epoch, insts, state, partial, flags, counts, start, mode, waveSum, sShared);
+        if (sc.posBytes && !sShared[0]) scatterInstance(insts[tl.inst], sc, start); // sShared[0]: step skipped
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1875,11 +1896,12 @@ __global__ void fillIntKernel(int *p, int64_t n, int v)
 // ===========================================================================
 hipError_t launchScan(int grid, hipStream_t s, const Inst *insts, InstState *state, LossPartial *partial,
                       const ScanTile *tiles, ScanCtl *ctl, unsigned long long *flags, int *counts, int *start,
-                      int mode, int *exactLen, int nQueues)
+                      int mode, int *exactLen, int nQueues, const ScanScatter &sc)
 {
     if (nQueues > kScanThreads) return hipErrorInvalidValue;
+    if (sc.posBytes != 0 && sc.posBytes != 8 && sc.posBytes != 16) return hipErrorInvalidValue;
     hipLaunchKernelGGL(scanKernel, dim3(grid), dim3(kScanThreads), 0, s, insts, state, partial, tiles, ctl, flags,
-                       counts, start, mode, exactLen, nQueues);
+                       counts, start, mode, exactLen, nQueues, sc);
     return hipGetLastError();
 }
 

@@ -389,7 +389,7 @@ private:
         VM_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
         try {
             VM_HIP(vmk::launchScan(gScan, stream_, dInst.p, dState.p, dPartial.p, dTiles.p, dScanCtl.p, dFlags.p,
-                                   dCounts.p, dStart.p, vmk::kModeIterate, dExactLen.p, 1));
+                                   dCounts.p, dStart.p, vmk::kModeIterate, dExactLen.p, 1, vmk::ScanScatter{}));
             VM_HIP(vmk::launchScatter<R>(gSite, stream_, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
                                          dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate,
                                          dScanCtl.p));
