@@ -376,7 +376,7 @@ def main():
                                "achieved_GBs": iter_bytes(S, T, C) / (ms_per_step / 1e3) / 1e9},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("batch512") if
-                     args.instances == 512 and args.seed == 2 else None, "kernel": "lossKernel<16,float>",
+                     args.instances == 512 and args.seed == 2 else None, "kernel": "lossKernel<16,float,false>",
                      "algorithmic_bytes": lb, "kernel_ms": loss_ms},
     }
     if not args.no_extra:
