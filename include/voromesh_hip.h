@@ -80,8 +80,8 @@ typedef struct vm_mesh {
     int            loops;
     int            unsafe_cells;
     int            overflow_cells;
-    float          gpu_ms;
-    float          host_ms;
+    float          gpu_ms;      /* grid build + clip kernels (the boundary kernel runs after them) */
+    float          host_ms;     /* host work after the device results arrived (CSR in slot order) */
 } vm_mesh;
 
 VM_API void        vm_default_params(vm_params *p);
