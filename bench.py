@@ -7,7 +7,7 @@ per iteration (one iteration captured as a HIP graph).  A "step" is one
 optimizer iteration of every instance.  Also measured: the same fits run one at
 a time through the plain per-instance path, the loss kernel's duration (HIP
 events on the batch's stream) for the roofline, and the CPU oracle on a bounded
-sample of the same workload.
+sample of the same workload (the oracle is used only for those CPU baselines).
 
 Prints one JSON line.  `value` = site-iterations per second of the batched form.
 """
@@ -174,18 +174,37 @@ def run_large(iters=500, K=16, n_sites=262144, n_targets=4194304):
                 iteration_frac=ib / (ms / iters / 1e3) / 1e9 / HBM_PEAK_GBS, grid_cell=ch["grid_cell"],
                 loss_paths_start=stats0, loss_paths_end=stats1, loss_kernel_traffic_bytes=measured_traffic("large262k"),
                 loss_kernel_algorithmic_bytes=lb,
-                final_mean_loss=b.status(0)["last_mean_loss"])
+                final_mean_loss=b.status(0)["last_mean_loss"], fp32_vs_fp64_gpu=large_precision(grids[0], r, tg, K))
+
+
+def large_precision(grid, r, tg, K):
+    """One loss pass of the large instance in FP32 and in the all-FP64 mode: norm-wise, max and mean gradient
+    error, and how many targets select a different nearest site or bisector partner (near-ties below FP32
+    resolution, whose terms dominate the norm-wise figure)."""
+    import vmhip
+    gw, gh, cs = vmhip.poisson_grid_dims(1.0, 1.0, r)
+    lp = {}
+    for prec in (0, 1):
+        o = vmhip.Optimizer(1.0, 1.0, r, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 / r, precision=prec)
+        o.set_targets(tg)
+        o.set_generators(grid, gw, gh, cs)
+        lp[prec] = o.loss_pass(len(tg), want_terms=True)
+        o.close()
+    g32, g64 = lp[0]["grad"], lp[1]["grad"]
+    ge = np.abs(g32 - g64)
+    flips = int(np.count_nonzero((lp[0]["nearest"] != lp[1]["nearest"]) | (lp[0]["partner"] != lp[1]["partner"])))
+    return dict(grad_rel_norm_err=float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+                grad_max_abs_err=float(ge.max()), grad_mean_abs_err=float(ge.mean()), selection_flips=flips)
 
 
 def run_precision(iters=5, K=16):
-    """Jittered 256x256 lattice (+-1e-7) and 524,288 circle targets: FP32 (FP64 clip and loss sum)
-    and all-FP64 GPU runs against the FP64 CPU oracle; clipped-cell topology mismatches."""
-    import oracle as O
+    """Jittered 256x256 lattice (+-1e-7) and 524,288 circle targets: the FP32 run (FP64 clip and loss sum)
+    against the all-FP64 GPU run: gradient, positions after `iters` steps, clipped-cell topology.  (Both runs
+    against the FP64 CPU oracle are tests: tests/test_gpu_parity.py.)"""
     import vmhip
     from vmhip import workloads as wl
     sites = wl.lattice(256, 1e-7, seed=3)
     tg = wl.circle(524288, radius=0.3, sigma=1e-4, seed=3)
-    out = {}
     runs = {}
     for prec in (0, 1):
         opt = vmhip.Optimizer(1.0, 1.0, 1 / 256, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 * 256,
@@ -196,31 +215,20 @@ def run_precision(iters=5, K=16):
         mesh0 = opt.extract()
         opt.step(iters)
         runs[prec] = (grad, mesh0, opt.generators().astype(np.float64))
-    s64 = sites.astype(np.float64)
-    r64 = O.loss_grad(s64, tg.astype(np.float64), K, 1.0, 1.0, 1 / 128, real=1)
-    lr = float(np.float32(np.float32(1e-3 * 256) * np.float32(1 / 256)))
-    sc = O.make_sched(lr, 1.0, 1.0, milestones=O.scale_milestones((80, 150, 200, 250), iters, iters))
-    box = (-2 / 256, -2 / 256, 1 + 2 / 256, 1 + 2 / 256)
-    p64, _, _, _, _ = O.fit(s64, tg.astype(np.float64), K, 1 / 128, sc, iters, box=box, real=1)
-    ref_clip = O.clip(s64, 1.0, 1.0, 1 / 128, real=1)
-    gnorm = np.linalg.norm(r64["grad"])
-    for prec, name in ((0, "fp32"), (1, "fp64")):
-        grad, mesh0, pos = runs[prec]
-        ge = np.abs(grad - r64["grad"])
-        pe = np.abs(pos - p64)
-        cnt = mesh0["cell_start"][1:] - mesh0["cell_start"][:-1]
-        mism = 0
-        for i in range(len(sites)):
-            a = mesh0["cell_labels"][mesh0["cell_start"][i]:mesh0["cell_start"][i + 1]]
-            b = ref_clip["labels"][i, :ref_clip["count"][i]]
-            mism += int(cnt[i] != ref_clip["count"][i] or not np.array_equal(a, b))
-        out[name] = dict(grad_rel_norm_err=float(np.linalg.norm(grad - r64["grad"]) / gnorm),
-                         grad_max_abs_err=float(ge.max()), grad_mean_abs_err=float(ge.mean()),
-                         pos_max_abs_err=float(pe.max()), pos_mean_abs_err=float(pe.mean()),
-                         topology_mismatches=mism, unsafe_cells=mesh0["unsafe_cells"],
-                         overflow_cells=mesh0["overflow_cells"])
-    out.update(sites=len(sites), targets=len(tg), iterations=iters)
-    return out
+    g64, m64, p64 = runs[1]
+    grad, mesh0, pos = runs[0]
+    ge, pe = np.abs(grad - g64), np.abs(pos - p64)
+    mism = 0
+    for i in range(len(sites)):
+        a = mesh0["cell_labels"][mesh0["cell_start"][i]:mesh0["cell_start"][i + 1]]
+        b = m64["cell_labels"][m64["cell_start"][i]:m64["cell_start"][i + 1]]
+        mism += int(not np.array_equal(a, b))
+    fp32 = dict(grad_rel_norm_err=float(np.linalg.norm(grad - g64) / np.linalg.norm(g64)),
+                grad_max_abs_err=float(ge.max()), grad_mean_abs_err=float(ge.mean()),
+                pos_max_abs_err=float(pe.max()), pos_mean_abs_err=float(pe.mean()),
+                topology_mismatches=mism, unsafe_cells=mesh0["unsafe_cells"], overflow_cells=mesh0["overflow_cells"])
+    return dict(fp32_vs_fp64_gpu=fp32, fp64_unsafe_cells=m64["unsafe_cells"], fp64_overflow_cells=m64["overflow_cells"],
+                sites=len(sites), targets=len(tg), iterations=iters)
 
 
 def run_local_edit(timeout=120):
@@ -239,13 +247,12 @@ def run_local_edit(timeout=120):
 
 def run_small(iters=200, K=16):
     """Plain single fit: Poisson r=0.025 seed 0, 8,192 star targets (sigma 1e-3), 200 iterations, clip;
-    also through the CPU oracle in full."""
-    import oracle as O
+    its CPU baseline is the oracle restatement over the same fit (cpu_baseline_small)."""
     import vmhip
     from vmhip import workloads as wl
     r = 0.025
-    grid = O.poisson(1.0, 1.0, r, 30, 0)
-    gw, gh, cs = O.poisson_dims(1.0, 1.0, r)
+    smp = vmhip.Sampler(1.0, 1.0, r, 30, 0)
+    grid, gw, gh, cs = smp.cells(), smp.grid_w, smp.grid_h, smp.cell_size
     tg = wl.star(8192, sigma=1e-3, seed=0)
     sites = wl.live_sites(grid)
     opt = vmhip.Optimizer(1.0, 1.0, r, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 / r, iterations=iters,
@@ -256,6 +263,17 @@ def run_small(iters=200, K=16):
     gpu_ms = opt.optimize()
     mesh = opt.extract()
     wall = time.perf_counter() - t0
+    out = dict(sites=len(sites), targets=len(tg), iterations=iters, gpu_ms=gpu_ms,
+               gpu_site_iters_per_s=len(sites) * iters / (gpu_ms / 1e3), extract_gpu_ms=mesh["gpu_ms"],
+               extract_host_ms=mesh["host_ms"], wall_s=wall, watertight=mesh["watertight"],
+               final_mean_loss=opt.evaluate_loss())
+    out.update(cpu_baseline_small(sites, tg, r, gw, gh, cs, K, iters))
+    return out
+
+
+def cpu_baseline_small(sites, tg, r, gw, gh, cs, K, iters):
+    """The oracle restatement timed over the same plain fit (200 iterations and the clip), on the host's cores."""
+    import oracle as O
     lr = float(np.float32(np.float32(1e-3 / r) * np.float32(r)))
     sc = O.make_sched(lr, 1.0, 1.0)
     m = np.float32(2) * np.float32(r)
@@ -264,13 +282,7 @@ def run_small(iters=200, K=16):
     ref, _, _, _, _ = O.fit(sites, tg, K, 2.0 / math.sqrt(len(sites)), sc, iters, box=box)
     O.clip(ref, 1.0, 1.0, 2.0 / math.sqrt(len(sites)))
     cpu_s = time.perf_counter() - t0
-    gpu_pos = opt.generators()[grid[:, 0] >= 0]
-    return dict(sites=len(sites), targets=len(tg), iterations=iters, gpu_ms=gpu_ms,
-                gpu_site_iters_per_s=len(sites) * iters / (gpu_ms / 1e3), extract_gpu_ms=mesh["gpu_ms"],
-                extract_host_ms=mesh["host_ms"], wall_s=wall, cpu_s=cpu_s, cpu_threads=O.num_threads(),
-                cpu_site_iters_per_s=len(sites) * iters / cpu_s, positions_bitexact_vs_oracle=bool(
-                    np.array_equal(gpu_pos, ref)), watertight=mesh["watertight"],
-                final_mean_loss=opt.evaluate_loss())
+    return dict(cpu_s=cpu_s, cpu_threads=O.num_threads(), cpu_site_iters_per_s=len(sites) * iters / cpu_s)
 
 
 def cpu_baseline(insts, K, iters=10, budget_sites=500_000, lr=1e-3):
