@@ -61,3 +61,13 @@ def test_invalid_arguments_are_rejected_before_the_device():
 
 def test_device_count_does_not_fail():
     assert vmhip.device_count() >= 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it exits non-zero with a message and no JSON line."""
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr) and "{" not in r.stdout
