@@ -575,3 +575,31 @@ def test_large_instance_full_size_bitexact():
     ref, _, _, lh, ch = O.fit(sites, targets, K, cell, O.make_sched(_lr(r), W, H), 3, box=_box(r, gw, gh, cs))
     assert np.array_equal(opt.generators()[so], ref)
     assert ch.min() > 4_000_000
+
+
+def test_boundary_box_sides_and_empty_occupancy():
+    """Every voxel occupied: every cell is inside and the boundary is the domain box (keys with box-side labels
+    -1..-4), one loop; no voxel occupied: no boundary.  Both bit-exact against the oracle's assembly."""
+    r, K = 0.04, 16
+    grid = O.poisson(W, H, r, 30, 5)
+    sites = wl.live_sites(grid)
+    b = vmhip.Batch()
+    b.add(W, H, sites, wl.star(8 * len(sites), seed=5), 1e-3, loss_k=K)
+    b.commit()
+    b.step(2)
+    live = b.sites(0)
+    tx = ty = 8
+    vox = np.float32(1.0 / 64)
+    for name, word in (("full", ~np.uint64(0)), ("empty", np.uint64(0))):
+        print(f"case: boundary with {name} occupancy", flush=True)
+        masks = np.full(tx * ty, word, np.uint64)
+        m = b.extract(0, masks, (tx, ty), vox)
+        ref = O.clip(live, W, H, 0.1, masks=masks, tiles=(tx, ty), voxel=vox)
+        assert np.array_equal(m["inside"], ref["inside"])
+        bd = O.boundary(ref["verts"], ref["labels"], ref["count"], ref["inside"])
+        assert np.array_equal(m["vertices"], bd["vertices"]) and np.array_equal(m["edges"], bd["edges"])
+        assert m["watertight"] == bd["watertight"] and m["loops"] == bd["loops"]
+        if name == "full":
+            assert m["inside"].all() and m["watertight"] and m["loops"] == 1 and len(m["edges"]) > 0
+        else:
+            assert not m["inside"].any() and len(m["edges"]) == 0
