@@ -272,7 +272,9 @@ int vm_set_occupancy(vm_handle *h, const uint64_t *masks, int tilesX, int tilesY
         h->masks.assign(masks, masks + n);
         h->tilesX = tilesX, h->tilesY = tilesY, h->voxel = voxelSize;
         h->dMasks.reserve(n);
-        if (n) VM_HIP(hipMemcpy(h->dMasks.p, masks, n * 8, hipMemcpyHostToDevice));
+        // On the optimizer's own stream (ordered after its kernels), then waited for.
+        if (n) VM_HIP(hipMemcpyAsync(h->dMasks.p, h->masks.data(), n * 8, hipMemcpyHostToDevice, h->stream));
+        VM_HIP(hipStreamSynchronize(h->stream));
     });
 }
 
@@ -287,8 +289,9 @@ int vm_update_occupancy(vm_handle *h, const uint64_t *masks, int tx0, int ty0, i
         for (int y = ty0; y < ty1; ++y)
             for (int x = tx0; x < tx1; ++x) h->masks[size_t(y) * h->tilesX + x] = masks[size_t(y) * h->tilesX + x];
         size_t pitch = size_t(h->tilesX) * 8, off = size_t(ty0) * h->tilesX + tx0;
-        VM_HIP(hipMemcpy2D(h->dMasks.p + off, pitch, h->masks.data() + off, pitch, size_t(tx1 - tx0) * 8, ty1 - ty0,
-                           hipMemcpyHostToDevice));
+        VM_HIP(hipMemcpy2DAsync(h->dMasks.p + off, pitch, h->masks.data() + off, pitch, size_t(tx1 - tx0) * 8,
+                                ty1 - ty0, hipMemcpyHostToDevice, h->stream));
+        VM_HIP(hipStreamSynchronize(h->stream));
     });
 }
 
