@@ -1112,6 +1112,8 @@ constexpr int kExactFirstRings = 2;   // the first round gathers rings 0..2 (25 
 template <class R> struct ExactLds {
     Key<R> cand[kExactCand];
     Key<R> top[32];
+    V2<R>  pos[kExactCand]; // each candidate's position (from the sorted copy), beside its key
+    V2<R>  topPos[32];
     int    cb[64];   // a round of ring cells: first sorted entry of each
     int    co[65];   // and the prefix of their site counts
 };
@@ -1124,11 +1126,11 @@ template <int K, class R> __device__ __forceinline__ int keepNearest(ExactLds<R>
         int          rank = 0;
 #pragma unroll 8
         for (int i = 0; i < nc; ++i) rank += lds.cand[i].lt(kj) ? 1 : 0;
-        if (rank < K) lds.top[rank] = kj;
+        if (rank < K) lds.top[rank] = kj, lds.topPos[rank] = lds.pos[j];
     }
     waveSync();
     const int nk = min(nc, K);
-    if (lane < nk) lds.cand[lane] = lds.top[lane];
+    if (lane < nk) lds.cand[lane] = lds.top[lane], lds.pos[lane] = lds.topPos[lane];
     waveSync();
     return nk;
 }
@@ -1192,6 +1194,7 @@ __device__ __forceinline__ int gatherKnn(ExactLds<KR> &lds, const Inst &in, cons
                     const int    site = lds.cb[lo] + (g - lds.co[lo]);
                     const V2<KR> q{KR(spos[site].x), KR(spos[site].y)};
                     lds.cand[nc + s] = Key<KR>::make(dist2(q, p), (uint32_t(sidx[site]) << kLdsSlotBits) | uint32_t(kLdsNone));
+                    lds.pos[nc + s]  = q;
                 }
                 waveSync();
                 nc += slice;
@@ -1247,10 +1250,10 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
         R           dd = rInf<R>();
         if (nc > 0) {
             tr.i0 = keySite(lds.cand[0].low());
-            tr.c0 = a.pos[so + tr.i0];
+            tr.c0 = lds.pos[0]; // the sorted copy of the positions: the same values as a.pos
             if (lane >= 1 && lane < nc) {
                 const int   j  = keySite(lds.cand[lane].low());
-                const V2<R> cj = a.pos[so + j];
+                const V2<R> cj = lds.pos[lane];
                 const V2<R> ev{cj.x - tr.c0.x, cj.y - tr.c0.y};
                 const R     l = sqrtR(dotR(ev, ev));
                 if (!(l < (sizeof(R) == 4 ? R(1e-12f) : R(1e-12)))) {
