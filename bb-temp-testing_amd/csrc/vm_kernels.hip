@@ -1109,28 +1109,37 @@ constexpr int kExactWaves      = 4;
 constexpr int kExactCand       = 256; // LDS keys per wavefront: the current K nearest + a batch of new sites
 constexpr int kExactFirstRings = 2;   // the first round gathers rings 0..2 (25 cells <= 64 lanes)
 
-template <class R> struct ExactLds {
+// P: each candidate's position (from the sorted copy) is kept beside its key, so
+// the term reads no position from HBM; the clip, whose occupancy is LDS-bound,
+// goes without.
+template <class R, bool P> struct ExactLds {
     Key<R> cand[kExactCand];
     Key<R> top[32];
-    V2<R>  pos[kExactCand]; // each candidate's position (from the sorted copy), beside its key
-    V2<R>  topPos[32];
+    V2<R>  pos[P ? kExactCand : 1];
+    V2<R>  topPos[P ? 32 : 1];
     int    cb[64];   // a round of ring cells: first sorted entry of each
     int    co[65];   // and the prefix of their site counts
 };
 
 // Sorts the K smallest of the distinct keys cand[0, nc) into cand[0, min(K, nc)).
-template <int K, class R> __device__ __forceinline__ int keepNearest(ExactLds<R> &lds, int nc, int lane)
+template <int K, class R, bool P> __device__ __forceinline__ int keepNearest(ExactLds<R, P> &lds, int nc, int lane)
 {
     for (int j = lane; j < nc; j += 64) {
         const Key<R> kj   = lds.cand[j];
         int          rank = 0;
 #pragma unroll 8
         for (int i = 0; i < nc; ++i) rank += lds.cand[i].lt(kj) ? 1 : 0;
-        if (rank < K) lds.top[rank] = kj, lds.topPos[rank] = lds.pos[j];
+        if (rank < K) {
+            lds.top[rank] = kj;
+            if constexpr (P) lds.topPos[rank] = lds.pos[j];
+        }
     }
     waveSync();
     const int nk = min(nc, K);
-    if (lane < nk) lds.cand[lane] = lds.top[lane], lds.pos[lane] = lds.topPos[lane];
+    if (lane < nk) {
+        lds.cand[lane] = lds.top[lane];
+        if constexpr (P) lds.pos[lane] = lds.topPos[lane];
+    }
     waveSync();
     return nk;
 }
@@ -1150,8 +1159,8 @@ __device__ __forceinline__ void ringCell(int r, int q, int cx, int cy, int &x, i
 // the 5 x 5 block (rings 0-2) in one round, then ring by ring in rounds of 64
 // cells; a ring ends the search when the K-th distance is below the distance to
 // the searched block's edge (less a rounding margin).
-template <int K, class R, class KR>
-__device__ __forceinline__ int gatherKnn(ExactLds<KR> &lds, const Inst &in, const V2<R> *spos, const int *sidx,
+template <int K, class R, class KR, bool P>
+__device__ __forceinline__ int gatherKnn(ExactLds<KR, P> &lds, const Inst &in, const V2<R> *spos, const int *sidx,
                                          const int *strt, V2<R> pr, int lane)
 {
     const int    gw = in.gw, gh = in.gh;
@@ -1194,7 +1203,7 @@ __device__ __forceinline__ int gatherKnn(ExactLds<KR> &lds, const Inst &in, cons
                     const int    site = lds.cb[lo] + (g - lds.co[lo]);
                     const V2<KR> q{KR(spos[site].x), KR(spos[site].y)};
                     lds.cand[nc + s] = Key<KR>::make(dist2(q, p), (uint32_t(sidx[site]) << kLdsSlotBits) | uint32_t(kLdsNone));
-                    lds.pos[nc + s]  = q;
+                    if constexpr (P) lds.pos[nc + s] = q;
                 }
                 waveSync();
                 nc += slice;
@@ -1218,9 +1227,9 @@ __device__ __forceinline__ int gatherKnn(ExactLds<KR> &lds, const Inst &in, cons
 template <int K, class R>
 __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
 {
-    __shared__ ExactLds<R> ldsAll[kExactWaves];
-    const int    wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    ExactLds<R> &lds  = ldsAll[wave];
+    __shared__ ExactLds<R, true> ldsAll[kExactWaves];
+    const int          wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    ExactLds<R, true> &lds  = ldsAll[wave];
     const int    lane = threadIdx.x & 63;
     const int    n    = *a.exactLen;
     const int    nw   = gridDim.x * kExactWaves;
@@ -1236,7 +1245,7 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
         const int      *sidx     = a.sidx + so;
         const int      *strt     = a.start + in.cellOff;
         const V2<R>     p        = a.tpos[it.tg];
-        const int       nc       = gatherKnn<K, R, R>(lds, in, spos, sidx, strt, p, lane);
+        const int       nc       = gatherKnn<K, R, R, true>(lds, in, spos, sidx, strt, p, lane);
         if (a.dbgKnn && lane < K)
             a.dbgKnn[int64_t(it.tg) * K + lane] = lane < nc ? keySite(lds.cand[lane].low()) : -1;
         if (listOnly) {
@@ -1446,7 +1455,7 @@ __device__ bool occupied(const uint64_t *masks, int tilesX, int tilesY, float in
 constexpr int kClipWaves = 4;
 
 struct ClipLds {
-    ExactLds<double> knn;
+    ExactLds<double, false> knn;
     double           px[kMaxCellVerts], py[kMaxCellVerts];
     int              pe[kMaxCellVerts];
 };
@@ -1505,7 +1514,7 @@ __global__ __launch_bounds__(64 * kClipWaves) void clipKernel(ClipArgs<R> a)
     const int   *strt = a.start + in.cellOff;
     for (int i = blockIdx.x * kClipWaves + wave; i < in.S; i += gridDim.x * kClipWaves) {
         const V2<R>  ci  = pos[i];
-        const int    nc  = gatherKnn<kCellK, R, double>(lds.knn, in, spos, sidx, strt, ci, lane);
+        const int    nc  = gatherKnn<kCellK, R, double, false>(lds.knn, in, spos, sidx, strt, ci, lane);
         const double cxd = double(ci.x), cyd = double(ci.y);
         const double x0 = -cxd, y0 = -cyd, x1 = double(in.W) - cxd, y1 = double(in.H) - cyd;
         // The box: lanes 0-3 hold (x0,y0), (x1,y0), (x1,y1), (x0,y1), edges -1..-4.
