@@ -381,10 +381,10 @@ int vm_batch_extract(vm_batch *b, int id, const uint64_t *masks, int tiles_x, in
         b->eng->pull();
         const vmrt::InstSpec &s = b->eng->specs[id];
         const int             n = int(s.pos.size() / 2);
-        VM_REQUIRE(int(c.count.size()) == n, "clip output does not match the instance");
+        VM_REQUIRE(c.S == n, "clip output does not match the instance");
         b->xGen.resize(size_t(n));
         for (int i = 0; i < n; ++i) b->xGen[size_t(i)] = vm_vec2{float(s.pos[2 * i]), float(s.pos[2 * i + 1])};
-        b->xInside.assign(c.inside.begin(), c.inside.end());
+        b->xInside.assign(c.inside, c.inside + n);
         b->xStart.assign(size_t(n) + 1, 0);
         for (int i = 0; i < n; ++i) b->xStart[size_t(i) + 1] = b->xStart[size_t(i)] + c.count[size_t(i)];
         const size_t nv = size_t(b->xStart[size_t(n)]);

@@ -666,6 +666,11 @@ public:
     DBuf<float>   clipVerts;
     DBuf<int>     clipLabels, clipCount, clipFlags;
     DBuf<uint8_t> clipInside;
+    // Pinned copies of the clip outputs (ClipOut views them) and of the
+    // boundary's: V, watertight, loops, overflow, E, then vertices and edges.
+    HBuf<float>   hClipVerts, hBndVerts;
+    HBuf<int>     hClipLabels, hClipCount, hClipFlags, hBndEdges, hBndRes;
+    HBuf<uint8_t> hClipInside;
     // Boundary assembly buffers, kept between calls like the clip's; their end
     // point capacity grows when a boundary does not fit.
     DBuf<int>         bndEcount, bndEoff, bndSlot, bndRep, bndVid, bndEdges, bndOut, bndIn, bndNxt, bndLab0, bndLab1,
@@ -679,13 +684,7 @@ public:
     // (site space: labels are instance site indices, which keeps slot order).
     // The outputs are copied at capacity, so the clip's synchronization serves
     // both in the usual case.
-    struct BndHost {
-        int                res[5] = {0, 0, 0, 0, 0}; // V, watertight, loops, overflow, E
-        std::vector<float> verts;
-        std::vector<int>   edges;
-    };
-
-    void boundaryEnqueue(int S, BndHost &h)
+    void boundaryEnqueue(int S)
     {
         const size_t cap = size_t(bndCap);
         size_t       tsz = 64;
@@ -702,23 +701,24 @@ public:
         b.vid = bndVid.p, b.vertices = bndVerts.p, b.edges = bndEdges.p, b.outDeg = bndOut.p, b.inDeg = bndIn.p;
         b.nxt = bndNxt.p, b.lab0 = bndLab0.p, b.lab1 = bndLab1.p, b.nxt1 = bndNxt1.p, b.result = bndResult.p;
         VM_HIP(vmk::launchBoundary(stream, b));
-        h.verts.resize(2 * cap), h.edges.resize(cap);
-        VM_HIP(hipMemcpyAsync(h.res, bndResult.p, sizeof(h.res), hipMemcpyDeviceToHost, stream));
-        VM_HIP(hipMemcpyAsync(h.verts.data(), bndVerts.p, h.verts.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
-        VM_HIP(hipMemcpyAsync(h.edges.data(), bndEdges.p, cap * sizeof(int), hipMemcpyDeviceToHost, stream));
+        hBndRes.reserve(5), hBndVerts.reserve(2 * cap), hBndEdges.reserve(cap);
+        VM_HIP(hipMemcpyAsync(hBndRes.p, bndResult.p, 5 * sizeof(int), hipMemcpyDeviceToHost, stream));
+        VM_HIP(hipMemcpyAsync(hBndVerts.p, bndVerts.p, 2 * cap * sizeof(float), hipMemcpyDeviceToHost, stream));
+        VM_HIP(hipMemcpyAsync(hBndEdges.p, bndEdges.p, cap * sizeof(int), hipMemcpyDeviceToHost, stream));
     }
 
     // After the copies completed: false (and a larger capacity) when the
     // boundary had more end points than the capacity.
-    bool boundaryFinish(BndHost &h, Boundary &out)
+    bool boundaryFinish(Boundary &out)
     {
-        if (h.res[3]) {
-            while (bndCap < 2 * h.res[4]) bndCap *= 2;
+        const int *res = hBndRes.p;
+        if (res[3]) {
+            while (bndCap < 2 * res[4]) bndCap *= 2;
             return false;
         }
-        out.vertices.assign(h.verts.begin(), h.verts.begin() + 2 * size_t(h.res[0]));
-        out.edges.assign(h.edges.begin(), h.edges.begin() + 2 * size_t(h.res[4]));
-        out.watertight = h.res[1] != 0, out.loops = h.res[2];
+        out.vertices.assign(hBndVerts.p, hBndVerts.p + 2 * size_t(res[0]));
+        out.edges.assign(hBndEdges.p, hBndEdges.p + 2 * size_t(res[4]));
+        out.watertight = res[1] != 0, out.loops = res[2];
         return true;
     }
 
@@ -745,29 +745,29 @@ public:
         VM_HIP(hipEventRecord(ev[6], stream));
         // Without occupancy no cell is inside and the boundary is empty.
         const bool haveBoundary = dMasks != nullptr && S > 0;
-        BndHost    bh;
-        if (haveBoundary) boundaryEnqueue(S, bh);
+        if (haveBoundary) boundaryEnqueue(S);
         bin();
-        out.verts.assign(size_t(S) * vmk::kMaxCellVerts * 2, 0.0f);
-        out.labels.assign(size_t(S) * vmk::kMaxCellVerts, 0);
-        out.count.assign(S, 0), out.inside.assign(S, 0);
-        int flags[2] = {0, 0};
+        const size_t nv = size_t(S) * vmk::kMaxCellVerts;
+        hClipVerts.reserve(2 * nv), hClipLabels.reserve(nv), hClipCount.reserve(S), hClipInside.reserve(S);
+        hClipFlags.reserve(2);
         if (S) {
-            VM_HIP(hipMemcpyAsync(out.verts.data(), dv.p, out.verts.size() * 4, hipMemcpyDeviceToHost, stream));
-            VM_HIP(hipMemcpyAsync(out.labels.data(), dl.p, out.labels.size() * 4, hipMemcpyDeviceToHost, stream));
-            VM_HIP(hipMemcpyAsync(out.count.data(), dc.p, S * 4, hipMemcpyDeviceToHost, stream));
-            VM_HIP(hipMemcpyAsync(out.inside.data(), di.p, S, hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(hClipVerts.p, dv.p, 2 * nv * sizeof(float), hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(hClipLabels.p, dl.p, nv * sizeof(int), hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(hClipCount.p, dc.p, size_t(S) * sizeof(int), hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(hClipInside.p, di.p, size_t(S), hipMemcpyDeviceToHost, stream));
         }
-        VM_HIP(hipMemcpyAsync(flags, df.p, 8, hipMemcpyDeviceToHost, stream));
+        VM_HIP(hipMemcpyAsync(hClipFlags.p, df.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
         VM_HIP(hipStreamSynchronize(stream));
         VM_HIP(hipEventElapsedTime(&out.gpuMs, ev[5], ev[6]));
-        out.unsafe = flags[0], out.overflow = flags[1];
+        out.S = S, out.verts = hClipVerts.p, out.labels = hClipLabels.p, out.count = hClipCount.p;
+        out.inside = hClipInside.p;
+        out.unsafe = hClipFlags.p[0], out.overflow = hClipFlags.p[1];
         out.bound  = Boundary{};
         out.bound.watertight = true;
-        if (haveBoundary && !boundaryFinish(bh, out.bound)) {
-            boundaryEnqueue(S, bh);
+        if (haveBoundary && !boundaryFinish(out.bound)) {
+            boundaryEnqueue(S);
             VM_HIP(hipStreamSynchronize(stream));
-            if (!boundaryFinish(bh, out.bound)) throw VmError(2, "boundary assembly: capacity did not grow");
+            if (!boundaryFinish(out.bound)) throw VmError(2, "boundary assembly: capacity did not grow");
         }
     }
 };

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <functional>
 #include <memory>
@@ -51,6 +52,32 @@ template <class T> struct DBuf {
     }
 };
 
+// Pinned host staging buffer (hipHostMalloc), so device-to-host copies run as
+// asynchronous DMA.  Pinning is slow, so a buffer that has to grow takes at
+// least half as much again as it held (contents lost).
+template <class T> struct HBuf {
+    T     *p = nullptr;
+    size_t n = 0;
+    HBuf() = default;
+    HBuf(const HBuf &) = delete;
+    HBuf &operator=(const HBuf &) = delete;
+    ~HBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void reserve(size_t cnt)
+    {
+        if (cnt <= n && p) return;
+        const size_t c = std::max<size_t>(cnt ? cnt : 1, n + n / 2);
+        release();
+        VM_HIP(hipHostMalloc(reinterpret_cast<void **>(&p), c * sizeof(T), hipHostMallocDefault));
+        n = c;
+    }
+};
+
 // Host description of one instance.  Positions and moments are kept in double
 // on the host so either device precision round-trips exactly.
 struct InstSpec {
@@ -91,14 +118,17 @@ struct Boundary {
     int                  loops      = 0;
 };
 
+// The clip's per-cell outputs are views of the engine's pinned staging
+// buffers, valid until its next clip.
 struct ClipOut {
-    std::vector<float>   verts;  // [S*24*2]
-    std::vector<int>     labels; // [S*24]
-    std::vector<int>     count;  // [S]
-    std::vector<uint8_t> inside; // [S]
-    int                  unsafe = 0, overflow = 0;
-    float                gpuMs = 0;
-    Boundary             bound;  // assembled on the device (site-index labels)
+    int            S      = 0;
+    const float   *verts  = nullptr; // [S*24*2]
+    const int     *labels = nullptr; // [S*24]
+    const int     *count  = nullptr; // [S]
+    const uint8_t *inside = nullptr; // [S]
+    int            unsafe = 0, overflow = 0;
+    float          gpuMs  = 0;
+    Boundary       bound; // assembled on the device (site-index labels)
 };
 
 class Engine {
