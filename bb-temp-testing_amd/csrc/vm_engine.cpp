@@ -208,7 +208,12 @@ template <class R> class EngineT final : public Engine {
 
 public:
     explicit EngineT(hipStream_t s) { stream = s; }
-    ~EngineT() override { destroyGraph(); }
+    ~EngineT() override
+    {
+        destroyGraph();
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e), e = nullptr;
+    }
 
     // ---- layout ---------------------------------------------------------------
     struct Layout {
@@ -238,6 +243,7 @@ public:
     DBuf<vmk::ExactItem> dExactQ;
     DBuf<int>            dExactLen; // one per group
     hipGraphExec_t       graph = nullptr, graphN = nullptr; // one iteration, kGraphIters iterations
+    bool                 stale1 = false, staleN = false;   // a commit changed their launches
     hipEvent_t           ev[8] = {};
     int                  gridScan = 1, gridSite = 1;
     std::vector<int>     gridLoss;
@@ -247,8 +253,6 @@ public:
         if (graph) (void)hipGraphExecDestroy(graph);
         if (graphN) (void)hipGraphExecDestroy(graphN);
         graph = graphN = nullptr;
-        for (auto &e : ev)
-            if (e) (void)hipEventDestroy(e), e = nullptr;
     }
 
     void ensureEvents()
@@ -326,7 +330,7 @@ public:
 
     void commit() override
     {
-        destroyGraph();
+        stale1 = staleN = true; // the graphs take the new launches before their next replay
         buildLayout();
         const int n = int(specs.size());
         dPos.reserve(L.sites), dM.reserve(L.sites), dV.reserve(L.sites), dGrad.reserve(2 * L.sites);
@@ -455,6 +459,9 @@ public:
     // 5 us on the queue, which dominates small fits.
     static constexpr int kGraphIters = 8;
 
+    // Captures `iters` iterations.  An existing exec of the same shape (the same
+    // launches, as after a local edit) takes the new parameters in place, which
+    // costs less than instantiating again; otherwise it is replaced.
     void buildGraph(hipGraphExec_t &out, int iters)
     {
         hipGraph_t g;
@@ -467,7 +474,16 @@ public:
             throw;
         }
         VM_HIP(hipStreamEndCapture(stream, &g));
-        hipError_t e = hipGraphInstantiate(&out, g, nullptr, nullptr, 0);
+        if (out) {
+            hipGraphNode_t           errNode = nullptr;
+            hipGraphExecUpdateResult res     = hipGraphExecUpdateError;
+            if (hipGraphExecUpdate(out, g, &errNode, &res) != hipSuccess || res != hipGraphExecUpdateSuccess) {
+                (void)hipGetLastError();
+                (void)hipGraphExecDestroy(out);
+                out = nullptr;
+            }
+        }
+        hipError_t e = out ? hipSuccess : hipGraphInstantiate(&out, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         VM_HIP(e);
     }
@@ -476,8 +492,8 @@ public:
     {
         if (dirty) commit();
         if (iterations <= 0) return 0.0f;
-        if (!graph) buildGraph(graph, 1);
-        if (iterations >= kGraphIters && !graphN) buildGraph(graphN, kGraphIters);
+        if (!graph || stale1) buildGraph(graph, 1), stale1 = false;
+        if (iterations >= kGraphIters && (!graphN || staleN)) buildGraph(graphN, kGraphIters), staleN = false;
         ensureEvents();
         VM_HIP(hipEventRecord(ev[5], stream));
         int k = 0;
