@@ -16,6 +16,7 @@
 #include "vm_engine.h"
 
 using vmrt::DBuf;
+using vmrt::HBuf;
 using vmrt::VmError;
 
 namespace {
@@ -53,6 +54,7 @@ struct vm_handle {
     int                            targetCapacity = 0;
     std::vector<uint64_t>          masks;
     DBuf<uint64_t>                 dMasks;
+    HBuf<uint64_t>                 hMaskRows; // pinned staging of occupancy rows
     int                            tilesX = 0, tilesY = 0;
     float                          voxel = 0;
     int                            iteration = 0, scheduleLength = 0;
@@ -288,9 +290,12 @@ int vm_update_occupancy(vm_handle *h, const uint64_t *masks, int tx0, int ty0, i
         VM_REQUIRE(masks, "null masks");
         for (int y = ty0; y < ty1; ++y)
             for (int x = tx0; x < tx1; ++x) h->masks[size_t(y) * h->tilesX + x] = masks[size_t(y) * h->tilesX + x];
-        size_t pitch = size_t(h->tilesX) * 8, off = size_t(ty0) * h->tilesX + tx0;
-        VM_HIP(hipMemcpy2DAsync(h->dMasks.p + off, pitch, h->masks.data() + off, pitch, size_t(tx1 - tx0) * 8,
-                                ty1 - ty0, hipMemcpyHostToDevice, h->stream));
+        // Whole rows [ty0, ty1) go up in one DMA from pinned memory (the host copy
+        // equals the device copy outside the rect).
+        const size_t off = size_t(ty0) * h->tilesX, n = size_t(ty1 - ty0) * h->tilesX;
+        h->hMaskRows.reserve(n);
+        std::memcpy(h->hMaskRows.p, h->masks.data() + off, n * sizeof(uint64_t));
+        VM_HIP(hipMemcpyAsync(h->dMasks.p + off, h->hMaskRows.p, n * sizeof(uint64_t), hipMemcpyHostToDevice, h->stream));
         VM_HIP(hipStreamSynchronize(h->stream));
     });
 }

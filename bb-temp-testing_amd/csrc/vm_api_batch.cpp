@@ -15,6 +15,7 @@
 #include "vm_engine.h"
 
 using vmrt::DBuf;
+using vmrt::HBuf;
 using vmrt::VmError;
 
 namespace {
@@ -77,6 +78,7 @@ struct vm_sampler {
     hipStream_t          stream = nullptr;
     hipEvent_t           e0 = nullptr, e1 = nullptr;
     DBuf<float>          dGrid;
+    HBuf<vm_vec2>        hRows; // pinned staging of resampled grid rows
     std::vector<vm_vec2> cells;
     PoissonRunner        runner;
     ~vm_sampler()
@@ -113,10 +115,16 @@ int vm_sampler_resample(vm_sampler *s, int cx0, int cy0, int cx1, int cy1, float
         else s->runner.run(s->stream, {p}, s->dGrid.p, s->trials);
         VM_HIP(hipEventRecord(s->e1, s->stream));
         s->roundBase += unsigned(s->trials);
-        size_t pitch = size_t(s->gw) * sizeof(vm_vec2), off = size_t(y0) * s->gw + x0;
-        VM_HIP(hipMemcpy2DAsync(s->cells.data() + off, pitch, reinterpret_cast<vm_vec2 *>(s->dGrid.p) + off, pitch,
-                                size_t(x1 - x0) * sizeof(vm_vec2), y1 - y0, hipMemcpyDeviceToHost, s->stream));
+        // Rows [y0, y1) come down in one DMA into pinned memory; the region's
+        // columns are then copied into the host grid.
+        const size_t gw = size_t(s->gw), n = size_t(y1 - y0) * gw;
+        s->hRows.reserve(n);
+        VM_HIP(hipMemcpyAsync(s->hRows.p, reinterpret_cast<vm_vec2 *>(s->dGrid.p) + size_t(y0) * gw, n * sizeof(vm_vec2),
+                              hipMemcpyDeviceToHost, s->stream));
         VM_HIP(hipStreamSynchronize(s->stream));
+        for (int y = y0; y < y1; ++y)
+            std::memcpy(s->cells.data() + size_t(y) * gw + x0, s->hRows.p + size_t(y - y0) * gw + x0,
+                        size_t(x1 - x0) * sizeof(vm_vec2));
         float ms = 0;
         VM_HIP(hipEventElapsedTime(&ms, s->e0, s->e1));
         if (gpuMs) *gpuMs = ms;
