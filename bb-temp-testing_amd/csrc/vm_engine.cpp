@@ -368,16 +368,38 @@ public:
         cnt[0] = n, cnt[1] = int(L.siteWork.size());
         for (size_t g = 0; g < L.groups.size(); ++g) cnt[2 + g] = L.groups[g].count;
 
+        // The host tables go up through one pinned staging buffer when they are
+        // small (each copy a DMA; a pageable copy is staged by the runtime on the
+        // host thread, which dominates a local edit's commit), else directly.
+        struct Up {
+            void       *dst;
+            const void *src;
+            size_t      bytes, off;
+        };
+        std::vector<Up> ups;
+        size_t          total = 0;
         auto up = [&](auto *dst, const auto &src) {
-            if (!src.empty())
-                VM_HIP(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(src[0]), hipMemcpyHostToDevice, stream));
+            if (src.empty()) return;
+            const size_t b = src.size() * sizeof(src[0]);
+            ups.push_back({dst, src.data(), b, total});
+            total += (b + 255) & ~size_t(255);
         };
         up(dPos.p, hp), up(dM.p, hm), up(dV.p, hv), up(dTpos.p, ht), up(dTslot.p, hts), up(dFrozen.p, hf);
         up(dInst.p, L.inst), up(dState.p, hs), up(dTiles.p, L.tiles), up(dSiteWork.p, L.siteWork);
         const std::vector<vmk::ScanCtl> ctl{{0, 0u, int(L.tiles.size()), L.dynTiles}};
         up(dScanCtl.p, ctl);
-        VM_HIP(hipMemsetAsync(dFlags.p, 0, dFlags.n * sizeof(unsigned long long), stream));
         up(dTargetWork.p, L.targetWork), up(dCnt.p, cnt);
+        const bool staged = total <= kPinnedUploadMax;
+        if (staged) hUpload.reserve(total);
+        for (const Up &u : ups) {
+            const void *src = u.src;
+            if (staged) {
+                std::memcpy(hUpload.p + u.off, u.src, u.bytes);
+                src = hUpload.p + u.off;
+            }
+            VM_HIP(hipMemcpyAsync(u.dst, src, u.bytes, hipMemcpyHostToDevice, stream));
+        }
+        VM_HIP(hipMemsetAsync(dFlags.p, 0, dFlags.n * sizeof(unsigned long long), stream));
         VM_HIP(hipMemsetAsync(dGrad.p, 0, dGrad.n * sizeof(vmk::GradT<R>), stream));
         VM_HIP(hipMemsetAsync(dPartial.p, 0, dPartial.n * sizeof(vmk::LossPartial), stream));
         VM_HIP(vmk::launchFillInt(stream, dCounts.p, int64_t(dCounts.n), 0));
@@ -678,6 +700,10 @@ public:
         VM_HIP(hipStreamSynchronize(stream));
         for (size_t i = 0; i < hd.size(); ++i) d2[i] = double(hd[i]);
     }
+
+    // Pinned staging of commit's uploads (layouts up to kPinnedUploadMax bytes).
+    static constexpr size_t kPinnedUploadMax = size_t(64) << 20;
+    HBuf<unsigned char>     hUpload;
 
     DBuf<float>   clipVerts;
     DBuf<int>     clipLabels, clipCount, clipFlags;
