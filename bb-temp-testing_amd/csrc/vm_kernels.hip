@@ -1110,8 +1110,8 @@ constexpr int kExactCand       = 256; // LDS keys per wavefront: the current K n
 constexpr int kExactFirstRings = 2;   // the first round gathers rings 0..2 (25 cells <= 64 lanes)
 
 // P: each candidate's position (from the sorted copy) is kept beside its key, so
-// the term reads no position from HBM; the clip, whose occupancy is LDS-bound,
-// goes without.
+// the term reads no position from HBM.  The clip and the FP64 exact selection,
+// whose occupancy is LDS-bound, go without.
 template <class R, bool P> struct ExactLds {
     Key<R> cand[kExactCand];
     Key<R> top[32];
@@ -1227,9 +1227,13 @@ __device__ __forceinline__ int gatherKnn(ExactLds<KR, P> &lds, const Inst &in, c
 template <int K, class R>
 __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
 {
-    __shared__ ExactLds<R, true> ldsAll[kExactWaves];
+    // FP32: positions beside the keys (a short queue, latency-bound).  FP64 runs
+    // send every target here, and the smaller LDS footprint (more workgroups per
+    // CU) is worth more than the dependent loads it saves.
+    constexpr bool P = sizeof(R) == 4;
+    __shared__ ExactLds<R, P> ldsAll[kExactWaves];
     const int          wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    ExactLds<R, true> &lds  = ldsAll[wave];
+    ExactLds<R, P>    &lds  = ldsAll[wave];
     const int    lane = threadIdx.x & 63;
     const int    n    = *a.exactLen;
     const int    nw   = gridDim.x * kExactWaves;
@@ -1245,7 +1249,7 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
         const int      *sidx     = a.sidx + so;
         const int      *strt     = a.start + in.cellOff;
         const V2<R>     p        = a.tpos[it.tg];
-        const int       nc       = gatherKnn<K, R, R, true>(lds, in, spos, sidx, strt, p, lane);
+        const int       nc       = gatherKnn<K, R, R, P>(lds, in, spos, sidx, strt, p, lane);
         if (a.dbgKnn && lane < K)
             a.dbgKnn[int64_t(it.tg) * K + lane] = lane < nc ? keySite(lds.cand[lane].low()) : -1;
         if (listOnly) {
@@ -1259,10 +1263,14 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
         R           dd = rInf<R>();
         if (nc > 0) {
             tr.i0 = keySite(lds.cand[0].low());
-            tr.c0 = lds.pos[0]; // the sorted copy of the positions: the same values as a.pos
+            // The sorted copy of the positions holds the same values as a.pos.
+            if constexpr (P) tr.c0 = lds.pos[0];
+            else tr.c0 = a.pos[so + tr.i0];
             if (lane >= 1 && lane < nc) {
                 const int   j  = keySite(lds.cand[lane].low());
-                const V2<R> cj = lds.pos[lane];
+                V2<R>       cj;
+                if constexpr (P) cj = lds.pos[lane];
+                else cj = a.pos[so + j];
                 const V2<R> ev{cj.x - tr.c0.x, cj.y - tr.c0.y};
                 const R     l = sqrtR(dotR(ev, ev));
                 if (!(l < (sizeof(R) == 4 ? R(1e-12f) : R(1e-12)))) {
