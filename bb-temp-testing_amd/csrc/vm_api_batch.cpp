@@ -181,6 +181,17 @@ int vm_sampler_verify(vm_sampler *s, int cx0, int cy0, int cx1, int cy1, int *ok
     });
 }
 
+int vm_sampler_load_cells(vm_sampler *s, const vm_vec2 *cells, int n)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(s && cells && size_t(n) == s->cells.size(), "n must be grid_w * grid_h");
+        std::copy(cells, cells + n, s->cells.begin());
+        VM_HIP(hipMemcpyAsync(s->dGrid.p, s->cells.data(), size_t(n) * sizeof(vm_vec2), hipMemcpyHostToDevice,
+                              s->stream));
+        VM_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
 int vm_sampler_dims(vm_sampler *s, int *gw, int *gh, float *cs)
 {
     return vmrt::guardedCall([&] {

@@ -77,7 +77,7 @@ EXPORTS = [
     "vm_set_generators", "vm_update_generators", "vm_begin_schedule", "vm_step", "vm_get_state",
     "vm_evaluate_loss", "vm_extract", "vm_get_generators", "vm_knn", "vm_loss_pass",
     "vm_sampler_create", "vm_sampler_destroy", "vm_sampler_resample", "vm_sampler_verify", "vm_sampler_dims",
-    "vm_sampler_cells", "vm_poisson_grid_dims", "vm_poisson_batch",
+    "vm_sampler_cells", "vm_sampler_load_cells", "vm_poisson_grid_dims", "vm_poisson_batch",
     "vm_batch_create", "vm_batch_destroy", "vm_batch_add", "vm_batch_commit", "vm_batch_step",
     "vm_batch_get_sites", "vm_batch_status", "vm_batch_extract", "vm_batch_choice", "vm_batch_profile", "vm_batch_loss_stats",
     "vm_serve",
@@ -132,6 +132,7 @@ def lib():
         "vm_sampler_verify": [vp, i32, i32, i32, i32, P(i32)],
         "vm_sampler_dims": [vp, P(i32), P(i32), P(f32)],
         "vm_sampler_cells": [vp, vp, i32],
+        "vm_sampler_load_cells": [vp, vp, i32],
         "vm_poisson_grid_dims": [f32, f32, f32, P(i32), P(i32), P(f32)],
         "vm_poisson_batch": [i32, vp, vp, vp, vp, i32, vp, vp, P(f32)],
         "vm_batch_create": [i32, P(vp)],
@@ -430,6 +431,11 @@ class Sampler:
         _check(lib().vm_sampler_cells(self._s, _ptr(out), n))
         return out
 
+    def load_cells(self, cells):
+        """Replaces the grid (host mirror and device copy); later resamples continue from it."""
+        c = _pts(cells)
+        _check(lib().vm_sampler_load_cells(self._s, _ptr(c), len(c)))
+
 
 def poisson_grid_dims(width, height, radius):
     gw, gh, cs = C.c_int(), C.c_int(), C.c_float()
@@ -438,7 +444,7 @@ def poisson_grid_dims(width, height, radius):
 
 
 def poisson_batch(widths, heights, radii, seeds, trials=30):
-    """Samples many Poisson-disk grids in one set of launches; returns (list of (gh*gw,2) grids, gpu_ms)."""
+    """Samples many Poisson-disk grids in one set of launches; returns (list of (gh*gw,2) grids, dims, gpu_ms)."""
     n = len(radii)
     w = np.ascontiguousarray(widths, np.float32)
     h = np.ascontiguousarray(heights, np.float32)

@@ -106,6 +106,20 @@ def log_uniform_sizes(count, lo, hi, seed):
     return np.round(np.exp(rng.uniform(math.log(lo), math.log(hi), count))).astype(np.int64)
 
 
+def batch_plan(n=512, seed=2, lo=256, hi=65536):
+    """The mixed batch's instances: Poisson radius (for a site count log-uniform in [lo, hi]) and sampler seed
+    of each (seed 2 * 2^20 + i for instance i)."""
+    sizes = log_uniform_sizes(n, lo, hi, seed)
+    radii = np.array([poisson_radius_for(int(s)) for s in sizes], np.float32)
+    seeds = (2 * (1 << 20) + np.arange(n)).astype(np.uint32)
+    return radii, seeds
+
+
+def batch_targets(sites, seed, per_site=8, sigma=5e-4):
+    """Targets of one batch instance: `per_site` per site on its random polygon (seeded by the sampler seed)."""
+    return polygon(per_site * len(sites), sigma=sigma, seed=int(seed))
+
+
 def poisson_arrivals(rate, duration, seed=4):
     rng = np.random.default_rng([0xA77, seed])
     t, out = 0.0, []

@@ -145,6 +145,10 @@ VM_API int vm_sampler_destroy(vm_sampler *s);
 VM_API int vm_sampler_resample(vm_sampler *s, int cx0, int cy0, int cx1, int cy1, float *gpu_ms);
 /* Verify(cx0, cy0, cx1, cy1)  poisson_disk.h:38, .cu:179-199; *ok = 1 if valid */
 VM_API int vm_sampler_verify(vm_sampler *s, int cx0, int cy0, int cx1, int cy1, int *ok);
+/* Replaces the sampler's grid (host mirror and device copy) with `cells`, grid_w * grid_h entries (x < 0:
+ * empty); later Resample calls continue from it.  No counterpart in the reference (its Cells() is read-only):
+ * restores a saved grid, and lets the tests plant conflicts that Verify must report. */
+VM_API int vm_sampler_load_cells(vm_sampler *s, const vm_vec2 *cells, int n);
 /* GridW(), GridH(), CellSize()  poisson_disk.h:44-50 */
 VM_API int vm_sampler_dims(vm_sampler *s, int *grid_w, int *grid_h, float *cell_size);
 /* Cells(): host mirror, grid_w * grid_h entries  poisson_disk.h:42 */

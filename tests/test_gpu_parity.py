@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from cases import VERIFY_R, clustered_sites, plant_conflict, verify_cases
 
 vmhip = pytest.importorskip("vmhip")
 from vmhip import workloads as wl  # noqa: E402
@@ -304,6 +305,60 @@ def test_poisson_sampler_bitexact(r, seed):
     ref = O.poisson(W, H, r, 30, seed, region=(3, 4, 20, 17), round_base=30, grid=ref)
     assert np.array_equal(s.cells(), ref)
     assert s.verify()
+
+
+def test_sampler_verify_reports_conflicts():
+    """The product's Verify (vm_sampler_verify) on planted grids, against the oracle and the hand-derived
+    expectations: conflicts closer than r, the strict r^2 boundary, the region window and its edge, and a
+    conflict planted into a sampled grid (invalid over it, valid over a window that misses it)."""
+    s = vmhip.Sampler(W, H, VERIFY_R, 30, 1)
+    assert (s.grid_w, s.grid_h) == (6, 6)
+    for name, grid, region, expected in verify_cases():
+        s.load_cells(grid)
+        assert np.array_equal(s.cells(), grid)
+        got = s.verify(*region)
+        assert got == expected == O.poisson_verify(W, H, VERIFY_R, grid, region=region), name
+    r = 0.02
+    s2 = vmhip.Sampler(W, H, r, 30, 9)
+    g = s2.cells()
+    assert s2.verify() and np.array_equal(g, O.poisson(W, H, r, 30, 9))
+    bad, j = plant_conflict(g, s2.grid_w, r)
+    s2.load_cells(bad)
+    cx, cy = j % s2.grid_w, j // s2.grid_w
+    assert not s2.verify() and not s2.verify(cx, cy, cx + 1, cy + 1)
+    far = (0, 0, 1, 1) if cx > 8 or cy > 8 else (s2.grid_w - 1, s2.grid_h - 1, s2.grid_w, s2.grid_h)
+    assert s2.verify(*far)
+    # A resample after the load continues from the loaded grid: the region is redrawn (valid again there),
+    # the rest of the grid is kept.
+    reg = (max(cx - 1, 0), max(cy - 1, 0), min(cx + 2, s2.grid_w), min(cy + 2, s2.grid_h))
+    s2.resample(*reg)
+    ref = O.poisson(W, H, r, 30, 9, region=reg, round_base=30, grid=bad.copy())
+    assert np.array_equal(s2.cells(), ref)
+    assert s2.verify()
+
+
+def test_clip_fallback_runs_on_gpu_bitexact():
+    """Coarse Poisson sites beside tight clusters: the cells next to a cluster cannot be certified from their 32
+    nearest sites, so the clip kernel's brute-force fallback runs (unsafe cells > 0, the same count as the
+    oracle's); every cell is bit-exact against the oracle's clip, in both precision modes."""
+    base = O.poisson(W, H, 0.1, 30, 3)
+    sites = clustered_sites(base[base[:, 0] >= 0])
+    ref = O.clip(sites, W, H, 0.2)
+    assert ref["unsafe"] > 0
+    for precision in (0, 1):
+        print(f"case: clip fallback, precision {precision}", flush=True)
+        b = vmhip.Batch(precision)
+        b.add(W, H, sites, wl.star(8 * len(sites), seed=3), 1e-3, loss_k=16)
+        b.commit()
+        m = b.extract(0)
+        assert m["unsafe_cells"] == ref["unsafe"] and m["overflow_cells"] == 0
+        cnt = m["cell_start"][1:] - m["cell_start"][:-1]
+        assert np.array_equal(cnt, ref["count"])
+        for k in range(len(sites)):
+            s0, n = m["cell_start"][k], ref["count"][k]
+            assert np.array_equal(m["cell_verts"][s0:s0 + n], ref["verts"][k, :n])
+            assert np.array_equal(m["cell_labels"][s0:s0 + n], ref["labels"][k, :n])
+        b.close()
 
 
 def test_poisson_batch_matches_single():

@@ -8,7 +8,9 @@ Every function is checked against something other than its own formula:
   * cell clipping                -- scipy.spatial.Voronoi (Qhull) neighbours, area sum,
                                     neighbour symmetry, vertex equidistance
   * boundary assembly            -- two-site hand case, watertight star
-  * Poisson dart thrower         -- Verify, determinism, region resample, yield
+  * Poisson dart thrower         -- determinism, region resample, yield; Verify on hand
+                                    grids (conflicts closer than r, the r^2 boundary, the
+                                    region window and its edge) and planted conflicts
 """
 import json
 import math
@@ -18,6 +20,7 @@ import numpy as np
 import pytest
 
 import oracle as O
+from cases import VERIFY_R, clustered_sites, plant_conflict, verify_cases
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -299,6 +302,47 @@ def test_poisson_valid_deterministic_and_region():
     live = g1[:, 0] >= 0
     cells = np.nonzero(live)[0]
     assert np.all((g1[live, 0] / cs).astype(int) == cells % gw)
+
+
+@pytest.mark.parametrize("name,grid,region,expected", verify_cases(), ids=[c[0] for c in verify_cases()])
+def test_poisson_verify_hand_cases(name, grid, region, expected):
+    assert O.poisson_verify(1, 1, VERIFY_R, grid, region=region) == expected
+
+
+def test_poisson_verify_planted_conflict():
+    """A valid sampled grid with one sample moved within r of its neighbour: invalid over the whole grid and
+    over a region containing it; still valid over a region whose window (expanded by 2 cells) misses it."""
+    r = 0.02
+    g = O.poisson(1, 1, r, 30, 9)
+    gw, gh, _ = O.poisson_dims(1, 1, r)
+    assert O.poisson_verify(1, 1, r, g)
+    bad, j = plant_conflict(g, gw, r)
+    cx, cy = j % gw, j // gw
+    assert not O.poisson_verify(1, 1, r, bad)
+    assert not O.poisson_verify(1, 1, r, bad, region=(cx, cy, cx + 1, cy + 1))
+    far = (0, 0, 1, 1) if cx > 8 or cy > 8 else (gw - 1, gh - 1, gw, gh)
+    assert O.poisson_verify(1, 1, r, bad, region=far)
+
+
+def test_clip_fallback_on_clustered_sites_matches_certified_clip():
+    """Coarse Poisson sites beside tight clusters of 40 sites: the cells next to a cluster cannot be certified
+    from their 32 nearest sites and take the brute-force fallback (unsafe > 0).  Their polygons have the same
+    neighbours as a clip whose lists are long enough to certify every cell (Kc = 128, no fallback), vertices
+    agree to 1e-6, and the areas sum to the domain's."""
+    base = O.poisson(1, 1, 0.1, 30, 3)
+    sites = clustered_sites(base[base[:, 0] >= 0])
+    a = O.clip(sites, 1.0, 1.0, 0.2)
+    b = O.clip(sites, 1.0, 1.0, 0.2, Kc=128)
+    assert a["unsafe"] > 0 and b["unsafe"] == 0 and a["overflow"] == 0
+    assert np.array_equal(a["count"], b["count"])
+    area = 0.0
+    for i in range(len(sites)):
+        n = a["count"][i]
+        assert set(a["labels"][i, :n]) == set(b["labels"][i, :n])
+        assert np.abs(np.sort(a["verts"][i, :n], 0) - np.sort(b["verts"][i, :n], 0)).max() < 1e-6
+        v = a["verts"][i, :n].astype(np.float64)
+        area += 0.5 * np.sum(v[:, 0] * np.roll(v[:, 1], -1) - np.roll(v[:, 0], -1) * v[:, 1])
+    assert area == pytest.approx(1.0, abs=1e-6)
 
 
 def test_poisson_yield():
