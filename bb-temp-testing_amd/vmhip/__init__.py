@@ -78,6 +78,7 @@ EXPORTS = [
     "vm_evaluate_loss", "vm_extract", "vm_get_generators", "vm_knn", "vm_loss_pass",
     "vm_sampler_create", "vm_sampler_destroy", "vm_sampler_resample", "vm_sampler_verify", "vm_sampler_dims",
     "vm_sampler_cells", "vm_sampler_load_cells", "vm_poisson_grid_dims", "vm_poisson_batch",
+    "vm_poisson_batch_noalloc",
     "vm_batch_create", "vm_batch_destroy", "vm_batch_add", "vm_batch_commit", "vm_batch_step",
     "vm_batch_get_sites", "vm_batch_status", "vm_batch_extract", "vm_batch_choice", "vm_batch_profile", "vm_batch_loss_stats",
     "vm_serve",
@@ -135,6 +136,7 @@ def lib():
         "vm_sampler_load_cells": [vp, vp, i32],
         "vm_poisson_grid_dims": [f32, f32, f32, P(i32), P(i32), P(f32)],
         "vm_poisson_batch": [i32, vp, vp, vp, vp, i32, vp, vp, P(f32)],
+        "vm_poisson_batch_noalloc": [i32, vp, vp, vp, vp, i32, vp, vp, P(f32)],
         "vm_batch_create": [i32, P(vp)],
         "vm_batch_destroy": [vp],
         "vm_batch_add": [vp, P(FitDesc), P(i32)],
@@ -443,8 +445,9 @@ def poisson_grid_dims(width, height, radius):
     return gw.value, gh.value, cs.value
 
 
-def poisson_batch(widths, heights, radii, seeds, trials=30):
-    """Samples many Poisson-disk grids in one set of launches; returns (list of (gh*gw,2) grids, dims, gpu_ms)."""
+def poisson_batch(widths, heights, radii, seeds, trials=30, noalloc=False):
+    """Samples many Poisson-disk grids in one set of launches; returns (list of (gh*gw,2) grids, dims, gpu_ms).
+    noalloc: through vm_poisson_batch_noalloc (fixed device array, no allocation; bounded sizes)."""
     n = len(radii)
     w = np.ascontiguousarray(widths, np.float32)
     h = np.ascontiguousarray(heights, np.float32)
@@ -458,8 +461,8 @@ def poisson_batch(widths, heights, radii, seeds, trials=30):
         total += gw * gh
     cells = np.zeros((max(total, 1), 2), np.float32)
     ms = C.c_float()
-    _check(lib().vm_poisson_batch(n, _ptr(w), _ptr(h), _ptr(r), _ptr(s), trials, _ptr(offs), _ptr(cells),
-                                  C.byref(ms)))
+    fn = lib().vm_poisson_batch_noalloc if noalloc else lib().vm_poisson_batch
+    _check(fn(n, _ptr(w), _ptr(h), _ptr(r), _ptr(s), trials, _ptr(offs), _ptr(cells), C.byref(ms)))
     grids = [cells[offs[i]:offs[i] + dims[i][0] * dims[i][1]] for i in range(n)]
     return grids, dims, ms.value
 

@@ -161,6 +161,18 @@ VM_API int vm_poisson_grid_dims(float width, float height, float radius, int *gr
 VM_API int vm_poisson_batch(int n, const float *w, const float *h, const float *r, const unsigned *seed, int trials,
                             const int64_t *offsets, vm_vec2 *cells, float *gpu_ms);
 
+/* vm_poisson_batch without any allocation: the instance table, the work tables and the grids live in a fixed
+ * 16 MiB device array owned by the library (one per device, set up on the first call made on that device, kept
+ * until the process ends); the call makes no hipMalloc / hipFree and no host heap allocation.  Same arguments,
+ * results and error behaviour as vm_poisson_batch, with two limits: at most 16,384 instances, and the grids plus
+ * the tables within the array (about 2 M grid cells in all; offsets[i] + grid_w * grid_h bounds the grids).
+ * Larger requests fail with VM_EINVAL before any GPU work.  Calls in one process are serialized; the array keeps
+ * the last call's contents (every grid is cleared before sampling, so results never depend on it).  No
+ * counterpart in the reference: for callers that cannot allocate (tight serving loops, harnesses that forbid
+ * allocation). */
+VM_API int vm_poisson_batch_noalloc(int n, const float *w, const float *h, const float *r, const unsigned *seed,
+                                    int trials, const int64_t *offsets, vm_vec2 *cells, float *gpu_ms);
+
 /* ---- voxel canvas: target slots and occupancy from shapes and disc edits -------
  * voromesh::Canvas (include/voromesh/canvas.h:27-89, src/voromesh/canvas.cpp of
  * the reference).  Host-only: no GPU call.  Voxel (i, j) covers [i, i+1) x
