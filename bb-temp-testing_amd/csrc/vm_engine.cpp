@@ -181,7 +181,12 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         }
         rho = std::sqrt(rho); // double: within an ulp, and rounded up twice in float below
         float rhof = std::nextafter(std::nextafter(float(rho), 1e30f), 1e30f);
-        targetWork.push_back({id, t, e - t, c % in.gw, qx, qy, rhof, c / in.gw});
+        vmk::TargetWork tw{};
+        tw.inst = id, tw.count = e - t, tw.cx = c % in.gw, tw.cy = c / in.gw;
+        tw.qx = qx, tw.qy = qy, tw.rho = rhof, tw.gw = in.gw, tw.gh = in.gh;
+        tw.flags   = (in.stageR & vmk::kWorkRingsMask) | (in.fraction < 1.0f ? vmk::kWorkSubsample : 0);
+        tw.tBegin  = targetOff + t, tw.cellOff = cellOff, tw.siteOff = siteOff;
+        targetWork.push_back(tw);
         // Lane l takes the chunk's targets l, l + 64, l + 128.  Interleave the
         // Morton order with stride H = ceil(n / 64), so the 64 targets of one
         // pass are spread over the chunk and fewer lanes of a pass add their

@@ -824,27 +824,57 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
     const uint32_t mask = sizeof(R) == 4 ? packMask() : 0u;
     for (int w = blockIdx.x * kLossWaves + wave; w < nWork; w += gridDim.x * kLossWaves) {
         const TargetWork wk = loadUniform(a.work + w);
-        const Inst       in = loadUniform(a.insts + wk.inst);
-        InstState       *st = a.state + wk.inst;
-        if (a.mode == kModeIterate && loadUniform(&st->done)) continue;
-        const int64_t so = in.siteOff;
+        const int64_t    so = wk.siteOff;
 
-        // ---- this lane's targets: lane + 64 h, h < kTargetsPerLane -------
-        V2<R> p[kTargetsPerLane];
-        bool  take[kTargetsPerLane];
-        bool  anyTake = false;
+        // Every load that needs only the work item goes out at once: the
+        // chunk's targets (lane + 64 h, h < kTargetsPerLane; their slot numbers
+        // when subsampled) and the cell starts of the staged block.  The
+        // instance record and state arrive meanwhile.
+        V2<R>    p[kTargetsPerLane];
+        uint32_t tsl[kTargetsPerLane];
+        bool     take[kTargetsPerLane];
+        const bool sub = (wk.flags & kWorkSubsample) != 0;
 #pragma unroll
         for (int h = 0; h < kTargetsPerLane; ++h) {
-            const int t = wk.begin + lane + 64 * h;
-            p[h] = V2<R>{R(-1), R(-1)}, take[h] = false;
-            if (lane + 64 * h < wk.count) {
-                p[h]    = a.tpos[in.targetOff + t];
-                take[h] = p[h].x >= R(0);
-                if (take[h] && in.useBox && a.mode == kModeIterate)
-                    take[h] = p[h].x >= R(in.bx0) && p[h].x <= R(in.bx1) && p[h].y >= R(in.by0) && p[h].y <= R(in.by1);
-                if (take[h] && a.mode == kModeIterate && in.fraction < 1.0f)
-                    take[h] = rand01_loss(uint32_t(a.tslot[in.targetOff + t]), uint32_t(loadUniform(&st->iter)), in.seed) < in.fraction;
+            const int64_t ti = wk.tBegin + (lane + 64 * h < wk.count ? lane + 64 * h : 0);
+            p[h]   = a.tpos[ti];
+            tsl[h] = sub ? uint32_t(a.tslot[ti]) : 0u;
+        }
+        // The block of rings 0..rs around the chunk's cell, [sx0,sx1] x [sy0,sy1]:
+        // lane r*snx + i reads the range of cell (sx0 + i, sy0 + r); lane / snx is
+        // a multiply and shift (exact for lane < 7 snx, snx <= 7).
+        const int *strt = a.start + wk.cellOff;
+        int        rs   = wk.flags & kWorkRingsMask;
+        int        sx0 = 0, sx1 = 0, sy0 = 0, sy1 = 0, snx = 1, sny = 1, cb = 0, ce = 0;
+        auto       blockStarts = [&]() {
+            sx0 = max(wk.cx - rs, 0), sx1 = min(wk.cx + rs, wk.gw - 1);
+            sy0 = max(wk.cy - rs, 0), sy1 = min(wk.cy + rs, wk.gh - 1);
+            snx = sx1 - sx0 + 1, sny = sy1 - sy0 + 1;
+            const int rowMul = snx == 1 ? 256 : snx == 2 ? 128 : snx == 3 ? 86 : snx == 4 ? 64 : snx == 5 ? 52
+                               : snx == 6 ? 43 : 37; // ceil(256 / snx)
+            cb = 0, ce = 0;
+            if (lane < snx * sny) {
+                const int r = (lane * rowMul) >> 8;
+                const int c = (sy0 + r) * wk.gw + sx0 + (lane - r * snx);
+                cb          = strt[c];
+                ce          = strt[c + 1];
             }
+        };
+        if constexpr (sizeof(R) == 4) blockStarts();
+
+        const Inst in = loadUniform(a.insts + wk.inst);
+        InstState *st = a.state + wk.inst;
+        if (a.mode == kModeIterate && loadUniform(&st->done)) continue;
+        bool anyTake = false;
+#pragma unroll
+        for (int h = 0; h < kTargetsPerLane; ++h) {
+            const bool valid = lane + 64 * h < wk.count;
+            if (!valid) p[h] = V2<R>{R(-1), R(-1)};
+            take[h] = valid && p[h].x >= R(0);
+            if (take[h] && in.useBox && a.mode == kModeIterate)
+                take[h] = p[h].x >= R(in.bx0) && p[h].x <= R(in.bx1) && p[h].y >= R(in.by0) && p[h].y <= R(in.by1);
+            if (take[h] && a.mode == kModeIterate && in.fraction < 1.0f)
+                take[h] = rand01_loss(tsl[h], uint32_t(loadUniform(&st->iter)), in.seed) < in.fraction;
             anyTake = anyTake || take[h];
         }
         if (__ballot(anyTake) == 0) {
@@ -858,9 +888,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
         int          total = 0; // sites of the staged block
         ScreenBounds sb{};
         if constexpr (sizeof(R) == 4) {
-            const int  *strt = a.start + in.cellOff;
-            const int   gw = in.gw, gh = in.gh;
-            const int   cx = wk.cx, cy = wk.cy;
+            const int   gw = wk.gw, gh = wk.gh;
             const R     cell = R(in.cell);
             const R     ext  = R(in.ext);
             const V2<R> q{R(wk.qx), R(wk.qy)};
@@ -868,60 +896,52 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             const R     rho = R(wk.rho) + mg;
             const int   e0 = lane, e1 = lane + 64;
             R           d0 = rInf<R>(), d1 = rInf<R>(), lo = R(0), hi = R(0), thr = R(0), thr2 = R(0);
-            // The block of rings 0..rs around the chunk's cell; a block that
-            // misses the candidate disc (next to the domain's edge) is retried
-            // once with kStageRetry rings.
-            for (int rs = in.stageR;; rs = kStageRetry) {
-                const int sx0 = max(cx - rs, 0), sx1 = min(cx + rs, gw - 1);
-                const int sy0 = max(cy - rs, 0), sy1 = min(cy + rs, gh - 1);
-                const int snx = sx1 - sx0 + 1, sny = sy1 - sy0 + 1;
-
-                // ---- stage the cell block [sx0,sx1] x [sy0,sy1] into LDS ----
-                // Lane r*snx + i reads the range of cell (sx0 + i, sy0 + r);
-                // lane / snx is a multiply and shift (exact for lane < 7 snx,
-                // snx <= 7).  Rows of the block are contiguous ranges of the
-                // sorted arrays: row r covers entries [rb[r], rb[r] + len) and
-                // LDS slots [roff[r], roff[r+1]).
-                const int rowMul = snx == 1 ? 256 : snx == 2 ? 128 : snx == 3 ? 86 : snx == 4 ? 64 : snx == 5 ? 52
-                                   : snx == 6 ? 43 : 37; // ceil(256 / snx)
-                int       b = 0, e = 0;
-                if (lane < snx * sny) {
-                    const int r = (lane * rowMul) >> 8;
-                    const int c = (sy0 + r) * gw + sx0 + (lane - r * snx);
-                    b           = strt[c];
-                    e           = strt[c + 1];
-                }
+            // A block that misses the candidate disc (next to the domain's edge)
+            // is retried once with kStageRetry rings.
+            for (;;) {
+                // ---- stage the block into LDS ----------------------------------
+                // Rows of the block are contiguous ranges of the sorted arrays:
+                // row r covers entries [rb[r], rb[r] + len) and LDS slots
+                // [roff[r], roff[r+1]).
                 constexpr int kRows = 2 * kStageRetry + 1;
                 int           rb[kRows], roff[kRows + 1];
                 roff[0] = 0;
 #pragma unroll
                 for (int r = 0; r < kRows; ++r) {
                     const int rr = min(r, sny - 1);
-                    rb[r]        = __builtin_amdgcn_readlane(b, rr * snx);
-                    roff[r + 1]  = roff[r] + (r < sny ? __builtin_amdgcn_readlane(e, rr * snx + snx - 1) - rb[r] : 0);
+                    rb[r]        = __builtin_amdgcn_readlane(cb, rr * snx);
+                    roff[r + 1]  = roff[r] + (r < sny ? __builtin_amdgcn_readlane(ce, rr * snx + snx - 1) - rb[r] : 0);
                 }
                 total = roff[kRows];
                 fast  = total <= kLossStage && total >= K;
                 if (fast) {
-                    // Row by row (the bounds are scalar): entry rb[r] + s goes to
-                    // slot roff[r] + s.
+                    // Slot d of the block (row-major over its cells) holds sorted
+                    // entry d + rb[r] - roff[r] of its row r (the last row with
+                    // roff[r] <= d).  Each lane takes slots lane and lane + 64 and
+                    // issues their four loads together; a lane past the end reads
+                    // slot 0's entry (total >= K >= 2) and keeps nothing.
+                    int s0 = lane + rb[0], s1 = lane + 64 + rb[0];
 #pragma unroll
-                    for (int r = 0; r < kRows; ++r) {
-                        if (r >= sny) break;
-                        for (int s = lane; s < roff[r + 1] - roff[r]; s += 64) {
-                            const int d         = roff[r] + s;
-                            lds.pos[d] = a.spos[so + rb[r] + s];
-                            lds.idx[d] = a.sidx[so + rb[r] + s];
-                        }
+                    for (int r = 1; r < kRows; ++r) {
+                        const int step = (rb[r] - roff[r]) - (rb[r - 1] - roff[r - 1]);
+                        s0 += lane >= roff[r] ? step : 0;
+                        s1 += lane + 64 >= roff[r] ? step : 0;
                     }
+                    const int     first = __builtin_amdgcn_readfirstlane(s0);
+                    const bool    in0 = e0 < total, in1 = e1 < total;
+                    const int64_t g0 = so + (in0 ? s0 : first), g1 = so + (in1 ? s1 : first);
+                    const V2<R>   q0 = a.spos[g0], q1 = a.spos[g1];
+                    const int     x0 = a.sidx[g0], x1 = a.sidx[g1];
+                    if (in0) lds.pos[e0] = q0, lds.idx[e0] = x0;
+                    if (in1) lds.pos[e1] = q1, lds.idx[e1] = x1;
                     waveSync();
                     // Squared distance from q to each staged site (two per lane).
-                    d0 = e0 < total ? dist2(lds.pos[e0], q) : rInf<R>();
-                    d1 = e1 < total ? dist2(lds.pos[e1], q) : rInf<R>();
+                    d0 = in0 ? dist2(q0, q) : rInf<R>();
+                    d1 = in1 ? dist2(q1, q) : rInf<R>();
                     // Bisection: lo < D^2 <= hi for D^2 = q's K-th smallest, i.e.
                     // count(d2 <= lo) < K <= count(d2 <= hi) over the staged sites.
                     lo = R(0);
-                    hi = waveMaxR<R>(fmax(e0 < total ? d0 : R(0), e1 < total ? d1 : R(0)));
+                    hi = waveMaxR<R>(fmax(in0 ? d0 : R(0), in1 ? d1 : R(0)));
                     for (int it = 0; it < kBisectSteps; ++it) {
                         R   mid = lo + R(0.5) * (hi - lo);
                         int c   = __popcll(__ballot(d0 <= mid)) + __popcll(__ballot(d1 <= mid));
@@ -942,6 +962,8 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                 }
                 if (fast || rs >= kStageRetry || total > kLossStage) break;
                 waveSync();
+                rs = kStageRetry;
+                blockStarts();
             }
             if (fast) {
                 // Compact the candidates (d2 <= thr^2) into LDS, the first band
@@ -1011,7 +1033,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
 #pragma unroll
         for (int h = 0; h < kTargetsPerLane; ++h) {
             if (__ballot(take[h] || (dbgKnn && lane + 64 * h < wk.count)) == 0) continue;
-            const int   t    = wk.begin + lane + 64 * h;
+            const int64_t t  = wk.tBegin + lane + 64 * h;
             LaneTerm<R> tr;
             bool        open = take[h];
             if constexpr (sizeof(R) == 4) {
@@ -1028,7 +1050,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                 base = __builtin_amdgcn_readfirstlane(base);
                 if (push)
                     a.exactQ[base + laneRank(pm)] =
-                        ExactItem{open ? wk.inst : int(uint32_t(wk.inst) | 0x80000000u), int(in.targetOff + t)};
+                        ExactItem{open ? wk.inst : int(uint32_t(wk.inst) | 0x80000000u), int(t)};
             }
             openCnt += __popcll(__ballot(open));
             const R lossT = tr.used ? tr.d * tr.d : R(0);
@@ -1038,10 +1060,10 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                          &lds.grad[2 * gradSlot(tr.sl0, lane & (gcopies - 1), gM)], __builtin_ldexp(1.0, in.gradShift));
             if (dbgKnn && lane + 64 * h < wk.count && !take[h]) {
 #pragma unroll
-                for (int k = 0; k < K; ++k) dbgKnn[(in.targetOff + t) * int64_t(K) + k] = -1;
+                for (int k = 0; k < K; ++k) dbgKnn[t * int64_t(K) + k] = -1;
             }
             if (dbgTerm && lane + 64 * h < wk.count && !open) {
-                const int64_t o   = in.targetOff + t;
+                const int64_t o   = t;
                 dbgTerm[o].i0   = tr.used ? tr.i0 : -1;
                 dbgTerm[o].j    = tr.used ? tr.bj : -1;
                 dbgTerm[o].loss = double(lossT);

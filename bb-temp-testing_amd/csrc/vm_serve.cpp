@@ -213,8 +213,8 @@ public:
         SlotInfo &si = slotInfo_[slot];
         si       = SlotInfo{so, to, co, S, T, p.cells, bytes, {}, {}};
         for (auto w : p.sw) si.sw.push_back({slot, w.begin, w.count, 0});
-        for (auto w : p.tw) {
-            w.inst = slot;
+        for (auto w : p.tw) { // prepared at offset 0: move to the pool ranges
+            w.inst = slot, w.tBegin += to, w.cellOff = co, w.siteOff = so;
             si.tw.push_back(w);
         }
         // Upload: positions, zero moments/gradient/counts, sorted targets, record, state.

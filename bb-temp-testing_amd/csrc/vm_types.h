@@ -74,13 +74,24 @@ struct SiteWork {
     int inst, begin, count, pad;
 };
 
-// Targets [begin, begin+count) of `inst` (sorted order), all in hash cell
-// (cx, cy); every one lies within `rho` of (qx, qy) (static: targets do not move).
+// Targets [tBegin, tBegin+count) of the pooled sorted target array, all of
+// instance `inst` and in its hash cell (cx, cy); every one lies within `rho`
+// of (qx, qy) (static: targets do not move).  The instance's grid width and
+// height, pooled offsets and staging rings are repeated from its record, so a
+// wavefront can issue its target and cell-start loads as soon as it holds the
+// work item, without waiting for the record.
+constexpr int kWorkRingsMask = 3;  // flags bits 0-1: staged cell rings
+constexpr int kWorkSubsample = 4;  // flags bit 2: subsampled targets (read their slot numbers)
 struct TargetWork {
-    int   inst, begin, count, cx;
-    float qx, qy, rho;
-    int   cy;
+    int     inst, count;
+    int     cx, cy;
+    float   qx, qy, rho;
+    int     gw, gh;
+    int     flags;
+    int64_t tBegin;
+    int64_t cellOff, siteOff;
 };
+static_assert(sizeof(TargetWork) == 64, "work items are four 16-byte words");
 
 // Scan tile: cells [begin, begin+count) of instance `inst` (first: begin == 0).
 // The tiles of one instance are consecutive in the table.

@@ -180,6 +180,7 @@ def test_serving_sample_full_size_law():
         alone, _ = vmhip.serve([reqs[i]], [0.0], hbm_budget_bytes=budget, max_iterations=iters, tol=0.0,
                                poll_every=5, want_sites=True)
         assert np.array_equal(alone[0]["sites"], mix[i]["sites"])
-        assert alone[0]["final_mean_loss"] == mix[i]["final_mean_loss"]
+        # the loss sum is an FP64 atomic reduction (order-dependent in its last bits); positions are exact
+        assert alone[0]["final_mean_loss"] == pytest.approx(mix[i]["final_mean_loss"], rel=1e-12)
         ref, _, _, _, _ = O.fit(reqs[i]["sites"], reqs[i]["targets"], 16, 2.0 / np.sqrt(sizes[i]), sc, iters)
         assert np.array_equal(mix[i]["sites"], ref), f"request {i} ({sizes[i]} sites) differs from the oracle"
