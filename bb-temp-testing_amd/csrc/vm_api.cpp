@@ -107,11 +107,12 @@ struct vm_handle {
         }
         {
             float cell, ce;
-            int   rs;
-            vmrt::tunedChoice(std::max(int(slotOfSite.size()), 1), W, H, s.K, &cell, &rs, &ce);
+            int   rs, tpl;
+            vmrt::tunedChoice(std::max(int(slotOfSite.size()), 1), W, H, s.K, &cell, &rs, &ce, &tpl);
             s.cell     = prm.grid_cell > 0.0f ? prm.grid_cell : cell;
             s.stageR   = prm.stage_rings > 0 ? prm.stage_rings : rs;
             s.chunkExt = ce;
+            s.tpl      = tpl;
         }
         eng->dirty = true;
     }
@@ -213,11 +214,12 @@ int vm_create(float width, float height, float spacing, const vm_params *params,
         s.fraction = h->prm.sample_fraction;
         s.seed     = h->prm.seed;
         float cell, ce;
-        int   rs;
-        vmrt::tunedChoice(1, width, height, s.K, &cell, &rs, &ce);
+        int   rs, tpl;
+        vmrt::tunedChoice(1, width, height, s.K, &cell, &rs, &ce, &tpl);
         s.cell     = h->prm.grid_cell > 0.0f ? h->prm.grid_cell : 2.0f * spacing;
         s.stageR   = h->prm.stage_rings > 0 ? h->prm.stage_rings : rs;
         s.chunkExt = ce;
+        s.tpl      = tpl;
         h->eng->add(std::move(s));
         *out = h.release();
     });
@@ -374,6 +376,15 @@ int vm_get_state(vm_handle *h, int *iteration, int *scheduleLength, float *lastB
         if (scheduleLength) *scheduleLength = h->scheduleLength;
         if (lastBatchLoss) *lastBatchLoss = h->lastBatchLoss;
         if (numSlots) *numSlots = h->numSlots();
+    });
+}
+
+int vm_graph_stats(vm_handle *h, int64_t *updates, int64_t *instantiations)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h, "null handle");
+        if (updates) *updates = h->eng->graphUpdates;
+        if (instantiations) *instantiations = h->eng->graphInstantiations;
     });
 }
 

@@ -299,14 +299,18 @@ InstSpec specFromDesc(const vm_fit_desc &d)
     s.lr = double(d.lr), s.gamma = double(d.gamma), s.beta1 = 0.9, s.beta2 = 0.999, s.eps = 1e-8;
     for (int k = 0; k < 4; ++k) s.milestones[k] = d.milestones[k];
     s.fraction = d.sample_fraction, s.seed = d.seed;
-    s.K        = d.loss_k > 0 ? d.loss_k : 16;
+    const int S = std::max(int(s.pos.size() / 2), 1);
+    s.K         = d.loss_k > 0 ? d.loss_k : tunedK(S);
     VM_REQUIRE(vmk::lossKSupported(s.K), "unsupported loss_k");
     float cell, ce;
-    int   rs;
-    tunedChoice(std::max(int(s.pos.size() / 2), 1), d.width, d.height, s.K, &cell, &rs, &ce);
+    int   rs, tpl;
+    tunedChoice(S, d.width, d.height, s.K, &cell, &rs, &ce, &tpl);
     s.cell     = d.grid_cell > 0 ? d.grid_cell : cell;
     s.stageR   = d.stage_rings > 0 ? d.stage_rings : rs;
     s.chunkExt = d.chunk_extent > 0 ? d.chunk_extent : ce;
+    VM_REQUIRE(d.targets_per_lane >= 0 && d.targets_per_lane <= vmk::kMaxTargetsPerLane, "targets_per_lane must be 0-4");
+    s.tpl = d.targets_per_lane > 0 ? d.targets_per_lane : tpl;
+    VM_REQUIRE(vmk::lossTplSupported(s.K, s.tpl, false), "targets_per_lane not compiled for this loss_k");
     s.maxIter = d.max_iterations, s.tol = d.tol;
     return s;
 }
@@ -481,7 +485,7 @@ int vm_batch_loss_stats(vm_batch *b, uint64_t *counters)
     });
 }
 
-int vm_batch_choice(vm_batch *b, int id, int *k, float *cell, int *rings, float *chunkExtent)
+int vm_batch_choice(vm_batch *b, int id, int *k, float *cell, int *rings, float *chunkExtent, int *targetsPerLane)
 {
     return vmrt::guardedCall([&] {
         VM_REQUIRE(b && id >= 0 && id < int(b->eng->specs.size()), "invalid arguments");
@@ -490,6 +494,7 @@ int vm_batch_choice(vm_batch *b, int id, int *k, float *cell, int *rings, float 
         if (cell) *cell = s.cell;
         if (rings) *rings = s.stageR;
         if (chunkExtent) *chunkExtent = s.chunkExt;
+        if (targetsPerLane) *targetsPerLane = b->precision == 1 ? vmk::kTargetsPerLane : s.tpl;
     });
 }
 

@@ -41,7 +41,15 @@ int guardedCall(const std::function<void()> &f)
 
 const char *lastError() { return t_lastError.c_str(); }
 
-void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR, float *chunkExt)
+static const TuneRow &tuneRow(int numSites)
+{
+    const int lg = int(std::floor(std::log2(double(std::max(numSites, 1)))));
+    return kTuneTable[std::min(std::max(lg, kTuneLog2Min), kTuneLog2Max) - kTuneLog2Min];
+}
+
+int tunedK(int numSites) { return tuneRow(numSites).k; }
+
+void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR, float *chunkExt, int *tpl)
 {
     // Mean site spacing a = sqrt(area / S).  The loss kernel's fast path needs
     // two staged rings to cover the K-NN radius (~a sqrt(K/pi)) plus the chunk
@@ -49,11 +57,11 @@ void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR
     // by the measured factor of the size class.
     const double a   = std::sqrt(double(W) * double(H) / double(std::max(numSites, 1)));
     const double f   = std::max(1.5, 0.5 * (std::sqrt(double(K) / 3.141592653589793) + 0.8));
-    const int    lg  = int(std::floor(std::log2(double(std::max(numSites, 1)))));
-    const auto  &row = kTuneTable[std::min(std::max(lg, kTuneLog2Min), kTuneLog2Max) - kTuneLog2Min];
+    const auto  &row = tuneRow(numSites);
     *cell            = float(f * row.cellScale * a);
     *stageR          = 2;
     *chunkExt        = row.chunkExtent;
+    if (tpl) *tpl = vmk::lossTplSupported(K, row.tpl, false) ? row.tpl : vmk::kTargetsPerLane;
 }
 
 namespace {
@@ -93,6 +101,8 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
     const int S = int(s.pos.size() / 2), T = int(s.tgt.size() / 2);
     if (S >= vmk::kMaxSitesPerInst) throw VmError(1, "instance has too many sites");
     if (!vmk::lossKSupported(s.K)) throw VmError(1, "unsupported loss_k " + std::to_string(s.K));
+    if (s.tpl < 1 || s.tpl > vmk::kMaxTargetsPerLane) throw VmError(1, "unsupported targets per lane");
+    const int chunkMax = 64 * s.tpl;
     if (!(s.cell > 0.0f)) throw VmError(1, "grid cell must be positive");
     in.W = s.W, in.H = s.H;
     in.cell = s.cell, in.inv = 1.0f / s.cell, in.ext = std::max(s.W, s.H);
@@ -114,7 +124,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
 
     // Targets sorted by hash cell, then by a 4x4-bit Morton key inside the cell,
     // then by input order; a chunk (one wavefront) stays in one cell, holds at
-    // most kTargetChunk targets and its bounding-box diagonal stays within the
+    // most 64 * tpl targets and its bounding-box diagonal stays within the
     // chunk extent (in mean spacings).
     std::vector<int>      cellOfT(T);
     std::vector<uint32_t> keyT(T);
@@ -165,7 +175,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
     for (int t = 0; t < T;) {
         int    c = cellOfT[perm[t]], e = t;
         double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
-        while (e < T && cellOfT[perm[e]] == c && e - t < vmk::kTargetChunk) {
+        while (e < T && cellOfT[perm[e]] == c && e - t < chunkMax) {
             double px = s.tgt[2 * perm[e]], py = s.tgt[2 * perm[e] + 1];
             double nx0 = std::min(x0, px), ny0 = std::min(y0, py), nx1 = std::max(x1, px), ny1 = std::max(y1, py);
             if (e > t && (nx1 - nx0) * (nx1 - nx0) + (ny1 - ny0) * (ny1 - ny0) > maxDiag * maxDiag) break;
@@ -187,7 +197,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         tw.flags   = (in.stageR & vmk::kWorkRingsMask) | (in.fraction < 1.0f ? vmk::kWorkSubsample : 0);
         tw.tBegin  = targetOff + t, tw.cellOff = cellOff, tw.siteOff = siteOff;
         targetWork.push_back(tw);
-        // Lane l takes the chunk's targets l, l + 64, l + 128.  Interleave the
+        // Lane l takes the chunk's targets l, l + 64, l + 128, ...  Interleave the
         // Morton order with stride H = ceil(n / 64), so the 64 targets of one
         // pass are spread over the chunk and fewer lanes of a pass add their
         // gradient to the same site (LDS atomics on one address serialize).
@@ -225,7 +235,7 @@ public:
         std::vector<vmk::Inst>       inst;
         std::vector<vmk::SiteWork>   siteWork;
         std::vector<vmk::TargetWork> targetWork; // grouped by K
-        struct Group { int K, begin, count; int64_t qOff, qLen; }; // the group's exact-selection queue
+        struct Group { int K, tpl, begin, count; int64_t qOff, qLen; }; // qOff/qLen: the group's exact-selection queue
         std::vector<Group>            groups;
         std::vector<vmk::ScanTile>    tiles; // scan tiles, instance by instance
         std::vector<std::vector<int>> perm; // per instance: sorted position -> input target index
@@ -283,6 +293,10 @@ public:
     {
         L = Layout();
         const int n = int(specs.size());
+        // Chunk shapes compiled for this precision (FP64 sends every target to
+        // the exact selection and has one).
+        for (auto &s : specs)
+            if (!vmk::lossTplSupported(s.K, s.tpl, sizeof(R) == 8)) s.tpl = vmk::kTargetsPerLane;
         L.inst.resize(n);
         L.perm.resize(n);
         std::vector<std::vector<vmk::TargetWork>> twByInst(n);
@@ -315,15 +329,15 @@ public:
                 }
                 if (pass == 0) L.dynTiles += nt;
             }
-        // Group target work by K (one loss launch per distinct K).
-        std::vector<int> ks;
-        for (auto &s : specs) ks.push_back(s.K);
+        // Group target work by (k, targets per lane): one loss launch per group.
+        std::vector<std::pair<int, int>> ks;
+        for (auto &s : specs) ks.push_back({s.K, s.tpl});
         std::sort(ks.begin(), ks.end());
         ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
-        for (int K : ks) {
-            typename Layout::Group g{K, int(L.targetWork.size()), 0, L.queue, 0};
+        for (auto [K, tpl] : ks) {
+            typename Layout::Group g{K, tpl, int(L.targetWork.size()), 0, L.queue, 0};
             for (int i = 0; i < n; ++i)
-                if (specs[i].K == K) {
+                if (specs[i].K == K && specs[i].tpl == tpl) {
                     L.targetWork.insert(L.targetWork.end(), twByInst[i].begin(), twByInst[i].end());
                     L.queue += L.inst[i].T; // every target of the group may be queued once per launch
                 }
@@ -473,7 +487,7 @@ public:
                                          dScanCtl.p));
         if (marks) VM_HIP(hipEventRecord(marks[2], stream));
         for (size_t g = 0; g < L.groups.size(); ++g)
-            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, lossArgs(int(g), vmk::kModeIterate, true)));
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, lossArgs(int(g), vmk::kModeIterate, true)));
         if (marks) VM_HIP(hipEventRecord(marks[3], stream));
         for (size_t g = 0; g < L.groups.size(); ++g)
             VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, lossArgs(int(g), vmk::kModeIterate, true)));
@@ -508,9 +522,16 @@ public:
                 (void)hipGetLastError();
                 (void)hipGraphExecDestroy(out);
                 out = nullptr;
+            } else {
+                ++graphUpdates;
             }
         }
-        hipError_t e = out ? hipSuccess : hipGraphInstantiate(&out, g, nullptr, nullptr, 0);
+        hipError_t e = hipSuccess;
+        if (!out) {
+            e = hipGraphInstantiate(&out, g, nullptr, nullptr, 0);
+            if (e == hipSuccess) ++graphInstantiations;
+            else out = nullptr;
+        }
         (void)hipGraphDestroy(g);
         VM_HIP(e);
     }
@@ -592,7 +613,7 @@ public:
         gridBuild(vmk::kModeEvaluate);
         for (size_t g = 0; g < L.groups.size(); ++g)
         {
-            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, lossArgs(int(g), vmk::kModeEvaluate, false)));
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, lossArgs(int(g), vmk::kModeEvaluate, false)));
             VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, lossArgs(int(g), vmk::kModeEvaluate, false)));
         }
         bin();
@@ -618,7 +639,7 @@ public:
         for (size_t g = 0; g < L.groups.size(); ++g) {
             auto a = lossArgs(int(g), vmk::kModeEvaluate, wantGrad);
             if (wantDbg && L.groups[g].K == K) a.dbgKnn = dK.p, a.dbgTerm = dT.p;
-            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, a));
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, a));
             VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, a));
         }
         std::vector<vmk::GradT<R>> hg;
@@ -670,7 +691,7 @@ public:
         for (size_t g = 0; g < L.groups.size(); ++g) {
             auto a  = lossArgs(int(g), vmk::kModeEvaluate, false);
             a.stats = dS.p;
-            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, gridLoss[g], stream, a));
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, a));
             VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, a));
         }
         bin();

@@ -94,6 +94,7 @@ struct InstSpec {
     float  cell = 0.0f;      // hash-grid cell
     int    stageR = 2;
     float  chunkExt = 1.6f;  // target chunk diagonal, in mean site spacings
+    int    tpl      = vmk::kTargetsPerLane; // loss kernel: targets per lane (chunks of up to 64 * tpl)
     int    useBox = 0;
     float  box[4] = {0, 0, 0, 0};
     int    maxIter = 0;
@@ -164,6 +165,9 @@ public:
     bool        dirty = true;
     bool        deviceNewer = false;
     hipStream_t stream = nullptr;
+    // Iteration-graph bookkeeping: captures taken by an existing exec in place
+    // (hipGraphExecUpdate) and execs instantiated (first use, or a new shape).
+    int64_t     graphUpdates = 0, graphInstantiations = 0;
 };
 
 // Boundary mesh of clipped cells (voro_optimizer.cu:798-854): every edge of an
@@ -192,9 +196,12 @@ constexpr float kSmallLayoutExtent = 0.7f;
 // a domain of extent `ext`.
 int gradShiftFor(int numTargets, float ext);
 
-// Default per-size-class choices (hash cell in mean spacings, staged rings).
-// The tuner's per-size-class choice (vm_tune_table.h): hash-grid cell, staged rings
-// and target chunk extent (mean spacings) for a fit of numSites sites with k = K.
-void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR, float *chunkExt);
+// The tuner's per-size-class choice (vm_tune_table.h) for a fit of numSites
+// sites with k = K: hash-grid cell, staged rings, target chunk extent (mean
+// spacings) and targets per lane (kept to what is compiled for K).
+void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR, float *chunkExt, int *tpl);
+// The tuner's k for the size class of a fit of numSites sites (the smallest k
+// whose loss terms equalled those of k = 32 on the class's tuning fits).
+int tunedK(int numSites);
 
 } // namespace vmrt

@@ -134,7 +134,8 @@ template <class R>
 hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstState *state, const SiteWork *work,
                          const int *nWork, const void *pos, const int *cellOf, const int *rank, const int *start,
                          void *spos, int *sidx, int mode, ScanCtl *ctl);
-template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const LossArgs<R> &a);
+// One loss launch over a group of chunks of fits sharing k and targets per lane.
+template <class R> hipError_t launchLoss(int K, int tpl, int grid, hipStream_t s, const LossArgs<R> &a);
 // The exact selection for the targets the loss launch queued; always follows it.
 template <class R> hipError_t launchExact(int K, hipStream_t s, const LossArgs<R> &a);
 template <class R> hipError_t launchAdamBin(int grid, hipStream_t s, const AdamArgs<R> &a);
@@ -162,5 +163,12 @@ inline int exactGridFor(int64_t targets, bool allExact)
 }
 
 inline bool lossKSupported(int K) { return K == 2 || K == 4 || K == 6 || K == 8 || K == 12 || K == 16 || K == 24 || K == 32; }
+// Targets per lane compiled for k: 2, 3 and 4 for the FP32 kernels of k = 8, 12, 16 (the tuned range); 3 for the
+// rest (FP64 runs send every target to the exact selection, so the chunk shape matters little there).
+inline bool lossTplSupported(int K, int tpl, bool fp64)
+{
+    if (tpl == kTargetsPerLane) return true;
+    return !fp64 && (tpl == 2 || tpl == 4) && (K == 8 || K == 12 || K == 16);
+}
 
 } // namespace vmk

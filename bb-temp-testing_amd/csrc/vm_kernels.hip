@@ -804,7 +804,7 @@ __device__ __forceinline__ int laneRank(uint64_t mask)
 // At most 72 VGPRs, so 7 wavefronts share a SIMD: the kernel is bound by issue
 // and by the latency of its per-chunk loads, and occupancy hides the latter
 // (K = 4 and 6 would spill at 72 and take 6).
-template <int K, class R, bool Diag>
+template <int K, int TPL, class R, bool Diag>
 __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K == 4 || K == 6 ? 6 : 7))) void lossKernel(LossArgs<R> a)
 {
     // Diagnostics (path counters, k-NN lists, per-target terms, exact-only
@@ -827,15 +827,15 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
         const int64_t    so = wk.siteOff;
 
         // Every load that needs only the work item goes out at once: the
-        // chunk's targets (lane + 64 h, h < kTargetsPerLane; their slot numbers
+        // chunk's targets (lane + 64 h, h < TPL; their slot numbers
         // when subsampled) and the cell starts of the staged block.  The
         // instance record and state arrive meanwhile.
-        V2<R>    p[kTargetsPerLane];
-        uint32_t tsl[kTargetsPerLane];
-        bool     take[kTargetsPerLane];
+        V2<R>    p[TPL];
+        uint32_t tsl[TPL];
+        bool     take[TPL];
         const bool sub = (wk.flags & kWorkSubsample) != 0;
 #pragma unroll
-        for (int h = 0; h < kTargetsPerLane; ++h) {
+        for (int h = 0; h < TPL; ++h) {
             const int64_t ti = wk.tBegin + (lane + 64 * h < wk.count ? lane + 64 * h : 0);
             p[h]   = a.tpos[ti];
             tsl[h] = sub ? uint32_t(a.tslot[ti]) : 0u;
@@ -867,7 +867,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
         if (a.mode == kModeIterate && loadUniform(&st->done)) continue;
         bool anyTake = false;
 #pragma unroll
-        for (int h = 0; h < kTargetsPerLane; ++h) {
+        for (int h = 0; h < TPL; ++h) {
             const bool valid = lane + 64 * h < wk.count;
             if (!valid) p[h] = V2<R>{R(-1), R(-1)};
             take[h] = valid && p[h].x >= R(0);
@@ -1031,7 +1031,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
         int      used    = 0;
         int      openCnt = 0;
 #pragma unroll
-        for (int h = 0; h < kTargetsPerLane; ++h) {
+        for (int h = 0; h < TPL; ++h) {
             if (__ballot(take[h] || (dbgKnn && lane + 64 * h < wk.count)) == 0) continue;
             const int64_t t  = wk.tBegin + lane + 64 * h;
             LaneTerm<R> tr;
@@ -1958,25 +1958,26 @@ hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstS
     return hipGetLastError();
 }
 
-template <class R> hipError_t launchLoss(int K, int grid, hipStream_t s, const LossArgs<R> &a)
+template <class R> hipError_t launchLoss(int K, int tpl, int grid, hipStream_t s, const LossArgs<R> &a)
 {
     const bool diag   = a.stats || a.dbgKnn || a.dbgTerm || a.exactOnly;
     const int  blocks = (grid + kLossWaves - 1) / kLossWaves;
-    switch (K) {
-#define VM_K(k)                                                                                                   \
-    case k:                                                                                                       \
+    if (!lossKSupported(K) || !lossTplSupported(K, tpl, sizeof(R) == 8)) return hipErrorInvalidValue;
+#define VM_KT(k, t)                                                                                               \
+    if (K == k && tpl == t) {                                                                                     \
         if (diag)                                                                                                 \
-            hipLaunchKernelGGL((lossKernel<k, R, true>), dim3(blocks), dim3(kLossThreads), 0, s, a);              \
+            hipLaunchKernelGGL((lossKernel<k, t, R, true>), dim3(blocks), dim3(kLossThreads), 0, s, a);           \
         else                                                                                                      \
-            hipLaunchKernelGGL((lossKernel<k, R, false>), dim3(blocks), dim3(kLossThreads), 0, s, a);             \
-        break;
-        VM_K(2) VM_K(4) VM_K(6) VM_K(8) VM_K(12) VM_K(16) VM_K(24) VM_K(32)
-#undef VM_K
-    default: return hipErrorInvalidValue;
+            hipLaunchKernelGGL((lossKernel<k, t, R, false>), dim3(blocks), dim3(kLossThreads), 0, s, a);          \
+        return hipGetLastError();                                                                                 \
     }
-    return hipGetLastError();
+    VM_KT(2, 3) VM_KT(4, 3) VM_KT(6, 3) VM_KT(8, 3) VM_KT(12, 3) VM_KT(16, 3) VM_KT(24, 3) VM_KT(32, 3)
+    if constexpr (sizeof(R) == 4) {
+        VM_KT(8, 2) VM_KT(8, 4) VM_KT(12, 2) VM_KT(12, 4) VM_KT(16, 2) VM_KT(16, 4)
+    }
+#undef VM_KT
+    return hipErrorInvalidValue;
 }
-
 template <class R> hipError_t launchExact(int K, hipStream_t s, const LossArgs<R> &a)
 {
     switch (K) {
@@ -2069,8 +2070,8 @@ template hipError_t launchScatter<float>(int, hipStream_t, const Inst *, const I
 template hipError_t launchScatter<double>(int, hipStream_t, const Inst *, const InstState *, const SiteWork *,
                                           const int *, const void *, const int *, const int *, const int *, void *,
                                           int *, int, ScanCtl *);
-template hipError_t launchLoss<float>(int, int, hipStream_t, const LossArgs<float> &);
-template hipError_t launchLoss<double>(int, int, hipStream_t, const LossArgs<double> &);
+template hipError_t launchLoss<float>(int, int, int, hipStream_t, const LossArgs<float> &);
+template hipError_t launchLoss<double>(int, int, int, hipStream_t, const LossArgs<double> &);
 template hipError_t launchExact<float>(int, hipStream_t, const LossArgs<float> &);
 template hipError_t launchExact<double>(int, hipStream_t, const LossArgs<double> &);
 template hipError_t launchAdamBin<float>(int, hipStream_t, const AdamArgs<float> &);

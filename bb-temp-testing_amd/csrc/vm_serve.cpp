@@ -165,6 +165,10 @@ public:
     }
     ~Server()
     {
+        // Graph replays and admission copies may still be queued (an exception
+        // can leave the loop after launch()): let them finish before the
+        // buffers and host images they use are released.
+        (void)hipStreamSynchronize(stream_);
         if (graph_) (void)hipGraphExecDestroy(graph_);
         (void)hipEventDestroy(ev0_);
         (void)hipEventDestroy(ev1_);
@@ -401,7 +405,7 @@ private:
             const char *ex = std::getenv("VM_LOSS_EXACT");
             la.exactOnly   = ex && ex[0] == '1';
             la.exactGrid   = vmk::exactGridFor(capTargets_, sizeof(R) == 8 || la.exactOnly);
-            VM_HIP(vmk::launchLoss<R>(K, gLoss, stream_, la));
+            VM_HIP(vmk::launchLoss<R>(K, vmk::kTargetsPerLane, gLoss, stream_, la));
             VM_HIP(vmk::launchExact<R>(K, stream_, la));
             VM_HIP(vmk::launchAdamBin<R>(gSite, stream_, adamArgs(1)));
         } catch (...) {
@@ -465,7 +469,7 @@ void serve(const vm_fit_desc *reqs, const double *arrival, int n, const vm_serve
         float cell = reqs[i].grid_cell, ce;
         int   rs;
         if (!(cell > 0.0f))
-            vmrt::tunedChoice(std::max(reqs[i].num_sites, 1), reqs[i].width, reqs[i].height, k, &cell, &rs, &ce);
+            vmrt::tunedChoice(std::max(reqs[i].num_sites, 1), reqs[i].width, reqs[i].height, k, &cell, &rs, &ce, nullptr);
         needC += RangeAlloc::round(int64_t(std::ceil(reqs[i].width / cell)) * int64_t(std::ceil(reqs[i].height / cell)) + 1);
     }
     Server<R> srv(cfg, n, needS, needT, needC);
@@ -495,6 +499,9 @@ void serve(const vm_fit_desc *reqs, const double *arrival, int n, const vm_serve
                     p->spec = vmrt::specFromDesc(reqs[i]);
                     p->spec.maxIter = cfg.max_iterations > 0 ? cfg.max_iterations : reqs[i].max_iterations;
                     p->spec.tol     = cfg.tol > 0 ? cfg.tol : reqs[i].tol;
+                    // One loss launch serves every resident request: one k, one chunk shape.
+                    p->spec.K   = K;
+                    p->spec.tpl = vmk::kTargetsPerLane;
                     vmrt::layoutInstance(p->spec, -1, 0, 0, 0, p->in, p->sw, p->tw, p->perm);
                     p->cells = int64_t(p->in.gw) * p->in.gh + 1;
                     Server<R>::buildImages(*p);
@@ -529,7 +536,14 @@ void serve(const vm_fit_desc *reqs, const double *arrival, int n, const vm_serve
         if (running) srv.launch(poll, K);
         // Admit prepared requests in arrival order.
         while (nextAdmit < n && ready[nextAdmit].load() != 0) {
-            if (ready[nextAdmit].load() == 2) throw VmError(VM_EINVAL, "request preparation failed: " + prepError);
+            if (ready[nextAdmit].load() == 2) {
+                std::string msg;
+                {
+                    std::lock_guard<std::mutex> lk(errMu);
+                    msg = prepError;
+                }
+                throw VmError(VM_EINVAL, "request preparation failed: " + msg);
+            }
             if (!srv.admit(nextAdmit, prep[nextAdmit])) {
                 if (srv.resident() == 0) throw VmError(VM_ENOMEM, "request does not fit the HBM budget");
                 break;

@@ -13,8 +13,10 @@ constexpr int kMaxCellVerts = 24; // clipped cell vertex budget (voro_optimizer.
 constexpr int kCellK        = 32; // clip neighbours before brute force (voro_optimizer.cu:55)
 constexpr int kBoxLabels    = 4;  // edge labels -1..-4 are the box sides (voro_optimizer.cu:58-59)
 constexpr int kSiteChunk    = 256;
-constexpr int kTargetsPerLane = 3;                      // loss kernel: targets per lane
-constexpr int kTargetChunk    = 64 * kTargetsPerLane;  // targets of one wavefront, all in one hash cell
+// Loss kernel: targets per lane (a wavefront takes a chunk of up to 64 * tpl
+// targets of one hash cell); a per-fit choice of the tuner among 2, 3 and 4.
+constexpr int kTargetsPerLane    = 3;
+constexpr int kMaxTargetsPerLane = 4;
 constexpr int kLdsSlotBits  = 10;  // low bits of a k-NN key: slot of the site in the staged LDS block
 constexpr int kLdsNone      = (1 << kLdsSlotBits) - 1;
 constexpr int kStageCap     = kLdsNone - 1; // staged sites per target chunk

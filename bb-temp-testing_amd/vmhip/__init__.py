@@ -52,7 +52,8 @@ class FitDesc(C.Structure):
                 ("targets", C.c_void_p), ("num_targets", C.c_int), ("lr", C.c_float), ("gamma", C.c_float),
                 ("milestones", C.c_int * 4), ("sample_fraction", C.c_float), ("seed", C.c_uint),
                 ("loss_k", C.c_int), ("grid_cell", C.c_float), ("stage_rings", C.c_int),
-                ("max_iterations", C.c_int), ("tol", C.c_float), ("chunk_extent", C.c_float)]
+                ("max_iterations", C.c_int), ("tol", C.c_float), ("chunk_extent", C.c_float),
+                ("targets_per_lane", C.c_int)]
 
 
 class ServeConfig(C.Structure):
@@ -75,7 +76,7 @@ EXPORTS = [
     "vm_default_params", "vm_last_error", "vm_version", "vm_device_count", "vm_copy_probe",
     "vm_create", "vm_destroy", "vm_set_targets", "vm_update_targets", "vm_set_occupancy", "vm_update_occupancy",
     "vm_set_generators", "vm_update_generators", "vm_begin_schedule", "vm_step", "vm_get_state",
-    "vm_evaluate_loss", "vm_extract", "vm_get_generators", "vm_knn", "vm_loss_pass",
+    "vm_evaluate_loss", "vm_extract", "vm_get_generators", "vm_knn", "vm_loss_pass", "vm_graph_stats",
     "vm_sampler_create", "vm_sampler_destroy", "vm_sampler_resample", "vm_sampler_verify", "vm_sampler_dims",
     "vm_sampler_cells", "vm_sampler_load_cells", "vm_poisson_grid_dims", "vm_poisson_batch",
     "vm_poisson_batch_noalloc",
@@ -127,6 +128,7 @@ def lib():
         "vm_get_generators": [vp, vp, i32],
         "vm_knn": [vp, vp, i32, i32, vp, vp],
         "vm_loss_pass": [vp, vp, vp, vp, vp, vp, vp, vp],
+        "vm_graph_stats": [vp, P(C.c_int64), P(C.c_int64)],
         "vm_sampler_create": [f32, f32, f32, i32, C.c_uint, P(vp)],
         "vm_sampler_destroy": [vp],
         "vm_sampler_resample": [vp, i32, i32, i32, i32, P(f32)],
@@ -145,7 +147,7 @@ def lib():
         "vm_batch_get_sites": [vp, i32, vp, i32],
         "vm_batch_status": [vp, i32, P(i32), P(i32), P(f64)],
         "vm_batch_extract": [vp, i32, vp, i32, i32, f32, P(Mesh)],
-        "vm_batch_choice": [vp, i32, P(i32), P(f32), P(i32), P(f32)],
+        "vm_batch_choice": [vp, i32, P(i32), P(f32), P(i32), P(f32), P(i32)],
         "vm_batch_profile": [vp, i32, vp],
         "vm_batch_loss_stats": [vp, vp],
         "vm_serve": [P(FitDesc), vp, i32, P(ServeConfig), vp, P(ServeStats)],
@@ -295,6 +297,12 @@ class Optimizer:
     def optimize(self) -> float:
         st = self.state()
         return self.step(st["schedule_length"] - st["iteration"])
+
+    def graph_stats(self):
+        """Iteration-graph captures taken in place by an existing exec, and execs instantiated."""
+        u, i = C.c_int64(), C.c_int64()
+        _check(lib().vm_graph_stats(self._h, C.byref(u), C.byref(i)))
+        return dict(updates=u.value, instantiations=i.value)
 
     def evaluate_loss(self) -> float:
         v = C.c_double()
@@ -468,7 +476,8 @@ def poisson_batch(widths, heights, radii, seeds, trials=30, noalloc=False):
 
 
 def fit_desc(width, height, sites, targets, lr, milestones=(80, 150, 200, 250), gamma=0.5, sample_fraction=1.0,
-             seed=1, loss_k=16, grid_cell=0.0, stage_rings=0, max_iterations=0, tol=0.0, chunk_extent=0.0):
+             seed=1, loss_k=16, grid_cell=0.0, stage_rings=0, max_iterations=0, tol=0.0, chunk_extent=0.0,
+             targets_per_lane=0):
     """A FitDesc plus the arrays it points to (keep the pair alive together)."""
     s, t = _pts(sites), _pts(targets)
     d = FitDesc()
@@ -481,6 +490,7 @@ def fit_desc(width, height, sites, targets, lr, milestones=(80, 150, 200, 250), 
     d.sample_fraction, d.seed, d.loss_k = sample_fraction, seed, loss_k
     d.grid_cell, d.stage_rings, d.max_iterations, d.tol = grid_cell, stage_rings, max_iterations, tol
     d.chunk_extent = chunk_extent
+    d.targets_per_lane = targets_per_lane
     return d, (s, t)
 
 
@@ -552,9 +562,11 @@ class Batch:
         return _mesh_dict(m)
 
     def choice(self, i):
-        k, cell, rings, ext = C.c_int(), C.c_float(), C.c_int(), C.c_float()
-        _check(lib().vm_batch_choice(self._b, i, C.byref(k), C.byref(cell), C.byref(rings), C.byref(ext)))
-        return dict(loss_k=k.value, grid_cell=cell.value, stage_rings=rings.value, chunk_extent=ext.value)
+        k, cell, rings, ext, tpl = C.c_int(), C.c_float(), C.c_int(), C.c_float(), C.c_int()
+        _check(lib().vm_batch_choice(self._b, i, C.byref(k), C.byref(cell), C.byref(rings), C.byref(ext),
+                                     C.byref(tpl)))
+        return dict(loss_k=k.value, grid_cell=cell.value, stage_rings=rings.value, chunk_extent=ext.value,
+                    targets_per_lane=tpl.value)
 
 
 def serve(requests, arrivals, hbm_budget_bytes=200e9, max_resident=0, max_iterations=300, tol=1e-6, poll_every=10,

@@ -1,9 +1,16 @@
-"""Tuner sweep for the loss kernel's geometry, per size class of fits.
+"""Tuner for the loss kernel, per size class of fits.
 
-For each size class (fits of 2^c .. 2^(c+1) - 1 sites), a batch of fits of that class (Poisson-disk sites, 8 targets
-per site on random polygons, k=16) is run on the GPU for every candidate pair (hash-grid cell scale, target chunk
-extent); the pair with the highest site-iterations/s is written to csrc/vm_tune_table.h, which the library compiles
-in.  Classes above the largest measured one reuse its row.  Run on a GPU box, then rebuild:
+For each size class (fits of 2^c .. 2^(c+1) - 1 sites), a batch of fits of that class (Poisson-disk sites, 8
+targets per site on random polygons) is run on the GPU, and the choices with the highest site-iterations/s are
+written to csrc/vm_tune_table.h, which the library compiles in.  Three stages per class:
+
+  1. geometry at k = 16 and 3 targets per lane: hash-grid cell scale x target chunk extent;
+  2. targets per lane (2, 3, 4) x chunk extent at the chosen cell scale;
+  3. k for fits that leave it to the tuner: among 8, 12 and 16, the fastest whose per-target loss terms (nearest
+     site, bisector partner, term) equal those of k = 32 on every target of the class's check fits, both at the
+     Poisson start and after 50 iterations (so the choice cannot change a result on them); 16 when none does.
+
+Classes above the largest measured one reuse its row.  Run on a GPU box, then rebuild:
 
     python -m vmhip.tune --out bb-temp-testing_amd/csrc/vm_tune_table.h
 """
@@ -17,8 +24,11 @@ import numpy as np
 import vmhip
 from vmhip import workloads as wl
 
-CELL_SCALES = (0.9, 1.0, 1.15)
-CHUNK_EXTENTS = (1.0, 1.6, 2.2, 3.0)
+CELL_SCALES = (0.8, 0.9, 1.0, 1.15, 1.3)
+CHUNK_EXTENTS = (1.6, 2.2, 3.0, 4.0)
+TARGETS_PER_LANE = (2, 3, 4)
+K_CANDIDATES = (8, 12, 16)
+K_REFERENCE = 32
 LOG2_MIN, LOG2_MAX = 8, 22
 
 
@@ -28,31 +38,64 @@ def base_cell(K):
 
 
 def make_class(c, budget_sites, max_fits, seed):
-    """Fits of 2^c .. 2^(c+1) - 1 sites (log-uniform), about budget_sites sites in total."""
+    """Fits of 2^c .. 2^(c+1) - 1 sites (log-uniform), about budget_sites sites in total, with their sampler grids."""
     lo, hi = 2 ** c, 2 ** (c + 1) - 1
     n = int(min(max_fits, max(1, budget_sites // int(1.5 * lo))))
     sizes = wl.log_uniform_sizes(n, lo, hi, seed)
     radii = np.array([wl.poisson_radius_for(int(s)) for s in sizes], np.float32)
     seeds = (7 * (1 << 20) + 4096 * c + np.arange(n)).astype(np.uint32)
-    grids, _, _ = vmhip.poisson_batch(np.ones(n), np.ones(n), radii, seeds)
+    grids, dims, _ = vmhip.poisson_batch(np.ones(n), np.ones(n), radii, seeds)
     fits = []
     for i, g in enumerate(grids):
         sites = wl.live_sites(g)
-        fits.append((sites, wl.polygon(8 * len(sites), sigma=5e-4, seed=int(seeds[i]))))
+        fits.append(dict(sites=sites, targets=wl.polygon(8 * len(sites), sigma=5e-4, seed=int(seeds[i])),
+                         grid=g, dims=dims[i], radius=float(radii[i])))
     return fits
 
 
-def measure(fits, K, scale, extent, iters):
+def measure(fits, K, scale, extent, tpl, iters):
     b = vmhip.Batch()
-    for sites, tg in fits:
-        a = math.sqrt(1.0 / len(sites))
-        b.add(1.0, 1.0, sites, tg, 1e-3, loss_k=K, grid_cell=scale * base_cell(K) * a, stage_rings=2,
-              chunk_extent=extent)
+    for f in fits:
+        a = math.sqrt(1.0 / len(f["sites"]))
+        b.add(1.0, 1.0, f["sites"], f["targets"], 1e-3, loss_k=K, grid_cell=scale * base_cell(K) * a, stage_rings=2,
+              chunk_extent=extent, targets_per_lane=tpl)
     b.commit()
-    b.step(3)
+    b.step(2)
     ms = b.step(iters)
     b.close()
-    return sum(len(s) for s, _ in fits) * iters / (ms / 1e3)
+    return sum(len(f["sites"]) for f in fits) * iters / (ms / 1e3)
+
+
+def terms(sites_slots, sw, sh, cell, targets, K, radius):
+    """Per-target (nearest, partner, term) of one loss pass at k = K on the optimizer's own path."""
+    opt = vmhip.Optimizer(1.0, 1.0, radius, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 / radius)
+    opt.set_targets(targets)
+    opt.set_generators(sites_slots, sw, sh, cell)
+    d = opt.loss_pass(len(targets))
+    opt.close()
+    return d["nearest"], d["partner"], d["loss_terms"]
+
+
+def k_agreement(fits, Ks, iters=50):
+    """For each K: True when k = K gives the same per-target terms as k = 32 on every target of the check fits,
+    at the start and after `iters` iterations of a k = 16 batch."""
+    b = vmhip.Batch()
+    for f in fits:
+        b.add(1.0, 1.0, f["sites"], f["targets"], 1e-3, loss_k=16)
+    b.commit()
+    b.step(iters)
+    ok = {K: True for K in Ks}
+    for i, f in enumerate(fits):
+        gw, gh, cs = f["dims"]
+        moved = b.sites(i)
+        for slots, sw, sh, cell in ((f["grid"], gw, gh, cs), (moved, len(moved), 1, 1.0 / len(moved))):
+            ref = terms(slots, sw, sh, cell, f["targets"], K_REFERENCE, f["radius"])
+            for K in Ks:
+                if ok[K]:
+                    got = terms(slots, sw, sh, cell, f["targets"], K, f["radius"])
+                    ok[K] = all(np.array_equal(x, y) for x, y in zip(ref, got))
+    b.close()
+    return ok
 
 
 def write_table(path, rows):
@@ -62,46 +105,71 @@ def write_table(path, rows):
              "#pragma once", "", "namespace vmrt {", "",
              "struct TuneRow {",
              "    float cellScale;   // multiple of the base hash-grid cell max(1.5, (sqrt(K/pi) + 0.8) / 2) spacings",
-             "    float chunkExtent; // target chunk diagonal, in mean site spacings", "};", "",
+             "    float chunkExtent; // target chunk diagonal, in mean site spacings",
+             "    int   tpl;         // loss kernel targets per lane",
+             "    int   k;           // loss neighbours when a fit leaves k to the tuner", "};", "",
              f"constexpr int     kTuneLog2Min = {LOG2_MIN};", f"constexpr int     kTuneLog2Max = {LOG2_MAX};",
              "constexpr TuneRow kTuneTable[kTuneLog2Max - kTuneLog2Min + 1] = {"]
     for c in range(LOG2_MIN, LOG2_MAX + 1):
-        s, e, rate = rows[c]
-        note = f"measured {rate:.3e} site-iters/s" if rate else "largest measured class"
-        lines.append(f"    {{{s:.2f}f, {e:.2f}f}}, // 2^{c}: {note}")
+        r = rows[c]
+        note = (f"{r['rate']:.3e} site-iters/s at k=16, {r['rate_k']:.3e} at k={r['k']}" if r.get("rate")
+                else "largest measured class")
+        lines.append(f"    {{{r['scale']:.2f}f, {r['extent']:.2f}f, {r['tpl']}, {r['k']}}}, // 2^{c}: {note}")
     lines += ["};", "", "} // namespace vmrt", ""]
     with open(path, "w") as f:
         f.write("\n".join(lines))
 
 
+def tune_class(c, args):
+    t0 = time.time()
+    budget = max(args.budget, int(1.5 * 2 ** c))
+    fits = make_class(c, budget, args.max_fits, seed=c)
+    say = lambda m: print(f"class 2^{c} ({len(fits)} fits): {m}", flush=True)  # noqa: E731
+    best = None
+    for s in CELL_SCALES:
+        rate = measure(fits, 16, s, 2.2, 3, args.iters)
+        say(f"cell x{s:.2f}, extent 2.2, tpl 3: {rate:.3e}")
+        if best is None or rate > best["rate"]:
+            best = dict(scale=s, extent=2.2, tpl=3, rate=rate)
+    for tpl in TARGETS_PER_LANE:
+        for e in CHUNK_EXTENTS:
+            if tpl == 3 and e == 2.2:
+                continue
+            rate = measure(fits, 16, best["scale"], e, tpl, args.iters)
+            say(f"cell x{best['scale']:.2f}, extent {e:.1f}, tpl {tpl}: {rate:.3e}")
+            if rate > best["rate"]:
+                best.update(extent=e, tpl=tpl, rate=rate)
+    # Check fits: a few of the class (one for classes of a million sites and more: each pass holds its k-NN lists).
+    check = fits[:min(len(fits), args.check_fits if c < 19 else 1)]
+    agree = k_agreement(check, [K for K in K_CANDIDATES if K != 16])
+    best.update(k=16, rate_k=best["rate"])
+    for K in K_CANDIDATES:
+        if K == 16:
+            continue
+        rate = measure(fits, K, best["scale"], best["extent"], best["tpl"], args.iters)
+        say(f"k {K}: {rate:.3e}, terms equal to k={K_REFERENCE}'s: {agree[K]}")
+        if agree[K] and rate > best["rate_k"]:
+            best.update(k=K, rate_k=rate)
+    say(f"best cell x{best['scale']:.2f}, extent {best['extent']:.1f}, tpl {best['tpl']}, k {best['k']} "
+        f"({time.time() - t0:.1f} s)")
+    return best
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--classes", default="8-18", help="measured size classes (log2 of the site count)")
-    ap.add_argument("--budget", type=int, default=1_000_000, help="sites per class")
+    ap.add_argument("--classes", default="8-20", help="measured size classes (log2 of the site count)")
+    ap.add_argument("--budget", type=int, default=600_000, help="sites per class")
     ap.add_argument("--max-fits", type=int, default=512)
-    ap.add_argument("--iters", type=int, default=20)
-    ap.add_argument("--k", type=int, default=16)
+    ap.add_argument("--check-fits", type=int, default=3, help="fits per class for the k agreement check")
+    ap.add_argument("--iters", type=int, default=10)
     args = ap.parse_args()
     c0, c1 = (int(x) for x in args.classes.split("-"))
-    rows = {}
-    for c in range(c0, c1 + 1):
-        t0 = time.time()
-        fits = make_class(c, args.budget, args.max_fits, seed=c)
-        best = None
-        for s in CELL_SCALES:
-            for e in CHUNK_EXTENTS:
-                rate = measure(fits, args.k, s, e, args.iters)
-                print(f"class 2^{c} ({len(fits)} fits): cell x{s:.2f}, extent {e:.1f}: {rate:.3e} site-iters/s",
-                      flush=True)
-                if best is None or rate > best[2]:
-                    best = (s, e, rate)
-        rows[c] = best
-        print(f"class 2^{c}: best cell x{best[0]:.2f}, extent {best[1]:.1f} ({time.time() - t0:.1f} s)", flush=True)
+    rows = {c: tune_class(c, args) for c in range(c0, c1 + 1)}
     for c in range(LOG2_MIN, LOG2_MAX + 1):
         if c not in rows:
             near = min(rows, key=lambda m: abs(m - c))
-            rows[c] = (rows[near][0], rows[near][1], None)
+            rows[c] = {k: v for k, v in rows[near].items() if k not in ("rate", "rate_k")}
     write_table(args.out, rows)
     print(f"wrote {args.out}")
 

@@ -136,6 +136,11 @@ VM_API int vm_knn(vm_handle *h, const vm_vec2 *queries, int num_queries, int k, 
 VM_API int vm_loss_pass(vm_handle *h, double *grad_xy, double *loss_sum, int64_t *count, int32_t *knn,
                         int32_t *nearest, int32_t *partner, double *loss_terms);
 
+/* The optimizer's iteration graphs so far: captures an existing exec took in place (hipGraphExecUpdate: a new
+ * layout with the same launches) and execs instantiated (first use, or launches of a new shape).  No
+ * counterpart in the reference (it re-instantiates after every capacity change, voro_optimizer.cu:554,618). */
+VM_API int vm_graph_stats(vm_handle *h, int64_t *updates, int64_t *instantiations);
+
 /* ---- poisson::Sampler ------------------------------------------------------ */
 typedef struct vm_sampler vm_sampler;
 /* Sampler(width, height, radius, trials, seed)  poisson_disk.h:24, .cu:119-132 (runs ResampleAll) */
@@ -223,6 +228,7 @@ typedef struct vm_fit_desc {
     int            max_iterations;  /* 0: no limit */
     float          tol;             /* stop when |dL/L| < tol between steps (0: off) */
     float          chunk_extent;    /* target chunk diagonal in mean site spacings (0: tuner choice) */
+    int            targets_per_lane; /* loss kernel targets per lane, 2-4 (0: tuner choice; FP64 batches use 3) */
 } vm_fit_desc;
 
 typedef struct vm_batch vm_batch;
@@ -243,9 +249,10 @@ VM_API int vm_batch_status(vm_batch *b, int id, int *iteration, int *done, doubl
  * vm_batch_extract on `b`.  No reference counterpart: the batched form of Extract (voro_optimizer.h:116). */
 VM_API int vm_batch_extract(vm_batch *b, int id, const uint64_t *masks, int tiles_x, int tiles_y, float voxel_size,
                             vm_mesh *out);
-/* Resolved per-instance choices: k, grid cell, staged rings, target chunk extent (mean spacings). */
+/* Resolved per-instance choices: k, grid cell, staged rings, target chunk extent (mean spacings), targets per
+ * lane of the loss kernel. */
 VM_API int vm_batch_choice(vm_batch *b, int id, int *loss_k, float *grid_cell, int *stage_rings,
-                           float *chunk_extent);
+                           float *chunk_extent, int *targets_per_lane);
 /* One evaluation pass of the loss kernel over every instance, counting its wavefronts by
  * path: [0] bounding-box fast path, [1] ring search over the staged block, [2] ring
  * search in global memory (staging overflow), [3] no active target; [4] candidates

@@ -7,7 +7,8 @@ not.  `set_generators` zeroes the moments and restarts the schedule, so an
 optimizer that ran before must then give the same positions, bit for bit, as a
 fresh one given the same generators.  Going from about 1,100 sites (one scan
 tile, scatter folded into the scan) to about 80,000 (several tiles, a scatter
-launch) changes the graph's shape; a second grid of that size keeps it.
+launch) changes the graph's shape; a second grid of that size keeps it.  The
+optimizer's graph counters show which path each re-layout took.
 """
 import numpy as np
 import pytest
@@ -47,10 +48,14 @@ def test_graph_update_after_relayout_matches_fresh_optimizer():
     reused = _optimizer(targets)
     reused.set_generators(*small)
     reused.step(9)  # one eight-iteration replay and one single-iteration replay
+    assert reused.graph_stats() == dict(updates=0, instantiations=2)
+    # new shape: both execs instantiated again; same shape: both taken in place
+    expect = [dict(updates=0, instantiations=4), dict(updates=2, instantiations=4)]
     for n, g in enumerate(large):
         print(f"case: re-layout {n} ({'new shape' if n == 0 else 'same shape'}) against a fresh optimizer", flush=True)
         reused.set_generators(*g)
         reused.step(9)
+        assert reused.graph_stats() == expect[n], f"re-layout {n}: {reused.graph_stats()}"
         fresh = _optimizer(targets)
         fresh.set_generators(*g)
         fresh.step(9)

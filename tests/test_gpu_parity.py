@@ -407,6 +407,27 @@ def test_batch_instances_match_plain():
         assert st["last_mean_loss"] == pytest.approx(lh[-1] / ch[-1], rel=1e-12)
 
 
+@pytest.mark.parametrize("K,tpl", [(8, 2), (8, 4), (12, 2), (12, 4), (16, 2), (16, 4)])
+def test_batch_targets_per_lane_bitexact(K, tpl):
+    """The loss kernel's other chunk shapes (2 and 4 targets per lane, chunks of up to 128 and 256 targets):
+    15 batched iterations of three fits bit-exact against the oracle."""
+    r, iters = 0.04, 15
+    b = vmhip.Batch()
+    cases = []
+    for i in range(3):
+        grid = O.poisson(W, H, r * (1 + 0.25 * i), 30, 70 + i)
+        sites = wl.live_sites(grid)
+        tg = wl.polygon(8 * len(sites), seed=80 + i)
+        cases.append((sites, tg))
+        b.add(W, H, sites, tg, 1e-3, loss_k=K, targets_per_lane=tpl)
+    b.commit()
+    assert all(b.choice(i)["targets_per_lane"] == tpl for i in range(3))
+    b.step(iters)
+    for i, (sites, tg) in enumerate(cases):
+        ref, _, _, _, _ = O.fit(sites, tg, K, 0.1, O.make_sched(1e-3, W, H), iters)
+        assert np.array_equal(b.sites(i), ref), f"fit {i}"
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_clip_lattice_topology_zero_mismatches(precision):
     """Precision stress: jittered 128x128 lattice; clipped cells (labels and vertices) equal the FP64
