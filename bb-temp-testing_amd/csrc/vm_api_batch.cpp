@@ -477,6 +477,25 @@ int vm_copy_probe(int64_t bytes, float *gpuMs)
     });
 }
 
+int vm_batch_loss_pass(vm_batch *b, double *gradXY, double *lossSum, int64_t *count, int32_t *nearest,
+                       int32_t *partner, double *lossTerms)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(b && b->committed, "batch not committed");
+        const bool    wantTerms = nearest || partner || lossTerms;
+        vmrt::DiagOut d;
+        b->eng->diagAll(gradXY != nullptr, wantTerms, d);
+        if (lossSum) *lossSum = d.lossSum;
+        if (count) *count = d.count;
+        if (gradXY) std::copy(d.grad.begin(), d.grad.end(), gradXY);
+        for (size_t t = 0; t < d.terms.size(); ++t) {
+            if (nearest) nearest[t] = d.terms[t].i0;
+            if (partner) partner[t] = d.terms[t].j;
+            if (lossTerms) lossTerms[t] = d.terms[t].loss;
+        }
+    });
+}
+
 int vm_batch_loss_stats(vm_batch *b, uint64_t *counters)
 {
     return vmrt::guardedCall([&] {

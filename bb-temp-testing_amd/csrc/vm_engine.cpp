@@ -226,8 +226,12 @@ public:
     ~EngineT() override
     {
         destroyGraph();
+        if (!side.empty()) (void)hipStreamSynchronize(stream);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e), e = nullptr;
+        for (auto &e : evJoin) (void)hipEventDestroy(e);
+        if (evFork) (void)hipEventDestroy(evFork);
+        for (auto &sd : side) (void)hipStreamDestroy(sd);
     }
 
     // ---- layout ---------------------------------------------------------------
@@ -260,6 +264,11 @@ public:
     hipGraphExec_t       graph = nullptr, graphN = nullptr; // one iteration, kGraphIters iterations
     bool                 stale1 = false, staleN = false;   // a commit changed their launches
     hipEvent_t           ev[8] = {};
+    // Loss groups after the first run on side streams forked from the main
+    // stream (and joined back before Adam), so their launches overlap.
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t>  evJoin;
+    hipEvent_t               evFork = nullptr;
     int                  gridScan = 1, gridSite = 1;
     std::vector<int>     gridLoss;
 
@@ -472,6 +481,21 @@ public:
                                          dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, mode, dScanCtl.p));
     }
 
+    void ensureSide(size_t groups)
+    {
+        while (side.size() + 1 < groups) {
+            hipStream_t sd;
+            VM_HIP(hipStreamCreateWithFlags(&sd, hipStreamNonBlocking));
+            side.push_back(sd);
+        }
+        while (evJoin.size() + 1 < groups) {
+            hipEvent_t e;
+            VM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            evJoin.push_back(e);
+        }
+        if (!evFork) VM_HIP(hipEventCreateWithFlags(&evFork, hipEventDisableTiming));
+    }
+
     void enqueueIteration(hipEvent_t *marks)
     {
         if (marks) VM_HIP(hipEventRecord(marks[0], stream));
@@ -486,11 +510,28 @@ public:
                                          dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, vmk::kModeIterate,
                                          dScanCtl.p));
         if (marks) VM_HIP(hipEventRecord(marks[2], stream));
-        for (size_t g = 0; g < L.groups.size(); ++g)
-            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, lossArgs(int(g), vmk::kModeIterate, true)));
-        if (marks) VM_HIP(hipEventRecord(marks[3], stream));
-        for (size_t g = 0; g < L.groups.size(); ++g)
-            VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, lossArgs(int(g), vmk::kModeIterate, true)));
+        if (L.groups.size() == 1) {
+            VM_HIP(vmk::launchLoss<R>(L.groups[0].K, L.groups[0].tpl, gridLoss[0], stream,
+                                      lossArgs(0, vmk::kModeIterate, true)));
+            if (marks) VM_HIP(hipEventRecord(marks[3], stream));
+            VM_HIP(vmk::launchExact<R>(L.groups[0].K, stream, lossArgs(0, vmk::kModeIterate, true)));
+        } else if (!L.groups.empty()) {
+            // Each group's loss and exact selection on its own stream (the groups
+            // touch disjoint instances); the main stream waits for all of them.
+            // With several groups the loss phase below covers both kernels.
+            ensureSide(L.groups.size());
+            VM_HIP(hipEventRecord(evFork, stream));
+            for (size_t g = 0; g < L.groups.size(); ++g) {
+                hipStream_t sg = g == 0 ? stream : side[g - 1];
+                if (g > 0) VM_HIP(hipStreamWaitEvent(sg, evFork, 0));
+                VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], sg,
+                                          lossArgs(int(g), vmk::kModeIterate, true)));
+                VM_HIP(vmk::launchExact<R>(L.groups[g].K, sg, lossArgs(int(g), vmk::kModeIterate, true)));
+                if (g > 0) VM_HIP(hipEventRecord(evJoin[g - 1], sg));
+            }
+            for (size_t g = 1; g < L.groups.size(); ++g) VM_HIP(hipStreamWaitEvent(stream, evJoin[g - 1], 0));
+            if (marks) VM_HIP(hipEventRecord(marks[3], stream));
+        }
         if (marks) VM_HIP(hipEventRecord(marks[4], stream));
         VM_HIP(vmk::launchAdamBin<R>(gridSite, stream, adamArgs(1)));
         if (marks) VM_HIP(hipEventRecord(marks[5], stream));
@@ -678,6 +719,55 @@ public:
                 std::copy_n(hk.begin() + size_t(t) * K, K, out.knn.begin() + size_t(o) * K);
                 out.terms[o] = ht[t];
             }
+        }
+    }
+
+    void diagAll(bool wantGrad, bool wantTerms, DiagOut &out) override
+    {
+        if (dirty) commit();
+        DBuf<vmk::TermDbg> dT;
+        if (wantTerms) dT.reserve(size_t(std::max<int64_t>(L.targets, 1)));
+        gridBuild(vmk::kModeEvaluate);
+        for (size_t g = 0; g < L.groups.size(); ++g) {
+            auto a = lossArgs(int(g), vmk::kModeEvaluate, wantGrad);
+            if (wantTerms) a.dbgTerm = dT.p;
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, a));
+            VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, a));
+        }
+        std::vector<vmk::GradT<R>> hg;
+        std::vector<vmk::TermDbg>  ht;
+        if (wantGrad && L.sites) {
+            hg.resize(2 * size_t(L.sites));
+            VM_HIP(hipMemcpyAsync(hg.data(), dGrad.p, hg.size() * sizeof(hg[0]), hipMemcpyDeviceToHost, stream));
+        }
+        if (wantTerms && L.targets) {
+            ht.resize(size_t(L.targets));
+            VM_HIP(hipMemcpyAsync(ht.data(), dT.p, ht.size() * sizeof(ht[0]), hipMemcpyDeviceToHost, stream));
+        }
+        std::vector<vmk::InstState> hs(specs.size());
+        if (!hs.empty())
+            VM_HIP(hipMemcpyAsync(hs.data(), dState.p, hs.size() * sizeof(hs[0]), hipMemcpyDeviceToHost, stream));
+        if (wantGrad) VM_HIP(hipMemsetAsync(dGrad.p, 0, dGrad.n * sizeof(vmk::GradT<R>), stream));
+        bin();
+        VM_HIP(hipStreamSynchronize(stream));
+        out = DiagOut{};
+        size_t nS = 0, nT = 0;
+        for (size_t i = 0; i < specs.size(); ++i) {
+            nS += size_t(L.inst[i].S), nT += size_t(L.inst[i].T);
+            out.lossSum += hs[i].evalLoss, out.count += hs[i].evalCount;
+        }
+        if (wantGrad) out.grad.reserve(2 * nS);
+        if (wantTerms) out.terms.assign(nT, vmk::TermDbg{-1, -1, 0.0});
+        size_t tBase = 0;
+        for (size_t i = 0; i < specs.size(); ++i) {
+            const vmk::Inst &in = L.inst[i];
+            if (wantGrad) {
+                const double inv = sizeof(R) == 4 ? std::ldexp(1.0, -in.gradShift) : 1.0;
+                for (int k = 0; k < 2 * in.S; ++k) out.grad.push_back(double(hg[2 * size_t(in.siteOff) + k]) * inv);
+            }
+            if (wantTerms)
+                for (int t = 0; t < in.T; ++t) out.terms[tBase + size_t(L.perm[i][t])] = ht[size_t(in.targetOff) + t];
+            tBase += size_t(in.T);
         }
     }
 

@@ -156,6 +156,9 @@ public:
     virtual void  readStates() = 0;                          // device -> specs[i].state
     virtual double evaluate(int id, int64_t *count) = 0;     // loss sum over all live targets of `id`
     virtual void  diag(int id, bool wantGrad, bool wantDbg, DiagOut &out) = 0;
+    // One evaluation loss pass over every instance: gradient sums [2 * sum S] and per-target terms [sum T]
+    // (instance by instance, live targets in input order; site indices instance-local).
+    virtual void  diagAll(bool wantGrad, bool wantTerms, DiagOut &out) = 0;
     virtual void  knnQuery(int id, const double *q, int nq, int K, int *idx, double *d2) = 0;
     virtual void  clip(int id, const uint64_t *dMasks, int tilesX, int tilesY, float voxel, ClipOut &out) = 0;
     virtual size_t deviceBytes() const = 0;

@@ -82,6 +82,7 @@ EXPORTS = [
     "vm_poisson_batch_noalloc",
     "vm_batch_create", "vm_batch_destroy", "vm_batch_add", "vm_batch_commit", "vm_batch_step",
     "vm_batch_get_sites", "vm_batch_status", "vm_batch_extract", "vm_batch_choice", "vm_batch_profile", "vm_batch_loss_stats",
+    "vm_batch_loss_pass",
     "vm_serve",
     "vm_canvas_create", "vm_canvas_destroy", "vm_canvas_rasterize", "vm_canvas_stamp_disc", "vm_canvas_dirty",
     "vm_canvas_dims", "vm_canvas_counts", "vm_canvas_slots", "vm_canvas_masks", "vm_canvas_voxel",
@@ -150,6 +151,7 @@ def lib():
         "vm_batch_choice": [vp, i32, P(i32), P(f32), P(i32), P(f32), P(i32)],
         "vm_batch_profile": [vp, i32, vp],
         "vm_batch_loss_stats": [vp, vp],
+        "vm_batch_loss_pass": [vp, vp, vp, vp, vp, vp, vp],
         "vm_serve": [P(FitDesc), vp, i32, P(ServeConfig), vp, P(ServeStats)],
         "vm_canvas_create": [f32, f32, f32, P(vp)],
         "vm_canvas_destroy": [vp],
@@ -502,6 +504,7 @@ class Batch:
         _check(lib().vm_batch_create(precision, C.byref(self._b)))
         self._keep = []
         self.sizes = []
+        self.target_counts = []
 
     def close(self):
         if self._b:
@@ -520,6 +523,7 @@ class Batch:
         _check(lib().vm_batch_add(self._b, C.byref(d), C.byref(i)))
         self._keep.append(keep)
         self.sizes.append(int((keep[0][:, 0] >= 0).sum()))
+        self.target_counts.append(int((keep[1][:, 0] >= 0).sum()))
         return i.value
 
     def commit(self):
@@ -546,6 +550,22 @@ class Batch:
         it, done, loss = C.c_int(), C.c_int(), C.c_double()
         _check(lib().vm_batch_status(self._b, i, C.byref(it), C.byref(done), C.byref(loss)))
         return dict(iteration=it.value, done=bool(done.value), last_mean_loss=loss.value)
+
+    def loss_pass(self, want_terms=True):
+        """One loss pass over every fit at its current sites: gradient sums and per-target terms, concatenated in
+        the order the fits were added (site indices instance-local)."""
+        nS = sum(self.sizes)
+        nT = sum(self.target_counts)
+        grad = np.zeros((nS, 2), np.float64)
+        ls, cnt = np.zeros(1, np.float64), np.zeros(1, np.int64)
+        near = part = terms = None
+        if want_terms:
+            near, part = np.zeros(nT, np.int32), np.zeros(nT, np.int32)
+            terms = np.zeros(nT, np.float64)
+        _check(lib().vm_batch_loss_pass(self._b, _ptr(grad), _ptr(ls), _ptr(cnt), _ptr(near), _ptr(part),
+                                        _ptr(terms)))
+        return dict(grad=grad, loss_sum=float(ls[0]), count=int(cnt[0]), nearest=near, partner=part,
+                    loss_terms=terms)
 
     def loss_stats(self):
         c = np.zeros(8, np.uint64)

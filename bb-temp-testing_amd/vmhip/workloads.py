@@ -38,24 +38,34 @@ def _on_polyline(verts: np.ndarray, n: int, sigma: float, rng: np.random.Generat
     return p.astype(np.float32)
 
 
-def star(n=8192, sigma=1e-3, seed=0, center=(0.5, 0.5), r_out=0.40, r_in=0.18):
-    rng = np.random.default_rng([0x5747, seed])
+def star_outline(center=(0.5, 0.5), r_out=0.40, r_in=0.18):
+    """The 5-point star's 10 vertices (outer vertex first, at angle pi/2)."""
     ang = math.pi / 2 + np.arange(10) * math.pi / 5
     rad = np.where(np.arange(10) % 2 == 0, r_out, r_in)
-    verts = np.stack([center[0] + rad * np.cos(ang), center[1] + rad * np.sin(ang)], axis=1)
-    return _on_polyline(verts, n, sigma, rng)
+    return np.stack([center[0] + rad * np.cos(ang), center[1] + rad * np.sin(ang)], axis=1)
+
+
+def star(n=8192, sigma=1e-3, seed=0, center=(0.5, 0.5), r_out=0.40, r_in=0.18):
+    rng = np.random.default_rng([0x5747, seed])
+    return _on_polyline(star_outline(center, r_out, r_in), n, sigma, rng)
+
+
+def circle_outline(radius=0.3, center=(0.5, 0.5), segments=4096):
+    a = np.arange(segments) * (2 * math.pi / segments)
+    return np.stack([center[0] + radius * np.cos(a), center[1] + radius * np.sin(a)], axis=1)
 
 
 def circle(n, radius=0.3, sigma=1e-4, seed=3, center=(0.5, 0.5), segments=4096):
     rng = np.random.default_rng([0xC1C, seed])
-    a = np.arange(segments) * (2 * math.pi / segments)
-    verts = np.stack([center[0] + radius * np.cos(a), center[1] + radius * np.sin(a)], axis=1)
-    return _on_polyline(verts, n, sigma, rng)
+    return _on_polyline(circle_outline(radius, center, segments), n, sigma, rng)
 
 
-def ellipses(n, count=64, sigma=2e-4, seed=1, segments=4096):
-    """`count` non-overlapping ellipses (bounding circles disjoint), points split by perimeter."""
-    rng = np.random.default_rng([0xE11, seed])
+def ellipse_outlines(count=64, seed=1, segments=4096):
+    """The `count` ellipse polylines of `ellipses` (same placement draws)."""
+    return _ellipse_shapes(count, np.random.default_rng([0xE11, seed]), segments)[0]
+
+
+def _ellipse_shapes(count, rng, segments):
     shapes = []
     tries = 0
     while len(shapes) < count:
@@ -77,20 +87,34 @@ def ellipses(n, count=64, sigma=2e-4, seed=1, segments=4096):
         v = np.stack([c[0] + ct * x - st * y, c[1] + st * x + ct * y], axis=1)
         polys.append(v)
         perims.append(np.hypot(*(np.roll(v, -1, 0) - v).T).sum())
-    perims = np.asarray(perims)
+    return polys, np.asarray(perims)
+
+
+def ellipses(n, count=64, sigma=2e-4, seed=1, segments=4096):
+    """`count` non-overlapping ellipses (bounding circles disjoint), points split by perimeter."""
+    rng = np.random.default_rng([0xE11, seed])
+    polys, perims = _ellipse_shapes(count, rng, segments)
     counts = np.floor(n * perims / perims.sum()).astype(np.int64)
     counts[: n - counts.sum()] += 1
     return np.concatenate([_on_polyline(v, int(k), sigma, rng) for v, k in zip(polys, counts)])
 
 
-def polygon(n, sigma=5e-4, seed=2, center=(0.5, 0.5)):
-    """Random simple polygon: 5-40 vertices at sorted U[0, 2pi) angles, radii U[0.15, 0.45]."""
-    rng = np.random.default_rng([0xB01, seed])
+def _polygon_verts(rng, center):
     m = int(rng.integers(5, 41))
     ang = np.sort(rng.uniform(0.0, 2 * math.pi, m))
     rad = rng.uniform(0.15, 0.45, m)
-    verts = np.stack([center[0] + rad * np.cos(ang), center[1] + rad * np.sin(ang)], axis=1)
-    return _on_polyline(verts, n, sigma, rng)
+    return np.stack([center[0] + rad * np.cos(ang), center[1] + rad * np.sin(ang)], axis=1)
+
+
+def polygon_outline(seed=2, center=(0.5, 0.5)):
+    """The vertices of `polygon`'s random simple polygon for the same seed."""
+    return _polygon_verts(np.random.default_rng([0xB01, seed]), center)
+
+
+def polygon(n, sigma=5e-4, seed=2, center=(0.5, 0.5)):
+    """Random simple polygon: 5-40 vertices at sorted U[0, 2pi) angles, radii U[0.15, 0.45]."""
+    rng = np.random.default_rng([0xB01, seed])
+    return _on_polyline(_polygon_verts(rng, center), n, sigma, rng)
 
 
 def lattice(side=256, jitter=1e-7, seed=3):

@@ -1,13 +1,22 @@
 """Benchmark: VoroMesh optimizer iterations on one MI355X.
 
-Default workload: 512 independent fits of uneven size (sites log-uniform in
-[256, 65,536], Poisson-disk initialised; 8 targets per site on a random simple
-polygon with 5e-4 jitter; k = 16; lr 1e-3), all advanced by one set of launches
-per iteration (one iteration captured as a HIP graph).  A "step" is one
-optimizer iteration of every instance.  Also measured: the same fits run one at
-a time through the plain per-instance path, the loss kernel's duration (HIP
-events on the batch's stream) for the roofline, and the CPU oracle on a bounded
-sample of the same workload (the oracle is used only for those CPU baselines).
+Default workload (`value`): 512 independent fits of uneven size (sites log-uniform in [256, 65,536], Poisson-disk
+initialised; 8 targets per site on a random simple polygon with 5e-4 jitter; lr 1e-3; k, grid cell, chunk extent and
+targets per lane from the tuner's table per size class), all advanced by one set of launches per iteration (one
+iteration captured as a HIP graph).  A "step" is one optimizer iteration of every instance.
+
+Also measured, each with its own section of the JSON line:
+  * the same fits one at a time through the plain per-instance path, and the batch with k fixed at 16;
+  * the gradient gate: FP32 gradient against the all-FP64 GPU run evaluated with the FP32 run's discrete choices
+    (nearest site, bisector partner), selection flips counted separately -- for the batch, the large instance and
+    the small fit;
+  * the large single instance (262,144-site class, 4,194,304 targets on 64 ellipses, 500 iterations, final clip),
+    with HBM GB/s for every kernel of an iteration;
+  * the small plain fit (1,024-site class, star, 200 iterations) and the precision stress (256^2 lattice);
+  * the boundary mesh of each with occupancy rasterized from its target shape;
+  * local edits (the headless scripted demo), and the serving loop;
+  * the CPU restatement (oracle) timed on the host's cores on bounded samples: the CPU baseline.
+The oracle is used only by the CPU-baseline legs.
 
 Prints one JSON line.  `value` = site-iterations per second of the batched form.
 """
@@ -27,6 +36,8 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "site-iters/s batched vs plain at <=1e-5 grad err vs FP64; p50/p99 fit latency"
+GRAD_GATE = 1e-5
+PHASES = ["scan", "scatter", "loss", "exact", "adam_bin"]
 
 
 def log(msg):
@@ -44,72 +55,172 @@ def cpu_model():
     return platform.processor()
 
 
-def make_batched(n_inst, seed, lo=256, hi=65536, per_site=8):
+# ---------------------------------------------------------------------------
+# Workloads
+# ---------------------------------------------------------------------------
+def make_batched(n_inst, seed):
     """Sites (Poisson, batched GPU sampler) and polygon targets for every instance."""
     import vmhip
     from vmhip import workloads as wl
-    sizes = wl.log_uniform_sizes(n_inst, lo, hi, seed)
-    radii = np.array([wl.poisson_radius_for(int(s)) for s in sizes], np.float32)
-    seeds = (2 * (1 << 20) + np.arange(n_inst)).astype(np.uint32)
+    radii, seeds = wl.batch_plan(n_inst, seed)
     grids, _, ms = vmhip.poisson_batch(np.ones(n_inst), np.ones(n_inst), radii, seeds)
     insts = []
     for i, g in enumerate(grids):
         sites = wl.live_sites(g)
-        tg = wl.polygon(per_site * len(sites), sigma=5e-4, seed=int(seeds[i]))
-        insts.append((sites, tg))
-    return insts, ms
+        insts.append((sites, wl.batch_targets(sites, seeds[i])))
+    return insts, seeds, ms
 
 
-def measured_traffic(key):
-    """HBM bytes per loss-kernel launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
-    (profiles/loss_kernel_traffic.json, calibrated on a 256 MiB copy), or None."""
-    path = os.path.join(ROOT, "profiles", "loss_kernel_traffic.json")
-    try:
-        with open(path) as f:
-            return float(json.load(f)[key]["traffic_bytes"])
-    except (OSError, KeyError, ValueError):
-        return None
+def occupancy(outlines, raster=1024):
+    """Occupancy masks of the union of closed outlines (disjoint shapes) in the unit square, rasterized by the
+    canvas (even-odd through voxel centres): (masks, tiles_x, tiles_y, voxel)."""
+    import vmhip
+    masks = None
+    for poly in outlines:
+        c = vmhip.Canvas(1.0, 1.0, 1.0 / raster)
+        c.rasterize(np.asarray(poly, np.float32))
+        m = c.inside_masks()
+        masks = m if masks is None else masks | m
+        tx, ty, vox = c.tiles_x, c.tiles_y, c.voxel_size
+        c.close()
+    return masks, tx, ty, vox
 
 
+def mesh_summary(m):
+    return dict(cells=int(len(m["cell_start"]) - 1), inside=int(np.count_nonzero(m["inside"])),
+                boundary_vertices=int(len(m["vertices"])), boundary_edges=int(len(m["edges"])),
+                watertight=bool(m["watertight"]), loops=int(m["loops"]), unsafe_cells=int(m["unsafe_cells"]),
+                overflow_cells=int(m["overflow_cells"]), gpu_ms=float(m["gpu_ms"]), host_ms=float(m["host_ms"]))
+
+
+# ---------------------------------------------------------------------------
+# Algorithmic bytes
+# ---------------------------------------------------------------------------
 def loss_bytes(S, T, C):
-    """Algorithmic HBM bytes of one loss-kernel launch: targets read (8 B), sorted sites
-    read (pos 8 B + index 4 B), cell starts read (4 B), gradient accumulated (8 B)."""
+    """One loss-kernel launch: targets read (8 B), sorted sites read (pos 8 B + index 4 B), cell starts read
+    (4 B), gradient accumulated (8 B per site)."""
     return 8 * T + 20 * S + 4 * C
 
 
 def iter_bytes(S, T, C):
-    """Algorithmic bytes of one whole iteration: 8T + 88S + 8C."""
+    """One whole iteration: 8T + 88S + 8C."""
     return 8 * T + 88 * S + 8 * C
 
 
-def run_batched(insts, K, steps, warmup, lr=1e-3):
+def phase_bytes(S, T, C):
+    """Algorithmic bytes of each kernel of an iteration: scan (cell count read, start written, count cleared:
+    12 B/cell), scatter (position 8, cell and rank 8, cell start 4 read; sorted position 8 and index 4 written:
+    32 B/site), loss (above), exact selection (data-dependent, not counted), Adam + bin (position, m, v read and
+    written 48, gradient read and cleared 32, cell and rank written 8: 88 B/site)."""
+    return dict(scan=12 * C, scatter=32 * S, loss=loss_bytes(S, T, C), exact=None, adam_bin=88 * S)
+
+
+def phase_report(phase_ms, S, T, C):
+    out = {}
+    for name, ms, b in zip(PHASES, phase_ms, [phase_bytes(S, T, C)[p] for p in PHASES]):
+        d = dict(ms=float(ms))
+        if b is not None and ms > 0:
+            d.update(bytes=int(b), GBs=b / (ms / 1e3) / 1e9, frac=b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS)
+        out[name] = d
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Gradient gate: FP32 against FP64 with the FP32 run's discrete choices
+# ---------------------------------------------------------------------------
+def term_grads64(sites, targets, t_idx, c0, cj):
+    """FP64 gradient contributions of the VoroLoss terms of targets t_idx with nearest site c0 and bisector partner cj
+    (voro_optimizer.cu:244-254): with u = p - c0, e = cj - c0, l = |e|, n = e / l, d = <u, e> / l - l / 2,
+    dL/dcj = 2 d ((u - <u, n> n) / l - n / 2) and dL/dc0 = -2 d n - dL/dcj."""
+    p = targets[t_idx].astype(np.float64)
+    a = sites[c0].astype(np.float64)
+    b = sites[cj].astype(np.float64)
+    u, e = p - a, b - a
+    ln = np.hypot(e[:, 0], e[:, 1])[:, None]
+    n = e / ln
+    un = np.sum(u * n, axis=1)[:, None]
+    d = np.sum(u * e, axis=1)[:, None] / ln - 0.5 * ln
+    dcj = (u - un * n) / ln - 0.5 * n
+    gj = 2 * d * dcj
+    g0 = -2 * d * n - gj
+    return g0, gj
+
+
+def gradient_gate(sites_list, targets_list, p32, p64):
+    """p32, p64: loss passes (dicts with grad [sum S, 2], nearest/partner [sum T]) over the same fits at the same sites
+    in FP32 and in the all-FP64 mode.  The FP64 gradient is re-evaluated with the FP32 run's choices where they
+    differ (flips: targets whose nearest site or partner differs), then compared norm-wise and per site."""
+    g32, g64 = p32["grad"].copy(), p64["grad"].copy()
+    n32, j32, n64, j64 = p32["nearest"], p32["partner"], p64["nearest"], p64["partner"]
+    flip = (n32 != n64) | (j32 != j64)
+    s_off = np.cumsum([0] + [len(s) for s in sites_list])
+    t_off = np.cumsum([0] + [len(t) for t in targets_list])
+    for i in np.unique(np.searchsorted(t_off, np.nonzero(flip)[0], side="right") - 1):
+        idx = np.nonzero(flip[t_off[i]:t_off[i + 1]])[0]
+        sites, tg = sites_list[i], targets_list[i]
+        for (near, part, sign) in ((n64, j64, -1.0), (n32, j32, 1.0)):
+            nn, pp = near[t_off[i] + idx], part[t_off[i] + idx]
+            ok = (nn >= 0) & (pp >= 0)
+            if not ok.any():
+                continue
+            g0, gj = term_grads64(sites, tg, idx[ok], nn[ok], pp[ok])
+            np.add.at(g64, s_off[i] + nn[ok], sign * g0)
+            np.add.at(g64, s_off[i] + pp[ok], sign * gj)
+    err = np.abs(g32 - g64)
+    rel = float(np.linalg.norm(g32 - g64) / max(np.linalg.norm(g64), 1e-300))
+    return dict(grad_rel_norm_err=rel, grad_max_abs_err=float(err.max()), grad_mean_abs_err=float(err.mean()),
+                grad_max_abs=float(np.abs(g64).max()), selection_flips=int(flip.sum()), targets=int(len(flip)),
+                tolerance=GRAD_GATE, passed=bool(rel <= GRAD_GATE))
+
+
+# ---------------------------------------------------------------------------
+# Sections
+# ---------------------------------------------------------------------------
+def run_batched(insts, steps, warmup, loss_k=0, lr=1e-3):
     import vmhip
     b = vmhip.Batch()
     for sites, tg in insts:
-        b.add(1.0, 1.0, sites, tg, lr, loss_k=K)
+        b.add(1.0, 1.0, sites, tg, lr, loss_k=loss_k)
     t0 = time.time()
     b.commit()
     log(f"  batch committed in {time.time() - t0:.1f} s")
     if warmup:
         b.step(warmup)
     ms = b.step(steps)
-    phase = b.profile(5)
-    b.paths = b.loss_stats()
+    return b, ms
+
+
+def batch_cells(b, n):
     cells = 0
-    for i in range(len(insts)):
-        ch = b.choice(i)
-        g = math.ceil(1.0 / ch["grid_cell"])
+    for i in range(n):
+        g = math.ceil(1.0 / b.choice(i)["grid_cell"])
         cells += g * g + 1
-    return b, ms, phase, cells
+    return cells
 
 
-def run_plain(insts, K, iters, lr=1e-3):
-    """The same fits one at a time: one plain per-instance engine per fit."""
+def batch_gate(b, insts):
+    """Gradient gate of the batch at its current sites: FP32 loss pass, then an all-FP64 batch of the same fits
+    (same sites, same k) and its loss pass."""
+    import vmhip
+    sites = [b.sites(i) for i in range(len(insts))]
+    tgs = [t for _, t in insts]
+    p32 = b.loss_pass()
+    b64 = vmhip.Batch(precision=1)
+    for i, s in enumerate(sites):
+        b64.add(1.0, 1.0, s, tgs[i], 1e-3, loss_k=b.choice(i)["loss_k"])
+    b64.commit()
+    p64 = b64.loss_pass()
+    b64.close()
+    return gradient_gate(sites, tgs, p32, p64)
+
+
+def run_plain(insts, iters, lr=1e-3):
+    """The same fits one at a time: one plain per-instance engine per fit (the tuner's choices, as in the batch)."""
     import vmhip
     total_ms, site_iters = 0.0, 0
     for sites, tg in insts:
         b = vmhip.Batch()
-        b.add(1.0, 1.0, sites, tg, lr, loss_k=K)
+        b.add(1.0, 1.0, sites, tg, lr, loss_k=0)
         b.commit()
         b.step(1)
         total_ms += b.step(iters)
@@ -118,38 +229,49 @@ def run_plain(insts, K, iters, lr=1e-3):
     return site_iters / (total_ms / 1e3), total_ms
 
 
-def run_serving(seconds, rate, seed=4, lo=512, hi=131072, budget=200e9, max_iters=300, tol=1e-6, K=16):
-    """Poisson arrivals at `rate`/s for `seconds`; each request a fit of log-uniform size with 8
-    targets per site on a random polygon, run to |dL/L| < tol or max_iters; 200 GB budget."""
+def batch_meshes(b, insts, seeds, count=4):
+    """Boundary meshes of a few batch fits (smallest, largest and two between) with occupancy from their polygons."""
+    from vmhip import workloads as wl
+    order = np.argsort([len(s) for s, _ in insts])
+    pick = [int(order[int(q * (len(order) - 1))]) for q in np.linspace(0, 1, count)]
+    out = []
+    for i in pick:
+        masks, tx, ty, vox = occupancy([wl.polygon_outline(int(seeds[i]))])
+        m = b.extract(i, masks, (tx, ty), vox)
+        out.append(dict(instance=i, sites=len(insts[i][0]), **mesh_summary(m)))
+    return out
+
+
+def single_gate(sites_grid, gw, gh, cs, r, tg, K, grid_cell=0.0):
+    """Gradient gate of one optimizer's fit at the given slots: FP32 and all-FP64 loss passes."""
     import vmhip
     from vmhip import workloads as wl
-    arr = wl.poisson_arrivals(rate, seconds, seed)
-    n = len(arr)
-    sizes = wl.log_uniform_sizes(n, lo, hi, seed + 1)
-    radii = np.array([wl.poisson_radius_for(int(s)) for s in sizes], np.float32)
-    seeds = (4 * (1 << 20) + np.arange(n)).astype(np.uint32)
-    grids, _, _ = vmhip.poisson_batch(np.ones(n), np.ones(n), radii, seeds)
-    reqs = []
-    for i, g in enumerate(grids):
-        sites = wl.live_sites(g)
-        reqs.append(dict(width=1.0, height=1.0, sites=sites, targets=wl.polygon(8 * len(sites), sigma=5e-4,
-                                                                              seed=int(seeds[i])),
-                         lr=1e-3, loss_k=K))
-    res, st = vmhip.serve(reqs, arr, hbm_budget_bytes=budget, max_iterations=max_iters, tol=tol, poll_every=10)
-    lat = np.array([r["latency_s"] for r in res])
-    its = np.array([r["iterations"] for r in res])
-    return dict(requests=n, arrival_rate=rate, window_s=seconds, wall_s=st["wall_s"],
-                p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
-                requests_per_s=n / st["wall_s"], site_iters_per_s=st["site_iters"] / st["wall_s"],
-                mean_iterations=float(its.mean()), max_resident=st["max_resident_seen"],
-                peak_resident_GB=st["peak_bytes"] / 1e9, sites_total=int(sum(len(r["sites"]) for r in reqs)))
+    lp = {}
+    for prec in (0, 1):
+        o = vmhip.Optimizer(1.0, 1.0, r, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 / r, precision=prec,
+                            grid_cell=grid_cell)
+        o.set_targets(tg)
+        o.set_generators(sites_grid, gw, gh, cs)
+        lp[prec] = o.loss_pass(len(tg))
+        o.close()
+    live = np.nonzero(sites_grid[:, 0] >= 0)[0]
+    site_of = -np.ones(len(sites_grid), np.int64)
+    site_of[live] = np.arange(len(live))
+    conv = {}
+    for prec in (0, 1):
+        d = lp[prec]
+        conv[prec] = dict(grad=d["grad"][live], nearest=np.where(d["nearest"] >= 0, site_of[np.maximum(d["nearest"], 0)], -1),
+                          partner=np.where(d["partner"] >= 0, site_of[np.maximum(d["partner"], 0)], -1))
+    return gradient_gate([wl.live_sites(sites_grid)], [tg], conv[0], conv[1])
 
 
 def run_large(iters=500, K=16, n_sites=262144, n_targets=4194304):
-    """One large instance: Poisson sites (~262k) and 4.2M targets on 64 ellipses (sigma 2e-4)."""
+    """One large instance: Poisson sites (~262k) and 4.2M targets on 64 ellipses (sigma 2e-4), k = 16 as the workload
+    states; 500 iterations; per-kernel GB/s; final clip with the ellipses' occupancy; gradient gate."""
     import vmhip
     from vmhip import workloads as wl
     r = wl.poisson_radius_for(n_sites)
+    gw, gh, cs = vmhip.poisson_grid_dims(1.0, 1.0, r)
     grids, _, _ = vmhip.poisson_batch([1.0], [1.0], [r], [1])
     sites = wl.live_sites(grids[0])
     tg = wl.ellipses(n_targets, count=64, sigma=2e-4, seed=1)
@@ -164,77 +286,94 @@ def run_large(iters=500, K=16, n_sites=262144, n_targets=4194304):
     ch = b.choice(0)
     g = math.ceil(1.0 / ch["grid_cell"])
     S, T, C = len(sites), len(tg), g * g + 1
-    lb = loss_bytes(S, T, C)
-    ib = iter_bytes(S, T, C)
-    return dict(sites=S, targets=T, iterations=iters, ms_per_iter=ms / iters,
-                site_iters_per_s=S * iters / (ms / 1e3),
-                phase_ms=dict(zip(["scan", "scatter", "loss", "exact", "adam_bin"], [float(x) for x in phase])),
+    masks, tx, ty, vox = occupancy(wl.ellipse_outlines(64, seed=1))
+    t0 = time.perf_counter()
+    mesh = b.extract(0, masks, (tx, ty), vox)
+    extract_wall = (time.perf_counter() - t0) * 1e3
+    fitted = b.sites(0)
+    b.close()
+    # gate at the fitted sites (a slot grid of the live sites)
+    gate = single_gate(fitted, len(fitted), 1, 1.0 / len(fitted), r, tg, K)
+    lb, ib = loss_bytes(S, T, C), iter_bytes(S, T, C)
+    return dict(sites=S, targets=T, hash_cells=C, iterations=iters, ms_per_iter=ms / iters,
+                site_iters_per_s=S * iters / (ms / 1e3), kernels=phase_report(phase, S, T, C),
                 loss_kernel_GBs=lb / (phase[2] / 1e3) / 1e9, loss_kernel_frac=lb / (phase[2] / 1e3) / 1e9 / HBM_PEAK_GBS,
-                iteration_GBs=ib / (ms / iters / 1e3) / 1e9,
+                iteration_bytes=ib, iteration_GBs=ib / (ms / iters / 1e3) / 1e9,
                 iteration_frac=ib / (ms / iters / 1e3) / 1e9 / HBM_PEAK_GBS, grid_cell=ch["grid_cell"],
-                loss_paths_start=stats0, loss_paths_end=stats1, loss_kernel_traffic_bytes=measured_traffic("large262k"),
-                loss_kernel_algorithmic_bytes=lb,
-                final_mean_loss=b.status(0)["last_mean_loss"], fp32_vs_fp64_gpu=large_precision(grids[0], r, tg, K))
+                choice=ch, loss_paths_start=stats0, loss_paths_end=stats1,
+                loss_kernel_algorithmic_bytes=lb, extract=dict(wall_ms=extract_wall, **mesh_summary(mesh)),
+                fp32_vs_fp64=gate)
 
 
-def large_precision(grid, r, tg, K):
-    """One loss pass of the large instance in FP32 and in the all-FP64 mode: norm-wise, max and mean gradient
-    error, and how many targets select a different nearest site or bisector partner (near-ties below FP32
-    resolution, whose terms dominate the norm-wise figure)."""
+def run_small(iters=200, K=16):
+    """Plain single fit: Poisson r=0.025 seed 0, 8,192 star targets (sigma 1e-3), 200 iterations, clip with the
+    star's occupancy; gradient gate at the fitted sites."""
     import vmhip
-    gw, gh, cs = vmhip.poisson_grid_dims(1.0, 1.0, r)
-    lp = {}
-    for prec in (0, 1):
-        o = vmhip.Optimizer(1.0, 1.0, r, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 / r, precision=prec)
-        o.set_targets(tg)
-        o.set_generators(grid, gw, gh, cs)
-        lp[prec] = o.loss_pass(len(tg), want_terms=True)
-        o.close()
-    g32, g64 = lp[0]["grad"], lp[1]["grad"]
-    ge = np.abs(g32 - g64)
-    flips = int(np.count_nonzero((lp[0]["nearest"] != lp[1]["nearest"]) | (lp[0]["partner"] != lp[1]["partner"])))
-    return dict(grad_rel_norm_err=float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
-                grad_max_abs_err=float(ge.max()), grad_mean_abs_err=float(ge.mean()), selection_flips=flips)
+    from vmhip import workloads as wl
+    r = 0.025
+    smp = vmhip.Sampler(1.0, 1.0, r, 30, 0)
+    grid, gw, gh, cs = smp.cells(), smp.grid_w, smp.grid_h, smp.cell_size
+    tg = wl.star(8192, sigma=1e-3, seed=0)
+    sites = wl.live_sites(grid)
+    cell = 2.0 / math.sqrt(len(sites))
+    opt = vmhip.Optimizer(1.0, 1.0, r, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 / r, iterations=iters,
+                          grid_cell=cell)
+    opt.set_targets(tg)
+    opt.set_generators(grid, gw, gh, cs)
+    masks, tx, ty, vox = occupancy([wl.star_outline()])
+    opt.set_occupancy(masks, tx, ty, vox)
+    t0 = time.perf_counter()
+    gpu_ms = opt.optimize()
+    mesh = opt.extract()
+    wall = time.perf_counter() - t0
+    fitted = opt.generators()
+    out = dict(sites=len(sites), targets=len(tg), iterations=iters, gpu_ms=gpu_ms,
+               gpu_site_iters_per_s=len(sites) * iters / (gpu_ms / 1e3), wall_s=wall,
+               final_mean_loss=opt.evaluate_loss(), extract=mesh_summary(mesh))
+    opt.close()
+    out["fp32_vs_fp64"] = single_gate(fitted, gw, gh, cs, r, tg, K, grid_cell=cell)
+    return out, (sites, tg, r, gw, gh, cs, K, iters, cell)
 
 
 def run_precision(iters=5, K=16):
     """Jittered 256x256 lattice (+-1e-7) and 524,288 circle targets: the FP32 run (FP64 clip and loss sum)
-    against the all-FP64 GPU run: gradient, positions after `iters` steps, clipped-cell topology.  (Both runs
-    against the FP64 CPU oracle are tests: tests/test_gpu_parity.py.)"""
+    against the all-FP64 GPU run: gradient (with the FP32 run's choices), positions after `iters` steps,
+    clipped-cell topology, and the circle's boundary.  (Both runs against the FP64 CPU oracle are tests.)"""
     import vmhip
     from vmhip import workloads as wl
     sites = wl.lattice(256, 1e-7, seed=3)
     tg = wl.circle(524288, radius=0.3, sigma=1e-4, seed=3)
+    masks, tx, ty, vox = occupancy([wl.circle_outline(0.3)])
     runs = {}
     for prec in (0, 1):
         opt = vmhip.Optimizer(1.0, 1.0, 1 / 256, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 * 256,
                               iterations=iters, precision=prec)
         opt.set_targets(tg)
         opt.set_generators(sites, 256, 256, 1 / 256)
-        grad = opt.loss_pass(len(tg), want_terms=False)["grad"]
+        opt.set_occupancy(masks, tx, ty, vox)
+        lp = opt.loss_pass(len(tg))
         mesh0 = opt.extract()
         opt.step(iters)
-        runs[prec] = (grad, mesh0, opt.generators().astype(np.float64))
-    g64, m64, p64 = runs[1]
-    grad, mesh0, pos = runs[0]
-    ge, pe = np.abs(grad - g64), np.abs(pos - p64)
+        runs[prec] = (lp, mesh0, opt.generators().astype(np.float64))
+        opt.close()
+    lp64, m64, p64 = runs[1]
+    lp32, mesh0, pos = runs[0]
+    pe = np.abs(pos - p64)
     mism = 0
     for i in range(len(sites)):
         a = mesh0["cell_labels"][mesh0["cell_start"][i]:mesh0["cell_start"][i + 1]]
         b = m64["cell_labels"][m64["cell_start"][i]:m64["cell_start"][i + 1]]
         mism += int(not np.array_equal(a, b))
-    fp32 = dict(grad_rel_norm_err=float(np.linalg.norm(grad - g64) / np.linalg.norm(g64)),
-                grad_max_abs_err=float(ge.max()), grad_mean_abs_err=float(ge.mean()),
-                pos_max_abs_err=float(pe.max()), pos_mean_abs_err=float(pe.mean()),
-                topology_mismatches=mism, unsafe_cells=mesh0["unsafe_cells"], overflow_cells=mesh0["overflow_cells"])
-    return dict(fp32_vs_fp64_gpu=fp32, fp64_unsafe_cells=m64["unsafe_cells"], fp64_overflow_cells=m64["overflow_cells"],
+    gate = gradient_gate([sites], [tg], lp32, lp64)
+    gate.update(pos_max_abs_err=float(pe.max()), pos_mean_abs_err=float(pe.mean()), topology_mismatches=mism)
+    return dict(fp32_vs_fp64=gate, extract_fp32=mesh_summary(mesh0), extract_fp64=mesh_summary(m64),
                 sites=len(sites), targets=len(tg), iterations=iters)
 
 
 def run_local_edit(timeout=120):
-    """The demo's scripted edit sequence, headless (tests/cpp/shot_demo, apps/voromesh_demo.cpp:526-570):
-    per-stamp latency of the local pipeline (canvas, upload, Poisson resample, 100 local iterations,
-    extract) on a 1024^2 canvas with Poisson r = 24 generators.  Run as a child process."""
+    """The demo's scripted edit sequence, headless (tests/cpp/shot_demo): per-stamp latency of the local pipeline
+    (canvas, upload, Poisson resample, 100 local iterations, extract) on a 1024^2 canvas with Poisson r = 24
+    generators.  Run as a child process."""
     import subprocess
     exe = os.path.join(ROOT, "tests", "cpp", "shot_demo")
     if not os.path.exists(exe):
@@ -245,67 +384,78 @@ def run_local_edit(timeout=120):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def run_small(iters=200, K=16):
-    """Plain single fit: Poisson r=0.025 seed 0, 8,192 star targets (sigma 1e-3), 200 iterations, clip;
-    its CPU baseline is the oracle restatement over the same fit (cpu_baseline_small)."""
+def run_serving(seconds, rate, seed=4, lo=512, hi=131072, budget=200e9, max_iters=300, tol=1e-6, K=16):
+    """Poisson arrivals at `rate`/s for `seconds`; each request a fit of log-uniform size with 8 targets per site on
+    a random polygon, run to |dL/L| < tol or max_iters; 200 GB budget."""
     import vmhip
     from vmhip import workloads as wl
-    r = 0.025
-    smp = vmhip.Sampler(1.0, 1.0, r, 30, 0)
-    grid, gw, gh, cs = smp.cells(), smp.grid_w, smp.grid_h, smp.cell_size
-    tg = wl.star(8192, sigma=1e-3, seed=0)
-    sites = wl.live_sites(grid)
-    opt = vmhip.Optimizer(1.0, 1.0, r, loss_k=K, sample_fraction=1.0, lr_per_spacing=1e-3 / r, iterations=iters,
-                          grid_cell=2.0 / math.sqrt(len(sites)))
-    opt.set_targets(tg)
-    opt.set_generators(grid, gw, gh, cs)
-    t0 = time.perf_counter()
-    gpu_ms = opt.optimize()
-    mesh = opt.extract()
-    wall = time.perf_counter() - t0
-    out = dict(sites=len(sites), targets=len(tg), iterations=iters, gpu_ms=gpu_ms,
-               gpu_site_iters_per_s=len(sites) * iters / (gpu_ms / 1e3), extract_gpu_ms=mesh["gpu_ms"],
-               extract_host_ms=mesh["host_ms"], wall_s=wall, watertight=mesh["watertight"],
-               final_mean_loss=opt.evaluate_loss())
-    out.update(cpu_baseline_small(sites, tg, r, gw, gh, cs, K, iters))
-    return out
+    arr = wl.poisson_arrivals(rate, seconds, seed)
+    n = len(arr)
+    sizes = wl.log_uniform_sizes(n, lo, hi, seed + 1)
+    radii = np.array([wl.poisson_radius_for(int(s)) for s in sizes], np.float32)
+    seeds = (4 * (1 << 20) + np.arange(n)).astype(np.uint32)
+    grids, _, _ = vmhip.poisson_batch(np.ones(n), np.ones(n), radii, seeds)
+    reqs = []
+    for i, g in enumerate(grids):
+        sites = wl.live_sites(g)
+        reqs.append(dict(width=1.0, height=1.0, sites=sites, targets=wl.batch_targets(sites, seeds[i]), lr=1e-3,
+                         loss_k=K))
+    res, st = vmhip.serve(reqs, arr, hbm_budget_bytes=budget, max_iterations=max_iters, tol=tol, poll_every=10)
+    lat = np.array([r["latency_s"] for r in res])
+    its = np.array([r["iterations"] for r in res])
+    return dict(requests=n, arrival_rate=rate, window_s=seconds, wall_s=st["wall_s"],
+                p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
+                requests_per_s=n / st["wall_s"], site_iters_per_s=st["site_iters"] / st["wall_s"],
+                mean_iterations=float(its.mean()), max_resident=st["max_resident_seen"],
+                peak_resident_GB=st["peak_bytes"] / 1e9, sites_total=int(sum(len(r["sites"]) for r in reqs)))
 
 
-def cpu_baseline_small(sites, tg, r, gw, gh, cs, K, iters):
-    """The oracle restatement timed over the same plain fit (200 iterations and the clip), on the host's cores."""
+# ---------------------------------------------------------------------------
+# CPU baseline (the oracle restatement on the host's cores)
+# ---------------------------------------------------------------------------
+def cpu_fit_rate(cases, iters, threads=None):
     import oracle as O
+    if threads:
+        O.set_threads(threads)
+    sc = O.make_sched(1e-3, 1.0, 1.0)
+    t0 = time.perf_counter()
+    si = 0
+    for s, t, K in cases:
+        O.fit(s, t, K, 2.0 / math.sqrt(len(s)), sc, iters)
+        si += len(s) * iters
+    dt = time.perf_counter() - t0
+    return si / dt, dt
+
+
+def cpu_baselines(insts, ks, small_case, large_case, prec_case, iters=2):
+    """Batch: every instance of the 512-fit workload at `iters` iterations on all host cores (the headline baseline),
+    and every 16th instance on one thread; the small fit in full (200 iterations and the clip); the large instance and
+    the precision stress at 3 iterations each."""
+    import oracle as O
+    threads = O.num_threads()
+    cases = [(s, t, k) for (s, t), k in zip(insts, ks)]
+    rate, dt = cpu_fit_rate(cases, iters)
+    rate1, dt1 = cpu_fit_rate(cases[::16], 1, threads=1)
+    O.set_threads(threads)
+    sites, tg, r, gw, gh, cs, K, sit, cell = small_case
     lr = float(np.float32(np.float32(1e-3 / r) * np.float32(r)))
-    sc = O.make_sched(lr, 1.0, 1.0)
     m = np.float32(2) * np.float32(r)
     box = (float(-m), float(-m), float(np.float32(gw) * np.float32(cs) + m), float(np.float32(gh) * np.float32(cs) + m))
     t0 = time.perf_counter()
-    ref, _, _, _, _ = O.fit(sites, tg, K, 2.0 / math.sqrt(len(sites)), sc, iters, box=box)
-    O.clip(ref, 1.0, 1.0, 2.0 / math.sqrt(len(sites)))
-    cpu_s = time.perf_counter() - t0
-    return dict(cpu_s=cpu_s, cpu_threads=O.num_threads(), cpu_site_iters_per_s=len(sites) * iters / cpu_s)
-
-
-def cpu_baseline(insts, K, iters=10, budget_sites=500_000, lr=1e-3):
-    import oracle as O
-    threads = O.num_threads()
-    sample, n = [], 0
-    for s, t in insts:
-        if n >= budget_sites:
-            break
-        sample.append((s, t))
-        n += len(s)
-    sc = O.make_sched(lr, 1.0, 1.0)
-    t0 = time.perf_counter()
-    si = 0
-    for s, t in sample:
-        cell = 2.0 / math.sqrt(len(s))
-        O.fit(s, t, K, cell, sc, iters)
-        si += len(s) * iters
-    dt = time.perf_counter() - t0
-    return dict(value=si / dt, unit="site-iters/s", cores=threads, kind="port",
-                sample=f"first {len(sample)} of {len(insts)} instances ({n} sites, {sum(len(t) for _, t in sample)} "
-                       f"targets), {iters} iterations each, oracle restatement (OpenMP {threads} threads, "
-                       f"{cpu_model()})")
+    ref, _, _, _, _ = O.fit(sites, tg, K, cell, O.make_sched(lr, 1.0, 1.0), sit, box=box)
+    O.clip(ref, 1.0, 1.0, cell)
+    small_s = time.perf_counter() - t0
+    lrate, ldt = cpu_fit_rate([large_case], 3)
+    prate, pdt = cpu_fit_rate([prec_case], 3)
+    return dict(value=rate, unit="site-iters/s", cores=threads, kind="port",
+                sample=f"all {len(insts)} instances of the batch ({sum(len(s) for s, _ in insts)} sites), {iters} "
+                       f"iterations each, {dt:.1f} s, oracle restatement (OpenMP {threads} threads, {cpu_model()})",
+                one_thread=dict(value=rate1, sample=f"every 16th instance ({len(cases[::16])}), 1 iteration, "
+                                                     f"{dt1:.1f} s"),
+                small_fit=dict(s=small_s, site_iters_per_s=len(sites) * sit / small_s,
+                               sample=f"{len(sites)} sites, {sit} iterations and the clip"),
+                large_instance=dict(site_iters_per_s=lrate, s=ldt, sample="3 iterations"),
+                precision_stress=dict(site_iters_per_s=prate, s=pdt, sample="3 iterations"))
 
 
 def main():
@@ -314,15 +464,14 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--instances", type=int, default=512)
-    ap.add_argument("--k", type=int, default=16)
     ap.add_argument("--plain-iters", type=int, default=20)
     ap.add_argument("--plain-instances", type=int, default=0, help="0: all instances")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--serve-seconds", type=float, default=0.25, help="arrival window of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
-    ap.add_argument("--no-extra", action="store_true", help="skip the large/small/precision sections")
-    ap.add_argument("--only", choices=["large", "small", "precision", "local_edit"], default=None,
+    ap.add_argument("--no-extra", action="store_true", help="skip the large/small/precision/local-edit sections")
+    ap.add_argument("--only", choices=["large", "small", "precision", "local_edit", "batch"], default=None,
                     help="run one section alone and print its JSON (profiling)")
     ap.add_argument("--large-iters", type=int, default=500)
     args = ap.parse_args()
@@ -336,27 +485,46 @@ def main():
     # A 256 MiB device copy: the known-size kernel the HBM traffic counters are calibrated on.
     copy_ms = vmhip.copy_probe(256 << 20)
     log(f"copy probe: 256 MiB in {copy_ms:.3f} ms ({2 * 256 * 1.048576 / copy_ms:.0f} GB/s read+write)")
-    if args.only:
-        fn = {"large": lambda: run_large(iters=args.large_iters), "small": run_small, "precision": run_precision,
-              "local_edit": run_local_edit}
+    if args.only and args.only != "batch":
+        fn = {"large": lambda: run_large(iters=args.large_iters), "small": lambda: run_small()[0],
+              "precision": run_precision, "local_edit": run_local_edit}
         print(json.dumps(fn[args.only](), default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
         return
     log(f"generating {args.instances} instances ...")
     t0 = time.time()
-    insts, poisson_ms = make_batched(args.instances, args.seed)
+    insts, seeds, poisson_ms = make_batched(args.instances, args.seed)
     S = sum(len(s) for s, _ in insts)
     T = sum(len(t) for _, t in insts)
     log(f"  {S} sites, {T} targets in {time.time() - t0:.1f} s (Poisson {poisson_ms:.1f} ms GPU)")
 
-    log(f"batched: warmup {args.warmup}, timed {args.steps} iterations ...")
-    b, ms, phase, C = run_batched(insts, args.k, args.steps, args.warmup)
+    log(f"batched (tuned k): warmup {args.warmup}, timed {args.steps} iterations ...")
+    b, ms = run_batched(insts, args.steps, args.warmup, loss_k=0)
     value = S * args.steps / (ms / 1e3)
     ms_per_step = ms / args.steps
+    phase = b.profile(5)
+    paths = b.loss_stats()
+    C = batch_cells(b, len(insts))
+    ks = [b.choice(i)["loss_k"] for i in range(len(insts))]
+    tpls = [b.choice(i)["targets_per_lane"] for i in range(len(insts))]
     log(f"  {ms_per_step:.3f} ms/iteration, {value:.3e} site-iters/s; phases ms {np.round(phase, 4).tolist()}")
+    log("  gradient gate (FP32 vs FP64 with the FP32 choices) ...")
+    gate = batch_gate(b, insts)
+    log(f"  {gate}")
+    meshes = batch_meshes(b, insts, seeds)
+    b.close()
+    if args.only == "batch":
+        print(json.dumps(dict(value=value, ms_per_step=ms_per_step, phases=phase_report(phase, S, T, C), gate=gate,
+                              paths=paths), default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
+        return
+
+    log("batched with k fixed at 16 ...")
+    b16, ms16 = run_batched(insts, args.steps, args.warmup, loss_k=16)
+    phase16 = b16.profile(5)
+    b16.close()
 
     npl = args.plain_instances or len(insts)
     log(f"plain: {npl} fits one at a time, {args.plain_iters} iterations each ...")
-    plain_rate, plain_ms = run_plain(insts[:npl], args.k, args.plain_iters)
+    plain_rate, plain_ms = run_plain(insts[:npl], args.plain_iters)
     log(f"  plain {plain_rate:.3e} site-iters/s")
 
     lb = loss_bytes(S, T, C)
@@ -376,29 +544,36 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded Poisson-disk sites, random-polygon targets)",
         "config": {"workload": "512-instance mixed batch: sites log-uniform [256, 65536], 8 targets/site on a "
-                               "random polygon (sigma 5e-4), k=16, lr 1e-3, one persistent set of launches per "
-                               "iteration (HIP graph)",
-                   "instances": len(insts), "sites": S, "targets": T, "hash_cells": C, "k": args.k},
+                               "random polygon (sigma 5e-4), lr 1e-3, k/cell/chunk shape tuned per size class, one "
+                               "persistent set of launches per iteration (HIP graph)",
+                   "instances": len(insts), "sites": S, "targets": T, "hash_cells": C,
+                   "k_histogram": {str(k): ks.count(k) for k in sorted(set(ks))},
+                   "targets_per_lane_histogram": {str(t): tpls.count(t) for t in sorted(set(tpls))}},
+        "grad_gate": gate,
         "plain_site_iters_per_s": plain_rate,
         "batched_over_plain": value / plain_rate,
-        "loss_paths": b.paths,
-        "phase_ms": {"scan": float(phase[0]), "scatter": float(phase[1]), "loss": float(phase[2]),
-                     "exact": float(phase[3]), "adam_bin": float(phase[4])},
+        "k16": {"site_iters_per_s": S * args.steps / (ms16 / 1e3), "ms_per_step": ms16 / args.steps,
+                "kernels": phase_report(phase16, S, T, C)},
+        "loss_paths": paths,
+        "kernels": phase_report(phase, S, T, C),
         "iteration_roofline": {"bytes": iter_bytes(S, T, C),
                                "achieved_GBs": iter_bytes(S, T, C) / (ms_per_step / 1e3) / 1e9},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("batch512") if
-                     args.instances == 512 and args.seed == 2 else None, "kernel": "lossKernel<16,float,false>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "lossKernel (batch)",
                      "algorithmic_bytes": lb, "kernel_ms": loss_ms},
+        "batch_meshes": meshes,
     }
+    traffic = measured_traffic()
+    if traffic and args.instances == 512 and args.seed == 2:
+        out["roofline"]["traffic"] = traffic
+    small_case = large_case = prec_case = None
     if not args.no_extra:
         log("config 1: large single instance ...")
         out["large_instance"] = run_large(iters=args.large_iters)
         li = out["large_instance"]
-        log(f"  {li['ms_per_iter']:.3f} ms/iter, {li['site_iters_per_s']:.3e} site-iters/s, loss kernel "
-            f"{li['loss_kernel_GBs']:.1f} GB/s")
-        log("config 0: small plain fit (GPU and CPU oracle) ...")
-        out["small"] = run_small()
+        log(f"  {li['ms_per_iter']:.4f} ms/iter, {li['site_iters_per_s']:.3e} site-iters/s; gate {li['fp32_vs_fp64']}")
+        log("config 0: small plain fit ...")
+        out["small"], small_case = run_small()
         log(f"  {out['small']}")
         log("config 3: precision stress ...")
         out["precision"] = run_precision()
@@ -410,6 +585,10 @@ def main():
             log(f"  {le['stamps']} stamps, total {le['total_ms']['mean']:.3f} ms mean per edit "
                 f"(optimize {le['optimize_ms']['mean']:.3f}, poisson {le['poisson_ms']['mean']:.3f}, "
                 f"extract {le['extract_ms']['mean']:.3f}); watertight {le['after']['watertight']}")
+        from vmhip import workloads as wl
+        large_case = (wl.live_sites(vmhip.poisson_batch([1.0], [1.0], [wl.poisson_radius_for(262144)], [1])[0][0]),
+                      wl.ellipses(4194304, count=64, sigma=2e-4, seed=1), 16)
+        prec_case = (wl.lattice(256, 1e-7, seed=3), wl.circle(524288, radius=0.3, sigma=1e-4, seed=3), 16)
     if args.serve_seconds > 0:
         log(f"serving: Poisson arrivals {args.serve_rate}/s for {args.serve_seconds} s ...")
         sv = run_serving(args.serve_seconds, args.serve_rate)
@@ -417,11 +596,22 @@ def main():
         out["serving"] = sv
         log(f"  {sv['requests']} requests, p50 {sv['p50_latency_s']:.3f} s, p99 {sv['p99_latency_s']:.3f} s, "
             f"{sv['site_iters_per_s']:.3e} site-iters/s")
-    if not args.no_cpu:
-        log("cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(insts, args.k)
+    if not args.no_cpu and small_case is not None:
+        log("cpu baselines (oracle) ...")
+        out["cpu_baseline"] = cpu_baselines(insts, ks, small_case, large_case, prec_case)
         log(f"  cpu {out['cpu_baseline']['value']:.3e} site-iters/s")
     print(json.dumps(out, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per batch loss-kernel launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    (profiles/loss_kernel_traffic.json, calibrated on a 256 MiB copy), or None."""
+    path = os.path.join(ROOT, "profiles", "loss_kernel_traffic.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)["batch512"]["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 if __name__ == "__main__":

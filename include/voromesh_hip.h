@@ -253,6 +253,13 @@ VM_API int vm_batch_extract(vm_batch *b, int id, const uint64_t *masks, int tile
  * lane of the loss kernel. */
 VM_API int vm_batch_choice(vm_batch *b, int id, int *loss_k, float *grid_cell, int *stage_rings,
                            float *chunk_extent, int *targets_per_lane);
+/* One loss pass over every instance at its current sites, as vm_loss_pass does for one optimizer (every live
+ * target, no subsampling): outputs (each optional) concatenated instance by instance in the order they were added:
+ * gradient sums [2 * sum of live sites], the loss sum and target count, and per live target (in input order) the
+ * nearest site, bisector partner (instance-local site indices, -1 if unused) and loss term.  No counterpart in
+ * the reference (diagnostics of batches: the FP32-vs-FP64 gradient gate). */
+VM_API int vm_batch_loss_pass(vm_batch *b, double *grad_xy, double *loss_sum, int64_t *count, int32_t *nearest,
+                              int32_t *partner, double *loss_terms);
 /* One evaluation pass of the loss kernel over every instance, counting its wavefronts by
  * path: [0] bounding-box fast path, [1] ring search over the staged block, [2] ring
  * search in global memory (staging overflow), [3] no active target; [4] candidates
