@@ -289,6 +289,8 @@ InstSpec specFromDesc(const vm_fit_desc &d)
     VM_REQUIRE((d.sites || d.num_sites == 0) && (d.targets || d.num_targets == 0), "null arrays");
     InstSpec s;
     s.W = d.width, s.H = d.height;
+    s.pos.reserve(2 * size_t(d.num_sites)), s.tgt.reserve(2 * size_t(d.num_targets));
+    s.tslot.reserve(size_t(d.num_targets));
     for (int i = 0; i < d.num_sites; ++i)
         if (d.sites[i].x >= 0.0f) s.pos.push_back(d.sites[i].x), s.pos.push_back(d.sites[i].y);
     for (int i = 0; i < d.num_targets; ++i)

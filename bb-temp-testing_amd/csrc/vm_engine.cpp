@@ -140,16 +140,33 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         cellOfT[t] = ccy * in.gw + ccx;
         keyT[t]    = mort;
     }
-    // Stable LSD radix sort on (cell << 8 | Morton key), 12-bit digits: input
-    // order breaks ties, as a stable comparison sort would.
-    {
-        const uint64_t maxKey = (uint64_t(std::max(in.gw * in.gh, 1)) << 8) - 1;
+    // Stable sort on (cell << 8 | Morton key), input order breaking ties.  A
+    // grid with no more cells than a few per target: two stable counting
+    // passes, LSD order -- by the 8-bit Morton key, then by cell.  Larger
+    // grids: LSD radix sort on the whole key, 12-bit digits.  Both give the
+    // order of a stable comparison sort on the key.
+    const size_t  nT     = static_cast<size_t>(T);
+    const int64_t nCells = int64_t(std::max(in.gw * in.gh, 1));
+    perm.resize(nT);
+    if (nCells <= 4 * int64_t(T) + 4096) {
+        std::vector<int> byKey(nT), count(size_t(std::max<int64_t>(nCells, 256)) + 1);
+        std::fill(count.begin(), count.begin() + 257, 0);
+        for (int t = 0; t < T; ++t) ++count[size_t(keyT[size_t(t)]) + 1];
+        for (int d = 0; d < 256; ++d) count[size_t(d) + 1] += count[size_t(d)];
+        for (int t = 0; t < T; ++t) byKey[size_t(count[size_t(keyT[size_t(t)])]++)] = t;
+        std::fill(count.begin(), count.end(), 0);
+        for (int t = 0; t < T; ++t) ++count[size_t(cellOfT[size_t(t)]) + 1];
+        for (int64_t c = 0; c < nCells; ++c) count[size_t(c) + 1] += count[size_t(c)];
+        for (size_t i = 0; i < nT; ++i) {
+            const int t = byKey[i];
+            perm[size_t(count[size_t(cellOfT[size_t(t)])]++)] = t;
+        }
+    } else {
+        const uint64_t maxKey = (uint64_t(nCells) << 8) - 1;
         int            bits   = 1;
         while (bits < 64 && (maxKey >> bits)) ++bits;
-        const size_t          nT = static_cast<size_t>(T);
         std::vector<uint64_t> key(nT), key2(nT);
         std::vector<int>      idx2(nT);
-        perm.resize(nT);
         for (int t = 0; t < T; ++t) key[size_t(t)] = (uint64_t(uint32_t(cellOfT[size_t(t)])) << 8) | keyT[size_t(t)];
         std::iota(perm.begin(), perm.end(), 0);
         constexpr int kDigit = 12, kBuckets = 1 << kDigit;
@@ -172,11 +189,18 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
     }
     const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(S, 1)));
     const double maxDiag = double(std::min(s.chunkExt > 0.0f ? s.chunkExt : 1.6f, extentCap)) * spacing;
+    // Targets and cells in sorted order, read sequentially by the chunk scan.
+    std::vector<double> sxT(nT), syT(nT);
+    std::vector<int>    scT(nT);
+    for (size_t t = 0; t < nT; ++t) {
+        const int o = perm[t];
+        sxT[t] = s.tgt[2 * size_t(o)], syT[t] = s.tgt[2 * size_t(o) + 1], scT[t] = cellOfT[size_t(o)];
+    }
     for (int t = 0; t < T;) {
-        int    c = cellOfT[perm[t]], e = t;
+        int    c = scT[size_t(t)], e = t;
         double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
-        while (e < T && cellOfT[perm[e]] == c && e - t < chunkMax) {
-            double px = s.tgt[2 * perm[e]], py = s.tgt[2 * perm[e] + 1];
+        while (e < T && scT[size_t(e)] == c && e - t < chunkMax) {
+            double px = sxT[size_t(e)], py = syT[size_t(e)];
             double nx0 = std::min(x0, px), ny0 = std::min(y0, py), nx1 = std::max(x1, px), ny1 = std::max(y1, py);
             if (e > t && (nx1 - nx0) * (nx1 - nx0) + (ny1 - ny0) * (ny1 - ny0) > maxDiag * maxDiag) break;
             x0 = nx0, y0 = ny0, x1 = nx1, y1 = ny1;
@@ -186,7 +210,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         float  qx = float(0.5 * (x0 + x1)), qy = float(0.5 * (y0 + y1));
         double rho = 0.0;
         for (int k = t; k < e; ++k) {
-            const double dx = s.tgt[2 * perm[k]] - double(qx), dy = s.tgt[2 * perm[k] + 1] - double(qy);
+            const double dx = sxT[size_t(k)] - double(qx), dy = syT[size_t(k)] - double(qy);
             rho = std::max(rho, dx * dx + dy * dy);
         }
         rho = std::sqrt(rho); // double: within an ulp, and rounded up twice in float below

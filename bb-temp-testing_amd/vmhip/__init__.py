@@ -58,7 +58,8 @@ class FitDesc(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_double), ("max_resident", C.c_int), ("max_iterations", C.c_int),
-                ("tol", C.c_float), ("poll_every", C.c_int), ("precision", C.c_int)]
+                ("tol", C.c_float), ("poll_every", C.c_int), ("precision", C.c_int),
+                ("max_resident_sites", C.c_double), ("initial_pool_bytes", C.c_double)]
 
 
 class ServeResult(C.Structure):
@@ -67,8 +68,24 @@ class ServeResult(C.Structure):
 
 
 class ServeStats(C.Structure):
-    _fields_ = [("wall_s", C.c_double), ("site_iters", C.c_double), ("completed", C.c_int),
-                ("max_resident_seen", C.c_int), ("peak_bytes", C.c_double)]
+    _fields_ = [("wall_s", C.c_double), ("site_iters", C.c_double), ("site_iters_run", C.c_double), ("completed", C.c_int),
+                ("max_resident_seen", C.c_int), ("peak_bytes", C.c_double), ("arrived", C.c_int),
+                ("admitted", C.c_int), ("pool_grows", C.c_int), ("pool_bytes", C.c_double),
+                ("gpu_busy_s", C.c_double), ("oldest_wait_s", C.c_double), ("prep_busy_s", C.c_double),
+                ("sample_busy_s", C.c_double), ("loop_busy_s", C.c_double), ("prep_wait_s", C.c_double),
+                ("upload_bytes", C.c_double),
+                ("prep_workers", C.c_int)]
+
+
+class ServeStream(C.Structure):
+    _fields_ = [("rate", C.c_double), ("duration_s", C.c_double), ("seed", C.c_uint64), ("min_sites", C.c_int),
+                ("max_sites", C.c_int), ("targets_per_site", C.c_int), ("sigma", C.c_float), ("lr", C.c_float),
+                ("loss_k", C.c_int), ("drain", C.c_int)]
+
+
+class ServeRecord(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("admit_s", C.c_double), ("done_s", C.c_double), ("num_sites", C.c_int),
+                ("num_targets", C.c_int), ("iterations", C.c_int), ("final_mean_loss", C.c_double)]
 
 
 # Every symbol include/voromesh_hip.h declares (checked by the CPU test suite).
@@ -83,7 +100,7 @@ EXPORTS = [
     "vm_batch_create", "vm_batch_destroy", "vm_batch_add", "vm_batch_commit", "vm_batch_step",
     "vm_batch_get_sites", "vm_batch_status", "vm_batch_extract", "vm_batch_choice", "vm_batch_profile", "vm_batch_loss_stats",
     "vm_batch_loss_pass",
-    "vm_serve",
+    "vm_serve", "vm_serve_synthetic", "vm_serve_synth_plan", "vm_serve_synth_targets",
     "vm_canvas_create", "vm_canvas_destroy", "vm_canvas_rasterize", "vm_canvas_stamp_disc", "vm_canvas_dirty",
     "vm_canvas_dims", "vm_canvas_counts", "vm_canvas_slots", "vm_canvas_masks", "vm_canvas_voxel",
 ]
@@ -153,6 +170,9 @@ def lib():
         "vm_batch_loss_stats": [vp, vp],
         "vm_batch_loss_pass": [vp, vp, vp, vp, vp, vp, vp],
         "vm_serve": [P(FitDesc), vp, i32, P(ServeConfig), vp, P(ServeStats)],
+        "vm_serve_synthetic": [P(ServeStream), P(ServeConfig), vp, i32, P(i32), P(ServeStats)],
+        "vm_serve_synth_plan": [P(ServeStream), C.c_int64, P(f64), P(i32), P(f32), P(C.c_uint)],
+        "vm_serve_synth_targets": [P(ServeStream), C.c_int64, i32, vp, i32],
         "vm_canvas_create": [f32, f32, f32, P(vp)],
         "vm_canvas_destroy": [vp],
         "vm_canvas_rasterize": [vp, vp, i32],
@@ -589,8 +609,21 @@ class Batch:
                     targets_per_lane=tpl.value)
 
 
+def _serve_config(hbm_budget_bytes, max_resident, max_iterations, tol, poll_every, precision, max_resident_sites,
+                  initial_pool_bytes):
+    cfg = ServeConfig()
+    cfg.hbm_budget_bytes, cfg.max_resident, cfg.max_iterations = hbm_budget_bytes, max_resident, max_iterations
+    cfg.tol, cfg.poll_every, cfg.precision = tol, poll_every, precision
+    cfg.max_resident_sites, cfg.initial_pool_bytes = max_resident_sites, initial_pool_bytes
+    return cfg
+
+
+def _stats_dict(st):
+    return {name: getattr(st, name) for name, _ in ServeStats._fields_}
+
+
 def serve(requests, arrivals, hbm_budget_bytes=200e9, max_resident=0, max_iterations=300, tol=1e-6, poll_every=10,
-          precision=0, want_sites=False):
+          precision=0, want_sites=False, max_resident_sites=0, initial_pool_bytes=0):
     """Continuous-batching request loop (vm_serve).
 
     requests: list of dicts with keys width, height, sites, targets, lr and optional fit_desc keywords.
@@ -613,9 +646,8 @@ def serve(requests, arrivals, hbm_budget_bytes=200e9, max_resident=0, max_iterat
             b = np.zeros((descs[i].num_sites, 2), np.float32)
             bufs.append(b)
             results[i].final_sites = b.ctypes.data
-    cfg = ServeConfig()
-    cfg.hbm_budget_bytes, cfg.max_resident, cfg.max_iterations = hbm_budget_bytes, max_resident, max_iterations
-    cfg.tol, cfg.poll_every, cfg.precision = tol, poll_every, precision
+    cfg = _serve_config(hbm_budget_bytes, max_resident, max_iterations, tol, poll_every, precision,
+                        max_resident_sites, initial_pool_bytes)
     stats = ServeStats()
     _check(lib().vm_serve(descs, _ptr(arr), n, C.byref(cfg), C.cast(results, C.c_void_p), C.byref(stats)))
     out = []
@@ -626,5 +658,61 @@ def serve(requests, arrivals, hbm_budget_bytes=200e9, max_resident=0, max_iterat
         if want_sites:
             o["sites"] = bufs[i]
         out.append(o)
-    return out, dict(wall_s=stats.wall_s, site_iters=stats.site_iters, completed=stats.completed,
-                     max_resident_seen=stats.max_resident_seen, peak_bytes=stats.peak_bytes)
+    return out, _stats_dict(stats)
+
+
+def serve_stream_desc(rate, duration_s, seed=4, min_sites=512, max_sites=131072, targets_per_site=8, sigma=5e-4,
+                      lr=1e-3, loss_k=16, drain=False):
+    """A vm_serve_stream: the synthetic request stream the library generates lazily."""
+    st = ServeStream()
+    st.rate, st.duration_s, st.seed = rate, duration_s, seed
+    st.min_sites, st.max_sites, st.targets_per_site = min_sites, max_sites, targets_per_site
+    st.sigma, st.lr, st.loss_k, st.drain = sigma, lr, loss_k, int(bool(drain))
+    return st
+
+
+RECORD_DTYPE = np.dtype([("arrival_s", np.float64), ("admit_s", np.float64), ("done_s", np.float64),
+                         ("num_sites", np.int32), ("num_targets", np.int32), ("iterations", np.int32),
+                         ("final_mean_loss", np.float64)], align=True)
+
+
+def serve_synthetic(stream, hbm_budget_bytes=200e9, max_resident=0, max_iterations=300, tol=1e-6, poll_every=10,
+                    precision=0, max_resident_sites=0, initial_pool_bytes=0, max_records=None):
+    """Serves a synthetic stream (vm_serve_synthetic); returns (records array of RECORD_DTYPE, stats dict)."""
+    assert RECORD_DTYPE.itemsize == C.sizeof(ServeRecord)
+    if max_records is None:
+        mean = stream.rate * stream.duration_s
+        max_records = int(mean + 10 * mean ** 0.5 + 64)
+    rec = np.zeros(max_records, RECORD_DTYPE)
+    cfg = _serve_config(hbm_budget_bytes, max_resident, max_iterations, tol, poll_every, precision,
+                        max_resident_sites, initial_pool_bytes)
+    n = C.c_int()
+    stats = ServeStats()
+    _check(lib().vm_serve_synthetic(C.byref(stream), C.byref(cfg), _ptr(rec), max_records, C.byref(n),
+                                    C.byref(stats)))
+    return rec[:n.value].copy(), _stats_dict(stats)
+
+
+def synth_plan(stream, index):
+    """(arrival_s, drawn site count, Poisson-disk radius, sampler seed) of request `index` of a stream."""
+    t, s, r, sd = C.c_double(), C.c_int(), C.c_float(), C.c_uint()
+    _check(lib().vm_serve_synth_plan(C.byref(stream), index, C.byref(t), C.byref(s), C.byref(r), C.byref(sd)))
+    return t.value, s.value, r.value, sd.value
+
+
+def synth_targets(stream, index, num_sites):
+    """Request `index`'s targets for its num_sites live sites."""
+    out = np.zeros((stream.targets_per_site * num_sites, 2), np.float32)
+    _check(lib().vm_serve_synth_targets(C.byref(stream), index, num_sites, _ptr(out), len(out)))
+    return out
+
+
+def synth_request(stream, index):
+    """Request `index` of a synthetic stream as the server generates it: (dict(width, height, sites, targets, lr,
+    loss_k) for vm_serve, arrival time)."""
+    t, _, r, sd = synth_plan(stream, index)
+    grids, _, _ = poisson_batch([1.0], [1.0], [r], [sd])
+    g = grids[0]
+    sites = g[g[:, 0] >= 0]
+    return dict(width=1.0, height=1.0, sites=sites, targets=synth_targets(stream, index, len(sites)), lr=stream.lr,
+                loss_k=stream.loss_k or 16), t

@@ -384,30 +384,76 @@ def run_local_edit(timeout=120):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def run_serving(seconds, rate, seed=4, lo=512, hi=131072, budget=200e9, max_iters=300, tol=1e-6, K=16):
-    """Poisson arrivals at `rate`/s for `seconds`; each request a fit of log-uniform size with 8 targets per site on
-    a random polygon, run to |dL/L| < tol or max_iters; 200 GB budget."""
+def serving_summary(rec, st, rate, seconds, warm=2.0):
+    """One window of the synthetic stream: throughput (site-iterations executed, and those of completed requests),
+    latency percentiles of the requests that arrived after `warm` s and completed, and the backlog at the end."""
+    done = rec["done_s"] >= 0
+    adm = rec["admit_s"] >= 0
+    sel = done & (rec["arrival_s"] >= warm)
+    lat = (rec["done_s"] - rec["arrival_s"])[sel]
+    pct = lambda a, q: float(np.percentile(a, q)) if len(a) else None  # noqa: E731
+    wall = st["wall_s"]
+    return dict(
+        arrival_rate=rate, window_s=seconds, wall_s=wall, arrived=st["arrived"], admitted=st["admitted"],
+        completed=st["completed"], backlog_at_end=st["arrived"] - st["admitted"],
+        in_flight_at_end=st["admitted"] - st["completed"], oldest_wait_at_end_s=st["oldest_wait_s"],
+        p50_latency_s=pct(lat, 50), p99_latency_s=pct(lat, 99), p50_queue_s=pct((rec["admit_s"] - rec["arrival_s"])[sel], 50),
+        requests_per_s=st["completed"] / wall, admissions_per_s=st["admitted"] / wall,
+        site_iters_per_s=st["site_iters_run"] / wall, completed_site_iters_per_s=st["site_iters"] / wall,
+        gpu_busy_frac=st["gpu_busy_s"] / wall, mean_sites=float(rec["num_sites"][adm].mean()) if adm.any() else None,
+        mean_iterations=float(rec["iterations"][done].mean()) if done.any() else None,
+        max_resident=st["max_resident_seen"], peak_resident_GB=st["peak_bytes"] / 1e9, pool_GB=st["pool_bytes"] / 1e9,
+        pool_grows=st["pool_grows"], upload_GB=st["upload_bytes"] / 1e9,
+        host=dict(prep_workers=st["prep_workers"], prep_s=st["prep_busy_s"], prep_wait_s=st["prep_wait_s"],
+                  sampler_s=st["sample_busy_s"], loop_s=st["loop_busy_s"]))
+
+
+def run_serving(seconds, rate, cap, seed=4, budget=200e9, max_iters=300, tol=1e-6):
+    """The serving stream: Poisson arrivals at `rate`/s for `seconds`, each request generated inside the library when
+    it is about to be admitted (site count log-uniform in [512, 131072], Poisson-disk sites, 8 targets per site on a
+    random polygon, sigma 5e-4), run to |dL/L| < tol or max_iters; 200 GB budget, at most `cap` resident sites.  The
+    window ends at `seconds` (no drain)."""
     import vmhip
-    from vmhip import workloads as wl
-    arr = wl.poisson_arrivals(rate, seconds, seed)
-    n = len(arr)
-    sizes = wl.log_uniform_sizes(n, lo, hi, seed + 1)
-    radii = np.array([wl.poisson_radius_for(int(s)) for s in sizes], np.float32)
-    seeds = (4 * (1 << 20) + np.arange(n)).astype(np.uint32)
-    grids, _, _ = vmhip.poisson_batch(np.ones(n), np.ones(n), radii, seeds)
-    reqs = []
-    for i, g in enumerate(grids):
-        sites = wl.live_sites(g)
-        reqs.append(dict(width=1.0, height=1.0, sites=sites, targets=wl.batch_targets(sites, seeds[i]), lr=1e-3,
-                         loss_k=K))
-    res, st = vmhip.serve(reqs, arr, hbm_budget_bytes=budget, max_iterations=max_iters, tol=tol, poll_every=10)
-    lat = np.array([r["latency_s"] for r in res])
-    its = np.array([r["iterations"] for r in res])
-    return dict(requests=n, arrival_rate=rate, window_s=seconds, wall_s=st["wall_s"],
-                p50_latency_s=float(np.percentile(lat, 50)), p99_latency_s=float(np.percentile(lat, 99)),
-                requests_per_s=n / st["wall_s"], site_iters_per_s=st["site_iters"] / st["wall_s"],
-                mean_iterations=float(its.mean()), max_resident=st["max_resident_seen"],
-                peak_resident_GB=st["peak_bytes"] / 1e9, sites_total=int(sum(len(r["sites"]) for r in reqs)))
+    stream = vmhip.serve_stream_desc(rate=rate, duration_s=seconds, seed=seed, drain=False)
+    rec, st = vmhip.serve_synthetic(stream, hbm_budget_bytes=budget, max_iterations=max_iters, tol=tol,
+                                    poll_every=10, max_resident_sites=cap)
+    return serving_summary(rec, st, rate, seconds), rec
+
+
+def serving_sustained(rec, seconds, warm=2.0, settle=4.0):
+    """Bounded p99: every request that arrived before the window's last `settle` s completed, and the p99 latency
+    of those arriving in the later half of [warm, seconds - settle] is within 1.5x of the earlier half's."""
+    end = seconds - settle
+    mid = 0.5 * (warm + end)
+    early = (rec["arrival_s"] >= warm) & (rec["arrival_s"] < mid)
+    late = (rec["arrival_s"] >= mid) & (rec["arrival_s"] < end)
+    if not (np.all(rec["done_s"][early | late] >= 0) and early.any() and late.any()):
+        return False, None, None
+    lat = rec["done_s"] - rec["arrival_s"]
+    p_early, p_late = float(np.percentile(lat[early], 99)), float(np.percentile(lat[late], 99))
+    return p_late <= 1.5 * p_early, p_early, p_late
+
+
+def run_serving_sweep(rates, seconds, cap):
+    out = []
+    for r in rates:
+        summ, rec = run_serving(seconds, r, cap)
+        ok, pe, pl = serving_sustained(rec, seconds)
+        summ.update(sustained=ok, p99_early_s=pe, p99_late_s=pl)
+        out.append(summ)
+    return out
+
+
+def serving_plain(count=16, iters=20, seed=4):
+    """The plain one-fit-at-a-time path on requests of the serving stream (regenerated from it): site-iters/s."""
+    import vmhip
+    stream = vmhip.serve_stream_desc(rate=2000.0, duration_s=60.0, seed=seed)
+    insts = []
+    for i in range(count):
+        req, _ = vmhip.synth_request(stream, i)
+        insts.append((req["sites"], req["targets"]))
+    rate, _ = run_plain(insts, iters)
+    return rate, float(np.mean([len(s) for s, _ in insts]))
 
 
 # ---------------------------------------------------------------------------
@@ -468,10 +514,13 @@ def main():
     ap.add_argument("--plain-instances", type=int, default=0, help="0: all instances")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--serve-seconds", type=float, default=0.25, help="arrival window of the serving run (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-cap", type=float, default=8e6, help="resident-site cap of the serving loop")
+    ap.add_argument("--serve-sweep", default="500,650,750,800,850", help="arrival rates of the sustainable-rate sweep")
+    ap.add_argument("--serve-sweep-seconds", type=float, default=14.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the large/small/precision/local-edit sections")
-    ap.add_argument("--only", choices=["large", "small", "precision", "local_edit", "batch"], default=None,
+    ap.add_argument("--only", choices=["large", "small", "precision", "local_edit", "batch", "serving"], default=None,
                     help="run one section alone and print its JSON (profiling)")
     ap.add_argument("--large-iters", type=int, default=500)
     args = ap.parse_args()
@@ -487,7 +536,7 @@ def main():
     log(f"copy probe: 256 MiB in {copy_ms:.3f} ms ({2 * 256 * 1.048576 / copy_ms:.0f} GB/s read+write)")
     if args.only and args.only != "batch":
         fn = {"large": lambda: run_large(iters=args.large_iters), "small": lambda: run_small()[0],
-              "precision": run_precision, "local_edit": run_local_edit}
+              "precision": run_precision, "local_edit": run_local_edit, "serving": lambda: serving_section(args)}
         print(json.dumps(fn[args.only](), default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
         return
     log(f"generating {args.instances} instances ...")
@@ -590,17 +639,47 @@ def main():
                       wl.ellipses(4194304, count=64, sigma=2e-4, seed=1), 16)
         prec_case = (wl.lattice(256, 1e-7, seed=3), wl.circle(524288, radius=0.3, sigma=1e-4, seed=3), 16)
     if args.serve_seconds > 0:
-        log(f"serving: Poisson arrivals {args.serve_rate}/s for {args.serve_seconds} s ...")
-        sv = run_serving(args.serve_seconds, args.serve_rate)
-        sv["vs_plain_one_at_a_time"] = sv["site_iters_per_s"] / plain_rate
-        out["serving"] = sv
-        log(f"  {sv['requests']} requests, p50 {sv['p50_latency_s']:.3f} s, p99 {sv['p99_latency_s']:.3f} s, "
-            f"{sv['site_iters_per_s']:.3e} site-iters/s")
+        out["serving"] = serving_section(args)
     if not args.no_cpu and small_case is not None:
         log("cpu baselines (oracle) ...")
         out["cpu_baseline"] = cpu_baselines(insts, ks, small_case, large_case, prec_case)
         log(f"  cpu {out['cpu_baseline']['value']:.3e} site-iters/s")
     print(json.dumps(out, default=lambda o: o.item() if hasattr(o, "item") else str(o)), flush=True)
+
+
+def serving_section(args):
+    """The serving stream at --serve-rate for --serve-seconds, the sustainable-rate sweep, the plain one-at-a-time
+    path on the same law, and the demand-vs-capacity arithmetic."""
+    log(f"serving: Poisson arrivals {args.serve_rate}/s for {args.serve_seconds} s, cap {args.serve_cap:.0f} sites ...")
+    sv, _ = run_serving(args.serve_seconds, args.serve_rate, args.serve_cap)
+    log(f"  {sv['arrived']} arrived, {sv['admitted']} admitted, {sv['completed']} completed; "
+        f"{sv['site_iters_per_s']:.3e} site-iters/s, p50 {sv['p50_latency_s']:.3f} s, p99 {sv['p99_latency_s']:.3f} s, "
+        f"backlog {sv['backlog_at_end']}")
+    rates = [float(r) for r in args.serve_sweep.split(",") if r]
+    log(f"serving sweep: {rates} for {args.serve_sweep_seconds} s each ...")
+    sweep = run_serving_sweep(rates, args.serve_sweep_seconds, args.serve_cap)
+    for r in sweep:
+        log(f"  {r['arrival_rate']:.0f}/s: p50 {r['p50_latency_s']}, p99 {r['p99_latency_s']}, sustained {r['sustained']}, "
+            f"backlog {r['backlog_at_end']}, {r['site_iters_per_s']:.3e} site-iters/s")
+    ok = [r["arrival_rate"] for r in sweep if r["sustained"]]
+    log("serving: plain one-fit-at-a-time path on 16 requests of the stream ...")
+    plain, plain_mean_sites = serving_plain()
+    es, eit = sv["mean_sites"], sv["mean_iterations"]
+    per_req = es * eit
+    return dict(
+        config=dict(arrival_rate=args.serve_rate, window_s=args.serve_seconds, sites="log-uniform [512, 131072]",
+                    targets_per_site=8, sigma=5e-4, stop="|dL/L| < 1e-6 or 300 iterations", hbm_budget_GB=200,
+                    max_resident_sites=args.serve_cap, k=16, generation="lazy, inside the library, at admission"),
+        window=sv, sweep=sweep, highest_sustained_rate=max(ok) if ok else None,
+        plain_one_at_a_time=dict(site_iters_per_s=plain, sample="16 requests of the stream, 20 iterations each",
+                                 mean_sites=plain_mean_sites),
+        vs_plain_one_at_a_time=sv["site_iters_per_s"] / plain,
+        demand=dict(mean_sites=es, mean_iterations=eit, site_iters_per_request=per_req,
+                    demand_site_iters_per_s=args.serve_rate * per_req,
+                    capacity_site_iters_per_s=sv["site_iters_per_s"],
+                    capacity_requests_per_s=sv["site_iters_per_s"] / per_req,
+                    plain_capacity_requests_per_s=plain / per_req,
+                    load_factor=args.serve_rate * per_req / sv["site_iters_per_s"]))
 
 
 def measured_traffic():

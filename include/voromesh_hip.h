@@ -274,12 +274,15 @@ VM_API int vm_batch_profile(vm_batch *b, int iterations, float *phase_ms);
 
 /* ---- continuous-batching request loop ---------------------------------------- */
 typedef struct vm_serve_config {
-    double   hbm_budget_bytes; /* device bytes available to resident fit state */
-    int      max_resident;     /* instance slots (0: unlimited) */
-    int      max_iterations;   /* per request (e.g. 300) */
-    float    tol;              /* per request relative loss-change stop */
-    int      poll_every;       /* iterations between completion polls */
+    double   hbm_budget_bytes;   /* device bytes available to resident fit state */
+    int      max_resident;       /* instance slots (0: unlimited) */
+    int      max_iterations;     /* per request (e.g. 300) */
+    float    tol;                /* per request relative loss-change stop */
+    int      poll_every;         /* iterations per launch between state readbacks */
     int      precision;
+    double   max_resident_sites; /* admission stops while resident sites would pass this (0: no cap;
+                                    a request is always admitted onto an empty device) */
+    double   initial_pool_bytes; /* device pools at start (0: 256 MiB); they grow on demand */
 } vm_serve_config;
 
 typedef struct vm_serve_result {
@@ -292,21 +295,81 @@ typedef struct vm_serve_result {
 
 typedef struct vm_serve_stats {
     double wall_s;
-    double site_iters;
+    double site_iters;        /* sum over completed requests of sites x iterations */
+    double site_iters_run;    /* site_iters plus the iterations run so far by requests still resident */
     int    completed;
     int    max_resident_seen;
-    double peak_bytes;
+    double peak_bytes;        /* largest resident charge against the budget */
+    int    arrived;           /* requests that arrived before the loop ended */
+    int    admitted;
+    int    pool_grows;        /* times a device pool was enlarged */
+    double pool_bytes;        /* device pool capacity at the end */
+    double gpu_busy_s;        /* device time of the iteration launches */
+    double oldest_wait_s;     /* at the end: age of the oldest request not yet admitted (0: none) */
+    double prep_busy_s;       /* host time of the preparation workers (sum over workers) */
+    double sample_busy_s;     /* host time of the site-sampling calls (synthetic stream) */
+    double loop_busy_s;       /* host time of the loop's retire / admit / upload work */
+    double prep_wait_s;       /* part of prep_busy_s spent waiting for sampled sites (synthetic stream) */
+    double upload_bytes;      /* bytes sent to the device for admissions and work tables */
+    int    prep_workers;
 } vm_serve_stats;
 
 /* Serves requests[i] arriving at arrival_s[i] seconds after the call starts
  * (non-decreasing).  Continuous batching: a request is prepared once it has
- * arrived, admitted in arrival order as soon as an instance slot and room
- * within hbm_budget_bytes are free, advanced together with every other
- * resident request by one set of launches per iteration, and retired when
- * its relative loss change drops below tol or it reaches max_iterations.
- * All requests share loss_k.  Returns when every request has completed. */
+ * arrived, admitted in arrival order as soon as an instance slot, room within
+ * hbm_budget_bytes and room under max_resident_sites are free, advanced
+ * together with every other resident request by one set of launches per
+ * iteration, and retired when its relative loss change drops below tol or it
+ * reaches max_iterations.  All requests share loss_k.  Returns when every
+ * request has completed. */
 VM_API int vm_serve(const vm_fit_desc *requests, const double *arrival_s, int n, const vm_serve_config *cfg,
                     vm_serve_result *results, vm_serve_stats *stats);
+
+/* A synthetic request stream, generated lazily inside the library: Poisson
+ * arrivals at `rate` per second during [0, duration_s); request i has a site
+ * count drawn log-uniform in [min_sites, max_sites], Poisson-disk sites in the
+ * unit square for it (the batched dart thrower, 30 rounds), and
+ * targets_per_site targets per live site on a random simple polygon (5-40
+ * vertices, sorted uniform angles, radii U[0.15, 0.45] around (0.5, 0.5)),
+ * jittered by N(0, sigma^2).  Every draw comes from a counter-based generator
+ * keyed by (seed, i), so a request is reproducible on its own
+ * (vm_serve_synth_plan, vm_serve_synth_targets). */
+typedef struct vm_serve_stream {
+    double   rate;
+    double   duration_s;
+    uint64_t seed;
+    int      min_sites, max_sites;
+    int      targets_per_site;
+    float    sigma;
+    float    lr;               /* Adam learning rate (milestones 80/150/200/250, gamma 0.5) */
+    int      loss_k;           /* shared by every request (0: 16) */
+    int      drain;            /* 1: serve every arrival to completion; 0: stop at duration_s */
+} vm_serve_stream;
+
+typedef struct vm_serve_record {
+    double arrival_s;
+    double admit_s;           /* -1: not admitted when the loop ended */
+    double done_s;            /* -1: not completed */
+    int    num_sites, num_targets;
+    int    iterations;
+    double final_mean_loss;
+} vm_serve_record;
+
+/* Runs the loop of vm_serve over a synthetic stream.  A request is generated
+ * (sites sampled, targets drawn, laid out) only once it has arrived and is
+ * among the next requests to be admitted, so a backlog costs only its arrival
+ * times.  records[i] describes arrival i; *num_records is the number of
+ * arrivals.  Fails with VM_EINVAL if the stream brings more than max_records
+ * arrivals.  No counterpart in the reference (the serving workload). */
+VM_API int vm_serve_synthetic(const vm_serve_stream *stream, const vm_serve_config *cfg, vm_serve_record *records,
+                              int max_records, int *num_records, vm_serve_stats *stats);
+/* Request `index` of a stream: arrival time, drawn site count, Poisson-disk radius and sampler seed (its sites are
+ * the live cells of vm_poisson_batch for that radius and seed in the unit square). */
+VM_API int vm_serve_synth_plan(const vm_serve_stream *stream, int64_t index, double *arrival_s, int *num_sites,
+                               float *radius, unsigned *sampler_seed);
+/* Request `index`'s targets for its num_sites live sites (targets_per_site * num_sites points). */
+VM_API int vm_serve_synth_targets(const vm_serve_stream *stream, int64_t index, int num_sites, vm_vec2 *targets,
+                                  int capacity);
 
 #ifdef __cplusplus
 }
