@@ -379,6 +379,17 @@ int vm_get_state(vm_handle *h, int *iteration, int *scheduleLength, float *lastB
     });
 }
 
+int vm_grad_saturation(vm_handle *h, int64_t *targets)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(h && targets, "invalid arguments");
+        *targets = 0;
+        if (h->numSlots() == 0) return;
+        h->eng->readStates();
+        *targets = h->spec().state.saturated;
+    });
+}
+
 int vm_graph_stats(vm_handle *h, int64_t *updates, int64_t *instantiations)
 {
     return vmrt::guardedCall([&] {

@@ -454,6 +454,17 @@ int vm_batch_status(vm_batch *b, int id, int *iteration, int *done, double *last
     });
 }
 
+int vm_batch_saturation(vm_batch *b, int64_t *targets)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(b && targets, "invalid arguments");
+        b->eng->readStates();
+        int64_t n = 0;
+        for (const auto &s : b->eng->specs) n += s.state.saturated;
+        *targets = n;
+    });
+}
+
 int vm_copy_probe(int64_t bytes, float *gpuMs)
 {
     return vmrt::guardedCall([&] {

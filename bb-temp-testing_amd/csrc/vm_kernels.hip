@@ -132,10 +132,14 @@ __device__ __forceinline__ double powi(double b, int e)
 // scale = 2^shift, shift in [8, 52].  The (finite) term is clamped to
 // +-2^(56 - shift) in FP32 before the exact scaling, which is the same as
 // clamping c * scale to +-2^56 after it.
+__device__ __forceinline__ float quantLimit(double scale) // 2^(56 - shift)
+{
+    const int g = int(uint64_t(__double_as_longlong(scale)) >> 52) - 1023;
+    return __uint_as_float(uint32_t(127 + 56 - g) << 23);
+}
 __device__ __forceinline__ long long quantize(float c, double scale)
 {
-    const int   g   = int(uint64_t(__double_as_longlong(scale)) >> 52) - 1023;
-    const float lim = __uint_as_float(uint32_t(127 + 56 - g) << 23); // 2^(56 - g)
+    const float lim = quantLimit(scale);
     return __double2ll_rn(double(__builtin_amdgcn_fmed3f(c, -lim, lim)) * scale);
 }
 
@@ -685,7 +689,7 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
 //   dd/dcj = (u - <u, n> n) / l - n / 2,   dd/dc0 = -n - dd/dcj,
 // scaled by 2d and added to the accumulators of cj (dj) and c0 (d0).
 template <class R>
-__device__ __forceinline__ void lossGrad(V2<R> u, V2<R> ev, R l, R d, GradT<R> *dj, GradT<R> *d0, double gscale)
+__device__ __forceinline__ int lossGrad(V2<R> u, V2<R> ev, R l, R d, GradT<R> *dj, GradT<R> *d0, double gscale)
 {
     const R     rl = R(1) / l;
     const V2<R> n{rl * ev.x, rl * ev.y};
@@ -693,10 +697,16 @@ __device__ __forceinline__ void lossGrad(V2<R> u, V2<R> ev, R l, R d, GradT<R> *
     const R     tx = fmaR(-un, n.x, u.x), ty = fmaR(-un, n.y, u.y);
     const V2<R> dcj{fmaR(rl, tx, R(-0.5) * n.x), fmaR(rl, ty, R(-0.5) * n.y)};
     const R     s = R(2) * d;
-    GradOps<R>::add(dj, s * dcj.x, gscale);
-    GradOps<R>::add(dj + 1, s * dcj.y, gscale);
-    GradOps<R>::add(d0, -(s * (n.x + dcj.x)), gscale);
-    GradOps<R>::add(d0 + 1, -(s * (n.y + dcj.y)), gscale);
+    const R     gx = s * dcj.x, gy = s * dcj.y, hx = -(s * (n.x + dcj.x)), hy = -(s * (n.y + dcj.y));
+    GradOps<R>::add(dj, gx, gscale);
+    GradOps<R>::add(dj + 1, gy, gscale);
+    GradOps<R>::add(d0, hx, gscale);
+    GradOps<R>::add(d0 + 1, hy, gscale);
+    // 1 when a component was clamped by the FP32 fixed-point scaling (the
+    // instance's `saturated` count reports it); FP64 sums are not clamped.
+    if constexpr (sizeof(R) == 4)
+        return int(fmax(fmax(fabs(gx), fabs(gy)), fmax(fabs(hx), fabs(hy))) > quantLimit(gscale));
+    return 0;
 }
 
 template <class R> __device__ __forceinline__ R relMargin();
@@ -1030,6 +1040,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
         double   lossSum = 0.0;
         int      used    = 0;
         int      openCnt = 0;
+        int      sat     = 0; // targets whose gradient terms were clamped
 #pragma unroll
         for (int h = 0; h < TPL; ++h) {
             if (__ballot(take[h] || (dbgKnn && lane + 64 * h < wk.count)) == 0) continue;
@@ -1055,7 +1066,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             openCnt += __popcll(__ballot(open));
             const R lossT = tr.used ? tr.d * tr.d : R(0);
             if (tr.used && a.grad)
-                lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.l, tr.d,
+                sat += lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.l, tr.d,
                          &lds.grad[2 * gradSlot(tr.slj, lane & (gcopies - 1), gM)],
                          &lds.grad[2 * gradSlot(tr.sl0, lane & (gcopies - 1), gM)], __builtin_ldexp(1.0, in.gradShift));
             if (dbgKnn && lane + 64 * h < wk.count && !take[h]) {
@@ -1082,6 +1093,10 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             }
         }
 
+        if (__ballot(sat != 0)) { // clamped gradient terms (rare): count them
+            const int ns = waveReduce32<int>(sat, 0, [](int x, int y) { return x + y; });
+            if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long *>(&st->saturated), (unsigned long long)ns);
+        }
         // ---- FP64 loss reduction: one atomic pair per wavefront ----------
         const double ls = waveSumDouble(lossSum);
         const int    uc = waveReduce32<int>(used, 0, [](int x, int y) { return x + y; });
@@ -1309,9 +1324,10 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
         const int      wl   = win ? __builtin_ctzll(win) : -1;
         const R        lossT = win ? best : R(0);
         if (lane == wl) {
-            if (a.grad)
+            if (a.grad &&
                 lossGrad(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, tr.ev, tr.l, tr.d, &a.grad[2 * (so + tr.bj)],
-                         &a.grad[2 * (so + tr.i0)], __builtin_ldexp(1.0, in.gradShift));
+                         &a.grad[2 * (so + tr.i0)], __builtin_ldexp(1.0, in.gradShift)))
+                atomicAdd(reinterpret_cast<unsigned long long *>(&a.state[inst].saturated), 1ull);
             if (a.mode == kModeIterate) {
                 LossPartial *pp = a.partial + int64_t(inst) * kLossPartials + (e & (kLossPartials - 1));
                 unsafeAtomicAdd(&pp->loss, double(lossT));

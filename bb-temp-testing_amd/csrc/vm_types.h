@@ -59,6 +59,7 @@ struct InstState {
     double  prevMean;  // mean loss of the previous step (convergence test)
     double  evalLoss;  // EvaluateLoss accumulators
     int64_t evalCount;
+    int64_t saturated; // FP32: targets whose gradient terms hit the fixed-point clamp (cumulative)
 };
 
 // Per-instance loss accumulators of one step, spread over kLossPartials cache

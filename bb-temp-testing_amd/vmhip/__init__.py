@@ -94,6 +94,7 @@ EXPORTS = [
     "vm_create", "vm_destroy", "vm_set_targets", "vm_update_targets", "vm_set_occupancy", "vm_update_occupancy",
     "vm_set_generators", "vm_update_generators", "vm_begin_schedule", "vm_step", "vm_get_state",
     "vm_evaluate_loss", "vm_extract", "vm_get_generators", "vm_knn", "vm_loss_pass", "vm_graph_stats",
+    "vm_grad_saturation", "vm_batch_saturation",
     "vm_sampler_create", "vm_sampler_destroy", "vm_sampler_resample", "vm_sampler_verify", "vm_sampler_dims",
     "vm_sampler_cells", "vm_sampler_load_cells", "vm_poisson_grid_dims", "vm_poisson_batch",
     "vm_poisson_batch_noalloc",
@@ -167,6 +168,8 @@ def lib():
         "vm_batch_extract": [vp, i32, vp, i32, i32, f32, P(Mesh)],
         "vm_batch_choice": [vp, i32, P(i32), P(f32), P(i32), P(f32), P(i32)],
         "vm_batch_profile": [vp, i32, vp],
+        "vm_batch_saturation": [vp, P(C.c_int64)],
+        "vm_grad_saturation": [vp, P(C.c_int64)],
         "vm_batch_loss_stats": [vp, vp],
         "vm_batch_loss_pass": [vp, vp, vp, vp, vp, vp, vp],
         "vm_serve": [P(FitDesc), vp, i32, P(ServeConfig), vp, P(ServeStats)],
@@ -325,6 +328,12 @@ class Optimizer:
         u, i = C.c_int64(), C.c_int64()
         _check(lib().vm_graph_stats(self._h, C.byref(u), C.byref(i)))
         return dict(updates=u.value, instantiations=i.value)
+
+    def grad_saturation(self):
+        """Targets whose FP32 fixed-point gradient terms were clamped so far (vm_grad_saturation)."""
+        n = C.c_int64()
+        _check(lib().vm_grad_saturation(self._h, C.byref(n)))
+        return n.value
 
     def evaluate_loss(self) -> float:
         v = C.c_double()
@@ -560,6 +569,12 @@ class Batch:
         out = np.zeros(5, np.float32)
         _check(lib().vm_batch_profile(self._b, iterations, _ptr(out)))
         return out
+
+    def saturation(self):
+        """Clamped FP32 gradient terms summed over the batch's fits (vm_batch_saturation)."""
+        n = C.c_int64()
+        _check(lib().vm_batch_saturation(self._b, C.byref(n)))
+        return n.value
 
     def sites(self, i):
         out = np.zeros((self.sizes[i], 2), np.float32)

@@ -291,6 +291,7 @@ def run_large(iters=500, K=16, n_sites=262144, n_targets=4194304):
     mesh = b.extract(0, masks, (tx, ty), vox)
     extract_wall = (time.perf_counter() - t0) * 1e3
     fitted = b.sites(0)
+    saturated = b.saturation()
     b.close()
     # gate at the fitted sites (a slot grid of the live sites)
     gate = single_gate(fitted, len(fitted), 1, 1.0 / len(fitted), r, tg, K)
@@ -302,7 +303,7 @@ def run_large(iters=500, K=16, n_sites=262144, n_targets=4194304):
                 iteration_frac=ib / (ms / iters / 1e3) / 1e9 / HBM_PEAK_GBS, grid_cell=ch["grid_cell"],
                 choice=ch, loss_paths_start=stats0, loss_paths_end=stats1,
                 loss_kernel_algorithmic_bytes=lb, extract=dict(wall_ms=extract_wall, **mesh_summary(mesh)),
-                fp32_vs_fp64=gate)
+                fp32_vs_fp64=gate, grad_clamped_targets=saturated)
 
 
 def run_small(iters=200, K=16):
@@ -329,7 +330,8 @@ def run_small(iters=200, K=16):
     fitted = opt.generators()
     out = dict(sites=len(sites), targets=len(tg), iterations=iters, gpu_ms=gpu_ms,
                gpu_site_iters_per_s=len(sites) * iters / (gpu_ms / 1e3), wall_s=wall,
-               final_mean_loss=opt.evaluate_loss(), extract=mesh_summary(mesh))
+               final_mean_loss=opt.evaluate_loss(), extract=mesh_summary(mesh),
+               grad_clamped_targets=opt.grad_saturation())
     opt.close()
     out["fp32_vs_fp64"] = single_gate(fitted, gw, gh, cs, r, tg, K, grid_cell=cell)
     return out, (sites, tg, r, gw, gh, cs, K, iters, cell)
@@ -354,10 +356,10 @@ def run_precision(iters=5, K=16):
         lp = opt.loss_pass(len(tg))
         mesh0 = opt.extract()
         opt.step(iters)
-        runs[prec] = (lp, mesh0, opt.generators().astype(np.float64))
+        runs[prec] = (lp, mesh0, opt.generators().astype(np.float64), opt.grad_saturation())
         opt.close()
-    lp64, m64, p64 = runs[1]
-    lp32, mesh0, pos = runs[0]
+    lp64, m64, p64, _ = runs[1]
+    lp32, mesh0, pos, saturated = runs[0]
     pe = np.abs(pos - p64)
     mism = 0
     for i in range(len(sites)):
@@ -367,7 +369,7 @@ def run_precision(iters=5, K=16):
     gate = gradient_gate([sites], [tg], lp32, lp64)
     gate.update(pos_max_abs_err=float(pe.max()), pos_mean_abs_err=float(pe.mean()), topology_mismatches=mism)
     return dict(fp32_vs_fp64=gate, extract_fp32=mesh_summary(mesh0), extract_fp64=mesh_summary(m64),
-                sites=len(sites), targets=len(tg), iterations=iters)
+                sites=len(sites), targets=len(tg), iterations=iters, grad_clamped_targets=saturated)
 
 
 def run_local_edit(timeout=120):
@@ -558,6 +560,7 @@ def main():
     log(f"  {ms_per_step:.3f} ms/iteration, {value:.3e} site-iters/s; phases ms {np.round(phase, 4).tolist()}")
     log("  gradient gate (FP32 vs FP64 with the FP32 choices) ...")
     gate = batch_gate(b, insts)
+    gate["grad_clamped_targets"] = b.saturation()
     log(f"  {gate}")
     meshes = batch_meshes(b, insts, seeds)
     b.close()

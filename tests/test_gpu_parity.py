@@ -679,3 +679,62 @@ def test_boundary_box_sides_and_empty_occupancy():
             assert m["inside"].all() and m["watertight"] and m["loops"] == 1 and len(m["edges"]) > 0
         else:
             assert not m["inside"].any() and len(m["edges"]) == 0
+
+
+def test_loss_pass_matches_committed_fixture():
+    """The GPU loss pass on tests/golden/config0_small.npz (r = 0.05, 1,024 star targets, k = 16): k-NN lists and
+    per-target terms bit-exact against the stored oracle values, the FP32 gradient within 1e-5 (norm-wise)."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "config0_small.npz"))
+    grid, targets = z["grid"], z["targets"]
+    gw, gh, cs = O.poisson_dims(W, H, 0.05)
+    opt = _optimizer(0.05, 16)
+    opt.set_targets(targets)
+    opt.set_generators(grid, gw, gh, cs)
+    d = opt.loss_pass(len(targets))
+    opt.close()
+    slot = _slots(grid)
+    assert np.array_equal(d["knn"], _map(z["knn"], slot))
+    assert np.array_equal(d["loss_terms"], z["loss_t"].astype(np.float64))
+    g = d["grad"][slot]
+    assert np.linalg.norm(g - z["grad"]) / np.linalg.norm(z["grad"]) < 1e-5
+
+
+@pytest.mark.parametrize("gap", [1e-3, 1e-6])
+def test_gradient_clamp_counted(gap):
+    """FP32 gradient sums are fixed point, each term clamped to 2^(56 - shift) first (shift 52 for 64 targets in
+    the unit square, so |term| <= 16).  A site pair `gap` apart with 64 targets on a ring of radius 0.03 around it:
+    at 1e-3 no term reaches the clamp; at 1e-6 the pair's terms (about 2 d |u_perp| / gap) pass it.  The reported
+    count equals the targets whose FP32 terms (the oracle's per-target terms, which the kernel computes bit for bit)
+    pass the clamp, the clamped sums equal the oracle's, and against the unclamped FP64 gradient only the pair's
+    two sites differ (by more than 1e-5 of the largest other gradient)."""
+    r, K = 0.1, 16
+    grid = O.poisson(W, H, r, 30, 5)
+    gw, gh, cs = O.poisson_dims(W, H, r)
+    so = _slots(grid)
+    g = grid.copy()
+    a, b = 10, 11
+    g[so[b]] = g[so[a]] + np.float32([gap, 0.0])
+    ang = np.linspace(0.0, 2 * np.pi, 64, endpoint=False) + 0.1
+    targets = (g[so[a]] + 0.03 * np.stack([np.cos(ang), np.sin(ang)], 1)).astype(np.float32)
+    sites = wl.live_sites(g)
+    opt = _optimizer(r, K)
+    opt.set_targets(targets)
+    opt.set_generators(g, gw, gh, cs)
+    d = opt.loss_pass(len(targets))
+    sat = opt.grad_saturation()
+    opt.close()
+    ref = O.loss_grad(sites, targets, K, W, H, 0.1, per_target=True)
+    expect = int((np.abs(ref["term_grad"]).max(axis=1) > 2.0 ** (56 - 52)).sum())
+    print(f"case: gap {gap}: {sat} clamped targets (oracle terms: {expect})", flush=True)
+    assert sat == expect
+    assert (sat > 0) == (gap < 1e-5)
+    grad = d["grad"][so]
+    assert np.array_equal(grad, ref["grad"])
+    r64 = O.loss_grad(sites.astype(np.float64), targets.astype(np.float64), K, W, H, 0.1, real=1)
+    err = np.abs(grad - r64["grad"]).max(axis=1)
+    others = np.setdiff1d(np.arange(len(sites)), [a, b])
+    tol = 1e-5 * np.abs(r64["grad"][others]).max()
+    assert err[others].max() <= tol
+    if sat:  # the clamped terms (16 against hundreds) largely cancel over the ring, but not to within tol
+        assert err[[a, b]].min() > tol

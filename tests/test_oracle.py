@@ -365,3 +365,16 @@ def test_clip_lattice_fp32_equals_fp64():
     assert np.array_equal(a["count"], b["count"]) and np.array_equal(a["labels"], b["labels"])
     assert np.array_equal(a["verts"], b["verts"])
     assert a["overflow"] == 0
+
+
+def test_oracle_reproduces_committed_config0_fixture():
+    """tests/golden/config0_small.npz (Poisson sites r = 0.05 seed 0, 1,024 star targets, k = 16; written by
+    make_golden.py) is reproduced bit for bit: sampler grid, brute-force k-NN lists, per-target terms, gradient."""
+    z = np.load(os.path.join(GOLDEN, "config0_small.npz"))
+    assert np.array_equal(O.poisson(1.0, 1.0, 0.05, 30, 0), z["grid"])
+    sites, targets = z["sites"], z["targets"]
+    idx, d2 = O.knn(sites, targets, 16, 1.0, 1.0, 0.1, brute=False)
+    assert np.array_equal(idx, z["knn"]) and np.array_equal(d2, z["knn_d2"])
+    r = O.loss_grad(sites, targets, 16, 1.0, 1.0, 0.1, per_target=True)
+    for key in ("c0", "bestj", "loss_t", "grad"):
+        assert np.array_equal(r[key], z[key]), key
