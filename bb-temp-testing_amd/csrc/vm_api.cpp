@@ -40,6 +40,16 @@ hipStream_t makeStream()
 // ===========================================================================
 // Plain optimizer (voromesh::VoroOptimizer)
 // ===========================================================================
+// Sites are kept in the reference's slot layout (voro_optimizer.cu:613-650):
+// instance 0's site i is generator slot i, empty slots (x < 0) stay in place
+// and are never binned or moved.  Targets are laid out sorted by hash cell.
+// While the engine is committed, local edits patch the device state without a
+// re-layout: UpdateGenerators copies the slot rectangle and bins again,
+// BeginSchedule clears the rectangle's moments and rewrites the instance
+// record, and UpdateTargets retires the dirty slots' sorted entries and lays
+// the new ones out as chunks of their own in room kept past the targets.  The
+// host work of an edit scales with the rectangle and the dirty slots; a full
+// re-layout happens when that room runs out or the slot arrays are replaced.
 struct vm_handle {
     float                          W = 1, H = 1, spacing = 1;
     vm_params                      prm{};
@@ -47,11 +57,17 @@ struct vm_handle {
     std::unique_ptr<vmrt::Engine>  eng;
     int                            slotsW = 0, slotsH = 0;
     float                          slotCellSize = 0;
-    std::vector<vm_vec2>           slotPos;        // generator slots (host mirror)
-    std::vector<vm_vec2>           slotM, slotV;   // Adam moments per slot
-    std::vector<int>               slotOfSite, siteOfSlot;
-    std::vector<vm_vec2>           targetSlots;
+    // Host mirrors of the slot arrays; stale while the device is newer.
+    std::vector<vm_vec2>           slotPos, slotM, slotV;
+    std::vector<vm_vec2>           targetSlots; // always current
     int                            targetCapacity = 0;
+    // Target slot -> pooled sorted position (-1: none), valid while committed.
+    std::vector<int64_t>           tposOf;
+    std::vector<uint32_t>          seen;
+    uint32_t                       seenGen = 0;
+    int                            liveTargets = 0;
+    bool                           patched  = false; // targets edited since the last layout
+    bool                           fullEdits = false; // every edit re-lays out (VM_EDIT_FULL=1)
     std::vector<uint64_t>          masks;
     DBuf<uint64_t>                 dMasks;
     HBuf<uint64_t>                 hMaskRows; // pinned staging of occupancy rows
@@ -76,45 +92,48 @@ struct vm_handle {
     vmrt::InstSpec &spec() { return eng->specs[0]; }
     int             numSlots() const { return slotsW * slotsH; }
 
-    // Device -> slot mirrors.
+    // Device -> host mirrors (committed engine with newer device state).
     void pullSlots()
     {
-        if (!eng->deviceNewer) return;
+        if (eng->dirty || !eng->deviceNewer) return;
         eng->pull();
         const vmrt::InstSpec &s = spec();
-        for (size_t i = 0; i < slotOfSite.size(); ++i) {
-            int sl       = slotOfSite[i];
-            slotPos[sl]  = vm_vec2{float(s.pos[2 * i]), float(s.pos[2 * i + 1])};
-            if (!s.m.empty()) slotM[sl] = vm_vec2{float(s.m[2 * i]), float(s.m[2 * i + 1])};
-            if (!s.v.empty()) slotV[sl] = vm_vec2{float(s.v[2 * i]), float(s.v[2 * i + 1])};
+        for (int i = 0; i < numSlots(); ++i) {
+            slotPos[size_t(i)] = vm_vec2{float(s.pos[2 * size_t(i)]), float(s.pos[2 * size_t(i) + 1])};
+            slotM[size_t(i)]   = vm_vec2{float(s.m[2 * size_t(i)]), float(s.m[2 * size_t(i) + 1])};
+            slotV[size_t(i)]   = vm_vec2{float(s.v[2 * size_t(i)]), float(s.v[2 * size_t(i) + 1])};
         }
     }
 
-    // Slot mirrors -> compact live sites of the instance.
-    void compactSites()
+    // The next use re-lays out everything from the host mirrors.
+    void markDirty()
+    {
+        if (!eng->dirty) pullSlots();
+        eng->dirty = true;
+    }
+
+    // Host mirrors -> instance spec (slot layout), before a full commit.
+    void buildSites()
     {
         vmrt::InstSpec &s = spec();
-        slotOfSite.clear();
-        siteOfSlot.assign(numSlots(), -1);
-        s.pos.clear(), s.m.clear(), s.v.clear();
-        for (int sl = 0; sl < numSlots(); ++sl) {
-            if (!liveX(slotPos[sl].x)) continue;
-            siteOfSlot[sl] = int(slotOfSite.size());
-            slotOfSite.push_back(sl);
-            s.pos.push_back(slotPos[sl].x), s.pos.push_back(slotPos[sl].y);
-            s.m.push_back(slotM[sl].x), s.m.push_back(slotM[sl].y);
-            s.v.push_back(slotV[sl].x), s.v.push_back(slotV[sl].y);
+        const size_t    n = size_t(numSlots());
+        s.pos.resize(2 * n), s.m.resize(2 * n), s.v.resize(2 * n);
+        int live = 0;
+        for (size_t i = 0; i < n; ++i) {
+            s.pos[2 * i] = slotPos[i].x, s.pos[2 * i + 1] = slotPos[i].y;
+            s.m[2 * i] = slotM[i].x, s.m[2 * i + 1] = slotM[i].y;
+            s.v[2 * i] = slotV[i].x, s.v[2 * i + 1] = slotV[i].y;
+            live += liveX(slotPos[i].x) ? 1 : 0;
         }
-        {
-            float cell, ce;
-            int   rs, tpl;
-            vmrt::tunedChoice(std::max(int(slotOfSite.size()), 1), W, H, s.K, &cell, &rs, &ce, &tpl);
-            s.cell     = prm.grid_cell > 0.0f ? prm.grid_cell : cell;
-            s.stageR   = prm.stage_rings > 0 ? prm.stage_rings : rs;
-            s.chunkExt = ce;
-            s.tpl      = tpl;
-        }
-        eng->dirty = true;
+        s.liveS = live;
+        float cell, ce;
+        int   rs, tpl;
+        vmrt::tunedChoice(std::max(live, 1), W, H, s.K, &cell, &rs, &ce, &tpl);
+        s.cell     = prm.grid_cell > 0.0f ? prm.grid_cell : cell;
+        s.stageR   = prm.stage_rings > 0 ? prm.stage_rings : rs;
+        s.chunkExt = ce;
+        s.tpl      = tpl;
+        s.frozen.clear();
     }
 
     void compactTargets()
@@ -126,14 +145,28 @@ struct vm_handle {
             s.tgt.push_back(targetSlots[t].x), s.tgt.push_back(targetSlots[t].y);
             s.tslot.push_back(t);
         }
-        eng->dirty = true;
+        liveTargets = int(s.tslot.size());
     }
 
-    void beginSchedule(int iterations, int cx0, int cy0, int cx1, int cy1)
+    // Commits a pending full layout (with room for local target edits) and
+    // maps target slots to their sorted positions.
+    void ensureCommitted()
     {
-        cx0 = std::max(cx0, 0), cy0 = std::max(cy0, 0);
-        cx1 = std::min(cx1, slotsW), cy1 = std::min(cy1, slotsH);
-        pullSlots();
+        if (!eng->dirty) return;
+        buildSites();
+        compactTargets();
+        const int64_t T = int64_t(spec().tslot.size());
+        eng->tailTargets = std::max<int64_t>(16384, T / 4);
+        eng->tailWork    = int(eng->tailTargets / 16 + 4096);
+        eng->commit();
+        tposOf.assign(size_t(std::max(targetCapacity, int(targetSlots.size()))), -1);
+        const std::vector<int> &perm = eng->permOf(0);
+        for (size_t t = 0; t < perm.size(); ++t) tposOf[size_t(spec().tslot[size_t(perm[t])])] = int64_t(t);
+        patched = false;
+    }
+
+    void setScheduleFields(int iterations)
+    {
         vmrt::InstSpec &s = spec();
         s.lr    = double(prm.lr_per_spacing * spacing);
         s.gamma = prm.gamma, s.beta1 = 0.9, s.beta2 = 0.999, s.eps = 1e-8;
@@ -143,27 +176,35 @@ struct vm_handle {
                                   : prm.milestones[k];
         float margin = 2.0f * spacing;
         s.useBox     = 1;
-        s.box[0] = float(cx0) * slotCellSize - margin, s.box[1] = float(cy0) * slotCellSize - margin;
-        s.box[2] = float(cx1) * slotCellSize + margin, s.box[3] = float(cy1) * slotCellSize + margin;
+        s.box[0] = float(rect[0]) * slotCellSize - margin, s.box[1] = float(rect[1]) * slotCellSize - margin;
+        s.box[2] = float(rect[2]) * slotCellSize + margin, s.box[3] = float(rect[3]) * slotCellSize + margin;
+        const bool all = rect[0] <= 0 && rect[1] <= 0 && rect[2] >= slotsW && rect[3] >= slotsH;
+        s.rectW = all ? 0 : slotsW;
+        for (int k = 0; k < 4; ++k) s.rect[k] = rect[k];
+    }
+
+    void beginSchedule(int iterations, int cx0, int cy0, int cx1, int cy1)
+    {
+        cx0 = std::max(cx0, 0), cy0 = std::max(cy0, 0);
+        cx1 = std::min(cx1, slotsW), cy1 = std::min(cy1, slotsH);
         rect[0] = cx0, rect[1] = cy0, rect[2] = cx1, rect[3] = cy1;
-        bool all = cx0 <= 0 && cy0 <= 0 && cx1 >= slotsW && cy1 >= slotsH;
-        if (cx0 < cx1 && cy0 < cy1)
-            for (int y = cy0; y < cy1; ++y)
-                for (int x = cx0; x < cx1; ++x) slotM[y * slotsW + x] = vm_vec2{0, 0}, slotV[y * slotsW + x] = vm_vec2{0, 0};
-        compactSites();
-        s.frozen.clear();
-        if (!all) {
-            s.frozen.assign(slotOfSite.size(), 1);
-            for (size_t i = 0; i < slotOfSite.size(); ++i) {
-                int sl = slotOfSite[i], x = sl % slotsW, y = sl / slotsW;
-                if (x >= cx0 && x < cx1 && y >= cy0 && y < cy1) s.frozen[i] = 0;
-            }
+        setScheduleFields(iterations);
+        if (eng->dirty || fullEdits) {
+            markDirty();
+            if (cx0 < cx1 && cy0 < cy1)
+                for (int y = cy0; y < cy1; ++y)
+                    for (int x = cx0; x < cx1; ++x)
+                        slotM[size_t(y) * slotsW + x] = vm_vec2{0, 0}, slotV[size_t(y) * slotsW + x] = vm_vec2{0, 0};
+            spec().state = vmk::InstState{};
+        } else {
+            // Moments of the rectangle cleared on the device; the record and a
+            // zero step state replace the instance's.
+            eng->uploadSiteRect(slotsW, cx0, cy0, cx1, cy1, nullptr, true);
+            eng->updateRecord(0, liveTargets, true);
         }
-        s.state        = vmk::InstState{};
         scheduleLength = iterations;
         iteration      = 0;
         lastBatchLoss  = 0.0f;
-        eng->dirty     = true;
     }
 };
 
@@ -206,6 +247,8 @@ int vm_create(float width, float height, float spacing, const vm_params *params,
         if (params) h->prm = *params;
         else vm_default_params(&h->prm);
         VM_REQUIRE(vmk::lossKSupported(h->prm.loss_k), "unsupported loss_k");
+        const char *full = std::getenv("VM_EDIT_FULL");
+        h->fullEdits     = full && full[0] == '1';
         h->stream = makeStream();
         h->eng    = vmrt::Engine::create(h->prm.precision, h->stream);
         vmrt::InstSpec s;
@@ -234,10 +277,9 @@ int vm_set_targets(vm_handle *h, const vm_vec2 *slots, int n)
 {
     return vmrt::guardedCall([&] {
         VM_REQUIRE(h && n >= 0 && (slots || n == 0), "invalid arguments");
-        h->pullSlots();
+        h->markDirty();
         h->targetCapacity = std::max(n, 1);
         h->targetSlots.assign(slots, slots + n);
-        h->compactTargets();
     });
 }
 
@@ -247,23 +289,57 @@ int vm_update_targets(vm_handle *h, const int32_t *dirty, int numDirty, const vm
         VM_REQUIRE(h && numDirty >= 0 && numSlots >= 0, "invalid arguments");
         if (numDirty == 0) return;
         VM_REQUIRE(dirty && slots, "null arrays");
-        h->pullSlots();
         if (numSlots > h->targetCapacity) {
             // Grow with slack and re-upload everything (voro_optimizer.cu:566-571).
+            h->markDirty();
             size_t cap = std::max(size_t(numSlots) * 3 / 2, size_t(numSlots) + 64);
             h->targetSlots.assign(slots, slots + numSlots);
             h->targetSlots.resize(cap, vm_vec2{-1.0f, -1.0f});
             h->targetCapacity = int(cap);
-        } else {
-            if (int(h->targetSlots.size()) < h->targetCapacity)
-                h->targetSlots.resize(h->targetCapacity, vm_vec2{NAN, NAN});
-            for (int k = 0; k < numDirty; ++k) {
-                int d = dirty[k];
-                VM_REQUIRE(d >= 0 && d < numSlots && d < h->targetCapacity, "dirty slot out of range");
-                h->targetSlots[d] = slots[d];
-            }
+            return;
         }
-        h->compactTargets();
+        if (int(h->targetSlots.size()) < h->targetCapacity)
+            h->targetSlots.resize(size_t(h->targetCapacity), vm_vec2{NAN, NAN});
+        for (int k = 0; k < numDirty; ++k) {
+            const int d = dirty[k];
+            VM_REQUIRE(d >= 0 && d < numSlots && d < h->targetCapacity, "dirty slot out of range");
+        }
+        if (h->eng->dirty || h->fullEdits || h->tposOf.size() < h->targetSlots.size()) {
+            h->markDirty();
+            for (int k = 0; k < numDirty; ++k) h->targetSlots[size_t(dirty[k])] = slots[dirty[k]];
+            return;
+        }
+        // Local edit: the dirty slots' sorted entries are retired and their new
+        // targets laid out as chunks of their own past the others.
+        if (h->seen.size() < h->targetSlots.size()) h->seen.resize(h->targetSlots.size(), 0);
+        ++h->seenGen;
+        std::vector<int64_t> kill;
+        std::vector<double>  addXY;
+        std::vector<int>     addSlot;
+        for (int k = 0; k < numDirty; ++k) {
+            const int d = dirty[k];
+            if (h->seen[size_t(d)] == h->seenGen) continue;
+            h->seen[size_t(d)] = h->seenGen;
+            if (h->tposOf[size_t(d)] >= 0) kill.push_back(h->tposOf[size_t(d)]);
+            if (liveX(slots[d].x)) addXY.push_back(slots[d].x), addXY.push_back(slots[d].y), addSlot.push_back(d);
+        }
+        std::vector<int64_t> pos;
+        if (!h->eng->appendTargets(addXY, addSlot, pos)) { // no room left: re-lay out at the next use
+            h->markDirty();
+            for (int k = 0; k < numDirty; ++k) h->targetSlots[size_t(dirty[k])] = slots[dirty[k]];
+            return;
+        }
+        h->eng->killTargets(kill);
+        h->liveTargets -= int(kill.size());
+        for (int k = 0; k < numDirty; ++k) {
+            const int d = dirty[k];
+            h->targetSlots[size_t(d)] = slots[d];
+            h->tposOf[size_t(d)]      = -1;
+        }
+        for (size_t k = 0; k < addSlot.size(); ++k) h->tposOf[size_t(addSlot[k])] = pos[k];
+        h->liveTargets += int(addSlot.size());
+        h->patched = true;
+        h->eng->updateRecord(0, h->liveTargets, false); // the gradient shift follows the live target count
     });
 }
 
@@ -308,6 +384,8 @@ int vm_set_generators(vm_handle *h, const vm_vec2 *slots, int slotsW, int slotsH
         VM_REQUIRE(h && slotsW >= 0 && slotsH >= 0, "invalid arguments");
         size_t n = size_t(slotsW) * slotsH;
         VM_REQUIRE(slots || n == 0, "null slots");
+        VM_REQUIRE(n < size_t(vmk::kMaxSitesPerInst), "too many generator slots");
+        h->eng->dirty = true; // replaced wholesale: nothing to pull
         h->slotsW = slotsW, h->slotsH = slotsH, h->slotCellSize = slotCellSize;
         h->slotPos.assign(slots, slots + n);
         h->slotM.assign(n, vm_vec2{0, 0});
@@ -326,21 +404,17 @@ int vm_update_generators(vm_handle *h, const vm_vec2 *slots, int cx0, int cy0, i
         cx1 = std::min(cx1, h->slotsW), cy1 = std::min(cy1, h->slotsH);
         if (cx0 >= cx1 || cy0 >= cy1) return;
         VM_REQUIRE(slots, "null slots");
-        h->pullSlots();
-        for (int y = cy0; y < cy1; ++y)
-            for (int x = cx0; x < cx1; ++x) h->slotPos[size_t(y) * h->slotsW + x] = slots[size_t(y) * h->slotsW + x];
-        h->compactSites();
-        // Frozen mask follows the current schedule rectangle.
-        vmrt::InstSpec &s = h->spec();
-        bool all = h->rect[0] <= 0 && h->rect[1] <= 0 && h->rect[2] >= h->slotsW && h->rect[3] >= h->slotsH;
-        s.frozen.clear();
-        if (!all) {
-            s.frozen.assign(h->slotOfSite.size(), 1);
-            for (size_t i = 0; i < h->slotOfSite.size(); ++i) {
-                int sl = h->slotOfSite[i], x = sl % h->slotsW, y = sl / h->slotsW;
-                if (x >= h->rect[0] && x < h->rect[2] && y >= h->rect[1] && y < h->rect[3]) s.frozen[i] = 0;
-            }
+        if (h->eng->dirty || h->fullEdits) {
+            h->markDirty();
+            for (int y = cy0; y < cy1; ++y)
+                for (int x = cx0; x < cx1; ++x)
+                    h->slotPos[size_t(y) * h->slotsW + x] = slots[size_t(y) * h->slotsW + x];
+            return;
         }
+        // Local edit: the rectangle's slots go up as one 2D copy; the sites are
+        // binned again (empty slots stay out of the grid).
+        h->eng->uploadSiteRect(h->slotsW, cx0, cy0, cx1, cy1, reinterpret_cast<const float *>(slots), false);
+        h->eng->rebin();
     });
 }
 
@@ -358,6 +432,7 @@ int vm_step(vm_handle *h, int iterations, float *gpuMs)
         VM_REQUIRE(h, "null handle");
         float ms = 0.0f;
         if (iterations > 0 && h->numSlots() > 0 && h->targetCapacity > 0) {
+            h->ensureCommitted();
             ms = h->eng->step(iterations);
             h->eng->readStates();
             const vmk::InstState &st = h->spec().state;
@@ -384,7 +459,7 @@ int vm_grad_saturation(vm_handle *h, int64_t *targets)
     return vmrt::guardedCall([&] {
         VM_REQUIRE(h && targets, "invalid arguments");
         *targets = 0;
-        if (h->numSlots() == 0) return;
+        if (h->numSlots() == 0 || h->eng->dirty) return;
         h->eng->readStates();
         *targets = h->spec().state.saturated;
     });
@@ -405,6 +480,7 @@ int vm_evaluate_loss(vm_handle *h, double *mean)
         VM_REQUIRE(h && mean, "invalid arguments");
         *mean = 0.0;
         if (h->numSlots() == 0 || h->targetCapacity == 0) return;
+        h->ensureCommitted();
         int64_t cnt = 0;
         double  sum = h->eng->evaluate(0, &cnt);
         *mean       = cnt > 0 ? sum / double(cnt) : 0.0;
@@ -432,33 +508,21 @@ int vm_extract(vm_handle *h, vm_mesh *out)
             out->cell_start = h->mStart.data();
             return;
         }
+        h->ensureCommitted();
         vmrt::ClipOut c;
         h->eng->clip(0, h->masks.empty() ? nullptr : h->dMasks.p, h->tilesX, h->tilesY, h->voxel, c);
         auto t0 = std::chrono::steady_clock::now();
         h->pullSlots();
         h->mGen = h->slotPos;
-        h->mInside.assign(n, 0);
-        std::vector<int> cnt(n, 0);
-        for (size_t i = 0; i < h->slotOfSite.size(); ++i) {
-            int sl          = h->slotOfSite[i];
-            cnt[sl]         = c.count[i];
-            h->mInside[sl]  = c.inside[i];
-        }
-        h->mStart.assign(n + 1, 0);
-        for (int s = 0; s < n; ++s) h->mStart[s + 1] = h->mStart[s] + cnt[s];
-        h->mVerts.resize(h->mStart[n]);
-        h->mLabels.resize(h->mStart[n]);
-        for (size_t i = 0; i < h->slotOfSite.size(); ++i) {
-            int sl = h->slotOfSite[i], o = h->mStart[sl];
-            for (int k = 0; k < c.count[i]; ++k) {
-                size_t src       = size_t(i) * vmk::kMaxCellVerts + k;
-                h->mVerts[o + k] = vm_vec2{c.verts[2 * src], c.verts[2 * src + 1]};
-                int l            = c.labels[src];
-                h->mLabels[o + k] = l >= 0 ? h->slotOfSite[l] : l;
-            }
-        }
-        // The boundary was assembled on the device in site space; live sites are
-        // in slot order, so its vertex numbering and edges are those of slot space.
+        // Site i is slot i (empty slots have no cell).
+        h->mInside.assign(c.inside, c.inside + n);
+        h->mStart.assign(c.start, c.start + n + 1);
+        const size_t nv = size_t(h->mStart[n]);
+        h->mVerts.resize(nv);
+        std::memcpy(h->mVerts.data(), c.verts, nv * sizeof(vm_vec2));
+        h->mLabels.assign(c.labels, c.labels + nv);
+        // The boundary was assembled on the device in site space, which is slot
+        // space: its vertex numbering and edges are those of the slots.
         const vmrt::Boundary &bd = c.bound;
         const int V = int(bd.vertices.size() / 2), E = int(bd.edges.size() / 2);
         h->mVertices.resize(size_t(V));
@@ -492,11 +556,9 @@ int vm_knn(vm_handle *h, const vm_vec2 *queries, int nq, int k, int32_t *idx, fl
         VM_REQUIRE(k >= 1 && k <= 32, "k must be in [1, 32]");
         std::vector<double> q(2 * size_t(nq)), dd(size_t(nq) * k);
         for (int i = 0; i < nq; ++i) q[2 * i] = queries[i].x, q[2 * i + 1] = queries[i].y;
-        h->eng->knnQuery(0, q.data(), nq, k, idx, dd.data());
-        for (size_t i = 0; i < dd.size(); ++i) {
-            d2[i]  = float(dd[i]);
-            idx[i] = idx[i] >= 0 ? h->slotOfSite[idx[i]] : -1;
-        }
+        h->ensureCommitted();
+        h->eng->knnQuery(0, q.data(), nq, k, idx, dd.data()); // site indices are slots
+        for (size_t i = 0; i < dd.size(); ++i) d2[i] = float(dd[i]);
     });
 }
 
@@ -505,18 +567,16 @@ int vm_loss_pass(vm_handle *h, double *gradXY, double *lossSum, int64_t *count, 
 {
     return vmrt::guardedCall([&] {
         VM_REQUIRE(h, "null handle");
-        bool         wantDbg = knn || nearest || partner || lossTerms;
+        bool wantDbg = knn || nearest || partner || lossTerms;
+        // Per-target diagnostics follow the laid-out order: edited targets are
+        // laid out again first.
+        if (wantDbg && h->patched) h->markDirty();
+        h->ensureCommitted();
         vmrt::DiagOut d;
         h->eng->diag(0, gradXY != nullptr, wantDbg, d);
         if (lossSum) *lossSum = d.lossSum;
         if (count) *count = d.count;
-        if (gradXY) {
-            std::fill(gradXY, gradXY + 2 * size_t(h->numSlots()), 0.0);
-            for (size_t i = 0; i < h->slotOfSite.size(); ++i) {
-                gradXY[2 * size_t(h->slotOfSite[i])]     = d.grad[2 * i];
-                gradXY[2 * size_t(h->slotOfSite[i]) + 1] = d.grad[2 * i + 1];
-            }
-        }
+        if (gradXY) std::copy(d.grad.begin(), d.grad.begin() + 2 * size_t(h->numSlots()), gradXY); // sites are slots
         if (wantDbg) {
             const vmrt::InstSpec &s = h->spec();
             const int K = s.K, nT = int(h->targetSlots.size());
@@ -524,7 +584,7 @@ int vm_loss_pass(vm_handle *h, double *gradXY, double *lossSum, int64_t *count, 
             if (nearest) std::fill(nearest, nearest + nT, -1);
             if (partner) std::fill(partner, partner + nT, -1);
             if (lossTerms) std::fill(lossTerms, lossTerms + nT, 0.0);
-            auto toSlot = [&](int i) { return i >= 0 ? h->slotOfSite[i] : -1; };
+            auto toSlot = [](int i) { return i >= 0 ? i : -1; };
             for (size_t t = 0; t < s.tslot.size(); ++t) {
                 int ts = s.tslot[t];
                 if (knn)

@@ -210,6 +210,17 @@ int vm_sampler_cells(vm_sampler *s, vm_vec2 *cells, int n)
     });
 }
 
+int vm_sampler_cells_region(vm_sampler *s, int cx0, int cy0, int cx1, int cy1, vm_vec2 *cells, int n)
+{
+    return vmrt::guardedCall([&] {
+        VM_REQUIRE(s && cells && size_t(n) == s->cells.size(), "n must be grid_w * grid_h");
+        const int x0 = std::max(cx0, 0), y0 = std::max(cy0, 0), x1 = std::min(cx1, s->gw), y1 = std::min(cy1, s->gh);
+        for (int y = y0; y < y1; ++y)
+            std::copy(s->cells.begin() + size_t(y) * s->gw + x0, s->cells.begin() + size_t(y) * s->gw + x1,
+                      cells + size_t(y) * s->gw + x0);
+    });
+}
+
 int vm_poisson_batch(int n, const float *w, const float *h, const float *r, const unsigned *seed, int trials,
                      const int64_t *offsets, vm_vec2 *cells, float *gpuMs)
 {
@@ -410,18 +421,10 @@ int vm_batch_extract(vm_batch *b, int id, const uint64_t *masks, int tiles_x, in
         b->xGen.resize(size_t(n));
         for (int i = 0; i < n; ++i) b->xGen[size_t(i)] = vm_vec2{float(s.pos[2 * i]), float(s.pos[2 * i + 1])};
         b->xInside.assign(c.inside, c.inside + n);
-        b->xStart.assign(size_t(n) + 1, 0);
-        for (int i = 0; i < n; ++i) b->xStart[size_t(i) + 1] = b->xStart[size_t(i)] + c.count[size_t(i)];
+        b->xStart.assign(c.start, c.start + n + 1);
         const size_t nv = size_t(b->xStart[size_t(n)]);
-        b->xVerts.resize(2 * nv), b->xLabels.resize(nv);
-        for (int i = 0; i < n; ++i) {
-            const size_t o = size_t(b->xStart[size_t(i)]), src = size_t(i) * vmk::kMaxCellVerts;
-            for (int k = 0; k < c.count[size_t(i)]; ++k) {
-                b->xVerts[2 * (o + k)]     = c.verts[2 * (src + k)];
-                b->xVerts[2 * (o + k) + 1] = c.verts[2 * (src + k) + 1];
-                b->xLabels[o + k]          = c.labels[src + k];
-            }
-        }
+        b->xVerts.assign(c.verts, c.verts + 2 * nv);
+        b->xLabels.assign(c.labels, c.labels + nv);
         b->xBound = std::move(c.bound);
         out->num_slots      = n;
         out->generators     = b->xGen.data();

@@ -111,7 +111,9 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
     in.gw = std::max(1, int(gwd)), in.gh = std::max(1, int(ghd));
     in.S = S, in.T = T, in.K = s.K;
     in.stageR     = std::min(std::max(s.stageR, 0), 2);
-    in.allMovable = s.frozen.empty() ? 1 : 0;
+    in.allMovable = s.frozen.empty() && s.rectW <= 0 ? 1 : 0;
+    in.rectW      = std::max(s.rectW, 0);
+    in.rx0 = s.rect[0], in.ry0 = s.rect[1], in.rx1 = s.rect[2], in.ry1 = s.rect[3];
     in.siteOff = siteOff, in.cellOff = cellOff, in.targetOff = targetOff;
     in.lr = s.lr, in.gamma = s.gamma, in.beta1 = s.beta1, in.beta2 = s.beta2, in.eps = s.eps;
     for (int k = 0; k < 4; ++k) in.milestones[k] = s.milestones[k];
@@ -187,7 +189,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
             key.swap(key2), perm.swap(idx2);
         }
     }
-    const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(S, 1)));
+    const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(s.liveS >= 0 ? s.liveS : S, 1)));
     const double maxDiag = double(std::min(s.chunkExt > 0.0f ? s.chunkExt : 1.6f, extentCap)) * spacing;
     // Targets and cells in sorted order, read sequentially by the chunk scan.
     std::vector<double> sxT(nT), syT(nT);
@@ -334,6 +336,7 @@ public:
         L.perm.resize(n);
         std::vector<std::vector<vmk::TargetWork>> twByInst(n);
         for (float cap : {1e30f, kSmallLayoutExtent}) {
+            extentCap_ = cap;
             L.sites = L.cells = L.targets = 0;
             L.siteWork.clear();
             size_t chunks = 0;
@@ -349,6 +352,9 @@ public:
             }
             if (chunks >= size_t(kMinLayoutChunks)) break;
         }
+        // A single instance may keep room past its targets for local edits.
+        tailBase_ = L.targets, tailUsed_ = 0;
+        if (n == 1 && tailTargets > 0) L.targets += align64(tailTargets);
         // Scan tiles: the tiles of multi-tile instances first (look-back), then
         // the single-tile instances.
         for (int pass = 0; pass < 2; ++pass)
@@ -375,7 +381,8 @@ public:
                     L.queue += L.inst[i].T; // every target of the group may be queued once per launch
                 }
             g.count = int(L.targetWork.size()) - g.begin;
-            g.qLen  = L.queue - g.qOff;
+            if (n == 1 && tailTargets > 0) L.queue += align64(tailTargets);
+            g.qLen = L.queue - g.qOff;
             L.groups.push_back(g);
         }
     }
@@ -392,12 +399,13 @@ public:
         dTpos.reserve(L.targets), dTslot.reserve(L.targets);
         dInst.reserve(n), dState.reserve(n), dPartial.reserve(size_t(n) * vmk::kLossPartials);
         dTiles.reserve(L.tiles.size()), dScanCtl.reserve(1), dFlags.reserve(L.tiles.size());
-        dSiteWork.reserve(L.siteWork.size()), dTargetWork.reserve(L.targetWork.size());
+        const bool tail = n == 1 && tailTargets > 0;
+        dSiteWork.reserve(L.siteWork.size()), dTargetWork.reserve(L.targetWork.size() + (tail ? size_t(tailWork) : 0));
         dCnt.reserve(2 + L.groups.size());
         dExactQ.reserve(std::max<int64_t>(L.queue, 1)), dExactLen.reserve(L.groups.size());
 
-        std::vector<V>       hp(L.sites), hm(L.sites), hv(L.sites), ht(L.targets);
-        std::vector<int>     hts(L.targets, 0);
+        std::vector<V>       hp(L.sites), hm(L.sites), hv(L.sites), ht(L.targets, V{R(-1), R(-1)});
+        std::vector<int>     hts(L.targets, -1);
         std::vector<uint8_t> hf(L.sites, 0);
         std::vector<vmk::InstState> hs(n);
         for (int i = 0; i < n; ++i) {
@@ -458,11 +466,130 @@ public:
         gridScan = std::max(1, std::min(int(L.tiles.size()), kMaxScanGrid));
         gridSite = std::max(1, std::min(int(L.siteWork.size()), kMaxSiteGrid));
         gridLoss.clear();
-        for (auto &g : L.groups) gridLoss.push_back(std::max(1, std::min(g.count, kMaxLossGrid)));
+        for (auto &g : L.groups) gridLoss.push_back(std::max(1, std::min(g.count + (tail ? tailWork : 0), kMaxLossGrid)));
         bin();
         VM_HIP(hipStreamSynchronize(stream));
         dirty       = false;
         deviceNewer = false;
+    }
+
+    // ---- local edits of instance 0 ---------------------------------------------
+    const std::vector<int> &permOf(int id) const override { return L.perm[size_t(id)]; }
+
+    void killTargets(const std::vector<int64_t> &pos) override
+    {
+        if (pos.empty()) return;
+        const size_t bytes = pos.size() * sizeof(int64_t);
+        hPatch.reserve(bytes);
+        std::memcpy(hPatch.p, pos.data(), bytes);
+        dPatchIdx.reserve(pos.size());
+        VM_HIP(hipMemcpyAsync(dPatchIdx.p, hPatch.p, bytes, hipMemcpyHostToDevice, stream));
+        VM_HIP(vmk::launchKillTargets<R>(stream, dTpos.p, dTslot.p, dPatchIdx.p, int(pos.size())));
+        VM_HIP(hipStreamSynchronize(stream));
+    }
+
+    bool appendTargets(const std::vector<double> &tgt, const std::vector<int> &tslot,
+                       std::vector<int64_t> &pos) override
+    {
+        const int64_t n = int64_t(tgt.size() / 2);
+        pos.assign(size_t(n), -1);
+        if (n == 0) return true;
+        if (dirty || specs.size() != 1 || L.groups.size() != 1 || tailUsed_ + n > tailTargets) return false;
+        // The new targets laid out alone with instance 0's geometry and chunk
+        // shape, at the next free tail position.
+        const InstSpec &s = specs[0];
+        InstSpec        sub;
+        sub.W = s.W, sub.H = s.H, sub.K = s.K, sub.cell = s.cell, sub.stageR = s.stageR;
+        sub.chunkExt = s.chunkExt, sub.tpl = s.tpl, sub.fraction = s.fraction;
+        sub.liveS = s.liveS >= 0 ? s.liveS : int(s.pos.size() / 2);
+        sub.tgt = tgt, sub.tslot = tslot;
+        const vmk::Inst              &in = L.inst[0];
+        vmk::Inst                     tin;
+        std::vector<vmk::SiteWork>    sw;
+        std::vector<vmk::TargetWork>  tw;
+        std::vector<int>              perm;
+        const int64_t                 at = tailBase_ + tailUsed_;
+        layoutInstance(sub, 0, in.siteOff, in.cellOff, at, tin, sw, tw, perm, extentCap_);
+        const size_t w0 = L.targetWork.size();
+        if (w0 + tw.size() > dTargetWork.n) return false;
+        std::vector<V>   ht(static_cast<size_t>(n));
+        std::vector<int> hs(static_cast<size_t>(n));
+        for (int64_t t = 0; t < n; ++t) {
+            const int o = perm[size_t(t)];
+            ht[size_t(t)] = V{R(tgt[2 * size_t(o)]), R(tgt[2 * size_t(o) + 1])};
+            hs[size_t(t)] = tslot[size_t(o)];
+            pos[size_t(o)] = at + t;
+        }
+        const int    cnt = int(w0 + tw.size());
+        const size_t bT = ht.size() * sizeof(V), bS = hs.size() * sizeof(int), bW = tw.size() * sizeof(tw[0]);
+        hPatch.reserve(bT + bS + bW + sizeof(int));
+        unsigned char *h = hPatch.p;
+        std::memcpy(h, ht.data(), bT), std::memcpy(h + bT, hs.data(), bS), std::memcpy(h + bT + bS, tw.data(), bW);
+        std::memcpy(h + bT + bS + bW, &cnt, sizeof(int));
+        VM_HIP(hipMemcpyAsync(dTpos.p + at, h, bT, hipMemcpyHostToDevice, stream));
+        VM_HIP(hipMemcpyAsync(dTslot.p + at, h + bT, bS, hipMemcpyHostToDevice, stream));
+        if (bW) VM_HIP(hipMemcpyAsync(dTargetWork.p + w0, h + bT + bS, bW, hipMemcpyHostToDevice, stream));
+        VM_HIP(hipMemcpyAsync(dCnt.p + 2, h + bT + bS + bW, sizeof(int), hipMemcpyHostToDevice, stream));
+        VM_HIP(hipStreamSynchronize(stream));
+        L.targetWork.insert(L.targetWork.end(), tw.begin(), tw.end());
+        L.groups[0].count = cnt;
+        tailUsed_ += n;
+        return true;
+    }
+
+    void uploadSiteRect(int slotsW, int x0, int y0, int x1, int y1, const float *xy, bool zeroMoments) override
+    {
+        if (dirty || x1 <= x0 || y1 <= y0) return;
+        const vmk::Inst &in = L.inst[0];
+        const int        w = x1 - x0, h = y1 - y0;
+        const size_t     pitch = size_t(slotsW) * sizeof(V), row = size_t(w) * sizeof(V);
+        const size_t     off   = size_t(in.siteOff) + size_t(y0) * slotsW + x0;
+        if (xy) {
+            hPatch.reserve(size_t(w) * h * sizeof(V));
+            V *st = reinterpret_cast<V *>(hPatch.p);
+            for (int y = 0; y < h; ++y)
+                for (int x = 0; x < w; ++x) {
+                    const size_t k = size_t(y0 + y) * slotsW + x0 + x;
+                    st[size_t(y) * w + x] = V{R(xy[2 * k]), R(xy[2 * k + 1])};
+                }
+            VM_HIP(hipMemcpy2DAsync(dPos.p + off, pitch, st, row, row, size_t(h), hipMemcpyHostToDevice, stream));
+        }
+        if (zeroMoments) {
+            VM_HIP(hipMemset2DAsync(dM.p + off, pitch, 0, row, size_t(h), stream));
+            VM_HIP(hipMemset2DAsync(dV.p + off, pitch, 0, row, size_t(h), stream));
+        }
+        VM_HIP(hipStreamSynchronize(stream));
+        deviceNewer = true;
+    }
+
+    void updateRecord(int id, int liveTargets, bool resetState) override
+    {
+        if (dirty) return;
+        const InstSpec &s  = specs[size_t(id)];
+        vmk::Inst      &in = L.inst[size_t(id)];
+        in.lr = s.lr, in.gamma = s.gamma, in.beta1 = s.beta1, in.beta2 = s.beta2, in.eps = s.eps;
+        for (int k = 0; k < 4; ++k) in.milestones[k] = s.milestones[k];
+        in.useBox = s.useBox, in.bx0 = s.box[0], in.by0 = s.box[1], in.bx1 = s.box[2], in.by1 = s.box[3];
+        in.allMovable = s.frozen.empty() && s.rectW <= 0 ? 1 : 0;
+        in.rectW      = std::max(s.rectW, 0);
+        in.rx0 = s.rect[0], in.ry0 = s.rect[1], in.rx1 = s.rect[2], in.ry1 = s.rect[3];
+        in.maxIter = s.maxIter, in.tol = s.tol;
+        in.gradShift = gradShiftFor(liveTargets, in.ext);
+        VM_HIP(hipMemcpyAsync(dInst.p + id, &in, sizeof(in), hipMemcpyHostToDevice, stream));
+        if (resetState) {
+            specs[size_t(id)].state = vmk::InstState{};
+            VM_HIP(hipMemcpyAsync(dState.p + id, &specs[size_t(id)].state, sizeof(vmk::InstState),
+                                  hipMemcpyHostToDevice, stream));
+        }
+        VM_HIP(hipStreamSynchronize(stream));
+    }
+
+    void rebin() override
+    {
+        if (dirty) return;
+        VM_HIP(vmk::launchFillInt(stream, dCounts.p, int64_t(dCounts.n), 0));
+        bin();
+        VM_HIP(hipStreamSynchronize(stream));
     }
 
     // ---- launch helpers -------------------------------------------------------
@@ -844,14 +971,20 @@ public:
     // Pinned staging of commit's uploads (layouts up to kPinnedUploadMax bytes).
     static constexpr size_t kPinnedUploadMax = size_t(64) << 20;
     HBuf<unsigned char>     hUpload;
+    // Local edits: pinned staging, pooled target indices to retire, the tail
+    // (first position, targets used) and the chunk extent cap of the last layout.
+    HBuf<unsigned char>     hPatch;
+    DBuf<int64_t>           dPatchIdx;
+    int64_t                 tailBase_ = 0, tailUsed_ = 0;
+    float                   extentCap_ = 1e30f;
 
-    DBuf<float>   clipVerts;
-    DBuf<int>     clipLabels, clipCount, clipFlags;
+    DBuf<float>   clipVerts, clipCVerts;
+    DBuf<int>     clipLabels, clipCount, clipFlags, clipStart, clipTiles, clipCLabels;
     DBuf<uint8_t> clipInside;
-    // Pinned copies of the clip outputs (ClipOut views them) and of the
+    // Pinned copies of the clip outputs (ClipOut views them: CSR) and of the
     // boundary's: V, watertight, loops, overflow, E, then vertices and edges.
     HBuf<float>   hClipVerts, hBndVerts;
-    HBuf<int>     hClipLabels, hClipCount, hClipFlags, hBndEdges, hBndRes;
+    HBuf<int>     hClipLabels, hClipStart, hClipFlags, hBndEdges, hBndRes;
     HBuf<uint8_t> hClipInside;
     // Boundary assembly buffers, kept between calls like the clip's; their end
     // point capacity grows when a boundary does not fit.
@@ -929,19 +1062,28 @@ public:
         const bool haveBoundary = dMasks != nullptr && S > 0;
         if (haveBoundary) boundaryEnqueue(S);
         bin();
+        // The cells go down as CSR (only the vertices that exist).
         const size_t nv = size_t(S) * vmk::kMaxCellVerts;
-        hClipVerts.reserve(2 * nv), hClipLabels.reserve(nv), hClipCount.reserve(S), hClipInside.reserve(S);
-        hClipFlags.reserve(2);
+        clipStart.reserve(size_t(S) + 1), clipTiles.reserve(size_t(std::max(vmk::csrTiles(S), 1)));
+        clipCVerts.reserve(2 * nv), clipCLabels.reserve(nv);
+        VM_HIP(vmk::launchClipCsr(stream, dv.p, dl.p, dc.p, S, clipStart.p, clipTiles.p, clipCVerts.p, clipCLabels.p));
+        hClipStart.reserve(size_t(S) + 1), hClipInside.reserve(S), hClipFlags.reserve(2);
         if (S) {
-            VM_HIP(hipMemcpyAsync(hClipVerts.p, dv.p, 2 * nv * sizeof(float), hipMemcpyDeviceToHost, stream));
-            VM_HIP(hipMemcpyAsync(hClipLabels.p, dl.p, nv * sizeof(int), hipMemcpyDeviceToHost, stream));
-            VM_HIP(hipMemcpyAsync(hClipCount.p, dc.p, size_t(S) * sizeof(int), hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(hClipStart.p, clipStart.p, (size_t(S) + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                                  stream));
             VM_HIP(hipMemcpyAsync(hClipInside.p, di.p, size_t(S), hipMemcpyDeviceToHost, stream));
         }
         VM_HIP(hipMemcpyAsync(hClipFlags.p, df.p, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
         VM_HIP(hipStreamSynchronize(stream));
+        const size_t nnz = S ? size_t(hClipStart.p[S]) : 0;
+        hClipVerts.reserve(std::max<size_t>(2 * nnz, 1)), hClipLabels.reserve(std::max<size_t>(nnz, 1));
+        if (nnz) {
+            VM_HIP(hipMemcpyAsync(hClipVerts.p, clipCVerts.p, 2 * nnz * sizeof(float), hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipMemcpyAsync(hClipLabels.p, clipCLabels.p, nnz * sizeof(int), hipMemcpyDeviceToHost, stream));
+            VM_HIP(hipStreamSynchronize(stream));
+        }
         VM_HIP(hipEventElapsedTime(&out.gpuMs, ev[5], ev[6]));
-        out.S = S, out.verts = hClipVerts.p, out.labels = hClipLabels.p, out.count = hClipCount.p;
+        out.S = S, out.verts = hClipVerts.p, out.labels = hClipLabels.p, out.start = hClipStart.p;
         out.inside = hClipInside.p;
         out.unsafe = hClipFlags.p[0], out.overflow = hClipFlags.p[1];
         out.bound  = Boundary{};

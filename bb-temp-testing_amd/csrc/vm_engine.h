@@ -100,6 +100,13 @@ struct InstSpec {
     int    maxIter = 0;
     float  tol = 0.0f;
     vmk::InstState state{};  // host copy of the device state
+    // Slot layouts (the plain optimizer's): pos holds every generator slot,
+    // empty ones with x < 0; liveS counts the live ones (-1: every entry is
+    // live), which set the mean spacing.  rectW > 0: only slots in
+    // [rect[0], rect[2]) x [rect[1], rect[3]) of a rectW-wide slot grid move.
+    int    liveS = -1;
+    int    rectW = 0;
+    int    rect[4] = {0, 0, 0, 0};
 };
 
 struct DiagOut {
@@ -123,9 +130,9 @@ struct Boundary {
 // buffers, valid until its next clip.
 struct ClipOut {
     int            S      = 0;
-    const float   *verts  = nullptr; // [S*24*2]
-    const int     *labels = nullptr; // [S*24]
-    const int     *count  = nullptr; // [S]
+    const int     *start  = nullptr; // [S+1]: cell i's vertices are [start[i], start[i+1])
+    const float   *verts  = nullptr; // [start[S]*2]
+    const int     *labels = nullptr; // [start[S]]
     const uint8_t *inside = nullptr; // [S]
     int            unsafe = 0, overflow = 0;
     float          gpuMs  = 0;
@@ -164,6 +171,31 @@ public:
     virtual size_t deviceBytes() const = 0;
     // One evaluation-mode loss pass with the loss kernel's path counters.
     virtual void lossStats(uint64_t out[8]) = 0;
+
+    // ---- local edits of instance 0 of a committed engine (the plain optimizer) ----
+    // Room kept past the targets at commit: `tailTargets` targets and
+    // `tailWork` chunks that appendTargets() lays out without a re-layout.
+    int64_t tailTargets = 0;
+    int     tailWork    = 0;
+    // Sorted position -> input index of instance `id`'s targets at the last commit.
+    virtual const std::vector<int> &permOf(int id) const = 0;
+    // Retires pooled targets (sorted positions): their entries become dead points.
+    virtual void killTargets(const std::vector<int64_t> &pos) = 0;
+    // Lays out extra targets of instance 0 (xy interleaved, with their slot
+    // numbers) as chunks of their own in the reserved tail; pos[k] receives the
+    // pooled position of target k.  False (nothing changed) when the tail is full.
+    virtual bool appendTargets(const std::vector<double> &tgt, const std::vector<int> &tslot,
+                               std::vector<int64_t> &pos) = 0;
+    // Generator slots [x0, x1) x [y0, y1) of instance 0 (slot layout, slotsW wide)
+    // from the row-major slot array xy (null: positions kept); zeroMoments also
+    // clears their Adam moments.
+    virtual void uploadSiteRect(int slotsW, int x0, int y0, int x1, int y1, const float *xy, bool zeroMoments) = 0;
+    // Instance id's record from its spec without a re-layout (learning rate,
+    // schedule, target box, movable rectangle; gradient shift for liveTargets)
+    // and, when asked, a zero step state.
+    virtual void updateRecord(int id, int liveTargets, bool resetState) = 0;
+    // Bins every site again (after positions were replaced).
+    virtual void rebin() = 0;
 
     bool        dirty = true;
     bool        deviceNewer = false;

@@ -143,10 +143,16 @@ template <class R>
 hipError_t launchKnnQuery(int K, hipStream_t s, const Inst *insts, int inst, const void *spos, const int *sidx,
                           const int *start, const void *q, int nq, int *outIdx, void *outD2);
 template <class R> hipError_t launchClip(int nSites, hipStream_t s, const ClipArgs<R> &a);
+// Per-cell clip output (kMaxCellVerts slots per cell) -> CSR: start[n + 1] and the
+// cells' vertices / labels back to back.  tileSum holds csrTiles(n) ints.
+int        csrTiles(int n);
+hipError_t launchClipCsr(hipStream_t s, const float *verts, const int *labels, const int *count, int n, int *start,
+                         int *tileSum, float *cverts, int *clabels);
 hipError_t launchPoissonClear(int grid, hipStream_t s, const PoiInst *insts, const PoiWork *work, int nWork, float *g);
 hipError_t launchPoissonPhase(int grid, hipStream_t s, const PoiInst *insts, const PoiWork *work, int nWork, float *g,
                               int px, int py, int round);
 hipError_t launchFillInt(hipStream_t s, int *p, int64_t n, int v);
+template <class R> hipError_t launchKillTargets(hipStream_t s, V2<R> *tpos, int *tslot, const int64_t *idx, int n);
 // One-launch resample of a region whose 2-cell halo fits kPoiRegionCells cells of LDS.
 constexpr int kPoiRegionCells   = 8000; // 64,000 B of float2
 constexpr int kPoiRegionThreads = 1024;

@@ -413,8 +413,10 @@ __device__ __forceinline__ void scanTile(const ScanTile &tl, int tile, bool dyn,
 __device__ void scatterInstance(const Inst &in, const ScanScatter &sc, const int *start)
 {
     for (int i = int(threadIdx.x); i < in.S; i += kScanThreads) {
-        const int64_t g = in.siteOff + i;
-        const int64_t d = in.siteOff + start[in.cellOff + sc.cellOf[g]] + sc.rank[g];
+        const int64_t g  = in.siteOff + i;
+        const int     ce = sc.cellOf[g];
+        if (ce < 0) continue; // empty slot
+        const int64_t d = in.siteOff + start[in.cellOff + ce] + sc.rank[g];
         if (sc.posBytes == 8)
             reinterpret_cast<int2 *>(sc.spos)[d] = reinterpret_cast<const int2 *>(sc.pos)[g];
         else
@@ -492,8 +494,10 @@ __global__ __launch_bounds__(kSiteChunk) void scatterKernel(const Inst *__restri
         if (mode == kModeIterate && state[wk.inst].done) continue;
         int i = wk.begin + int(threadIdx.x);
         if (i >= wk.begin + wk.count) continue;
-        int64_t g = in.siteOff + i;
-        int     s = start[in.cellOff + cellOf[g]] + rank[g];
+        int64_t   g  = in.siteOff + i;
+        const int ce = cellOf[g];
+        if (ce < 0) continue; // empty slot
+        int s = start[in.cellOff + ce] + rank[g];
         spos[in.siteOff + s] = pos[g];
         sidx[in.siteOff + s] = i;
     }
@@ -1378,6 +1382,9 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
             p = a.pos[g];
             if (a.doAdam) gx = a.grad[2 * g], gy = a.grad[2 * g + 1], mo = a.m[g], vo = a.v[g];
         }
+        // An empty slot (x < 0 or NaN: slot layouts keep them in place) is
+        // never moved and never binned (voro_optimizer.cu:168-177, 268-301).
+        const bool site = mine && p.x >= R(0);
         if (a.doAdam) {
             // Fold the step's loss partials: every chunk needs the target count,
             // the instance's first chunk also publishes the totals.
@@ -1400,7 +1407,10 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
                 V2<R>           gr{GradOps<R>::toReal(gx, ginv), GradOps<R>::toReal(gy, ginv)};
                 a.grad[2 * g]     = GradT<R>(0);
                 a.grad[2 * g + 1] = GradT<R>(0);
-                bool movable      = in.allMovable || !a.frozen[g];
+                bool movable      = site && (in.allMovable ||
+                                        (in.rectW > 0 ? (i % in.rectW >= in.rx0 && i % in.rectW < in.rx1 &&
+                                                         i / in.rectW >= in.ry0 && i / in.rectW < in.ry1)
+                                                      : !a.frozen[g]));
                 if (movable) {
                     const int t  = loadUniform(&st->iter);
                     R         lr = R(in.lr);
@@ -1426,16 +1436,18 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
             }
         }
         int c = 0;
-        if (mine) {
+        if (site) {
             const R inv = R(1) / R(in.cell);
             c           = cellCoord(p.y, inv, in.gh) * in.gw + cellCoord(p.x, inv, in.gw);
             a.cellOf[g] = c;
+        } else if (mine) {
+            a.cellOf[g] = -1;
         }
         // Rank of the site within its cell for the counting sort (any order
         // within a cell gives the same results: the k-NN order is (d2, index)).
         const int wv = int(threadIdx.x) >> 6;
-        const int lo = waveReduce32<int>(mine ? c : INT_MAX, INT_MAX, [](int x, int y) { return min(x, y); });
-        const int hi = waveReduce32<int>(mine ? c : -1, -1, [](int x, int y) { return max(x, y); });
+        const int lo = waveReduce32<int>(site ? c : INT_MAX, INT_MAX, [](int x, int y) { return min(x, y); });
+        const int hi = waveReduce32<int>(site ? c : -1, -1, [](int x, int y) { return max(x, y); });
         if ((threadIdx.x & 63) == 0) sLo[wv] = lo, sHi[wv] = hi;
         __syncthreads();
         int cLo = sLo[0], cHi = sHi[0];
@@ -1445,15 +1457,15 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
             const int span = cHi - cLo + 1;
             for (int b = int(threadIdx.x); b < span; b += kSiteChunk) sBin[b] = 0;
             __syncthreads();
-            const int local = mine ? atomicAdd(&sBin[c - cLo], 1) : 0;
+            const int local = site ? atomicAdd(&sBin[c - cLo], 1) : 0;
             __syncthreads();
             for (int b = int(threadIdx.x); b < span; b += kSiteChunk) {
                 const int n = sBin[b];
                 if (n) sBin[b] = atomicAdd(&a.counts[in.cellOff + cLo + b], n);
             }
             __syncthreads();
-            if (mine) a.rank[g] = sBin[c - cLo] + local;
-        } else if (mine) {
+            if (site) a.rank[g] = sBin[c - cLo] + local;
+        } else if (site) {
             a.rank[g] = atomicAdd(&a.counts[in.cellOff + c], 1);
         }
         __syncthreads(); // sCount, sBin and the bounds are rewritten by the next chunk
@@ -1560,6 +1572,10 @@ __global__ __launch_bounds__(64 * kClipWaves) void clipKernel(ClipArgs<R> a)
     const int   *strt = a.start + in.cellOff;
     for (int i = blockIdx.x * kClipWaves + wave; i < in.S; i += gridDim.x * kClipWaves) {
         const V2<R>  ci  = pos[i];
+        if (!(ci.x >= R(0))) { // empty slot: no cell (voro_optimizer.cu:361-437 skips it)
+            if (lane == 0) a.inside[i] = 0, a.count[i] = 0;
+            continue;
+        }
         const int    nc  = gatherKnn<kCellK, R, double, false>(lds.knn, in, spos, sidx, strt, ci, lane);
         const double cxd = double(ci.x), cyd = double(ci.y);
         const double x0 = -cxd, y0 = -cyd, x1 = double(in.W) - cxd, y1 = double(in.H) - cyd;
@@ -1596,7 +1612,8 @@ __global__ __launch_bounds__(64 * kClipWaves) void clipKernel(ClipArgs<R> a)
                     if (!seen) {
                         const V2<R>  cj = pos[j];
                         const double rx = double(cj.x) - cxd, ry = double(cj.y) - cyd;
-                        cand            = !(rx * rx + ry * ry > 4.0 * r2max); // r2max only shrinks: rechecked below
+                        // r2max only shrinks: rechecked below.  Empty slots are no neighbours.
+                        cand = cj.x >= R(0) && !(rx * rx + ry * ry > 4.0 * r2max);
                     }
                 }
                 for (uint64_t m = __ballot(cand); m; m &= m - 1) {
@@ -2036,6 +2053,80 @@ template <class R> hipError_t launchClip(int nSites, hipStream_t s, const ClipAr
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Clipped cells -> CSR: exclusive scan of the vertex counts (tiles of
+// kCsrTile cells, then the tile totals), and a gather of every cell's vertices
+// and labels to its offset, so only the vertices that exist leave the device.
+// ---------------------------------------------------------------------------
+constexpr int kCsrThreads = 256, kCsrItems = 16, kCsrTile = kCsrThreads * kCsrItems;
+
+__global__ __launch_bounds__(kCsrThreads) void csrScanKernel(const int *__restrict__ count, int n, int *start,
+                                                             int *tileSum)
+{
+    __shared__ int wsum[kCsrThreads / 64];
+    const int      base = blockIdx.x * kCsrTile + int(threadIdx.x) * kCsrItems;
+    int            v[kCsrItems], tot = 0;
+#pragma unroll
+    for (int j = 0; j < kCsrItems; ++j) v[j] = base + j < n ? count[base + j] : 0, tot += v[j];
+    const int inc = waveInclusiveSum(tot);
+    if ((threadIdx.x & 63) == 63) wsum[threadIdx.x >> 6] = inc;
+    __syncthreads();
+    int pre = 0, btot = 0;
+#pragma unroll
+    for (int k = 0; k < kCsrThreads / 64; ++k) pre += k < int(threadIdx.x >> 6) ? wsum[k] : 0, btot += wsum[k];
+    int ex = pre + inc - tot;
+#pragma unroll
+    for (int j = 0; j < kCsrItems; ++j) {
+        if (base + j < n) start[base + j] = ex;
+        ex += v[j];
+    }
+    if (threadIdx.x == 0) tileSum[blockIdx.x] = btot;
+}
+
+// Exclusive scan of the (few) tile totals in place; the grand total goes to start[n].
+__global__ void csrTileKernel(int *tileSum, int nTiles, int *start, int n)
+{
+    if (threadIdx.x != 0) return;
+    int s = 0;
+    for (int t = 0; t < nTiles; ++t) {
+        const int c = tileSum[t];
+        tileSum[t]  = s;
+        s += c;
+    }
+    start[n] = s;
+}
+
+__global__ void csrGatherKernel(const float *__restrict__ verts, const int *__restrict__ labels,
+                                const int *__restrict__ count, int n, int *start, const int *__restrict__ tileSum,
+                                float *cverts, int *clabels)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int o = start[i] + tileSum[i / kCsrTile];
+        start[i]    = o;
+        for (int k = 0; k < count[i]; ++k) {
+            const int64_t src = int64_t(i) * kMaxCellVerts + k;
+            cverts[2 * int64_t(o + k)]     = verts[2 * src];
+            cverts[2 * int64_t(o + k) + 1] = verts[2 * src + 1];
+            clabels[o + k]                 = labels[src];
+        }
+    }
+}
+
+int csrTiles(int n) { return (n + kCsrTile - 1) / kCsrTile; }
+
+hipError_t launchClipCsr(hipStream_t s, const float *verts, const int *labels, const int *count, int n, int *start,
+                         int *tileSum, float *cverts, int *clabels)
+{
+    if (n <= 0) return hipSuccess;
+    const int nt = csrTiles(n);
+    hipLaunchKernelGGL(csrScanKernel, dim3(nt), dim3(kCsrThreads), 0, s, count, n, start, tileSum);
+    hipLaunchKernelGGL(csrTileKernel, dim3(1), dim3(64), 0, s, tileSum, nt, start, n);
+    const int g = std::min((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(csrGatherKernel, dim3(g), dim3(256), 0, s, verts, labels, count, n, start, tileSum, cverts,
+                       clabels);
+    return hipGetLastError();
+}
+
 hipError_t launchBoundary(hipStream_t s, const BoundaryArgs &a)
 {
     if (a.S <= 0) return hipSuccess;
@@ -2071,6 +2162,28 @@ hipError_t launchPoissonPhase(int grid, hipStream_t s, const PoiInst *insts, con
     hipLaunchKernelGGL(poissonPhaseKernel, dim3(grid), dim3(256), 0, s, insts, work, nWork, g, px, py, round);
     return hipGetLastError();
 }
+
+// Marks pooled targets dead (position (-1, -1), slot -1): the loss kernel takes
+// no target with x < 0.  Local edits retire a target's sorted entry this way.
+template <class R> __global__ void killTargetsKernel(V2<R> *tpos, int *tslot, const int64_t *idx, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int64_t t = idx[i];
+        tpos[t]         = V2<R>{R(-1), R(-1)};
+        tslot[t]        = -1;
+    }
+}
+
+template <class R> hipError_t launchKillTargets(hipStream_t s, V2<R> *tpos, int *tslot, const int64_t *idx, int n)
+{
+    if (n <= 0) return hipSuccess;
+    const int blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(killTargetsKernel<R>, dim3(unsigned(blocks > 1024 ? 1024 : blocks)), dim3(256), 0, s, tpos,
+                       tslot, idx, n);
+    return hipGetLastError();
+}
+template hipError_t launchKillTargets<float>(hipStream_t, V2<float> *, int *, const int64_t *, int);
+template hipError_t launchKillTargets<double>(hipStream_t, V2<double> *, int *, const int64_t *, int);
 
 hipError_t launchFillInt(hipStream_t s, int *p, int64_t n, int v)
 {

@@ -46,7 +46,10 @@ struct Inst {
     int     maxIter;      // 0: no limit
     float   tol;          // relative loss-change convergence threshold (0: off)
     int     gradShift;    // FP32 runs: gradient terms are summed as round(term * 2^gradShift) in int64
-    int     pad[2];
+    // Sites in slot layout (slot i at (i % rectW, i / rectW)): only slots in
+    // [rx0, rx1) x [ry0, ry1) move.  rectW == 0: the frozen mask decides.
+    int     rectW, rx0, ry0, rx1, ry1;
+    int     pad;
 };
 
 // Written by the device every iteration (voro_optimizer.cu:81-85 IterState,

@@ -96,7 +96,7 @@ EXPORTS = [
     "vm_evaluate_loss", "vm_extract", "vm_get_generators", "vm_knn", "vm_loss_pass", "vm_graph_stats",
     "vm_grad_saturation", "vm_batch_saturation",
     "vm_sampler_create", "vm_sampler_destroy", "vm_sampler_resample", "vm_sampler_verify", "vm_sampler_dims",
-    "vm_sampler_cells", "vm_sampler_load_cells", "vm_poisson_grid_dims", "vm_poisson_batch",
+    "vm_sampler_cells", "vm_sampler_cells_region", "vm_sampler_load_cells", "vm_poisson_grid_dims", "vm_poisson_batch",
     "vm_poisson_batch_noalloc",
     "vm_batch_create", "vm_batch_destroy", "vm_batch_add", "vm_batch_commit", "vm_batch_step",
     "vm_batch_get_sites", "vm_batch_status", "vm_batch_extract", "vm_batch_choice", "vm_batch_profile", "vm_batch_loss_stats",
@@ -154,6 +154,7 @@ def lib():
         "vm_sampler_verify": [vp, i32, i32, i32, i32, P(i32)],
         "vm_sampler_dims": [vp, P(i32), P(i32), P(f32)],
         "vm_sampler_cells": [vp, vp, i32],
+        "vm_sampler_cells_region": [vp, i32, i32, i32, i32, vp, i32],
         "vm_sampler_load_cells": [vp, vp, i32],
         "vm_poisson_grid_dims": [f32, f32, f32, P(i32), P(i32), P(f32)],
         "vm_poisson_batch": [i32, vp, vp, vp, vp, i32, vp, vp, P(f32)],

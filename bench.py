@@ -372,18 +372,34 @@ def run_precision(iters=5, K=16):
                 sites=len(sites), targets=len(tg), iterations=iters, grad_clamped_targets=saturated)
 
 
-def run_local_edit(timeout=120):
-    """The demo's scripted edit sequence, headless (tests/cpp/shot_demo): per-stamp latency of the local pipeline
-    (canvas, upload, Poisson resample, 100 local iterations, extract) on a 1024^2 canvas with Poisson r = 24
-    generators.  Run as a child process."""
+def _shot(args, timeout, full=False):
     import subprocess
     exe = os.path.join(ROOT, "tests", "cpp", "shot_demo")
-    if not os.path.exists(exe):
-        return {"skipped": "tests/cpp/shot_demo not built"}
-    r = subprocess.run([exe, "24", "1024", "100", "--json"], capture_output=True, text=True, timeout=timeout)
+    env = dict(os.environ, VM_EDIT_FULL="1" if full else "0")
+    r = subprocess.run([exe, *args, "--json"], capture_output=True, text=True, timeout=timeout, env=env)
     if r.returncode != 0:
         raise RuntimeError(f"shot_demo failed ({r.returncode}): {r.stderr[-500:]}")
     return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def run_local_edit(timeout=240):
+    """The demo's scripted edit sequence, headless (tests/cpp/shot_demo): per-stamp latency of the local pipeline
+    (canvas, upload, Poisson resample, 100 local iterations, extract) on a 1024^2 canvas with Poisson r = 24
+    generators; and on a 4096^2 canvas with r = 1.63 (889^2 generator slots, about 263k live sites, the large
+    instance's size), with local edits patched in place and, for comparison, with every edit re-laid out.
+    Run as child processes."""
+    if not os.path.exists(os.path.join(ROOT, "tests", "cpp", "shot_demo")):
+        return {"skipped": "tests/cpp/shot_demo not built"}
+    out = _shot(["24", "1024", "100"], timeout)
+    big = _shot(["1.63", "4096", "100"], timeout)
+    big_full = _shot(["1.63", "4096", "100"], timeout, full=True)
+    keys = ("canvas_ms", "upload_ms", "poisson_ms", "optimize_ms", "extract_ms", "total_ms")
+    out["large_canvas"] = dict(radius=1.63, raster=4096, generator_slots=big["generator_slots"],
+                               live_sites=big["heart"]["live"], stamps=big["stamps"],
+                               incremental={k: big[k]["mean"] for k in keys},
+                               relayout_every_edit={k: big_full[k]["mean"] for k in keys},
+                               watertight_after=big["after"]["watertight"])
+    return out
 
 
 def serving_summary(rec, st, rate, seconds, warm=2.0):

@@ -39,7 +39,10 @@ public:
     {
         float ms = 0.0f;
         check(vm_sampler_resample(mS, cx0, cy0, cx1, cy1, &ms), "Resample");
-        pull();
+        // Only the region changed: refresh that part of the host mirror.
+        check(vm_sampler_cells_region(mS, cx0, cy0, cx1, cy1, reinterpret_cast<vm_vec2 *>(mCells.data()),
+                                      int(mCells.size())),
+              "Cells");
         return ms;
     }
     float ResampleAll() { return Resample(0, 0, mGridW, mGridH); }

@@ -164,6 +164,10 @@ VM_API int vm_sampler_load_cells(vm_sampler *s, const vm_vec2 *cells, int n);
 VM_API int vm_sampler_dims(vm_sampler *s, int *grid_w, int *grid_h, float *cell_size);
 /* Cells(): host mirror, grid_w * grid_h entries  poisson_disk.h:42 */
 VM_API int vm_sampler_cells(vm_sampler *s, vm_vec2 *cells, int n);
+/* Cells() restricted to [cx0, cx1) x [cy0, cy1) (clamped): only those entries of `cells` (grid_w * grid_h) are
+ * written.  No counterpart in the reference (its Resample refreshes the host mirror itself): lets a caller's
+ * mirror follow a Resample in work proportional to the region. */
+VM_API int vm_sampler_cells_region(vm_sampler *s, int cx0, int cy0, int cx1, int cy1, vm_vec2 *cells, int n);
 
 /* Many independent samplers in one set of launches (batched initialisation).
  * Instance i samples [0,w[i]]x[0,h[i]] with radius r[i] and seed seed[i], `trials`
