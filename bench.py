@@ -168,9 +168,27 @@ def gradient_gate(sites_list, targets_list, p32, p64):
             np.add.at(g64, s_off[i] + pp[ok], sign * gj)
     err = np.abs(g32 - g64)
     rel = float(np.linalg.norm(g32 - g64) / max(np.linalg.norm(g64), 1e-300))
-    return dict(grad_rel_norm_err=rel, grad_max_abs_err=float(err.max()), grad_mean_abs_err=float(err.mean()),
-                grad_max_abs=float(np.abs(g64).max()), selection_flips=int(flip.sum()), targets=int(len(flip)),
-                tolerance=GRAD_GATE, passed=bool(rel <= GRAD_GATE))
+    out = dict(grad_rel_norm_err=rel, grad_max_abs_err=float(err.max()), grad_mean_abs_err=float(err.mean()),
+               grad_max_abs=float(np.abs(g64).max()), selection_flips=int(flip.sum()), targets=int(len(flip)),
+               tolerance=GRAD_GATE, passed=bool(rel <= GRAD_GATE))
+    if t_off[-1] <= 5_000_000:
+        # Conditioning: a site's gradient is a sum of terms that cancel near convergence, and a sum in FP32 can
+        # only be as exact as eps times the sum of its terms' magnitudes.  Per site and component, the error
+        # over that magnitude (FP64 terms with the FP32 run's choices).
+        mass = np.zeros_like(g64)
+        for i in range(len(sites_list)):
+            nn, pp = n32[t_off[i]:t_off[i + 1]], j32[t_off[i]:t_off[i + 1]]
+            idx = np.nonzero((nn >= 0) & (pp >= 0))[0]
+            if len(idx):
+                g0, gj = term_grads64(sites_list[i], targets_list[i], idx, nn[idx], pp[idx])
+                np.add.at(mass, s_off[i] + nn[idx], np.abs(g0))
+                np.add.at(mass, s_off[i] + pp[idx], np.abs(gj))
+        has = mass > 0
+        ratio = err[has] / mass[has]
+        out.update(grad_err_over_term_mass_max=float(ratio.max()) if ratio.size else 0.0,
+                   grad_err_over_term_mass_mean=float(ratio.mean()) if ratio.size else 0.0,
+                   grad_rel_norm_to_term_mass=float(np.linalg.norm(g32 - g64) / max(np.linalg.norm(mass), 1e-300)))
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -323,6 +341,7 @@ def run_small(iters=200, K=16):
     opt.set_generators(grid, gw, gh, cs)
     masks, tx, ty, vox = occupancy([wl.star_outline()])
     opt.set_occupancy(masks, tx, ty, vox)
+    gate_start = single_gate(grid, gw, gh, cs, r, tg, K, grid_cell=cell)  # at the Poisson start
     t0 = time.perf_counter()
     gpu_ms = opt.optimize()
     mesh = opt.extract()
@@ -333,7 +352,8 @@ def run_small(iters=200, K=16):
                final_mean_loss=opt.evaluate_loss(), extract=mesh_summary(mesh),
                grad_clamped_targets=opt.grad_saturation())
     opt.close()
-    out["fp32_vs_fp64"] = single_gate(fitted, gw, gh, cs, r, tg, K, grid_cell=cell)
+    out["fp32_vs_fp64_start"] = gate_start
+    out["fp32_vs_fp64"] = single_gate(fitted, gw, gh, cs, r, tg, K, grid_cell=cell)  # at the fitted sites
     return out, (sites, tg, r, gw, gh, cs, K, iters, cell)
 
 
