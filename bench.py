@@ -561,6 +561,7 @@ def main():
     ap.add_argument("--only", choices=["large", "small", "precision", "local_edit", "batch", "serving"], default=None,
                     help="run one section alone and print its JSON (profiling)")
     ap.add_argument("--large-iters", type=int, default=500)
+    ap.add_argument("--no-gate", action="store_true", help="skip the FP64 gate and the meshes (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU layout is a design "
@@ -594,11 +595,13 @@ def main():
     ks = [b.choice(i)["loss_k"] for i in range(len(insts))]
     tpls = [b.choice(i)["targets_per_lane"] for i in range(len(insts))]
     log(f"  {ms_per_step:.3f} ms/iteration, {value:.3e} site-iters/s; phases ms {np.round(phase, 4).tolist()}")
-    log("  gradient gate (FP32 vs FP64 with the FP32 choices) ...")
-    gate = batch_gate(b, insts)
-    gate["grad_clamped_targets"] = b.saturation()
-    log(f"  {gate}")
-    meshes = batch_meshes(b, insts, seeds)
+    gate, meshes = None, None
+    if not args.no_gate:
+        log("  gradient gate (FP32 vs FP64 with the FP32 choices) ...")
+        gate = batch_gate(b, insts)
+        gate["grad_clamped_targets"] = b.saturation()
+        log(f"  {gate}")
+        meshes = batch_meshes(b, insts, seeds)
     b.close()
     if args.only == "batch":
         print(json.dumps(dict(value=value, ms_per_step=ms_per_step, phases=phase_report(phase, S, T, C), gate=gate,
