@@ -175,6 +175,15 @@ public:
     int                    NumSlots() const { return mSlotsW * mSlotsH; }
     const OptimizerParams &Params() const { return mParams; }
 
+    // Not in the reference: how many FP32 gradient terms had a component clamped before the fixed-point sum
+    // (0 unless two neighbouring sites nearly coincide).
+    int64_t GradSaturation() const
+    {
+        int64_t n = 0;
+        detail::check(vm_grad_saturation(mH, &n), "GradSaturation");
+        return n;
+    }
+
 private:
     struct State
     {

@@ -143,14 +143,15 @@ int main()
     int   live = 0;
     for (Vec2 g : mesh2.generators) live += g.x >= 0;
 
-    bool ok = ok1 && ok2 && samplesOk && loss1 < loss0 && opt.Iteration() == opt.ScheduleLength();
+    long long clamped = (long long)opt.GradSaturation();
+    bool ok = ok1 && ok2 && samplesOk && loss1 < loss0 && opt.Iteration() == opt.ScheduleLength() && clamped == 0;
     std::printf("{\"ok\": %s, \"slots\": %d, \"live\": %d, \"loss0\": %.6e, \"loss1\": %.6e, \"global_ms\": %.3f, "
                 "\"watertight\": [%d, %d], \"loops\": [%d, %d], \"unsafe\": [%d, %d], \"overflow\": [%d, %d], "
                 "\"boundary_edges\": [%zu, %zu], \"poisson_ms\": %.3f, \"local_ms\": %.3f, \"extract_gpu_ms\": %.3f, "
-                "\"extract_host_ms\": %.3f, \"iteration\": %d}\n",
+                "\"extract_host_ms\": %.3f, \"iteration\": %d, \"grad_clamped\": %lld}\n",
                 ok ? "true" : "false", opt.NumSlots(), live, loss0, loss1, globalMs, mesh.watertight,
                 mesh2.watertight, mesh.loops, mesh2.loops, mesh.unsafeCells, mesh2.unsafeCells, mesh.overflowCells,
                 mesh2.overflowCells, mesh.edges.size(), mesh2.edges.size(), poissonMs, localMs, mesh2.gpuMs,
-                mesh2.hostMs, opt.Iteration());
+                mesh2.hostMs, opt.Iteration(), clamped);
     return ok ? 0 : 1;
 }
