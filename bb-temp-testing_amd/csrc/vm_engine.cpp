@@ -189,6 +189,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
             key.swap(key2), perm.swap(idx2);
         }
     }
+    const size_t firstWork = targetWork.size();
     const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(s.liveS >= 0 ? s.liveS : S, 1)));
     const double maxDiag = double(std::min(s.chunkExt > 0.0f ? s.chunkExt : 1.6f, extentCap)) * spacing;
     // Targets and cells in sorted order, read sequentially by the chunk scan.
@@ -236,6 +237,21 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         }
         t = e;
     }
+    // Chunks in Morton order of their cells (chunks of one cell keep their
+    // order): runs of consecutive chunks, which the loss kernel hands to one
+    // XCD, then cover compact patches of the grid.
+    auto spread = [](uint32_t v) { // bits 0..15 of v to the even bits
+        v &= 0xFFFFu;
+        v = (v | (v << 8)) & 0x00FF00FFu, v = (v | (v << 4)) & 0x0F0F0F0Fu;
+        v = (v | (v << 2)) & 0x33333333u, v = (v | (v << 1)) & 0x55555555u;
+        return v;
+    };
+    if (in.gw <= 65536 && in.gh <= 65536)
+        std::stable_sort(targetWork.begin() + std::ptrdiff_t(firstWork), targetWork.end(),
+                         [&](const vmk::TargetWork &a, const vmk::TargetWork &b) {
+                             return (spread(uint32_t(a.cx)) | (spread(uint32_t(a.cy)) << 1)) <
+                                    (spread(uint32_t(b.cx)) | (spread(uint32_t(b.cy)) << 1));
+                         });
 }
 
 namespace {

@@ -523,6 +523,15 @@ __global__ __launch_bounds__(kSiteChunk) void scatterKernel(const Inst *__restri
 // ---------------------------------------------------------------------------
 constexpr int kLossWaves     = 4;
 constexpr int kLossThreads   = 64 * kLossWaves;
+// Workgroups go to the 8 XCDs in turn (workgroup b to XCD b % 8), and each XCD
+// has its own L2.  Work items are handed out in runs of kXcdRun consecutive
+// items per XCD, runs of successive XCDs interleaved, so neighbouring chunks
+// (whose staged blocks share sites and cell starts) meet in one L2 while every
+// XCD still gets an even share.  The launch rounds the grid up to a multiple
+// of 2 kXcdRun workgroups, which makes the mapping a permutation of each
+// round of gridDim.x * kLossWaves items.
+constexpr int kXcds          = 8;
+constexpr int kXcdRun        = 16;
 constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane (block of <= 5 x 5 cells)
 constexpr int kBisectSteps   = 9;
 constexpr int kStageRetry    = 3;   // rings of the retried block (7 x 7 cells <= 64 lanes)   // brackets q's K-th distance^2 to 1/512 of the block's largest
@@ -553,9 +562,12 @@ struct ScreenBounds {
     int   mp1;  // candidates [0, mp1) hold every site that can be a target's nearest
     int   mp2;  // candidates [mp2, mp) lie farther than r2 from q (outer band)
     int   mp;   // all candidates, padded to whole groups
-    float t2;   // d2(p, w) below t2 proves w is one of p's K nearest
+    int   nb;   // sites in the two inner bands (candidates [0, nb))
     float r2;   // inner radius of the outer band (lower bound)
     float rho;  // >= |p - q| for every target of the chunk
+    float qx, qy; // chunk centre q
+    float mg;   // absolute rounding margin
+    float sl;   // sqrt(lo) (1 - 4e-6): fewer than K staged sites lie within it of q
 };
 
 // Packed screening keys: |v| (a 32-bit float) with its low 7 bits replaced by
@@ -596,11 +608,11 @@ __device__ __forceinline__ void keepTwo(uint32_t &a1, uint32_t &a2, uint32_t k)
 //      packed; the two smallest are kept.  c0 itself, sites on top of it and
 //      the sentinels give NaN and never win.
 //      Candidates of the outer band (farther than r2 from q) are skipped when
-//      their lower bound (r2 - rho - |p - c0|) / 2 exceeds every active lane's
-//      runner-up.
-//   3. The winner w must be one of the K nearest: d2(w) < t2 proves it (fewer
-//      than K sites lie within sqrt(t2) + |p - q| of q), else at most K
-//      candidates with d2 <= d2(w).
+//      their lower bound (r2 - |p - q| - |p - c0|) / 2 exceeds every active
+//      lane's runner-up.
+//   3. The winner w must be one of the K nearest: |p - w| + |p - q| below
+//      sqrt(lo) proves it (fewer than K sites lie within sqrt(lo) of q), else
+//      at most K candidates with d2 <= d2(w).
 // Error bound (u = p - c0, e = c_i - c0, ee and ue the rounded |e|^2 and
 // <u, e> both computations share, delta = ue / sqrt(ee) - sqrt(ee) / 2): the
 // fma forms ue - ee / 2 with one rounding, so with rsq accurate to 2 ulp
@@ -634,6 +646,13 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     // uB >= |u|: the key's value bounds d2 from above, and v_sqrt_f32 is within
     // 1 ulp on normal inputs (smaller ones are raised to 2^-100).
     const float     uB = __builtin_amdgcn_sqrtf(fmaxf(__uint_as_float(b1 | kIdxMask), 0x1p-100f)) * (1.0f + 0x1p-20f);
+    // dq >= |p - q| for this target (at most rho): the rounded distance, raised
+    // past the errors of dist2 and v_sqrt_f32, plus the margin.  For K >= 16 the
+    // chunk-wide rho is used: the outer band and the count are rarely needed
+    // there, and the per-target bound cost more than it saved (measured).
+    const float dq = K >= 16 ? sb.rho
+                             : fminf(__builtin_amdgcn_sqrtf(dist2(p, V2<float>{sb.qx, sb.qy})) * (1.0f + 0x1p-20f) + sb.mg,
+                                     sb.rho);
     uint32_t        a1 = ~0u, a2 = ~0u;
     auto            screenGroups = [&](int g0, int g1) {
 #pragma unroll 1
@@ -649,12 +668,12 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
         }
     };
     screenGroups(0, sb.mp2);
-    // Outer band: |d_i| >= (|p - c_i| - |p - c0|) / 2 >= (r2 - rho - |u|) / 2.
+    // Outer band: |d_i| >= (|p - c_i| - |p - c0|) / 2 >= (r2 - |p - q| - |u|) / 2.
     // When that exceeds the runner-up (with room for the screen's error) in
     // every lane, no outer candidate can be the winner or the runner-up.
     {
         const float v2  = __uint_as_float(a2 & ~kIdxMask);
-        const float lb  = 0.5f * (sb.r2 * (1.0f - 1e-5f) - sb.rho - uB * (1.0f + 1e-5f));
+        const float lb  = 0.5f * (sb.r2 * (1.0f - 1e-5f) - dq - uB * (1.0f + 1e-5f));
         const bool  out = v2 * (1.0f + 0x1p-10f) + uB * 0x1p-14f < lb; // false when v2 is NaN (no runner-up)
         if (__ballot(active && !out)) screenGroups(sb.mp2, sb.mp);
     }
@@ -665,13 +684,22 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
               (a2 >= kPackInf || __builtin_fmaf(v2, -0x1p-13f, v2) > __builtin_fmaf(v1, 0x1p-13f, v1) + uB * 0x1p-17f);
     const V2<float> cw  = lds.pos[w];
     const float     d2w = dist2(cw, p);
-    bool            member = d2w * (1.0f + 4e-6f) < sb.t2;
+    // Every site within |p - w| of p lies within |p - w| + |p - q| of q, and
+    // fewer than K staged sites lie within sqrt(lo) of q.
+    const float     tq     = sb.sl - dq - sb.mg;
+    bool            member = d2w * (1.0f + 4e-6f) < (tq > 0.0f ? tq * tq : 0.0f);
     if (__ballot(ok && !member)) {
-        int cnt = 0;
+        // w's rank in p's (d2, index) order is at most the number of candidates
+        // with d2 <= d2(w).  The two inner bands hold nb sites; when nb < K only
+        // the outer band is counted, and rank <= nb + (outer ones with d2 <=
+        // d2(w)) (w itself is among the nb or among those).
+        const bool outerOnly = sb.nb < K;
+        int        cnt       = outerOnly ? sb.nb : 0;
 #pragma unroll 1
-        for (int g = 0; g < sb.mp; g += kScreenGroup) {
+        for (int g = outerOnly ? sb.nb & ~(kScreenGroup - 1) : 0; g < sb.mp; g += kScreenGroup) {
 #pragma unroll
-            for (int k = 0; k < kScreenGroup; ++k) cnt += dist2(lds.pos[g + k], p) <= d2w ? 1 : 0;
+            for (int k = 0; k < kScreenGroup; ++k)
+                cnt += (!outerOnly || g + k >= sb.nb) && dist2(lds.pos[g + k], p) <= d2w ? 1 : 0;
         }
         member = cnt <= K;
     }
@@ -836,7 +864,11 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
     LossLds<R> &lds  = ldsAll[wave];
     const int  nWork = *a.nWork;
     const uint32_t mask = sizeof(R) == 4 ? packMask() : 0u;
-    for (int w = blockIdx.x * kLossWaves + wave; w < nWork; w += gridDim.x * kLossWaves) {
+    const int  xcdWave = (int(blockIdx.x) / kXcds) * kLossWaves + wave; // the wavefront's rank on its XCD
+    const int  first   = gridDim.x % (2 * kXcdRun) == 0
+                             ? ((xcdWave / kXcdRun) * kXcds + int(blockIdx.x) % kXcds) * kXcdRun + xcdWave % kXcdRun
+                             : int(blockIdx.x) * kLossWaves + wave;
+    for (int w = first; w < nWork; w += gridDim.x * kLossWaves) {
         const TargetWork wk = loadUniform(a.work + w);
         const int64_t    so = wk.siteOff;
 
@@ -1031,10 +1063,10 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                 sb.mp1     = __builtin_amdgcn_readfirstlane((nA + kScreenGroup - 1) & ~(kScreenGroup - 1));
                 sb.mp2     = __builtin_amdgcn_readfirstlane((nB + kScreenGroup - 1) & ~(kScreenGroup - 1));
                 sb.mp      = __builtin_amdgcn_readfirstlane(mp);
+                sb.nb      = __builtin_amdgcn_readfirstlane(nB);
                 sb.r2      = sqrtR(r2sq);
                 sb.rho     = rho;
-                const R tq = sqrtR(lo) * R(1 - 4e-6) - rho - mg;
-                sb.t2      = tq > R(0) ? tq * tq : R(0);
+                sb.qx = q.x, sb.qy = q.y, sb.mg = mg, sb.sl = sqrtR(lo) * R(1 - 4e-6);
             }
         }
 
@@ -1994,7 +2026,7 @@ hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstS
 template <class R> hipError_t launchLoss(int K, int tpl, int grid, hipStream_t s, const LossArgs<R> &a)
 {
     const bool diag   = a.stats || a.dbgKnn || a.dbgTerm || a.exactOnly;
-    const int  blocks = (grid + kLossWaves - 1) / kLossWaves;
+    const int  blocks = ((grid + kLossWaves - 1) / kLossWaves + 2 * kXcdRun - 1) / (2 * kXcdRun) * (2 * kXcdRun);
     if (!lossKSupported(K) || !lossTplSupported(K, tpl, sizeof(R) == 8)) return hipErrorInvalidValue;
 #define VM_KT(k, t)                                                                                               \
     if (K == k && tpl == t) {                                                                                     \
