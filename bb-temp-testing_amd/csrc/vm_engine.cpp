@@ -846,7 +846,8 @@ public:
         gridBuild(vmk::kModeEvaluate);
         for (size_t g = 0; g < L.groups.size(); ++g) {
             auto a = lossArgs(int(g), vmk::kModeEvaluate, wantGrad);
-            if (wantDbg && L.groups[g].K == K) a.dbgKnn = dK.p, a.dbgTerm = dT.p;
+            if (wantDbg && L.groups[g].K == K)
+                a.dbgKnn = dK.p, a.dbgTerm = dT.p, a.exactGrid = vmk::exactGridFor(L.groups[g].qLen, true);
             VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, a));
             VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, a));
         }

@@ -160,11 +160,13 @@ bool          poissonRegionFits(const PoiInst &pi);
 hipError_t    launchPoissonRegion(hipStream_t s, const PoiInst &pi, float *grid, int trials);
 
 // Workgroups for the exact-selection launch of a group with `targets` targets:
-// about one wavefront per 512 targets, so the few the screen leaves open spread
-// widely; every target when all take the exact path (FP64, VM_LOSS_EXACT).
+// every target takes it in FP64 runs, with VM_LOSS_EXACT and for the k-NN list
+// diagnostics (one wavefront per target); otherwise only the targets of chunks
+// that could not be staged do (the screen's open targets take the selection
+// inside their chunk), about one wavefront per 8192 targets.
 inline int exactGridFor(int64_t targets, bool allExact)
 {
-    const int64_t g = allExact ? (targets + 3) / 4 : (targets + 2047) / 2048;
+    const int64_t g = allExact ? (targets + 3) / 4 : (targets + 32767) / 32768;
     return int(g < 16 ? 16 : (g > 4096 ? 4096 : g));
 }
 

@@ -541,6 +541,8 @@ template <class R> struct LossLds {
     int      idx[kLossStage];
     uint32_t klo[kLossStage]; // key low word (index << kLdsSlotBits | slot) of each compacted candidate
     GradT<R> grad[2 * kLossStage];
+    uint64_t xkey[kLossStage]; // exact selection inside the chunk: (d2, index) keys of the candidates
+    int      xtop[32];         // and the slots of the K nearest, in order
 };
 
 // Outcome of one lane's VoroLoss term: nearest site c0 (instance index i0, LDS
@@ -846,6 +848,60 @@ __device__ __forceinline__ int laneRank(uint64_t mask)
 // At most 72 VGPRs, so 7 wavefronts share a SIMD: the kernel is bound by issue
 // and by the latency of its per-chunk loads, and occupancy hides the latter
 // (K = 4 and 6 would spill at 72 and take 6).
+// The exact selection for one target pt of a staged chunk, by the whole
+// wavefront, when the screen left it open.  The chunk's m candidates hold pt's
+// K nearest sites (the candidate disc), so they are ranked by the (d2, index)
+// keys of the exact kernel and lane k evaluates the reference's term for
+// neighbour k; the smallest (d^2, k) wins (voro_optimizer.cu:225-242), exactly
+// as in the exact kernel.  The winner's gradient goes to the chunk's LDS
+// accumulators.  Returns the winning lane (-1: every pair degenerate); that
+// lane's outputs are the term's loss, the nearest site's and the partner's
+// instance indices and whether a gradient component was clamped.
+template <int K>
+__device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float> pt, int lane, int gcopies, int gM,
+                                            bool grad, double gscale, float &lossT, int &i0, int &j, int &sat)
+{
+    for (int s = lane; s < m; s += 64) lds.xkey[s] = Key<float>::make(dist2(lds.pos[s], pt), lds.klo[s]).k;
+    waveSync();
+    for (int s = lane; s < m; s += 64) {
+        const uint64_t ks   = lds.xkey[s];
+        int            rank = 0;
+#pragma unroll 4
+        for (int t = 0; t < m; ++t) rank += lds.xkey[t] < ks ? 1 : 0;
+        if (rank < K) lds.xtop[rank] = s;
+    }
+    waveSync();
+    const int       s0 = lds.xtop[0];
+    const V2<float> c0 = lds.pos[s0];
+    const V2<float> u{pt.x - c0.x, pt.y - c0.y};
+    float           dd = rInf<float>(), d = 0.0f, l = 0.0f;
+    V2<float>       ev{0.0f, 0.0f};
+    int             sj = 0;
+    bool            ok = false;
+    if (lane >= 1 && lane < K) {
+        sj                 = lds.xtop[lane];
+        const V2<float> cj = lds.pos[sj];
+        ev                 = V2<float>{cj.x - c0.x, cj.y - c0.y};
+        l                  = sqrtR(dotR(ev, ev));
+        if (!(l < 1e-12f)) {
+            d  = fmaR(-0.5f, l, dotR(u, ev) / l);
+            dd = d * d;
+            ok = true;
+        }
+    }
+    const float    best = waveMinR<float>(dd);
+    const uint64_t win  = __ballot(ok && dd == best);
+    const int      wl   = win ? __builtin_ctzll(win) : -1;
+    if (lane == wl) {
+        lossT = best, i0 = lds.idx[s0], j = lds.idx[sj];
+        sat   = grad ? lossGrad(u, ev, l, d, &lds.grad[2 * gradSlot(sj, lane & (gcopies - 1), gM)],
+                              &lds.grad[2 * gradSlot(s0, lane & (gcopies - 1), gM)], gscale)
+                     : 0;
+    }
+    waveSync();
+    return wl;
+}
+
 template <int K, int TPL, class R, bool Diag>
 __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K == 4 || K == 6 ? 6 : 7))) void lossKernel(LossArgs<R> a)
 {
@@ -1070,6 +1126,9 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             }
         }
 
+        // Open targets of a staged chunk take the exact selection inside the
+        // chunk; the k-NN list diagnostics and exact-only runs queue them.
+        const bool inl = fast && !exactOnly && !dbgKnn;
         // ---- each target: screened term, or queued for the exact selection
         //      (for the k-NN list diagnostics every taken target is queued; bit
         //      31 of the instance marks a term already settled here) ------------
@@ -1089,7 +1148,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                     tr.used = take[h] && !open ? tr.used : 0;
                 }
             }
-            const bool     push = open || (dbgKnn && take[h]);
+            const bool     push = (open && !inl) || (dbgKnn && take[h]);
             const uint64_t pm   = __ballot(push);
             if (pm) {
                 int base = 0;
@@ -1117,6 +1176,27 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             }
             lossSum += double(lossT);
             used += tr.used;
+            if constexpr (sizeof(R) == 4) {
+                // Targets the screen left open: the exact selection inside the chunk.
+                uint64_t om = inl ? __ballot(open) : 0ull;
+                while (om) {
+                    const int l = __builtin_ctzll(om);
+                    om &= om - 1;
+                    const V2<float> pt{__int_as_float(__builtin_amdgcn_readlane(__float_as_int(p[h].x), l)),
+                                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p[h].y), l))};
+                    float     xl = 0.0f;
+                    int       xi0 = -1, xj = -1, xs = 0;
+                    const int wl = exactInChunk<K>(lds, m, pt, lane, gcopies, gM, a.grad != nullptr,
+                                                   __builtin_ldexp(1.0, in.gradShift), xl, xi0, xj, xs);
+                    if (lane == wl) lossSum += double(xl), used += 1, sat += xs;
+                    if (dbgTerm && lane == (wl >= 0 ? wl : 0)) {
+                        const int64_t o = wk.tBegin + l + 64 * h;
+                        dbgTerm[o].i0   = wl >= 0 ? xi0 : -1;
+                        dbgTerm[o].j    = wl >= 0 ? xj : -1;
+                        dbgTerm[o].loss = wl >= 0 ? double(xl) : 0.0;
+                    }
+                }
+            }
         }
         if (stats && lane == 0) {
             unsigned long long *sp = stats + 8 * (blockIdx.x & 63); // 64 lines, summed on the host
