@@ -287,30 +287,43 @@ __device__ __forceinline__ void publishTile(unsigned long long *f, uint32_t epoc
     __hip_atomic_store(f, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Begins the step of instance `id` (first tile); returns 1 when it is skipped.
-__device__ __forceinline__ int beginStep(const Inst &in, InstState *s, int mode)
+// What the start of a step reads from the instance's record and state, loaded
+// in one round (the decision below then waits for nothing else).
+struct StepIn {
+    int     iter, done, maxIter;
+    float   tol;
+    int64_t count;
+    double  loss, prevMean;
+};
+__device__ __forceinline__ StepIn stepLoad(const Inst &in, const InstState *s)
 {
-    int d = s->done;
-    if (mode == kModeIterate) {
-        if (!d) {
-            if (s->iter > 0 && s->count > 0 && in.tol > 0.0f) {
-                double mean = s->loss / double(s->count);
-                if (s->iter > 1 && fabs(mean - s->prevMean) < double(in.tol) * fabs(s->prevMean)) d = 1;
-                s->prevMean = mean;
-            }
-            if (in.maxIter > 0 && s->iter >= in.maxIter) d = 1;
-            if (!d) {
-                s->iter += 1;
-                s->count = 0;
-                s->loss  = 0.0;
-            }
-            s->done = d;
-        }
-        return d;
+    return StepIn{s->iter, s->done, in.maxIter, in.tol, s->count, s->loss, s->prevMean};
+}
+
+// Begins the step of an instance (first tile); returns 1 when it is skipped.
+__device__ __forceinline__ int beginStep(const StepIn &v, InstState *s, int mode)
+{
+    if (mode != kModeIterate) {
+        s->evalLoss  = 0.0;
+        s->evalCount = 0;
+        return 0;
     }
-    s->evalLoss  = 0.0;
-    s->evalCount = 0;
-    return 0;
+    int d = v.done;
+    if (!d) {
+        if (v.iter > 0 && v.count > 0 && v.tol > 0.0f) {
+            const double mean = v.loss / double(v.count);
+            if (v.iter > 1 && fabs(mean - v.prevMean) < double(v.tol) * fabs(v.prevMean)) d = 1;
+            s->prevMean = mean;
+        }
+        if (v.maxIter > 0 && v.iter >= v.maxIter) d = 1;
+        if (!d) {
+            s->iter  = v.iter + 1;
+            s->count = 0;
+            s->loss  = 0.0;
+        }
+        s->done = d;
+    }
+    return d;
 }
 
 // One scan tile: exclusive scan of counts into starts (and zero the counts).
@@ -328,6 +341,9 @@ __device__ __forceinline__ void scanTile(const ScanTile &tl, int tile, bool dyn,
 
     // Each thread owns kScanItems consecutive cells (16-byte vector loads and
     // stores: cell offsets and tile starts are multiples of 64).
+    // The first tile's thread 0 reads the step's inputs now, beside the counts.
+    StepIn si{};
+    if (threadIdx.x == 0 && tl.first) si = stepLoad(in, state + tl.inst);
     const int  i0   = int(threadIdx.x) * kScanItems;
     const bool full = i0 + kScanItems <= n;
     int        v[kScanItems];
@@ -359,7 +375,7 @@ __device__ __forceinline__ void scanTile(const ScanTile &tl, int tile, bool dyn,
     if (threadIdx.x == 0) {
         int skip = 0, prefix = 0;
         if (tl.first) {
-            skip = beginStep(in, state + tl.inst, mode);
+            skip = beginStep(si, state + tl.inst, mode);
             if (dyn) publishTile(&flags[tile], epoch, skip ? kTileSkip : kTilePrefix, btot);
         } else {
             publishTile(&flags[tile], epoch, kTileAggregate, btot);
