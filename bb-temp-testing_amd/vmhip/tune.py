@@ -91,9 +91,10 @@ def k_agreement(fits, Ks, iters=50):
     b.step(iters)
     ok = {K: True for K in Ks}
     for i, f in enumerate(fits):
-        gw, gh, cs = f["dims"]
-        moved = b.sites(i)
-        for slots, sw, sh, cell in ((f["grid"], gw, gh, cs), (moved, len(moved), 1, 1.0 / len(moved))):
+        # Live sites as a row of slots (a sampler grid of the largest classes has more slots than an optimizer
+        # takes; empty slots play no part in the terms).
+        start, moved = f["sites"], b.sites(i)
+        for slots, sw, sh, cell in ((start, len(start), 1, 1.0 / len(start)), (moved, len(moved), 1, 1.0 / len(moved))):
             ref = terms(slots, sw, sh, cell, f["targets"], K_REFERENCE, f["radius"])
             for K in Ks:
                 if ok[K]:
