@@ -738,3 +738,31 @@ def test_gradient_clamp_counted(gap):
     assert err[others].max() <= tol
     if sat:  # the clamped terms (16 against hundreds) largely cancel over the ring, but not to within tol
         assert err[[a, b]].min() > tol
+
+
+@pytest.mark.parametrize("cell_factor,rings", [(3.0, 1), (4.0, 2), (0.5, 2)])
+def test_unstaged_chunks_bitexact(cell_factor, rings):
+    """Grid shapes whose staged blocks overflow the loss kernel's stage or miss the candidate disc: those chunks
+    queue their targets for the exact-selection kernel's ring search.  The batch loss pass (per-target terms, no
+    k-NN lists) and five optimizer steps are bit-exact against the oracle."""
+    r, K, iters = 0.025, 16, 5
+    grid, _, targets = _config0(r)
+    sites = wl.live_sites(grid)
+    spacing = (1.0 / len(sites)) ** 0.5
+    print(f"case: unstaged chunks, cell {cell_factor} spacings, {rings} staged rings", flush=True)
+    b = vmhip.Batch()
+    b.add(W, H, sites, targets, 1e-3, loss_k=K, grid_cell=cell_factor * spacing, stage_rings=rings)
+    b.commit()
+    st = b.loss_stats()
+    assert st["ring_staged_waves"] + st["ring_global_waves"] > 0
+    d = b.loss_pass()
+    ref = O.loss_grad(sites, targets, K, W, H, 0.1, per_target=True)
+    assert np.array_equal(d["nearest"], ref["c0"])
+    assert np.array_equal(d["partner"], ref["bestj"])
+    assert np.array_equal(d["loss_terms"], ref["loss_t"].astype(np.float64))
+    assert np.array_equal(d["grad"], ref["grad"])
+    b.step(iters)
+    sc = O.make_sched(1e-3, W, H)
+    fit, _, _, lh, ch = O.fit(sites, targets, K, 0.1, sc, iters)
+    assert np.array_equal(b.sites(0), fit)
+    assert b.status(0)["last_mean_loss"] == pytest.approx(lh[-1] / ch[-1], rel=1e-12)
