@@ -552,13 +552,32 @@ constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane 
 constexpr int kBisectSteps   = 9;
 constexpr int kStageRetry    = 3;   // rings of the retried block (7 x 7 cells <= 64 lanes)   // brackets q's K-th distance^2 to 1/512 of the block's largest
 
+// Screen table of a staged chunk: one row per distinct nearest site c_r of the
+// chunk's targets, one column per inner-band candidate c_i.  Entry (r, i) holds
+// the unit vector n from c_r to c_i and -|c_i - c_r| / 2, so the signed
+// distance of a target p with nearest site c_r to their bisector is
+// fma(ux, nx, fma(uy, ny, off)) with u = p - c_r.  Rows are kTblPad entries
+// longer than the columns, which puts the rows a wavefront reads at once in
+// different LDS banks.
+constexpr int kTblEntries = 160;
+constexpr int kTblPad     = 2;
+
 template <class R> struct LossLds {
     V2<R>    pos[kLossStage];
     int      idx[kLossStage];
-    uint32_t klo[kLossStage]; // key low word (index << kLdsSlotBits | slot) of each compacted candidate
     GradT<R> grad[2 * kLossStage];
-    uint64_t xkey[kLossStage]; // exact selection inside the chunk: (d2, index) keys of the candidates
-    int      xtop[32];         // and the slots of the K nearest, in order
+    // The table lives while the chunk's terms are screened; the exact
+    // selection inside the chunk runs afterwards and reuses the space.
+    union {
+        struct {
+            uint64_t xkey[kLossStage]; // (d2, index) keys of the candidates
+            int      xtop[32];         // slots of the K nearest, in order
+        } x;
+        struct {
+            float2 n[kTblEntries];
+            float  off[kTblEntries];
+        } t;
+    } u;
 };
 
 // Outcome of one lane's VoroLoss term: nearest site c0 (instance index i0, LDS
@@ -586,6 +605,12 @@ struct ScreenBounds {
     float qx, qy; // chunk centre q
     float mg;   // absolute rounding margin
     float sl;   // sqrt(lo) (1 - 4e-6): fewer than K staged sites lie within it of q
+    // Screen table (rows = the distinct nearest sites, as a mask of their
+    // slots; cols = the row stride in entries, the first mp2 of each row
+    // used); rowMask == 0: no table, every candidate is screened with the
+    // per-candidate formula.
+    uint32_t rowMask;
+    int      cols;
 };
 
 // Packed screening keys: |v| (a 32-bit float) with its low 7 bits replaced by
@@ -612,33 +637,60 @@ __device__ __forceinline__ void keepTwo(uint32_t &a1, uint32_t &a2, uint32_t k)
     a1 = min(a1, k);
 }
 
-// FP32 screening of one lane's VoroLoss term over the wave's compacted
-// candidates (a superset of the lane's K nearest sites).
-//   1. The nearest candidate, c0, searched among the first band (sites within
-//      D1 + 2 rho of q, D1 = q's nearest distance: a target's nearest site is no
-//      farther from q) with packed keys of d2.  The packing truncates d2 by at
-//      most 2^-16 relative: when the two smallest keys differ above the slot
-//      bits, the smallest is strictly the nearest (d2 below the other's
-//      truncated value), which is the (d2, index) order's first; otherwise the
-//      lane takes the exact selection.
-//   2. An approximation a_i of |distance to the bisector (c0, c_i)| with the
-//      hardware reciprocal square root, a_i = rsq(|e|^2) (<u, e> - |e|^2 / 2),
-//      packed; the two smallest are kept.  c0 itself, sites on top of it and
-//      the sentinels give NaN and never win.
+// Nearest candidate of a target p among the first band of the wave's compacted
+// candidates (sites within D1 + 2 rho of q, D1 = q's nearest distance: a
+// target's nearest site is no farther from q), by packed keys of d2.  The
+// packing truncates d2 by at most 2^-16 relative: when the two smallest keys
+// differ above the slot bits, the smallest is strictly the nearest (d2 below
+// the other's truncated value), which is the (d2, index) order's first;
+// otherwise `ok` is false and the lane takes the exact selection.
+__device__ __forceinline__ int nearestCandidate(const LossLds<float> &lds, const ScreenBounds &sb, V2<float> p,
+                                                uint32_t mask, bool &ok)
+{
+    constexpr uint32_t kIdxMask = uint32_t(kLossStage - 1);
+    uint32_t b1 = ~0u, b2 = ~0u;
+#pragma unroll 1
+    for (int g = 0; g < sb.mp1; g += kScreenGroup) {
+#pragma unroll
+        for (int k = 0; k < kScreenGroup; ++k) keepTwo(b1, b2, packKey(mask, dist2(lds.pos[g + k], p), g + k));
+    }
+    ok = (b1 ^ b2) > kIdxMask;
+    return int(b1 & kIdxMask);
+}
+
+// FP32 screening of one lane's VoroLoss term, given its nearest candidate c0
+// (slot s0, certified when nearOk), over the wave's compacted candidates (a
+// superset of the lane's K nearest sites).
+//   1. For every candidate c_i, an approximation a_i of |distance from p to
+//      the bisector of (c0, c_i)|, packed with the slot in its low 7 bits; the
+//      two smallest are kept.  c0 itself, sites on top of it and the padding
+//      give NaN or infinity and never win.
+//      The first mp2 candidates, when the chunk has a screen table: a_i =
+//      |fma(ux, nx, fma(uy, ny, off))| from the row of c0 (u = p - c0).
+//      Without a table, and for the outer band: a_i = rsq(ee) (ue - ee / 2)
+//      with ee = |e|^2 and ue = <u, e> (e = c_i - c0) rounded as the reference
+//      rounds them.
 //      Candidates of the outer band (farther than r2 from q) are skipped when
 //      their lower bound (r2 - |p - q| - |p - c0|) / 2 exceeds every active
 //      lane's runner-up.
-//   3. The winner w must be one of the K nearest: |p - w| + |p - q| below
+//   2. The winner w must be one of the K nearest: |p - w| + |p - q| below
 //      sqrt(lo) proves it (fewer than K sites lie within sqrt(lo) of q), else
 //      at most K candidates with d2 <= d2(w).
-// Error bound (u = p - c0, e = c_i - c0, ee and ue the rounded |e|^2 and
-// <u, e> both computations share, delta = ue / sqrt(ee) - sqrt(ee) / 2): the
-// fma forms ue - ee / 2 with one rounding, so with rsq accurate to 2 ulp
-// |a_i - delta| <= 1.5 2^-22 |delta|; the reference's d_i = RN(RN(ue / l) -
-// l / 2) has |d_i - delta| <= 2^-22.9 |u| + 2^-25 l + 2^-24 |d_i|, and l <= 4
-// max(|u|, |d_i|) (if l > 4 |u| then d_i >= (l - 2|u|) / 2 >= l / 4).  Hence
-// |a_i - |d_i|| <= 2^-20 (|u| + |d_i|); packing truncates a_i by at most 2^-16
-// relative.  So
+// Error bounds (eps = 2^-24).  Let delta_i = <p - m_i, n_i> be the exact signed
+// distance (m_i the exact midpoint of c0 and c_i, n_i the exact unit normal).
+// The reference's d_i = RN(RN(ue / l) - RN(l / 2)) has |d_i - delta_i| <=
+// 11 eps |u| + 4 eps |delta_i| <= 2^-20 (|u| + |delta_i|): u and e carry
+// relative errors eps, ue and ee 2 eps more each, l = RN(sqrt(ee)) is within
+// 3 eps l of |e|, and |e| <= 2 (|u| + |delta_i|).
+//  * Per-candidate formula: the fma forms ue - ee / 2 with one rounding from
+//    the same ee and ue, and with rsq accurate to 2 ulp |a_i - |d_i|| <=
+//    2^-20 (|u| + |d_i|).
+//  * Table: e rsq(ee) is within 1.5 2^-22 + 2^-23 < 2^-21 of n_i, the half
+//    length 0.5 ee rsq(ee) within 2^-21 L / 2 of L / 2 (L = |c_i - c0|), u
+//    within eps |u|, and the two fma add eps (|u| + L / 2) + eps |a_i|; with
+//    L <= 2 (|u| + |delta_i|), |a_i - delta_i| <= 2^-19.9 |u| + 2^-20.9
+//    |delta_i| + eps |a_i|.
+// Packing truncates a_i by at most 2^-16 relative.  So (both forms)
 //   a_2 (1 - 2^-13) > a_1 (1 + 2^-13) + 2^-17 |u|
 // implies |d_j| > |d_w| (1 + 2^-22) for every other candidate j: RN(d_j^2) >
 // RN(d_w^2) strictly, and the reference's loop over the K nearest (smallest
@@ -647,23 +699,15 @@ __device__ __forceinline__ void keepTwo(uint32_t &a1, uint32_t &a2, uint32_t k)
 // the caller runs the exact selection for that lane.
 template <int K>
 __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const ScreenBounds &sb, V2<float> p, bool active,
-                                           uint32_t mask, LaneTerm<float> &o)
+                                           uint32_t mask, int s0, bool nearOk, LaneTerm<float> &o)
 {
     constexpr uint32_t kIdxMask = uint32_t(kLossStage - 1);
     static_assert((kLossStage & (kLossStage - 1)) == 0, "slot mask");
-    uint32_t b1 = ~0u, b2 = ~0u;
-#pragma unroll 1
-    for (int g = 0; g < sb.mp1; g += kScreenGroup) {
-#pragma unroll
-        for (int k = 0; k < kScreenGroup; ++k) keepTwo(b1, b2, packKey(mask, dist2(lds.pos[g + k], p), g + k));
-    }
-    const bool      nearOk = (b1 ^ b2) > kIdxMask;
-    const int       s0     = int(b1 & kIdxMask);
-    const V2<float> c0     = lds.pos[s0];
+    const V2<float> c0 = lds.pos[s0];
     const V2<float> u{p.x - c0.x, p.y - c0.y};
-    // uB >= |u|: the key's value bounds d2 from above, and v_sqrt_f32 is within
+    // uB >= |u|: |u|^2 <= RN(|u|^2) (1 + 2^-23), and v_sqrt_f32 is within
     // 1 ulp on normal inputs (smaller ones are raised to 2^-100).
-    const float     uB = __builtin_amdgcn_sqrtf(fmaxf(__uint_as_float(b1 | kIdxMask), 0x1p-100f)) * (1.0f + 0x1p-20f);
+    const float     uB = __builtin_amdgcn_sqrtf(fmaxf(dotR(u, u), 0x1p-100f)) * (1.0f + 0x1p-20f);
     // dq >= |p - q| for this target (at most rho): the rounded distance, raised
     // past the errors of dist2 and v_sqrt_f32, plus the margin.  For K >= 16 the
     // chunk-wide rho is used: the outer band and the count are rarely needed
@@ -685,7 +729,22 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
             }
         }
     };
-    screenGroups(0, sb.mp2);
+    if (sb.rowMask) {
+        // The row of c0: lanes whose target is not taken read row 0.
+        const int     row = active ? __builtin_popcount(sb.rowMask & ((1u << s0) - 1u)) * sb.cols : 0;
+        const float2 *tn  = lds.u.t.n + row;
+        const float  *to  = lds.u.t.off + row;
+#pragma unroll 1
+        for (int g = 0; g < sb.mp2; g += kScreenGroup) {
+#pragma unroll
+            for (int k = 0; k < kScreenGroup; ++k) {
+                const float2 n = tn[g + k];
+                keepTwo(a1, a2, packKey(mask, __builtin_fmaf(u.x, n.x, __builtin_fmaf(u.y, n.y, to[g + k])), g + k));
+            }
+        }
+    } else {
+        screenGroups(0, sb.mp2);
+    }
     // Outer band: |d_i| >= (|p - c_i| - |p - c0|) / 2 >= (r2 - |p - q| - |u|) / 2.
     // When that exceeds the runner-up (with room for the screen's error) in
     // every lane, no outer candidate can be the winner or the runner-up.
@@ -699,7 +758,8 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     const float v1 = __uint_as_float(a1 & ~kIdxMask);
     const float v2 = __uint_as_float(a2 & ~kIdxMask);
     bool        ok = nearOk && a1 < kPackInf &&
-              (a2 >= kPackInf || __builtin_fmaf(v2, -0x1p-13f, v2) > __builtin_fmaf(v1, 0x1p-13f, v1) + uB * 0x1p-17f);
+              (a2 >= kPackInf ||
+               __builtin_fmaf(v2, -0x1p-13f, v2) > __builtin_fmaf(v1, 0x1p-13f, v1) + uB * 0x1p-17f);
     const V2<float> cw  = lds.pos[w];
     const float     d2w = dist2(cw, p);
     // Every site within |p - w| of p lies within |p - w| + |p - q| of q, and
@@ -729,9 +789,36 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     o.l            = sqrtNormal(ee);
     ok             = ok && ee >= 0x1p-90f && !(o.l < 1e-12f);
     o.d   = fmaR(-0.5f, o.l, dotR(u, o.ev) / o.l);
-    o.i0  = keySite(lds.klo[s0]), o.sl0 = s0, o.c0 = c0;
-    o.bj  = keySite(lds.klo[w]), o.slj = w, o.used = 1;
+    o.i0  = lds.idx[s0], o.sl0 = s0, o.c0 = c0;
+    o.bj  = lds.idx[w], o.slj = w, o.used = 1;
     return ok;
+}
+
+// Screen table rows for the distinct nearest sites of a staged chunk (slots in
+// rowMask, all below 32) over its first `cols` candidates, rows `stride`
+// entries apart; candidates from slot m on are padding and get NaN entries.
+// Every lane of the wave takes part.
+__device__ __forceinline__ void buildScreenTable(LossLds<float> &lds, uint32_t rowMask, int cols, int stride, int m,
+                                                 int lane)
+{
+    const float nan = __builtin_nanf("");
+    int         row = 0;
+    for (uint32_t rm = rowMask; rm; rm &= rm - 1u, row += stride) {
+        const V2<float> c0 = lds.pos[__builtin_ctz(rm)];
+        for (int i = lane; i < cols; i += 64) {
+            float nx = nan, ny = nan, off = nan;
+            if (i < m) {
+                const V2<float> c  = lds.pos[i];
+                const float     ex = c.x - c0.x, ey = c.y - c0.y;
+                const float     ee = __builtin_fmaf(ey, ey, ex * ex);
+                const float     rs = __builtin_amdgcn_rsqf(ee); // c_i == c0: infinity, and the entry is NaN
+                nx = ex * rs, ny = ey * rs;
+                off = -0.5f * (ee * rs);
+            }
+            lds.u.t.n[row + i]   = float2{nx, ny};
+            lds.u.t.off[row + i] = off;
+        }
+    }
 }
 
 // Gradient of the term d^2 for the pair (c0, cj) (voro_optimizer.cu:244-254).
@@ -877,17 +964,18 @@ template <int K>
 __device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float> pt, int lane, int gcopies, int gM,
                                             bool grad, double gscale, float &lossT, int &i0, int &j, int &sat)
 {
-    for (int s = lane; s < m; s += 64) lds.xkey[s] = Key<float>::make(dist2(lds.pos[s], pt), lds.klo[s]).k;
+    for (int s = lane; s < m; s += 64)
+        lds.u.x.xkey[s] = Key<float>::make(dist2(lds.pos[s], pt), (uint32_t(lds.idx[s]) << kLdsSlotBits) | uint32_t(s)).k;
     waveSync();
     for (int s = lane; s < m; s += 64) {
-        const uint64_t ks   = lds.xkey[s];
+        const uint64_t ks   = lds.u.x.xkey[s];
         int            rank = 0;
 #pragma unroll 4
-        for (int t = 0; t < m; ++t) rank += lds.xkey[t] < ks ? 1 : 0;
-        if (rank < K) lds.xtop[rank] = s;
+        for (int t = 0; t < m; ++t) rank += lds.u.x.xkey[t] < ks ? 1 : 0;
+        if (rank < K) lds.u.x.xtop[rank] = s;
     }
     waveSync();
-    const int       s0 = lds.xtop[0];
+    const int       s0 = lds.u.x.xtop[0];
     const V2<float> c0 = lds.pos[s0];
     const V2<float> u{pt.x - c0.x, pt.y - c0.y};
     float           dd = rInf<float>(), d = 0.0f, l = 0.0f;
@@ -895,7 +983,7 @@ __device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float
     int             sj = 0;
     bool            ok = false;
     if (lane >= 1 && lane < K) {
-        sj                 = lds.xtop[lane];
+        sj                 = lds.u.x.xtop[lane];
         const V2<float> cj = lds.pos[sj];
         ev                 = V2<float>{cj.x - c0.x, cj.y - c0.y};
         l                  = sqrtR(dotR(ev, ev));
@@ -1108,20 +1196,11 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                 if (dst0 >= 0) p0 = lds.pos[e0], i0v = lds.idx[e0];
                 if (dst1 >= 0) p1 = lds.pos[e1], i1v = lds.idx[e1];
                 waveSync();
-                if (dst0 >= 0) {
-                    lds.pos[dst0] = p0, lds.idx[dst0] = i0v;
-                    lds.klo[dst0] = (uint32_t(i0v) << kLdsSlotBits) | uint32_t(dst0);
-                }
-                if (dst1 >= 0) {
-                    lds.pos[dst1] = p1, lds.idx[dst1] = i1v;
-                    lds.klo[dst1] = (uint32_t(i1v) << kLdsSlotBits) | uint32_t(dst1);
-                }
+                if (dst0 >= 0) lds.pos[dst0] = p0, lds.idx[dst0] = i0v;
+                if (dst1 >= 0) lds.pos[dst1] = p1, lds.idx[dst1] = i1v;
                 // Pad to whole screening groups with far sentinels.
                 const int mp = (m + kScreenGroup - 1) & ~(kScreenGroup - 1);
-                if (lane < mp - m) {
-                    lds.pos[m + lane] = V2<R>{R(kFarSite), R(kFarSite)};
-                    lds.klo[m + lane] = ~0u;
-                }
+                if (lane < mp - m) lds.pos[m + lane] = V2<R>{R(kFarSite), R(kFarSite)};
                 // Clear the gradient copies.
                 gcopies = m <= kLossStage / 4 ? 4 : (m <= kLossStage / 2 ? 2 : 1);
                 gM      = m <= kLossStage / 4 ? kLossStage / 4 : (m <= kLossStage / 2 ? kLossStage / 2 : kLossStage);
@@ -1145,6 +1224,35 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
         // Open targets of a staged chunk take the exact selection inside the
         // chunk; the k-NN list diagnostics and exact-only runs queue them.
         const bool inl = fast && !exactOnly && !dbgKnn;
+        const bool scr = fast && !exactOnly; // screened terms (FP32 only)
+        // ---- nearest candidate of every target, then the screen table over
+        //      the chunk's distinct nearest sites (when their slots are below
+        //      32 and the rows fit) ---------------------------------------------
+        uint32_t nearPack = 0; // pass h: bits 8h..8h+6 the nearest slot, bit 8h+7 set when certified
+        if constexpr (sizeof(R) == 4) {
+            if (scr) {
+                uint32_t rows = 0;
+                bool     high = false;
+#pragma unroll
+                for (int h = 0; h < TPL; ++h) {
+                    if (__ballot(take[h]) == 0) continue;
+                    bool      nok;
+                    const int s0 = nearestCandidate(lds, sb, p[h], mask, nok);
+                    nearPack |= uint32_t(s0 | (nok ? 0x80 : 0)) << (8 * h);
+                    if (take[h]) {
+                        if (s0 < 32) rows |= 1u << s0;
+                        else high = true;
+                    }
+                }
+                const uint32_t rowMask = waveReduce32<uint32_t>(rows, 0u, [](uint32_t x, uint32_t y) { return x | y; });
+                const int      nrows   = __builtin_popcount(rowMask);
+                if (__ballot(high) == 0 && nrows * (sb.mp2 + kTblPad) <= kTblEntries) {
+                    buildScreenTable(lds, rowMask, sb.mp2, sb.mp2 + kTblPad, m, lane);
+                    sb.rowMask = rowMask, sb.cols = sb.mp2 + kTblPad;
+                    waveSync();
+                }
+            }
+        }
         // ---- each target: screened term, or queued for the exact selection
         //      (for the k-NN list diagnostics every taken target is queued; bit
         //      31 of the instance marks a term already settled here) ------------
@@ -1152,15 +1260,18 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
         int      used    = 0;
         int      openCnt = 0;
         int      sat     = 0; // targets whose gradient terms were clamped
+        uint64_t openM[TPL]; // open targets of each pass (exact selection inside the chunk)
 #pragma unroll
         for (int h = 0; h < TPL; ++h) {
+            openM[h] = 0;
             if (__ballot(take[h] || (dbgKnn && lane + 64 * h < wk.count)) == 0) continue;
             const int64_t t  = wk.tBegin + lane + 64 * h;
             LaneTerm<R> tr;
             bool        open = take[h];
             if constexpr (sizeof(R) == 4) {
-                if (fast && !exactOnly) {
-                    if (screenTerm<K>(lds, sb, p[h], take[h], mask, tr)) open = false;
+                if (scr) {
+                    const uint32_t np = nearPack >> (8 * h);
+                    if (screenTerm<K>(lds, sb, p[h], take[h], mask, int(np & 0x7Fu), (np & 0x80u) != 0, tr)) open = false;
                     tr.used = take[h] && !open ? tr.used : 0;
                 }
             }
@@ -1192,9 +1303,14 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             }
             lossSum += double(lossT);
             used += tr.used;
-            if constexpr (sizeof(R) == 4) {
-                // Targets the screen left open: the exact selection inside the chunk.
-                uint64_t om = inl ? __ballot(open) : 0ull;
+            if constexpr (sizeof(R) == 4) openM[h] = inl ? __ballot(open) : 0ull;
+        }
+        // ---- targets the screen left open: the exact selection inside the
+        //      chunk (its keys take the table's space) --------------------------
+        if constexpr (sizeof(R) == 4) {
+#pragma unroll
+            for (int h = 0; h < TPL; ++h) {
+                uint64_t om = openM[h];
                 while (om) {
                     const int l = __builtin_ctzll(om);
                     om &= om - 1;
