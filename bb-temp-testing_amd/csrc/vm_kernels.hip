@@ -83,6 +83,24 @@ __device__ __forceinline__ double fmaR(double a, double b, double c) { return __
 __device__ __forceinline__ float  sqrtR(float a) { return sqrtf(a); }
 __device__ __forceinline__ double sqrtR(double a) { return sqrt(a); }
 
+// Correctly rounded quotients a / b for 2^-60 <= |b| <= 2^60 and a == 0 or
+// 2^-70 <= |a| <= 2^60 (so the quotient and both residuals below stay normal):
+// the core of the division sequence (reciprocal, one refinement, quotient and
+// two residual corrections), whose operand scaling and special-case fixup
+// leave such operands unchanged.  recipRefined(b) is the refined reciprocal
+// the quotients by b share.
+__device__ __forceinline__ float recipRefined(float b)
+{
+    const float r = __builtin_amdgcn_rcpf(b);
+    return __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+}
+__device__ __forceinline__ float divCore(float a, float b, float r1)
+{
+    const float q0 = a * r1;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q0, a), r1, q0);
+    return __builtin_fmaf(__builtin_fmaf(-b, q1, a), r1, q1);
+}
+
 // Correctly rounded sqrt for finite x >= 2^-96: v_sqrt_f32 is within 1 ulp,
 // and the residuals x - s' s of its neighbours s' pick the rounded root (the
 // same steps sqrtf takes there, without its rescaling of tiny inputs and its
@@ -586,7 +604,7 @@ template <class R> struct LossLds {
 template <class R> struct LaneTerm {
     int   i0 = -1, sl0 = kLdsNone, bj = -1, slj = kLdsNone, used = 0;
     V2<R> c0{R(0), R(0)}, ev{R(0), R(0)};
-    R     d = R(0), l = R(0);
+    R     d = R(0), l = R(0), rl = R(0); // rl = RN(1 / l)
 };
 
 // Screening walks the compacted candidates in groups of kScreenGroup; the list
@@ -788,7 +806,14 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     const float ee = dotR(o.ev, o.ev);
     o.l            = sqrtNormal(ee);
     ok             = ok && ee >= 0x1p-90f && !(o.l < 1e-12f);
-    o.d   = fmaR(-0.5f, o.l, dotR(u, o.ev) / o.l);
+    // ue / l and 1 / l from one refined reciprocal (1e-12 <= l <= 2 ext
+    // wherever the term is used); a numerator below 2^-70 other than 0 takes
+    // the full division.
+    const float ue = dotR(u, o.ev);
+    const float r1 = recipRefined(o.l);
+    if (__ballot(ok && ue != 0.0f && fabsf(ue) < 0x1p-70f)) o.d = fmaR(-0.5f, o.l, ue / o.l);
+    else o.d = fmaR(-0.5f, o.l, divCore(ue, o.l, r1));
+    o.rl  = divCore(1.0f, o.l, r1);
     o.i0  = lds.idx[s0], o.sl0 = s0, o.c0 = c0;
     o.bj  = lds.idx[w], o.slj = w, o.used = 1;
     return ok;
@@ -825,10 +850,10 @@ __device__ __forceinline__ void buildScreenTable(LossLds<float> &lds, uint32_t r
 // With u = p - c0, e = cj - c0, l = |e|, n = e / l:
 //   dd/dcj = (u - <u, n> n) / l - n / 2,   dd/dc0 = -n - dd/dcj,
 // scaled by 2d and added to the accumulators of cj (dj) and c0 (d0).
+// rl is the correctly rounded 1 / l.
 template <class R>
-__device__ __forceinline__ int lossGrad(V2<R> u, V2<R> ev, R l, R d, GradT<R> *dj, GradT<R> *d0, double gscale)
+__device__ __forceinline__ int lossGrad(V2<R> u, V2<R> ev, R rl, R d, GradT<R> *dj, GradT<R> *d0, double gscale)
 {
-    const R     rl = R(1) / l;
     const V2<R> n{rl * ev.x, rl * ev.y};
     const R     un = dotR(u, n);
     const R     tx = fmaR(-un, n.x, u.x), ty = fmaR(-un, n.y, u.y);
@@ -998,7 +1023,7 @@ __device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float
     const int      wl   = win ? __builtin_ctzll(win) : -1;
     if (lane == wl) {
         lossT = best, i0 = lds.idx[s0], j = lds.idx[sj];
-        sat   = grad ? lossGrad(u, ev, l, d, &lds.grad[2 * gradSlot(sj, lane & (gcopies - 1), gM)],
+        sat   = grad ? lossGrad(u, ev, 1.0f / l, d, &lds.grad[2 * gradSlot(sj, lane & (gcopies - 1), gM)],
                               &lds.grad[2 * gradSlot(s0, lane & (gcopies - 1), gM)], gscale)
                      : 0;
     }
@@ -1288,7 +1313,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             openCnt += __popcll(__ballot(open));
             const R lossT = tr.used ? tr.d * tr.d : R(0);
             if (tr.used && a.grad)
-                sat += lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.l, tr.d,
+                sat += lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.rl, tr.d,
                          &lds.grad[2 * gradSlot(tr.slj, lane & (gcopies - 1), gM)],
                          &lds.grad[2 * gradSlot(tr.sl0, lane & (gcopies - 1), gM)], __builtin_ldexp(1.0, in.gradShift));
             if (dbgKnn && lane + 64 * h < wk.count && !take[h]) {
@@ -1573,7 +1598,7 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
         const R        lossT = win ? best : R(0);
         if (lane == wl) {
             if (a.grad &&
-                lossGrad(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, tr.ev, tr.l, tr.d, &a.grad[2 * (so + tr.bj)],
+                lossGrad(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, tr.ev, R(1) / tr.l, tr.d, &a.grad[2 * (so + tr.bj)],
                          &a.grad[2 * (so + tr.i0)], __builtin_ldexp(1.0, in.gradShift)))
                 atomicAdd(reinterpret_cast<unsigned long long *>(&a.state[inst].saturated), 1ull);
             if (a.mode == kModeIterate) {
