@@ -390,29 +390,51 @@ __device__ __forceinline__ void scanTile(const ScanTile &tl, int tile, bool dyn,
     }
 
     // ---- tile prefix (look-back over the instance's earlier tiles) ---------
-    if (threadIdx.x == 0) {
+    // Wave 0 reads a window of up to 64 predecessor flags at once (lane l the
+    // flag of tile j0 - l); the nearest tile with a prefix (or a skip) ends
+    // the walk once every flag before it has its aggregate, otherwise the
+    // window's aggregates are added and the window moves back.  The instance's
+    // first tile always publishes a prefix or a skip.
+    if (threadIdx.x < 64) {
         int skip = 0, prefix = 0;
         if (tl.first) {
-            skip = beginStep(si, state + tl.inst, mode);
-            if (dyn) publishTile(&flags[tile], epoch, skip ? kTileSkip : kTilePrefix, btot);
-        } else {
-            publishTile(&flags[tile], epoch, kTileAggregate, btot);
-            for (int j = tile - 1;; --j) {
-                unsigned long long f;
-                do {
-                    f = __hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                } while (uint32_t(f >> 34) != (epoch & 0x3FFFFFFFu) || ((f >> 32) & 3u) == 0);
-                const uint32_t status = uint32_t(f >> 32) & 3u;
-                if (status == kTileSkip) {
-                    skip = 1;
-                    break;
-                }
-                prefix += int(uint32_t(f));
-                if (status == kTilePrefix) break;
+            if (lane == 0) {
+                skip = beginStep(si, state + tl.inst, mode);
+                if (dyn) publishTile(&flags[tile], epoch, skip ? kTileSkip : kTilePrefix, btot);
             }
-            publishTile(&flags[tile], epoch, skip ? kTileSkip : kTilePrefix, prefix + btot);
+        } else {
+            if (lane == 0) publishTile(&flags[tile], epoch, kTileAggregate, btot);
+            const int      jmin = tile - tl.begin / kScanTileCells; // the instance's first tile
+            const uint32_t ep   = epoch & 0x3FFFFFFFu;
+            for (int j0 = tile - 1;;) {
+                const int          j  = j0 - lane;
+                unsigned long long f  = 0;
+                if (j >= jmin) f = __hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t st    = uint32_t(f >> 32) & 3u;
+                const bool     valid = j >= jmin && uint32_t(f >> 34) == ep && st != 0u;
+                const uint64_t have  = __ballot(valid);
+                const uint64_t stop  = __ballot(valid && st != kTileAggregate);
+                const uint64_t inWin = __ballot(j >= jmin);
+                const uint64_t need  = stop ? (~0ull >> (63 - __builtin_ctzll(stop))) : inWin;
+                if ((have & need) != need) { // a flag the walk needs is not published yet
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                const int v   = (need >> lane) & 1ull ? int(uint32_t(f)) : 0;
+                const int sum = __builtin_amdgcn_readlane(waveInclusiveSum(v), 63);
+                if (!stop) {
+                    prefix += sum;
+                    j0 -= 64;
+                    continue;
+                }
+                const int p = __builtin_ctzll(stop);
+                skip        = __builtin_amdgcn_readlane(int(st), p) == int(kTileSkip);
+                prefix += sum;
+                break;
+            }
+            if (lane == 0) publishTile(&flags[tile], epoch, skip ? kTileSkip : kTilePrefix, prefix + btot);
         }
-        sShared[0] = skip, sShared[1] = prefix;
+        if (lane == 0) sShared[0] = skip, sShared[1] = prefix;
     }
     __syncthreads();
     if (!sShared[0]) {
