@@ -15,19 +15,19 @@ struct TuneRow {
 constexpr int     kTuneLog2Min = 8;
 constexpr int     kTuneLog2Max = 22;
 constexpr TuneRow kTuneTable[kTuneLog2Max - kTuneLog2Min + 1] = {
-    {1.00f, 2.20f, 3, 8}, // 2^8: 2.041e+09 site-iters/s at k=16, 2.104e+09 at k=8
-    {1.00f, 2.20f, 3, 8}, // 2^9: 2.778e+09 site-iters/s at k=16, 2.921e+09 at k=8
-    {1.00f, 2.20f, 3, 12}, // 2^10: 3.433e+09 site-iters/s at k=16, 3.596e+09 at k=12
-    {1.00f, 2.20f, 3, 12}, // 2^11: 3.835e+09 site-iters/s at k=16, 4.070e+09 at k=12
-    {1.00f, 1.60f, 3, 12}, // 2^12: 4.314e+09 site-iters/s at k=16, 4.458e+09 at k=12
-    {1.00f, 4.00f, 3, 16}, // 2^13: 4.380e+09 site-iters/s at k=16, 4.380e+09 at k=16
-    {1.00f, 2.20f, 3, 12}, // 2^14: 3.705e+09 site-iters/s at k=16, 3.871e+09 at k=12
-    {0.90f, 3.00f, 3, 12}, // 2^15: 4.891e+09 site-iters/s at k=16, 5.034e+09 at k=12
-    {1.00f, 2.20f, 3, 12}, // 2^16: 4.901e+09 site-iters/s at k=16, 5.045e+09 at k=12
-    {0.90f, 2.20f, 3, 12}, // 2^17: 5.003e+09 site-iters/s at k=16, 5.317e+09 at k=12
-    {1.00f, 2.20f, 3, 16}, // 2^18: 4.252e+09 site-iters/s at k=16, 4.252e+09 at k=16
-    {0.80f, 2.20f, 3, 16}, // 2^19: 4.609e+09 site-iters/s at k=16, 4.609e+09 at k=16
-    {0.90f, 2.20f, 3, 16}, // 2^20: 5.499e+09 site-iters/s at k=16, 5.499e+09 at k=16
+    {1.00f, 2.20f, 3, 8}, // 2^8: 1.991e+09 site-iters/s at k=16, 2.035e+09 at k=8
+    {1.00f, 2.20f, 3, 8}, // 2^9: 2.751e+09 site-iters/s at k=16, 2.847e+09 at k=8
+    {1.00f, 2.20f, 3, 12}, // 2^10: 3.462e+09 site-iters/s at k=16, 3.578e+09 at k=12
+    {1.00f, 2.20f, 3, 12}, // 2^11: 3.890e+09 site-iters/s at k=16, 4.057e+09 at k=12
+    {1.00f, 1.60f, 3, 12}, // 2^12: 4.371e+09 site-iters/s at k=16, 4.505e+09 at k=12
+    {1.00f, 2.20f, 3, 16}, // 2^13: 4.435e+09 site-iters/s at k=16, 4.435e+09 at k=16
+    {1.00f, 1.60f, 3, 12}, // 2^14: 3.865e+09 site-iters/s at k=16, 4.076e+09 at k=12
+    {1.00f, 2.20f, 3, 12}, // 2^15: 4.808e+09 site-iters/s at k=16, 5.103e+09 at k=12
+    {1.00f, 2.20f, 3, 16}, // 2^16: 4.968e+09 site-iters/s at k=16, 4.968e+09 at k=16
+    {0.90f, 2.20f, 3, 12}, // 2^17: 5.113e+09 site-iters/s at k=16, 5.255e+09 at k=12
+    {1.00f, 2.20f, 3, 16}, // 2^18: 4.358e+09 site-iters/s at k=16, 4.358e+09 at k=16
+    {0.90f, 2.20f, 3, 16}, // 2^19: 4.632e+09 site-iters/s at k=16, 4.632e+09 at k=16
+    {0.90f, 2.20f, 3, 16}, // 2^20: 5.726e+09 site-iters/s at k=16, 5.726e+09 at k=16
     {0.90f, 2.20f, 3, 16}, // 2^21: largest measured class
     {0.90f, 2.20f, 3, 16}, // 2^22: largest measured class
 };
