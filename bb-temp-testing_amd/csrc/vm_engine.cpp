@@ -302,7 +302,8 @@ public:
     DBuf<vmk::SiteWork>  dSiteWork;
     DBuf<vmk::TargetWork> dTargetWork;
     DBuf<vmk::ExactItem> dExactQ;
-    DBuf<int>            dExactLen; // one per group
+    DBuf<int>            dExactLen; // one per group, then the drain's two counters (claimed, finished)
+    DBuf<vmk::ExactQueue> dQueues;  // each group's queue offset and k (the FP32 drain in the Adam launch)
     hipGraphExec_t       graph = nullptr, graphN = nullptr; // one iteration, kGraphIters iterations
     bool                 stale1 = false, staleN = false;   // a commit changed their launches
     hipEvent_t           ev[8] = {};
@@ -336,7 +337,7 @@ public:
         b += dFrozen.n + dInst.n * sizeof(vmk::Inst) + dState.n * sizeof(vmk::InstState) +
              dPartial.n * sizeof(vmk::LossPartial);
         b += dSiteWork.n * sizeof(vmk::SiteWork) + dTargetWork.n * sizeof(vmk::TargetWork);
-        b += dExactQ.n * sizeof(vmk::ExactItem) + dExactLen.n * sizeof(int);
+        b += dExactQ.n * sizeof(vmk::ExactItem) + dExactLen.n * sizeof(int) + dQueues.n * sizeof(vmk::ExactQueue);
         return b;
     }
 
@@ -418,7 +419,8 @@ public:
         const bool tail = n == 1 && tailTargets > 0;
         dSiteWork.reserve(L.siteWork.size()), dTargetWork.reserve(L.targetWork.size() + (tail ? size_t(tailWork) : 0));
         dCnt.reserve(2 + L.groups.size());
-        dExactQ.reserve(std::max<int64_t>(L.queue, 1)), dExactLen.reserve(L.groups.size());
+        dExactQ.reserve(std::max<int64_t>(L.queue, 1)), dExactLen.reserve(L.groups.size() + 2);
+        dQueues.reserve(std::max<size_t>(L.groups.size(), 1));
 
         std::vector<V>       hp(L.sites), hm(L.sites), hv(L.sites), ht(L.targets, V{R(-1), R(-1)});
         std::vector<int>     hts(L.targets, -1);
@@ -465,6 +467,9 @@ public:
         const std::vector<vmk::ScanCtl> ctl{{0, 0u, int(L.tiles.size()), L.dynTiles}};
         up(dScanCtl.p, ctl);
         up(dTargetWork.p, L.targetWork), up(dCnt.p, cnt);
+        std::vector<vmk::ExactQueue> queues;
+        for (const auto &g : L.groups) queues.push_back(vmk::ExactQueue{g.qOff, g.K, 0});
+        up(dQueues.p, queues);
         const bool staged = total <= kPinnedUploadMax;
         if (staged) hUpload.reserve(total);
         for (const Up &u : ups) {
@@ -623,12 +628,24 @@ public:
         return a;
     }
 
-    vmk::AdamArgs<R> adamArgs(int doAdam)
+    // FP32 iterations drain the exact-selection queues in the Adam launch.
+    static constexpr bool kDrainInAdam = sizeof(R) == 4;
+
+    vmk::AdamArgs<R> adamArgs(int doAdam, bool drain = false)
     {
         vmk::AdamArgs<R> a{};
         a.insts = dInst.p, a.state = dState.p, a.work = dSiteWork.p, a.nWork = dCnt.p + 1;
         a.pos = dPos.p, a.m = dM.p, a.v = dV.p, a.grad = dGrad.p, a.frozen = dFrozen.p, a.partial = dPartial.p;
         a.cellOf = dCellOf.p, a.rank = dRank.p, a.counts = dCounts.p, a.doAdam = doAdam;
+        if (drain && !L.groups.empty()) {
+            a.drainQueues = int(L.groups.size());
+            a.queues      = dQueues.p;
+            a.drainCtl    = dExactLen.p + L.groups.size();
+            a.la          = lossArgs(0, vmk::kModeIterate, true);
+            a.la.exactQ   = dExactQ.p;
+            a.la.exactLen = dExactLen.p;
+            a.la.exactOnly = 0;
+        }
         return a;
     }
 
@@ -642,7 +659,7 @@ public:
         if (L.dynTiles == 0)
             sc = vmk::ScanScatter{dPos.p, dCellOf.p, dRank.p, dSpos.p, dSidx.p, int(sizeof(V))};
         VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dTiles.p, dScanCtl.p, dFlags.p,
-                               dCounts.p, dStart.p, mode, dExactLen.p, int(L.groups.size()), sc));
+                               dCounts.p, dStart.p, mode, dExactLen.p, int(L.groups.size()) + 2, sc));
         if (L.dynTiles != 0)
             VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
                                          dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, mode, dScanCtl.p));
@@ -670,7 +687,7 @@ public:
         if (L.dynTiles == 0)
             sc = vmk::ScanScatter{dPos.p, dCellOf.p, dRank.p, dSpos.p, dSidx.p, int(sizeof(V))};
         VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dTiles.p, dScanCtl.p, dFlags.p,
-                               dCounts.p, dStart.p, vmk::kModeIterate, dExactLen.p, int(L.groups.size()), sc));
+                               dCounts.p, dStart.p, vmk::kModeIterate, dExactLen.p, int(L.groups.size()) + 2, sc));
         if (marks) VM_HIP(hipEventRecord(marks[1], stream));
         if (L.dynTiles != 0)
             VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
@@ -681,7 +698,7 @@ public:
             VM_HIP(vmk::launchLoss<R>(L.groups[0].K, L.groups[0].tpl, gridLoss[0], stream,
                                       lossArgs(0, vmk::kModeIterate, true)));
             if (marks) VM_HIP(hipEventRecord(marks[3], stream));
-            VM_HIP(vmk::launchExact<R>(L.groups[0].K, stream, lossArgs(0, vmk::kModeIterate, true)));
+            if (!kDrainInAdam) VM_HIP(vmk::launchExact<R>(L.groups[0].K, stream, lossArgs(0, vmk::kModeIterate, true)));
         } else if (!L.groups.empty()) {
             // Each group's loss and exact selection on its own stream (the groups
             // touch disjoint instances); the main stream waits for all of them.
@@ -693,14 +710,14 @@ public:
                 if (g > 0) VM_HIP(hipStreamWaitEvent(sg, evFork, 0));
                 VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], sg,
                                           lossArgs(int(g), vmk::kModeIterate, true)));
-                VM_HIP(vmk::launchExact<R>(L.groups[g].K, sg, lossArgs(int(g), vmk::kModeIterate, true)));
+                if (!kDrainInAdam) VM_HIP(vmk::launchExact<R>(L.groups[g].K, sg, lossArgs(int(g), vmk::kModeIterate, true)));
                 if (g > 0) VM_HIP(hipEventRecord(evJoin[g - 1], sg));
             }
             for (size_t g = 1; g < L.groups.size(); ++g) VM_HIP(hipStreamWaitEvent(stream, evJoin[g - 1], 0));
             if (marks) VM_HIP(hipEventRecord(marks[3], stream));
         }
         if (marks) VM_HIP(hipEventRecord(marks[4], stream));
-        VM_HIP(vmk::launchAdamBin<R>(gridSite, stream, adamArgs(1)));
+        VM_HIP(vmk::launchAdamBin<R>(gridSite, stream, adamArgs(1, kDrainInAdam)));
         if (marks) VM_HIP(hipEventRecord(marks[5], stream));
     }
 

@@ -1454,7 +1454,7 @@ template <class R, bool P> struct ExactLds {
 };
 
 // Sorts the K smallest of the distinct keys cand[0, nc) into cand[0, min(K, nc)).
-template <int K, class R, bool P> __device__ __forceinline__ int keepNearest(ExactLds<R, P> &lds, int nc, int lane)
+template <class R, bool P> __device__ __forceinline__ int keepNearest(ExactLds<R, P> &lds, int nc, int lane, int K)
 {
     for (int j = lane; j < nc; j += 64) {
         const Key<R> kj   = lds.cand[j];
@@ -1491,9 +1491,9 @@ __device__ __forceinline__ void ringCell(int r, int q, int cx, int cy, int &x, i
 // the 5 x 5 block (rings 0-2) in one round, then ring by ring in rounds of 64
 // cells; a ring ends the search when the K-th distance is below the distance to
 // the searched block's edge (less a rounding margin).
-template <int K, class R, class KR, bool P>
+template <class R, class KR, bool P>
 __device__ __forceinline__ int gatherKnn(ExactLds<KR, P> &lds, const Inst &in, const V2<R> *spos, const int *sidx,
-                                         const int *strt, V2<R> pr, int lane)
+                                         const int *strt, V2<R> pr, int lane, int K)
 {
     const int    gw = in.gw, gh = in.gh;
     const R      inv    = R(1) / R(in.cell);
@@ -1540,10 +1540,10 @@ __device__ __forceinline__ int gatherKnn(ExactLds<KR, P> &lds, const Inst &in, c
                 waveSync();
                 nc += slice;
                 s0 += slice;
-                if (nc == kExactCand) nc = keepNearest<K>(lds, nc, lane);
+                if (nc == kExactCand) nc = keepNearest(lds, nc, lane, K);
             }
         }
-        nc = keepNearest<K>(lds, nc, lane);
+        nc = keepNearest(lds, nc, lane, K);
         const int x0 = cx - r, x1 = cx + r, y0 = cy - r, y1 = cy + r;
         if (x0 <= 0 && y0 <= 0 && x1 >= gw - 1 && y1 >= gh - 1) break;
         if (nc == K) {
@@ -1554,6 +1554,79 @@ __device__ __forceinline__ int gatherKnn(ExactLds<KR, P> &lds, const Inst &in, c
         }
     }
     return nc;
+}
+
+// The exact selection for queue entry e (item `it`) by one wavefront, with k
+// = K nearest sites.
+template <class R, bool P>
+__device__ __forceinline__ void exactItem(const LossArgs<R> &a, ExactLds<R, P> &lds, const ExactItem it, int e, int lane,
+                                          int K)
+{
+    const bool      listOnly = it.inst < 0;
+    const int       inst     = it.inst & 0x7FFFFFFF;
+    const Inst      in       = loadUniform(a.insts + inst);
+    const int64_t   so       = in.siteOff;
+    const V2<R>    *spos     = a.spos + so;
+    const int      *sidx     = a.sidx + so;
+    const int      *strt     = a.start + in.cellOff;
+    const V2<R>     p        = a.tpos[it.tg];
+    const int       nc       = gatherKnn<R, R, P>(lds, in, spos, sidx, strt, p, lane, K);
+    if (a.dbgKnn && lane < K)
+        a.dbgKnn[int64_t(it.tg) * K + lane] = lane < nc ? keySite(lds.cand[lane].low()) : -1;
+    if (listOnly) {
+        waveSync();
+        return;
+    }
+
+    // ---- the term: lane k evaluates neighbour k ----------------------------
+    LaneTerm<R> tr;
+    bool        ok = false;
+    R           dd = rInf<R>();
+    if (nc > 0) {
+        tr.i0 = keySite(lds.cand[0].low());
+        // The sorted copy of the positions holds the same values as a.pos.
+        if constexpr (P) tr.c0 = lds.pos[0];
+        else tr.c0 = a.pos[so + tr.i0];
+        if (lane >= 1 && lane < nc) {
+            const int   j  = keySite(lds.cand[lane].low());
+            V2<R>       cj;
+            if constexpr (P) cj = lds.pos[lane];
+            else cj = a.pos[so + j];
+            const V2<R> ev{cj.x - tr.c0.x, cj.y - tr.c0.y};
+            const R     l = sqrtR(dotR(ev, ev));
+            if (!(l < (sizeof(R) == 4 ? R(1e-12f) : R(1e-12)))) {
+                tr.d = fmaR(R(-0.5), l, dotR(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, ev) / l);
+                tr.ev = ev, tr.l = l, tr.bj = j;
+                dd    = tr.d * tr.d;
+                ok    = true;
+            }
+        }
+    }
+    waveSync();
+    const R        best = waveMinR<R>(dd);
+    const uint64_t win  = __ballot(ok && dd == best);
+    const int      wl   = win ? __builtin_ctzll(win) : -1;
+    const R        lossT = win ? best : R(0);
+    if (lane == wl) {
+        if (a.grad &&
+            lossGrad(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, tr.ev, R(1) / tr.l, tr.d, &a.grad[2 * (so + tr.bj)],
+                     &a.grad[2 * (so + tr.i0)], __builtin_ldexp(1.0, in.gradShift)))
+            atomicAdd(reinterpret_cast<unsigned long long *>(&a.state[inst].saturated), 1ull);
+        if (a.mode == kModeIterate) {
+            LossPartial *pp = a.partial + int64_t(inst) * kLossPartials + (e & (kLossPartials - 1));
+            unsafeAtomicAdd(&pp->loss, double(lossT));
+            atomicAdd(&pp->count, 1ull);
+        } else {
+            InstState *st = a.state + inst;
+            unsafeAtomicAdd(&st->evalLoss, double(lossT));
+            atomicAdd((unsigned long long *)&st->evalCount, 1ull);
+        }
+    }
+    if (a.dbgTerm && lane == (win ? wl : 0)) {
+        a.dbgTerm[it.tg].i0   = win ? tr.i0 : -1;
+        a.dbgTerm[it.tg].j    = win ? tr.bj : -1;
+        a.dbgTerm[it.tg].loss = double(lossT);
+    }
 }
 
 template <int K, class R>
@@ -1571,74 +1644,7 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
     const int    nw   = gridDim.x * kExactWaves;
     // Entry e goes to wavefront e % nw, numbered across workgroups first, so a
     // short queue spreads over the chip.
-    for (int e = blockIdx.x + gridDim.x * wave; e < n; e += nw) {
-        const ExactItem it       = loadUniform(a.exactQ + e);
-        const bool      listOnly = it.inst < 0;
-        const int       inst     = it.inst & 0x7FFFFFFF;
-        const Inst      in       = loadUniform(a.insts + inst);
-        const int64_t   so       = in.siteOff;
-        const V2<R>    *spos     = a.spos + so;
-        const int      *sidx     = a.sidx + so;
-        const int      *strt     = a.start + in.cellOff;
-        const V2<R>     p        = a.tpos[it.tg];
-        const int       nc       = gatherKnn<K, R, R, P>(lds, in, spos, sidx, strt, p, lane);
-        if (a.dbgKnn && lane < K)
-            a.dbgKnn[int64_t(it.tg) * K + lane] = lane < nc ? keySite(lds.cand[lane].low()) : -1;
-        if (listOnly) {
-            waveSync();
-            continue;
-        }
-
-        // ---- the term: lane k evaluates neighbour k ------------------------
-        LaneTerm<R> tr;
-        bool        ok = false;
-        R           dd = rInf<R>();
-        if (nc > 0) {
-            tr.i0 = keySite(lds.cand[0].low());
-            // The sorted copy of the positions holds the same values as a.pos.
-            if constexpr (P) tr.c0 = lds.pos[0];
-            else tr.c0 = a.pos[so + tr.i0];
-            if (lane >= 1 && lane < nc) {
-                const int   j  = keySite(lds.cand[lane].low());
-                V2<R>       cj;
-                if constexpr (P) cj = lds.pos[lane];
-                else cj = a.pos[so + j];
-                const V2<R> ev{cj.x - tr.c0.x, cj.y - tr.c0.y};
-                const R     l = sqrtR(dotR(ev, ev));
-                if (!(l < (sizeof(R) == 4 ? R(1e-12f) : R(1e-12)))) {
-                    tr.d = fmaR(R(-0.5), l, dotR(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, ev) / l);
-                    tr.ev = ev, tr.l = l, tr.bj = j;
-                    dd    = tr.d * tr.d;
-                    ok    = true;
-                }
-            }
-        }
-        waveSync();
-        const R        best = waveMinR<R>(dd);
-        const uint64_t win  = __ballot(ok && dd == best);
-        const int      wl   = win ? __builtin_ctzll(win) : -1;
-        const R        lossT = win ? best : R(0);
-        if (lane == wl) {
-            if (a.grad &&
-                lossGrad(V2<R>{p.x - tr.c0.x, p.y - tr.c0.y}, tr.ev, R(1) / tr.l, tr.d, &a.grad[2 * (so + tr.bj)],
-                         &a.grad[2 * (so + tr.i0)], __builtin_ldexp(1.0, in.gradShift)))
-                atomicAdd(reinterpret_cast<unsigned long long *>(&a.state[inst].saturated), 1ull);
-            if (a.mode == kModeIterate) {
-                LossPartial *pp = a.partial + int64_t(inst) * kLossPartials + (e & (kLossPartials - 1));
-                unsafeAtomicAdd(&pp->loss, double(lossT));
-                atomicAdd(&pp->count, 1ull);
-            } else {
-                InstState *st = a.state + inst;
-                unsafeAtomicAdd(&st->evalLoss, double(lossT));
-                atomicAdd((unsigned long long *)&st->evalCount, 1ull);
-            }
-        }
-        if (a.dbgTerm && lane == (win ? wl : 0)) {
-            a.dbgTerm[it.tg].i0   = win ? tr.i0 : -1;
-            a.dbgTerm[it.tg].j    = win ? tr.bj : -1;
-            a.dbgTerm[it.tg].loss = double(lossT);
-        }
-    }
+    for (int e = blockIdx.x + gridDim.x * wave; e < n; e += nw) exactItem<R, P>(a, lds, loadUniform(a.exactQ + e), e, lane, K);
 }
 // ---------------------------------------------------------------------------
 // 4. Adam + MultiStepLR + clamp (voro_optimizer.cu:268-301), then bin the new
@@ -1650,12 +1656,65 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
 // takes one global atomic per occupied cell; otherwise one per site.
 constexpr int kBinSpan = 2048;
 
+// FP32 iterations: the exact selection of the targets the loss launches queued
+// runs at the start of the Adam launch instead of in launches of its own.
+// Every wavefront claims queue entries (over all the queues) until none is
+// left, then waits until every claimed entry is finished; a wavefront only
+// waits once all entries are claimed, and a claimed entry is finished without
+// waiting, so the wait ends.  After it the gradient and loss partials hold
+// every term, and no site has moved yet.  The queues are almost always empty:
+// then this is one look at their lengths.
+template <class R>
+__device__ void drainExact(const AdamArgs<R> &a, ExactLds<R, false> &lds, int lane)
+{
+    int total = 0;
+    for (int q = 0; q < a.drainQueues; ++q) total += loadUniform(a.la.exactLen + q);
+    if (total == 0) return;
+    // Every lane takes part in the counter atomics, lane 0 with 1 and the rest
+    // with 0, so no branch on the lane surrounds them and the claimed entry
+    // (lane 0's result) is read with every lane active.
+    const int one = lane == 0 ? 1 : 0;
+    for (;;) {
+        const int e = __builtin_amdgcn_readfirstlane(atomicAdd(a.drainCtl, one));
+        if (e >= total) break;
+        int q = 0, base = 0;
+        for (;; ++q) {
+            const int n = loadUniform(a.la.exactLen + q);
+            if (e < base + n || q == a.drainQueues - 1) break;
+            base += n;
+        }
+        const ExactQueue eq = loadUniform(a.queues + q);
+        exactItem<R, false>(a.la, lds, loadUniform(a.la.exactQ + eq.qOff + (e - base)), e, lane, eq.K);
+        __hip_atomic_fetch_add(a.drainCtl + 1, one, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    for (;;) {
+        const int d = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(a.drainCtl + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+        if (d >= total) break;
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+template <class R> struct AdamLds {
+    union {
+        int               bin[kBinSpan];
+        ExactLds<R, false> ex[sizeof(R) == 4 ? kSiteChunk / 64 : 1]; // the drain (FP32 only)
+    } u;
+};
+
 template <class R>
 __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
 {
-    __shared__ long long sCount;
-    __shared__ int       sBin[kBinSpan];
-    __shared__ int       sLo[kSiteChunk / 64], sHi[kSiteChunk / 64];
+    __shared__ long long  sCount;
+    __shared__ AdamLds<R> sU;
+    int *const            sBin = sU.u.bin;
+    __shared__ int        sLo[kSiteChunk / 64], sHi[kSiteChunk / 64];
+    if constexpr (sizeof(R) == 4) {
+        if (a.drainQueues > 0) {
+            drainExact<R>(a, sU.u.ex[threadIdx.x >> 6], int(threadIdx.x & 63));
+            __syncthreads();
+        }
+    }
     const int nWork = *a.nWork;
     for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
         const SiteWork wk = loadUniform(a.work + w);
@@ -1867,7 +1926,7 @@ __global__ __launch_bounds__(64 * kClipWaves) void clipKernel(ClipArgs<R> a)
             if (lane == 0) a.inside[i] = 0, a.count[i] = 0;
             continue;
         }
-        const int    nc  = gatherKnn<kCellK, R, double, false>(lds.knn, in, spos, sidx, strt, ci, lane);
+        const int    nc  = gatherKnn<R, double, false>(lds.knn, in, spos, sidx, strt, ci, lane, kCellK);
         const double cxd = double(ci.x), cyd = double(ci.y);
         const double x0 = -cxd, y0 = -cyd, x1 = double(in.W) - cxd, y1 = double(in.H) - cyd;
         // The box: lanes 0-3 hold (x0,y0), (x1,y0), (x1,y1), (x0,y1), edges -1..-4.

@@ -121,6 +121,13 @@ struct ExactItem {
     int inst, tg;
 };
 
+// The exact-selection queue of one loss group: its entries start at entry
+// qOff of the queue array, and their terms use the K nearest sites.
+struct ExactQueue {
+    int64_t qOff;
+    int     K, pad;
+};
+
 // Poisson dart thrower: one record per instance grid being (re)sampled.
 struct PoiInst {
     float    W, H, radius, cellSize;
