@@ -1687,9 +1687,16 @@ __device__ void drainExact(const AdamArgs<R> &a, ExactLds<R, false> &lds, int la
         exactItem<R, false>(a.la, lds, loadUniform(a.la.exactQ + eq.qOff + (e - base)), e, lane, eq.K);
         __hip_atomic_fetch_add(a.drainCtl + 1, one, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
+    // The wait polls with relaxed loads and ends without an acquire fence (whose
+    // cache invalidation, once per wavefront of the launch, cost more than the
+    // drain): everything the entries changed -- gradient sums and loss partials --
+    // was changed by device-scope atomics, which act at the memory side and are
+    // complete before the release increment that counts their entry, and this
+    // launch reads those lines only after the wait, so no cache holds a stale
+    // copy of them.
     for (;;) {
         const int d = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(a.drainCtl + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+            __hip_atomic_load(a.drainCtl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         if (d >= total) break;
         __builtin_amdgcn_s_sleep(2);
     }
