@@ -124,52 +124,40 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
 
     for (int b = 0; b < S; b += vmk::kSiteChunk) siteWork.push_back({id, b, std::min(vmk::kSiteChunk, S - b), 0});
 
-    // Targets sorted by hash cell, then by a 4x4-bit Morton key inside the cell,
-    // then by input order; a chunk (one wavefront) stays in one cell, holds at
-    // most 64 * tpl targets and its bounding-box diagonal stays within the
-    // chunk extent (in mean spacings).
-    std::vector<int>      cellOfT(T);
-    std::vector<uint32_t> keyT(T);
+    // Targets sorted by a Morton key over the grid at 1/16 of a cell (the bits
+    // of the fine x and y coordinates interleaved), input order breaking ties.
+    // A chunk (one wavefront) takes consecutive targets of that order, at most
+    // 64 * tpl, while its bounding-box diagonal stays within the chunk extent
+    // (in mean spacings).  Chunks cross cell boundaries: the loss kernel stages
+    // the cells around the one that holds the chunk's centre and checks that
+    // they cover every target's candidate disc.
+    auto spread32 = [](uint64_t v) { // bits 0..31 of v to the even bits
+        v &= 0xFFFFFFFFull;
+        v = (v | (v << 16)) & 0x0000FFFF0000FFFFull;
+        v = (v | (v << 8)) & 0x00FF00FF00FF00FFull;
+        v = (v | (v << 4)) & 0x0F0F0F0F0F0F0F0Full;
+        v = (v | (v << 2)) & 0x3333333333333333ull;
+        v = (v | (v << 1)) & 0x5555555555555555ull;
+        return v;
+    };
+    const size_t          nT = static_cast<size_t>(T);
+    std::vector<uint64_t> key(nT), key2(nT);
     for (int t = 0; t < T; ++t) {
         float fx = float(s.tgt[2 * t]) * in.inv, fy = float(s.tgt[2 * t + 1]) * in.inv;
         int   ccx = cellCoordF(float(s.tgt[2 * t]), in.inv, in.gw);
         int   ccy = cellCoordF(float(s.tgt[2 * t + 1]), in.inv, in.gh);
         int   sx  = std::min(std::max(int((fx - float(ccx)) * 16.0f), 0), 15);
         int   sy  = std::min(std::max(int((fy - float(ccy)) * 16.0f), 0), 15);
-        uint32_t mort = 0;
-        for (int bit = 0; bit < 4; ++bit)
-            mort |= ((uint32_t(sx >> bit) & 1u) << (2 * bit)) | ((uint32_t(sy >> bit) & 1u) << (2 * bit + 1));
-        cellOfT[t] = ccy * in.gw + ccx;
-        keyT[t]    = mort;
+        key[size_t(t)] = spread32(uint64_t(ccx) * 16 + uint64_t(sx)) | (spread32(uint64_t(ccy) * 16 + uint64_t(sy)) << 1);
     }
-    // Stable sort on (cell << 8 | Morton key), input order breaking ties.  A
-    // grid with no more cells than a few per target: two stable counting
-    // passes, LSD order -- by the 8-bit Morton key, then by cell.  Larger
-    // grids: LSD radix sort on the whole key, 12-bit digits.  Both give the
-    // order of a stable comparison sort on the key.
-    const size_t  nT     = static_cast<size_t>(T);
-    const int64_t nCells = int64_t(std::max(in.gw * in.gh, 1));
+    // LSD radix sort on the key, 12-bit digits: the order of a stable
+    // comparison sort.
     perm.resize(nT);
-    if (nCells <= 4 * int64_t(T) + 4096) {
-        std::vector<int> byKey(nT), count(size_t(std::max<int64_t>(nCells, 256)) + 1);
-        std::fill(count.begin(), count.begin() + 257, 0);
-        for (int t = 0; t < T; ++t) ++count[size_t(keyT[size_t(t)]) + 1];
-        for (int d = 0; d < 256; ++d) count[size_t(d) + 1] += count[size_t(d)];
-        for (int t = 0; t < T; ++t) byKey[size_t(count[size_t(keyT[size_t(t)])]++)] = t;
-        std::fill(count.begin(), count.end(), 0);
-        for (int t = 0; t < T; ++t) ++count[size_t(cellOfT[size_t(t)]) + 1];
-        for (int64_t c = 0; c < nCells; ++c) count[size_t(c) + 1] += count[size_t(c)];
-        for (size_t i = 0; i < nT; ++i) {
-            const int t = byKey[i];
-            perm[size_t(count[size_t(cellOfT[size_t(t)])]++)] = t;
-        }
-    } else {
-        const uint64_t maxKey = (uint64_t(nCells) << 8) - 1;
+    {
+        const uint64_t maxKey = spread32(uint64_t(in.gw) * 16 - 1) | (spread32(uint64_t(in.gh) * 16 - 1) << 1);
         int            bits   = 1;
         while (bits < 64 && (maxKey >> bits)) ++bits;
-        std::vector<uint64_t> key(nT), key2(nT);
-        std::vector<int>      idx2(nT);
-        for (int t = 0; t < T; ++t) key[size_t(t)] = (uint64_t(uint32_t(cellOfT[size_t(t)])) << 8) | keyT[size_t(t)];
+        std::vector<int> idx2(nT);
         std::iota(perm.begin(), perm.end(), 0);
         constexpr int kDigit = 12, kBuckets = 1 << kDigit;
         std::vector<int> count(kBuckets);
@@ -189,23 +177,46 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
             key.swap(key2), perm.swap(idx2);
         }
     }
-    const size_t firstWork = targetWork.size();
     const double spacing = std::sqrt(double(s.W) * double(s.H) / double(std::max(s.liveS >= 0 ? s.liveS : S, 1)));
     const double maxDiag = double(std::min(s.chunkExt > 0.0f ? s.chunkExt : 1.6f, extentCap)) * spacing;
-    // Targets and cells in sorted order, read sequentially by the chunk scan.
+    // Target positions in sorted order, read sequentially by the chunk scan.
     std::vector<double> sxT(nT), syT(nT);
-    std::vector<int>    scT(nT);
     for (size_t t = 0; t < nT; ++t) {
         const int o = perm[t];
-        sxT[t] = s.tgt[2 * size_t(o)], syT[t] = s.tgt[2 * size_t(o) + 1], scT[t] = cellOfT[size_t(o)];
+        sxT[t] = s.tgt[2 * size_t(o)], syT[t] = s.tgt[2 * size_t(o) + 1];
     }
+    // The loss kernel stages the block of cells within `rs` rings of the cell
+    // holding the chunk centre q; it covers the candidate disc (radius D + 2 rho,
+    // D = q's K-th neighbour distance, rho = the chunk's radius) when the block
+    // reaches farther than that from q on every side that is not the grid's
+    // edge.  `room` is that reach; D is estimated as sqrt(K / pi) spacings, the
+    // radius of the disc that holds K sites at the mean density.
+    const double dEst = std::sqrt(double(s.K) / 3.141592653589793) * spacing;
+    auto         room = [&](double qx, double qy, int rs) {
+        const double c  = double(in.cell);
+        const int    hx = cellCoordF(float(qx), in.inv, in.gw), hy = cellCoordF(float(qy), in.inv, in.gh);
+        double       r  = 1e300;
+        if (hx - rs > 0) r = std::min(r, qx - double(hx - rs) * c);
+        if (hx + rs < in.gw - 1) r = std::min(r, double(hx + rs + 1) * c - qx);
+        if (hy - rs > 0) r = std::min(r, qy - double(hy - rs) * c);
+        if (hy + rs < in.gh - 1) r = std::min(r, double(hy + rs + 1) * c - qy);
+        return r;
+    };
     for (int t = 0; t < T;) {
-        int    c = scT[size_t(t)], e = t;
+        int    e  = t;
         double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
-        while (e < T && scT[size_t(e)] == c && e - t < chunkMax) {
+        while (e < T && e - t < chunkMax) {
             double px = sxT[size_t(e)], py = syT[size_t(e)];
             double nx0 = std::min(x0, px), ny0 = std::min(y0, py), nx1 = std::max(x1, px), ny1 = std::max(y1, py);
-            if (e > t && (nx1 - nx0) * (nx1 - nx0) + (ny1 - ny0) * (ny1 - ny0) > maxDiag * maxDiag) break;
+            if (e > t) {
+                const double dx = nx1 - nx0, dy = ny1 - ny0;
+                if (dx * dx + dy * dy > maxDiag * maxDiag) break;
+                // A chunk stops growing before even three rings would miss its
+                // disc (with a fifth of D to spare).
+                if (room(0.5 * (nx0 + nx1), 0.5 * (ny0 + ny1), vmk::kStageRetry) <
+                    1.2 * dEst + std::sqrt(dx * dx + dy * dy))
+                    break;
+            }
             x0 = nx0, y0 = ny0, x1 = nx1, y1 = ny1;
             ++e;
         }
@@ -219,9 +230,13 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         rho = std::sqrt(rho); // double: within an ulp, and rounded up twice in float below
         float rhof = std::nextafter(std::nextafter(float(rho), 1e30f), 1e30f);
         vmk::TargetWork tw{};
-        tw.inst = id, tw.count = e - t, tw.cx = c % in.gw, tw.cy = c / in.gw;
+        tw.inst = id, tw.count = e - t;
+        tw.cx = cellCoordF(qx, in.inv, in.gw), tw.cy = cellCoordF(qy, in.inv, in.gh);
         tw.qx = qx, tw.qy = qy, tw.rho = rhof, tw.gw = in.gw, tw.gh = in.gh;
-        tw.flags   = (in.stageR & vmk::kWorkRingsMask) | (in.fraction < 1.0f ? vmk::kWorkSubsample : 0);
+        // Two rings, or three where two would likely miss the disc (the kernel
+        // retries with the other count when its choice fails).
+        const int rings = in.stageR == 2 && room(qx, qy, 2) < dEst + 2.0 * rho ? vmk::kStageRetry : in.stageR;
+        tw.flags   = (rings & vmk::kWorkRingsMask) | (in.fraction < 1.0f ? vmk::kWorkSubsample : 0);
         tw.tBegin  = targetOff + t, tw.cellOff = cellOff, tw.siteOff = siteOff;
         targetWork.push_back(tw);
         // Lane l takes the chunk's targets l, l + 64, l + 128, ...  Interleave the
@@ -237,21 +252,9 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         }
         t = e;
     }
-    // Chunks in Morton order of their cells (chunks of one cell keep their
-    // order): runs of consecutive chunks, which the loss kernel hands to one
-    // XCD, then cover compact patches of the grid.
-    auto spread = [](uint32_t v) { // bits 0..15 of v to the even bits
-        v &= 0xFFFFu;
-        v = (v | (v << 8)) & 0x00FF00FFu, v = (v | (v << 4)) & 0x0F0F0F0Fu;
-        v = (v | (v << 2)) & 0x33333333u, v = (v | (v << 1)) & 0x55555555u;
-        return v;
-    };
-    if (in.gw <= 65536 && in.gh <= 65536)
-        std::stable_sort(targetWork.begin() + std::ptrdiff_t(firstWork), targetWork.end(),
-                         [&](const vmk::TargetWork &a, const vmk::TargetWork &b) {
-                             return (spread(uint32_t(a.cx)) | (spread(uint32_t(a.cy)) << 1)) <
-                                    (spread(uint32_t(b.cx)) | (spread(uint32_t(b.cy)) << 1));
-                         });
+    // The chunks follow the Morton order of their targets: runs of consecutive
+    // chunks, which the loss kernel hands to one XCD, cover compact patches of
+    // the grid.
 }
 
 namespace {
@@ -302,8 +305,7 @@ public:
     DBuf<vmk::SiteWork>  dSiteWork;
     DBuf<vmk::TargetWork> dTargetWork;
     DBuf<vmk::ExactItem> dExactQ;
-    DBuf<int>            dExactLen; // one per group, then the drain's two counters (claimed, finished)
-    DBuf<vmk::ExactQueue> dQueues;  // each group's queue offset and k (the FP32 drain in the Adam launch)
+    DBuf<int>            dExactLen; // one per group
     hipGraphExec_t       graph = nullptr, graphN = nullptr; // one iteration, kGraphIters iterations
     bool                 stale1 = false, staleN = false;   // a commit changed their launches
     hipEvent_t           ev[8] = {};
@@ -337,7 +339,7 @@ public:
         b += dFrozen.n + dInst.n * sizeof(vmk::Inst) + dState.n * sizeof(vmk::InstState) +
              dPartial.n * sizeof(vmk::LossPartial);
         b += dSiteWork.n * sizeof(vmk::SiteWork) + dTargetWork.n * sizeof(vmk::TargetWork);
-        b += dExactQ.n * sizeof(vmk::ExactItem) + dExactLen.n * sizeof(int) + dQueues.n * sizeof(vmk::ExactQueue);
+        b += dExactQ.n * sizeof(vmk::ExactItem) + dExactLen.n * sizeof(int);
         return b;
     }
 
@@ -419,8 +421,7 @@ public:
         const bool tail = n == 1 && tailTargets > 0;
         dSiteWork.reserve(L.siteWork.size()), dTargetWork.reserve(L.targetWork.size() + (tail ? size_t(tailWork) : 0));
         dCnt.reserve(2 + L.groups.size());
-        dExactQ.reserve(std::max<int64_t>(L.queue, 1)), dExactLen.reserve(L.groups.size() + 2);
-        dQueues.reserve(std::max<size_t>(L.groups.size(), 1));
+        dExactQ.reserve(std::max<int64_t>(L.queue, 1)), dExactLen.reserve(L.groups.size());
 
         std::vector<V>       hp(L.sites), hm(L.sites), hv(L.sites), ht(L.targets, V{R(-1), R(-1)});
         std::vector<int>     hts(L.targets, -1);
@@ -467,9 +468,6 @@ public:
         const std::vector<vmk::ScanCtl> ctl{{0, 0u, int(L.tiles.size()), L.dynTiles}};
         up(dScanCtl.p, ctl);
         up(dTargetWork.p, L.targetWork), up(dCnt.p, cnt);
-        std::vector<vmk::ExactQueue> queues;
-        for (const auto &g : L.groups) queues.push_back(vmk::ExactQueue{g.qOff, g.K, 0});
-        up(dQueues.p, queues);
         const bool staged = total <= kPinnedUploadMax;
         if (staged) hUpload.reserve(total);
         for (const Up &u : ups) {
@@ -628,24 +626,12 @@ public:
         return a;
     }
 
-    // FP32 iterations drain the exact-selection queues in the Adam launch.
-    static constexpr bool kDrainInAdam = sizeof(R) == 4;
-
-    vmk::AdamArgs<R> adamArgs(int doAdam, bool drain = false)
+    vmk::AdamArgs<R> adamArgs(int doAdam)
     {
         vmk::AdamArgs<R> a{};
         a.insts = dInst.p, a.state = dState.p, a.work = dSiteWork.p, a.nWork = dCnt.p + 1;
         a.pos = dPos.p, a.m = dM.p, a.v = dV.p, a.grad = dGrad.p, a.frozen = dFrozen.p, a.partial = dPartial.p;
         a.cellOf = dCellOf.p, a.rank = dRank.p, a.counts = dCounts.p, a.doAdam = doAdam;
-        if (drain && !L.groups.empty()) {
-            a.drainQueues = int(L.groups.size());
-            a.queues      = dQueues.p;
-            a.drainCtl    = dExactLen.p + L.groups.size();
-            a.la          = lossArgs(0, vmk::kModeIterate, true);
-            a.la.exactQ   = dExactQ.p;
-            a.la.exactLen = dExactLen.p;
-            a.la.exactOnly = 0;
-        }
         return a;
     }
 
@@ -659,7 +645,7 @@ public:
         if (L.dynTiles == 0)
             sc = vmk::ScanScatter{dPos.p, dCellOf.p, dRank.p, dSpos.p, dSidx.p, int(sizeof(V))};
         VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dTiles.p, dScanCtl.p, dFlags.p,
-                               dCounts.p, dStart.p, mode, dExactLen.p, int(L.groups.size()) + 2, sc));
+                               dCounts.p, dStart.p, mode, dExactLen.p, int(L.groups.size()), sc));
         if (L.dynTiles != 0)
             VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
                                          dCellOf.p, dRank.p, dStart.p, dSpos.p, dSidx.p, mode, dScanCtl.p));
@@ -687,7 +673,7 @@ public:
         if (L.dynTiles == 0)
             sc = vmk::ScanScatter{dPos.p, dCellOf.p, dRank.p, dSpos.p, dSidx.p, int(sizeof(V))};
         VM_HIP(vmk::launchScan(gridScan, stream, dInst.p, dState.p, dPartial.p, dTiles.p, dScanCtl.p, dFlags.p,
-                               dCounts.p, dStart.p, vmk::kModeIterate, dExactLen.p, int(L.groups.size()) + 2, sc));
+                               dCounts.p, dStart.p, vmk::kModeIterate, dExactLen.p, int(L.groups.size()), sc));
         if (marks) VM_HIP(hipEventRecord(marks[1], stream));
         if (L.dynTiles != 0)
             VM_HIP(vmk::launchScatter<R>(gridSite, stream, dInst.p, dState.p, dSiteWork.p, dCnt.p + 1, dPos.p,
@@ -698,7 +684,7 @@ public:
             VM_HIP(vmk::launchLoss<R>(L.groups[0].K, L.groups[0].tpl, gridLoss[0], stream,
                                       lossArgs(0, vmk::kModeIterate, true)));
             if (marks) VM_HIP(hipEventRecord(marks[3], stream));
-            if (!kDrainInAdam) VM_HIP(vmk::launchExact<R>(L.groups[0].K, stream, lossArgs(0, vmk::kModeIterate, true)));
+            VM_HIP(vmk::launchExact<R>(L.groups[0].K, stream, lossArgs(0, vmk::kModeIterate, true)));
         } else if (!L.groups.empty()) {
             // Each group's loss and exact selection on its own stream (the groups
             // touch disjoint instances); the main stream waits for all of them.
@@ -710,14 +696,14 @@ public:
                 if (g > 0) VM_HIP(hipStreamWaitEvent(sg, evFork, 0));
                 VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], sg,
                                           lossArgs(int(g), vmk::kModeIterate, true)));
-                if (!kDrainInAdam) VM_HIP(vmk::launchExact<R>(L.groups[g].K, sg, lossArgs(int(g), vmk::kModeIterate, true)));
+                VM_HIP(vmk::launchExact<R>(L.groups[g].K, sg, lossArgs(int(g), vmk::kModeIterate, true)));
                 if (g > 0) VM_HIP(hipEventRecord(evJoin[g - 1], sg));
             }
             for (size_t g = 1; g < L.groups.size(); ++g) VM_HIP(hipStreamWaitEvent(stream, evJoin[g - 1], 0));
             if (marks) VM_HIP(hipEventRecord(marks[3], stream));
         }
         if (marks) VM_HIP(hipEventRecord(marks[4], stream));
-        VM_HIP(vmk::launchAdamBin<R>(gridSite, stream, adamArgs(1, kDrainInAdam)));
+        VM_HIP(vmk::launchAdamBin<R>(gridSite, stream, adamArgs(1)));
         if (marks) VM_HIP(hipEventRecord(marks[5], stream));
     }
 

@@ -64,14 +64,6 @@ template <class R> struct AdamArgs {
     const LossPartial   *partial; // folded into InstState (count, loss) before the update
     int                 *cellOf, *rank, *counts;
     int                  doAdam; // 0: only bin the current positions
-    // FP32 iterations: the exact selection of the targets the loss launches
-    // queued runs at the start of this launch (drainQueues queues, described by
-    // queues[]; la carries the loss arguments with la.exactQ / la.exactLen the
-    // bases of the queue array and of the lengths); 0: the exact launches ran.
-    int                  drainQueues;
-    const ExactQueue    *queues;
-    int                 *drainCtl; // [2] entries claimed, entries finished (cleared by the scan)
-    LossArgs<R>          la;
 };
 
 template <class R> struct ClipArgs {

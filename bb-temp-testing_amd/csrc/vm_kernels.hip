@@ -588,9 +588,8 @@ constexpr int kLossThreads   = 64 * kLossWaves;
 // round of gridDim.x * kLossWaves items.
 constexpr int kXcds          = 8;
 constexpr int kXcdRun        = 16;
-constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane (block of <= 5 x 5 cells)
-constexpr int kBisectSteps   = 9;
-constexpr int kStageRetry    = 3;   // rings of the retried block (7 x 7 cells <= 64 lanes)   // brackets q's K-th distance^2 to 1/512 of the block's largest
+constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane (block of <= 7 x 7 cells)
+constexpr int kBisectSteps   = 9;   // brackets q's K-th distance^2 to 1/512 of the block's largest
 
 // Screen table of a staged chunk: one row per distinct nearest site c_r of the
 // chunk's targets, one column per inner-band candidate c_i.  Entry (r, i) holds
@@ -1149,8 +1148,10 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             const R     rho = R(wk.rho) + mg;
             const int   e0 = lane, e1 = lane + 64;
             R           d0 = rInf<R>(), d1 = rInf<R>(), lo = R(0), hi = R(0), thr = R(0), thr2 = R(0);
-            // A block that misses the candidate disc (next to the domain's edge)
-            // is retried once with kStageRetry rings.
+            // One retry: a block that misses the candidate disc is staged again
+            // with kStageRetry rings, a block of kStageRetry rings that holds
+            // too many sites with two.
+            bool retried = false;
             for (;;) {
                 // ---- stage the block into LDS ----------------------------------
                 // Rows of the block are contiguous ranges of the sorted arrays:
@@ -1213,9 +1214,16 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                     if (sy1 < gh - 1) edge = fmin(edge, R(sy1 + 1) * cell - q.y);
                     fast = edge - R(2) * mg > thr;
                 }
-                if (fast || rs >= kStageRetry || total > kLossStage) break;
+                if (fast || retried) break;
+                if (total > kLossStage) {
+                    if (rs < kStageRetry) break;
+                    rs = 2;
+                } else {
+                    if (rs >= kStageRetry) break;
+                    rs = kStageRetry;
+                }
+                retried = true;
                 waveSync();
-                rs = kStageRetry;
                 blockStarts();
             }
             if (fast) {
@@ -1656,72 +1664,12 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
 // takes one global atomic per occupied cell; otherwise one per site.
 constexpr int kBinSpan = 2048;
 
-// FP32 iterations: the exact selection of the targets the loss launches queued
-// runs at the start of the Adam launch instead of in launches of its own.
-// Every wavefront claims queue entries (over all the queues) until none is
-// left, then waits until every claimed entry is finished; a wavefront only
-// waits once all entries are claimed, and a claimed entry is finished without
-// waiting, so the wait ends.  After it the gradient and loss partials hold
-// every term, and no site has moved yet.  The queues are almost always empty:
-// then this is one look at their lengths.
-template <class R>
-__device__ void drainExact(const AdamArgs<R> &a, ExactLds<R, false> &lds, int lane)
-{
-    int total = 0;
-    for (int q = 0; q < a.drainQueues; ++q) total += loadUniform(a.la.exactLen + q);
-    if (total == 0) return;
-    // Every lane takes part in the counter atomics, lane 0 with 1 and the rest
-    // with 0, so no branch on the lane surrounds them and the claimed entry
-    // (lane 0's result) is read with every lane active.
-    const int one = lane == 0 ? 1 : 0;
-    for (;;) {
-        const int e = __builtin_amdgcn_readfirstlane(atomicAdd(a.drainCtl, one));
-        if (e >= total) break;
-        int q = 0, base = 0;
-        for (;; ++q) {
-            const int n = loadUniform(a.la.exactLen + q);
-            if (e < base + n || q == a.drainQueues - 1) break;
-            base += n;
-        }
-        const ExactQueue eq = loadUniform(a.queues + q);
-        exactItem<R, false>(a.la, lds, loadUniform(a.la.exactQ + eq.qOff + (e - base)), e, lane, eq.K);
-        __hip_atomic_fetch_add(a.drainCtl + 1, one, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    // The wait polls with relaxed loads and ends without an acquire fence (whose
-    // cache invalidation, once per wavefront of the launch, cost more than the
-    // drain): everything the entries changed -- gradient sums and loss partials --
-    // was changed by device-scope atomics, which act at the memory side and are
-    // complete before the release increment that counts their entry, and this
-    // launch reads those lines only after the wait, so no cache holds a stale
-    // copy of them.
-    for (;;) {
-        const int d = __builtin_amdgcn_readfirstlane(
-            __hip_atomic_load(a.drainCtl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-        if (d >= total) break;
-        __builtin_amdgcn_s_sleep(2);
-    }
-}
-
-template <class R> struct AdamLds {
-    union {
-        int               bin[kBinSpan];
-        ExactLds<R, false> ex[sizeof(R) == 4 ? kSiteChunk / 64 : 1]; // the drain (FP32 only)
-    } u;
-};
-
 template <class R>
 __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
 {
-    __shared__ long long  sCount;
-    __shared__ AdamLds<R> sU;
-    int *const            sBin = sU.u.bin;
-    __shared__ int        sLo[kSiteChunk / 64], sHi[kSiteChunk / 64];
-    if constexpr (sizeof(R) == 4) {
-        if (a.drainQueues > 0) {
-            drainExact<R>(a, sU.u.ex[threadIdx.x >> 6], int(threadIdx.x & 63));
-            __syncthreads();
-        }
-    }
+    __shared__ long long sCount;
+    __shared__ int       sBin[kBinSpan];
+    __shared__ int       sLo[kSiteChunk / 64], sHi[kSiteChunk / 64];
     const int nWork = *a.nWork;
     for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
         const SiteWork wk = loadUniform(a.work + w);

@@ -87,6 +87,7 @@ struct SiteWork {
 // wavefront can issue its target and cell-start loads as soon as it holds the
 // work item, without waiting for the record.
 constexpr int kWorkRingsMask = 3;  // flags bits 0-1: staged cell rings
+constexpr int kStageRetry    = 3;  // the most rings a block has (7 x 7 cells <= 64 lanes)
 constexpr int kWorkSubsample = 4;  // flags bit 2: subsampled targets (read their slot numbers)
 struct TargetWork {
     int     inst, count;
@@ -119,13 +120,6 @@ struct ScanCtl {
 // (bit 31 set: only its k-NN list is wanted) and packed target index.
 struct ExactItem {
     int inst, tg;
-};
-
-// The exact-selection queue of one loss group: its entries start at entry
-// qOff of the queue array, and their terms use the K nearest sites.
-struct ExactQueue {
-    int64_t qOff;
-    int     K, pad;
 };
 
 // Poisson dart thrower: one record per instance grid being (re)sampled.
