@@ -589,7 +589,12 @@ constexpr int kLossThreads   = 64 * kLossWaves;
 constexpr int kXcds          = 8;
 constexpr int kXcdRun        = 16;
 constexpr int kLossStage     = 128; // staged sites per wavefront, two per lane (block of <= 7 x 7 cells)
-constexpr int kBisectSteps   = 9;   // brackets q's K-th distance^2 to 1/512 of the block's largest
+// Bisection of q's K-th squared distance over the bit patterns of the staged
+// squared distances (non-negative floats order as their patterns): from
+// [largest / 2^8, largest] (or [0, largest] when K sites lie below the lower
+// end), 10 halvings leave a bracket of 2^16 patterns, 1/128 of a binade.
+constexpr int      kBisectSteps = 10;
+constexpr uint32_t kBisectSpan  = 8u << 23; // eight binades
 
 // Screen table of a staged chunk: one row per distinct nearest site c_r of the
 // chunk's targets, one column per inner-band candidate c_i.  Entry (r, i) holds
@@ -1193,14 +1198,26 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                     d0 = in0 ? dist2(q0, q) : rInf<R>();
                     d1 = in1 ? dist2(q1, q) : rInf<R>();
                     // Bisection: lo < D^2 <= hi for D^2 = q's K-th smallest, i.e.
-                    // count(d2 <= lo) < K <= count(d2 <= hi) over the staged sites.
-                    lo = R(0);
-                    hi = waveMaxR<R>(fmax(in0 ? d0 : R(0), in1 ? d1 : R(0)));
-                    for (int it = 0; it < kBisectSteps; ++it) {
-                        R   mid = lo + R(0.5) * (hi - lo);
-                        int c   = __popcll(__ballot(d0 <= mid)) + __popcll(__ballot(d1 <= mid));
-                        if (c >= K) hi = mid;
-                        else lo = mid;
+                    // count(d2 <= lo) < K <= count(d2 <= hi) over the staged sites,
+                    // on the squared distances' bit patterns (uniform: scalar
+                    // arithmetic, two vector compares per step).
+                    {
+                        const uint32_t b0 = __builtin_bit_cast(uint32_t, float(d0));
+                        const uint32_t b1 = __builtin_bit_cast(uint32_t, float(d1));
+                        auto count = [&](uint32_t x) {
+                            return __popcll(__ballot(b0 <= x)) + __popcll(__ballot(b1 <= x));
+                        };
+                        uint32_t hiB = waveReduce32<uint32_t>(max(in0 ? b0 : 0u, in1 ? b1 : 0u), 0u,
+                                                              [](uint32_t x, uint32_t y) { return max(x, y); });
+                        uint32_t loB = hiB > kBisectSpan ? hiB - kBisectSpan : 0u;
+                        if (count(loB) >= K) loB = 0u;
+                        for (int it = 0; it < kBisectSteps; ++it) {
+                            const uint32_t mid = loB + ((hiB - loB) >> 1);
+                            if (count(mid) >= K) hiB = mid;
+                            else loB = mid;
+                        }
+                        lo = R(__builtin_bit_cast(float, loB));
+                        hi = R(__builtin_bit_cast(float, hiB));
                     }
                     const R D = sqrtR(hi) + mg;
                     thr       = D + R(2) * rho + mg;
