@@ -323,7 +323,7 @@ InstSpec specFromDesc(const vm_fit_desc &d)
     s.chunkExt = d.chunk_extent > 0 ? d.chunk_extent : ce;
     VM_REQUIRE(d.targets_per_lane >= 0 && d.targets_per_lane <= vmk::kMaxTargetsPerLane, "targets_per_lane must be 0-4");
     s.tpl = d.targets_per_lane > 0 ? d.targets_per_lane : tpl;
-    VM_REQUIRE(vmk::lossTplSupported(s.K, s.tpl, false), "targets_per_lane not compiled for this loss_k");
+    VM_REQUIRE(vmk::lossTplSupported(s.tpl, false), "targets_per_lane must be 2, 3 or 4");
     s.maxIter = d.max_iterations, s.tol = d.tol;
     return s;
 }

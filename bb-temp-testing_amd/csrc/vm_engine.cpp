@@ -61,7 +61,7 @@ void tunedChoice(int numSites, float W, float H, int K, float *cell, int *stageR
     *cell            = float(f * row.cellScale * a);
     *stageR          = 2;
     *chunkExt        = row.chunkExtent;
-    if (tpl) *tpl = vmk::lossTplSupported(K, row.tpl, false) ? row.tpl : vmk::kTargetsPerLane;
+    if (tpl) *tpl = vmk::lossTplSupported(row.tpl, false) ? row.tpl : vmk::kTargetsPerLane;
 }
 
 namespace {
@@ -284,7 +284,7 @@ public:
         std::vector<vmk::Inst>       inst;
         std::vector<vmk::SiteWork>   siteWork;
         std::vector<vmk::TargetWork> targetWork; // grouped by K
-        struct Group { int K, tpl, begin, count; int64_t qOff, qLen; }; // qOff/qLen: the group's exact-selection queue
+        struct Group { int tpl, begin, count; int64_t qOff, qLen; }; // qOff/qLen: the group's exact-selection queue
         std::vector<Group>            groups;
         std::vector<vmk::ScanTile>    tiles; // scan tiles, instance by instance
         std::vector<std::vector<int>> perm; // per instance: sorted position -> input target index
@@ -350,7 +350,7 @@ public:
         // Chunk shapes compiled for this precision (FP64 sends every target to
         // the exact selection and has one).
         for (auto &s : specs)
-            if (!vmk::lossTplSupported(s.K, s.tpl, sizeof(R) == 8)) s.tpl = vmk::kTargetsPerLane;
+            if (!vmk::lossTplSupported(s.tpl, sizeof(R) == 8)) s.tpl = vmk::kTargetsPerLane;
         L.inst.resize(n);
         L.perm.resize(n);
         std::vector<std::vector<vmk::TargetWork>> twByInst(n);
@@ -387,15 +387,16 @@ public:
                 }
                 if (pass == 0) L.dynTiles += nt;
             }
-        // Group target work by (k, targets per lane): one loss launch per group.
-        std::vector<std::pair<int, int>> ks;
-        for (auto &s : specs) ks.push_back({s.K, s.tpl});
-        std::sort(ks.begin(), ks.end());
-        ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
-        for (auto [K, tpl] : ks) {
-            typename Layout::Group g{K, tpl, int(L.targetWork.size()), 0, L.queue, 0};
+        // Group target work by targets per lane: one loss launch per group (the
+        // kernel reads each fit's k from its record, so fits of every k share one).
+        std::vector<int> tpls;
+        for (auto &s : specs) tpls.push_back(s.tpl);
+        std::sort(tpls.begin(), tpls.end());
+        tpls.erase(std::unique(tpls.begin(), tpls.end()), tpls.end());
+        for (int tpl : tpls) {
+            typename Layout::Group g{tpl, int(L.targetWork.size()), 0, L.queue, 0};
             for (int i = 0; i < n; ++i)
-                if (specs[i].K == K && specs[i].tpl == tpl) {
+                if (specs[i].tpl == tpl) {
                     L.targetWork.insert(L.targetWork.end(), twByInst[i].begin(), twByInst[i].end());
                     L.queue += L.inst[i].T; // every target of the group may be queued once per launch
                 }
@@ -617,7 +618,7 @@ public:
         vmk::LossArgs<R> a{};
         a.insts = dInst.p, a.state = dState.p, a.work = dTargetWork.p + L.groups[g].begin, a.nWork = dCnt.p + 2 + g;
         a.pos = dPos.p, a.spos = dSpos.p, a.sidx = dSidx.p, a.start = dStart.p, a.tpos = dTpos.p, a.tslot = dTslot.p;
-        a.grad = wantGrad ? dGrad.p : nullptr, a.partial = dPartial.p, a.mode = mode;
+        a.grad = wantGrad ? dGrad.p : nullptr, a.partial = dPartial.p, a.mode = mode, a.dbgInst = -1;
         // VM_LOSS_EXACT=1 sends every lane through the exact k-NN selection (parity checks of that path).
         const char *ex = std::getenv("VM_LOSS_EXACT");
         a.exactOnly    = ex && ex[0] == '1';
@@ -681,10 +682,10 @@ public:
                                          dScanCtl.p));
         if (marks) VM_HIP(hipEventRecord(marks[2], stream));
         if (L.groups.size() == 1) {
-            VM_HIP(vmk::launchLoss<R>(L.groups[0].K, L.groups[0].tpl, gridLoss[0], stream,
+            VM_HIP(vmk::launchLoss<R>(L.groups[0].tpl, gridLoss[0], stream,
                                       lossArgs(0, vmk::kModeIterate, true)));
             if (marks) VM_HIP(hipEventRecord(marks[3], stream));
-            VM_HIP(vmk::launchExact<R>(L.groups[0].K, stream, lossArgs(0, vmk::kModeIterate, true)));
+            VM_HIP(vmk::launchExact<R>(stream, lossArgs(0, vmk::kModeIterate, true)));
         } else if (!L.groups.empty()) {
             // Each group's loss and exact selection on its own stream (the groups
             // touch disjoint instances); the main stream waits for all of them.
@@ -694,9 +695,9 @@ public:
             for (size_t g = 0; g < L.groups.size(); ++g) {
                 hipStream_t sg = g == 0 ? stream : side[g - 1];
                 if (g > 0) VM_HIP(hipStreamWaitEvent(sg, evFork, 0));
-                VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], sg,
+                VM_HIP(vmk::launchLoss<R>(L.groups[g].tpl, gridLoss[g], sg,
                                           lossArgs(int(g), vmk::kModeIterate, true)));
-                VM_HIP(vmk::launchExact<R>(L.groups[g].K, sg, lossArgs(int(g), vmk::kModeIterate, true)));
+                VM_HIP(vmk::launchExact<R>(sg, lossArgs(int(g), vmk::kModeIterate, true)));
                 if (g > 0) VM_HIP(hipEventRecord(evJoin[g - 1], sg));
             }
             for (size_t g = 1; g < L.groups.size(); ++g) VM_HIP(hipStreamWaitEvent(stream, evJoin[g - 1], 0));
@@ -824,8 +825,8 @@ public:
         gridBuild(vmk::kModeEvaluate);
         for (size_t g = 0; g < L.groups.size(); ++g)
         {
-            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, lossArgs(int(g), vmk::kModeEvaluate, false)));
-            VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, lossArgs(int(g), vmk::kModeEvaluate, false)));
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].tpl, gridLoss[g], stream, lossArgs(int(g), vmk::kModeEvaluate, false)));
+            VM_HIP(vmk::launchExact<R>(stream, lossArgs(int(g), vmk::kModeEvaluate, false)));
         }
         bin();
         vmk::InstState st;
@@ -843,16 +844,15 @@ public:
         DBuf<vmk::TermDbg> dT;
         int                K = in.K;
         if (wantDbg) {
-            dK.reserve(size_t(L.targets) * K);
+            dK.reserve(std::max<size_t>(size_t(in.T) * K, 1));
             dT.reserve(size_t(L.targets));
         }
         gridBuild(vmk::kModeEvaluate);
         for (size_t g = 0; g < L.groups.size(); ++g) {
             auto a = lossArgs(int(g), vmk::kModeEvaluate, wantGrad);
-            if (wantDbg && L.groups[g].K == K)
-                a.dbgKnn = dK.p, a.dbgTerm = dT.p, a.exactGrid = vmk::exactGridFor(L.groups[g].qLen, true);
-            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, a));
-            VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, a));
+            if (wantDbg) a.dbgKnn = dK.p, a.dbgInst = id, a.dbgTerm = dT.p, a.exactGrid = vmk::exactGridFor(L.groups[g].qLen, true);
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].tpl, gridLoss[g], stream, a));
+            VM_HIP(vmk::launchExact<R>(stream, a));
         }
         std::vector<vmk::GradT<R>> hg;
         if (wantGrad) {
@@ -866,7 +866,7 @@ public:
         if (wantDbg) {
             hk.resize(size_t(in.T) * K), ht.resize(in.T);
             if (in.T) {
-                VM_HIP(hipMemcpyAsync(hk.data(), dK.p + in.targetOff * K, hk.size() * sizeof(int),
+                VM_HIP(hipMemcpyAsync(hk.data(), dK.p, hk.size() * sizeof(int),
                                       hipMemcpyDeviceToHost, stream));
                 VM_HIP(hipMemcpyAsync(ht.data(), dT.p + in.targetOff, ht.size() * sizeof(vmk::TermDbg),
                                       hipMemcpyDeviceToHost, stream));
@@ -902,8 +902,8 @@ public:
         for (size_t g = 0; g < L.groups.size(); ++g) {
             auto a = lossArgs(int(g), vmk::kModeEvaluate, wantGrad);
             if (wantTerms) a.dbgTerm = dT.p;
-            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, a));
-            VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, a));
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].tpl, gridLoss[g], stream, a));
+            VM_HIP(vmk::launchExact<R>(stream, a));
         }
         std::vector<vmk::GradT<R>> hg;
         std::vector<vmk::TermDbg>  ht;
@@ -952,8 +952,8 @@ public:
         for (size_t g = 0; g < L.groups.size(); ++g) {
             auto a  = lossArgs(int(g), vmk::kModeEvaluate, false);
             a.stats = dS.p;
-            VM_HIP(vmk::launchLoss<R>(L.groups[g].K, L.groups[g].tpl, gridLoss[g], stream, a));
-            VM_HIP(vmk::launchExact<R>(L.groups[g].K, stream, a));
+            VM_HIP(vmk::launchLoss<R>(L.groups[g].tpl, gridLoss[g], stream, a));
+            VM_HIP(vmk::launchExact<R>(stream, a));
         }
         bin();
         std::vector<unsigned long long> h(64 * 8);

@@ -42,7 +42,8 @@ template <class R> struct LossArgs {
     GradT<R>         *grad;   // accumulated gradient [2 per site], null: loss only
     LossPartial      *partial; // [instances * kLossPartials]: step loss sums (iterate mode)
     int               mode;   // kModeIterate / kModeEvaluate
-    int              *dbgKnn; // optional [T*K] k-NN lists (instance-local indices)
+    int              *dbgKnn; // optional k-NN lists of instance dbgInst's targets ([T * K], instance-local indices)
+    int               dbgInst;
     TermDbg          *dbgTerm;
     unsigned long long *stats; // optional [64 x 8] path counters: fast, ring (staged), ring (global),
                                // idle, candidates, staged sites, fast waves with an unscreened lane,
@@ -134,10 +135,11 @@ template <class R>
 hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstState *state, const SiteWork *work,
                          const int *nWork, const void *pos, const int *cellOf, const int *rank, const int *start,
                          void *spos, int *sidx, int mode, ScanCtl *ctl);
-// One loss launch over a group of chunks of fits sharing k and targets per lane.
-template <class R> hipError_t launchLoss(int K, int tpl, int grid, hipStream_t s, const LossArgs<R> &a);
+// One loss launch over a group of chunks of fits sharing targets per lane (each
+// fit's k comes from its instance record).
+template <class R> hipError_t launchLoss(int tpl, int grid, hipStream_t s, const LossArgs<R> &a);
 // The exact selection for the targets the loss launch queued; always follows it.
-template <class R> hipError_t launchExact(int K, hipStream_t s, const LossArgs<R> &a);
+template <class R> hipError_t launchExact(hipStream_t s, const LossArgs<R> &a);
 template <class R> hipError_t launchAdamBin(int grid, hipStream_t s, const AdamArgs<R> &a);
 template <class R>
 hipError_t launchKnnQuery(int K, hipStream_t s, const Inst *insts, int inst, const void *spos, const int *sidx,
@@ -171,12 +173,11 @@ inline int exactGridFor(int64_t targets, bool allExact)
 }
 
 inline bool lossKSupported(int K) { return K == 2 || K == 4 || K == 6 || K == 8 || K == 12 || K == 16 || K == 24 || K == 32; }
-// Targets per lane compiled for k: 2, 3 and 4 for the FP32 kernels of k = 8, 12, 16 (the tuned range); 3 for the
-// rest (FP64 runs send every target to the exact selection, so the chunk shape matters little there).
-inline bool lossTplSupported(int K, int tpl, bool fp64)
+// Targets per lane compiled: 2, 3 and 4 for FP32 runs (the kernels take k from each instance's record); 3 for
+// FP64 runs (they send every target to the exact selection, so the chunk shape matters little there).
+inline bool lossTplSupported(int tpl, bool fp64)
 {
-    if (tpl == kTargetsPerLane) return true;
-    return !fp64 && (tpl == 2 || tpl == 4) && (K == 8 || K == 12 || K == 16);
+    return tpl == kTargetsPerLane || (!fp64 && (tpl == 2 || tpl == 4));
 }
 
 } // namespace vmk

@@ -741,9 +741,8 @@ __device__ __forceinline__ int nearestCandidate(const LossLds<float> &lds, const
 // d^2, ties to the earlier key) picks w.  The exact term is then computed for w
 // alone with the reference's formula.  Returns false when any condition fails;
 // the caller runs the exact selection for that lane.
-template <int K>
 __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const ScreenBounds &sb, V2<float> p, bool active,
-                                           uint32_t mask, int s0, bool nearOk, LaneTerm<float> &o)
+                                           uint32_t mask, int s0, bool nearOk, int K, LaneTerm<float> &o)
 {
     constexpr uint32_t kIdxMask = uint32_t(kLossStage - 1);
     static_assert((kLossStage & (kLossStage - 1)) == 0, "slot mask");
@@ -756,9 +755,9 @@ __device__ __forceinline__ bool screenTerm(const LossLds<float> &lds, const Scre
     // past the errors of dist2 and v_sqrt_f32, plus the margin.  For K >= 16 the
     // chunk-wide rho is used: the outer band and the count are rarely needed
     // there, and the per-target bound cost more than it saved (measured).
-    const float dq = K >= 16 ? sb.rho
-                             : fminf(__builtin_amdgcn_sqrtf(dist2(p, V2<float>{sb.qx, sb.qy})) * (1.0f + 0x1p-20f) + sb.mg,
-                                     sb.rho);
+    float dq = sb.rho;
+    if (K < 16) // uniform
+        dq = fminf(__builtin_amdgcn_sqrtf(dist2(p, V2<float>{sb.qx, sb.qy})) * (1.0f + 0x1p-20f) + sb.mg, sb.rho);
     uint32_t        a1 = ~0u, a2 = ~0u;
     auto            screenGroups = [&](int g0, int g1) {
 #pragma unroll 1
@@ -999,9 +998,6 @@ __device__ __forceinline__ int laneRank(uint64_t mask)
     return int(__builtin_amdgcn_mbcnt_hi(uint32_t(mask >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(mask), 0u)));
 }
 
-// At most 72 VGPRs, so 7 wavefronts share a SIMD: the kernel is bound by issue
-// and by the latency of its per-chunk loads, and occupancy hides the latter
-// (K = 4 and 6 would spill at 72 and take 6).
 // The exact selection for one target pt of a staged chunk, by the whole
 // wavefront, when the screen left it open.  The chunk's m candidates hold pt's
 // K nearest sites (the candidate disc), so they are ranked by the (d2, index)
@@ -1011,8 +1007,7 @@ __device__ __forceinline__ int laneRank(uint64_t mask)
 // accumulators.  Returns the winning lane (-1: every pair degenerate); that
 // lane's outputs are the term's loss, the nearest site's and the partner's
 // instance indices and whether a gradient component was clamped.
-template <int K>
-__device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float> pt, int lane, int gcopies, int gM,
+__device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float> pt, int lane, int K, int gcopies, int gM,
                                             bool grad, double gscale, float &lossT, int &i0, int &j, int &sat)
 {
     for (int s = lane; s < m; s += 64)
@@ -1057,8 +1052,12 @@ __device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float
     return wl;
 }
 
-template <int K, int TPL, class R, bool Diag>
-__global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K == 4 || K == 6 ? 6 : 7))) void lossKernel(LossArgs<R> a)
+// At most 72 VGPRs, so 7 wavefronts share a SIMD: the kernel is bound by issue
+// and by the latency of its per-chunk loads, and occupancy hides the latter.
+// k comes from each work item's instance record, so one launch serves fits of
+// every k.
+template <int TPL, class R, bool Diag>
+__global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(7))) void lossKernel(LossArgs<R> a)
 {
     // Diagnostics (path counters, k-NN lists, per-target terms, exact-only
     // runs) exist only in the Diag instantiation.
@@ -1120,6 +1119,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
         if constexpr (sizeof(R) == 4) blockStarts();
 
         const Inst in = loadUniform(a.insts + wk.inst);
+        const int  K  = in.K; // neighbours of the instance's terms
         InstState *st = a.state + wk.inst;
         if (a.mode == kModeIterate && loadUniform(&st->done)) continue;
         bool anyTake = false;
@@ -1343,7 +1343,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
             if constexpr (sizeof(R) == 4) {
                 if (scr) {
                     const uint32_t np = nearPack >> (8 * h);
-                    if (screenTerm<K>(lds, sb, p[h], take[h], mask, int(np & 0x7Fu), (np & 0x80u) != 0, tr)) open = false;
+                    if (screenTerm(lds, sb, p[h], take[h], mask, int(np & 0x7Fu), (np & 0x80u) != 0, K, tr)) open = false;
                     tr.used = take[h] && !open ? tr.used : 0;
                 }
             }
@@ -1363,9 +1363,8 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                 sat += lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.rl, tr.d,
                          &lds.grad[2 * gradSlot(tr.slj, lane & (gcopies - 1), gM)],
                          &lds.grad[2 * gradSlot(tr.sl0, lane & (gcopies - 1), gM)], __builtin_ldexp(1.0, in.gradShift));
-            if (dbgKnn && lane + 64 * h < wk.count && !take[h]) {
-#pragma unroll
-                for (int k = 0; k < K; ++k) dbgKnn[t * int64_t(K) + k] = -1;
+            if (dbgKnn && wk.inst == a.dbgInst && lane + 64 * h < wk.count && !take[h]) {
+                for (int k = 0; k < K; ++k) dbgKnn[(t - in.targetOff) * K + k] = -1;
             }
             if (dbgTerm && lane + 64 * h < wk.count && !open) {
                 const int64_t o   = t;
@@ -1390,7 +1389,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(K 
                                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p[h].y), l))};
                     float     xl = 0.0f;
                     int       xi0 = -1, xj = -1, xs = 0;
-                    const int wl = exactInChunk<K>(lds, m, pt, lane, gcopies, gM, a.grad != nullptr,
+                    const int wl = exactInChunk(lds, m, pt, lane, K, gcopies, gM, a.grad != nullptr,
                                                    __builtin_ldexp(1.0, in.gradShift), xl, xi0, xj, xs);
                     if (lane == wl) lossSum += double(xl), used += 1, sat += xs;
                     if (dbgTerm && lane == (wl >= 0 ? wl : 0)) {
@@ -1581,23 +1580,23 @@ __device__ __forceinline__ int gatherKnn(ExactLds<KR, P> &lds, const Inst &in, c
     return nc;
 }
 
-// The exact selection for queue entry e (item `it`) by one wavefront, with k
-// = K nearest sites.
+// The exact selection for queue entry e (item `it`) by one wavefront, with the
+// K nearest sites of the entry's instance.
 template <class R, bool P>
-__device__ __forceinline__ void exactItem(const LossArgs<R> &a, ExactLds<R, P> &lds, const ExactItem it, int e, int lane,
-                                          int K)
+__device__ __forceinline__ void exactItem(const LossArgs<R> &a, ExactLds<R, P> &lds, const ExactItem it, int e, int lane)
 {
     const bool      listOnly = it.inst < 0;
     const int       inst     = it.inst & 0x7FFFFFFF;
     const Inst      in       = loadUniform(a.insts + inst);
+    const int       K        = in.K;
     const int64_t   so       = in.siteOff;
     const V2<R>    *spos     = a.spos + so;
     const int      *sidx     = a.sidx + so;
     const int      *strt     = a.start + in.cellOff;
     const V2<R>     p        = a.tpos[it.tg];
     const int       nc       = gatherKnn<R, R, P>(lds, in, spos, sidx, strt, p, lane, K);
-    if (a.dbgKnn && lane < K)
-        a.dbgKnn[int64_t(it.tg) * K + lane] = lane < nc ? keySite(lds.cand[lane].low()) : -1;
+    if (a.dbgKnn && inst == a.dbgInst && lane < K)
+        a.dbgKnn[(int64_t(it.tg) - in.targetOff) * K + lane] = lane < nc ? keySite(lds.cand[lane].low()) : -1;
     if (listOnly) {
         waveSync();
         return;
@@ -1654,7 +1653,7 @@ __device__ __forceinline__ void exactItem(const LossArgs<R> &a, ExactLds<R, P> &
     }
 }
 
-template <int K, class R>
+template <class R>
 __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
 {
     // FP32: positions beside the keys (a short queue, latency-bound).  FP64 runs
@@ -1669,7 +1668,7 @@ __global__ __launch_bounds__(64 * kExactWaves) void exactKernel(LossArgs<R> a)
     const int    nw   = gridDim.x * kExactWaves;
     // Entry e goes to wavefront e % nw, numbered across workgroups first, so a
     // short queue spreads over the chip.
-    for (int e = blockIdx.x + gridDim.x * wave; e < n; e += nw) exactItem<R, P>(a, lds, loadUniform(a.exactQ + e), e, lane, K);
+    for (int e = blockIdx.x + gridDim.x * wave; e < n; e += nw) exactItem<R, P>(a, lds, loadUniform(a.exactQ + e), e, lane);
 }
 // ---------------------------------------------------------------------------
 // 4. Adam + MultiStepLR + clamp (voro_optimizer.cu:268-301), then bin the new
@@ -2313,35 +2312,29 @@ hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstS
     return hipGetLastError();
 }
 
-template <class R> hipError_t launchLoss(int K, int tpl, int grid, hipStream_t s, const LossArgs<R> &a)
+template <class R> hipError_t launchLoss(int tpl, int grid, hipStream_t s, const LossArgs<R> &a)
 {
     const bool diag   = a.stats || a.dbgKnn || a.dbgTerm || a.exactOnly;
     const int  blocks = ((grid + kLossWaves - 1) / kLossWaves + 2 * kXcdRun - 1) / (2 * kXcdRun) * (2 * kXcdRun);
-    if (!lossKSupported(K) || !lossTplSupported(K, tpl, sizeof(R) == 8)) return hipErrorInvalidValue;
-#define VM_KT(k, t)                                                                                               \
-    if (K == k && tpl == t) {                                                                                     \
+    if (!lossTplSupported(tpl, sizeof(R) == 8)) return hipErrorInvalidValue;
+#define VM_T(t)                                                                                                   \
+    if (tpl == t) {                                                                                               \
         if (diag)                                                                                                 \
-            hipLaunchKernelGGL((lossKernel<k, t, R, true>), dim3(blocks), dim3(kLossThreads), 0, s, a);           \
+            hipLaunchKernelGGL((lossKernel<t, R, true>), dim3(blocks), dim3(kLossThreads), 0, s, a);              \
         else                                                                                                      \
-            hipLaunchKernelGGL((lossKernel<k, t, R, false>), dim3(blocks), dim3(kLossThreads), 0, s, a);          \
+            hipLaunchKernelGGL((lossKernel<t, R, false>), dim3(blocks), dim3(kLossThreads), 0, s, a);             \
         return hipGetLastError();                                                                                 \
     }
-    VM_KT(2, 3) VM_KT(4, 3) VM_KT(6, 3) VM_KT(8, 3) VM_KT(12, 3) VM_KT(16, 3) VM_KT(24, 3) VM_KT(32, 3)
+    VM_T(3)
     if constexpr (sizeof(R) == 4) {
-        VM_KT(8, 2) VM_KT(8, 4) VM_KT(12, 2) VM_KT(12, 4) VM_KT(16, 2) VM_KT(16, 4)
+        VM_T(2) VM_T(4)
     }
-#undef VM_KT
+#undef VM_T
     return hipErrorInvalidValue;
 }
-template <class R> hipError_t launchExact(int K, hipStream_t s, const LossArgs<R> &a)
+template <class R> hipError_t launchExact(hipStream_t s, const LossArgs<R> &a)
 {
-    switch (K) {
-#define VM_K(k)                                                                                                   \
-    case k: hipLaunchKernelGGL((exactKernel<k, R>), dim3(a.exactGrid), dim3(64 * kExactWaves), 0, s, a); break;
-        VM_K(2) VM_K(4) VM_K(6) VM_K(8) VM_K(12) VM_K(16) VM_K(24) VM_K(32)
-#undef VM_K
-    default: return hipErrorInvalidValue;
-    }
+    hipLaunchKernelGGL((exactKernel<R>), dim3(a.exactGrid), dim3(64 * kExactWaves), 0, s, a);
     return hipGetLastError();
 }
 
@@ -2521,10 +2514,10 @@ template hipError_t launchScatter<float>(int, hipStream_t, const Inst *, const I
 template hipError_t launchScatter<double>(int, hipStream_t, const Inst *, const InstState *, const SiteWork *,
                                           const int *, const void *, const int *, const int *, const int *, void *,
                                           int *, int, ScanCtl *);
-template hipError_t launchLoss<float>(int, int, int, hipStream_t, const LossArgs<float> &);
-template hipError_t launchLoss<double>(int, int, int, hipStream_t, const LossArgs<double> &);
-template hipError_t launchExact<float>(int, hipStream_t, const LossArgs<float> &);
-template hipError_t launchExact<double>(int, hipStream_t, const LossArgs<double> &);
+template hipError_t launchLoss<float>(int, int, hipStream_t, const LossArgs<float> &);
+template hipError_t launchLoss<double>(int, int, hipStream_t, const LossArgs<double> &);
+template hipError_t launchExact<float>(hipStream_t, const LossArgs<float> &);
+template hipError_t launchExact<double>(hipStream_t, const LossArgs<double> &);
 template hipError_t launchAdamBin<float>(int, hipStream_t, const AdamArgs<float> &);
 template hipError_t launchAdamBin<double>(int, hipStream_t, const AdamArgs<double> &);
 template hipError_t launchKnnQuery<float>(int, hipStream_t, const Inst *, int, const void *, const int *, const int *,

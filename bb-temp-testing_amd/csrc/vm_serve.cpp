@@ -1029,8 +1029,8 @@ private:
             const char *ex = std::getenv("VM_LOSS_EXACT");
             la.exactOnly   = ex && ex[0] == '1';
             la.exactGrid   = vmk::exactGridFor(targets_.cap(), sizeof(R) == 8 || la.exactOnly);
-            VM_HIP(vmk::launchLoss<R>(K_, vmk::kTargetsPerLane, gLoss, stream_, la));
-            VM_HIP(vmk::launchExact<R>(K_, stream_, la));
+            VM_HIP(vmk::launchLoss<R>(vmk::kTargetsPerLane, gLoss, stream_, la));
+            VM_HIP(vmk::launchExact<R>(stream_, la));
             VM_HIP(vmk::launchAdamBin<R>(gSite, stream_, adamArgs(1)));
         } catch (...) {
             hipGraph_t dummy;
