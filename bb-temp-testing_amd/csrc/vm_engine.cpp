@@ -192,6 +192,10 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
     // edge.  `room` is that reach; D is estimated as sqrt(K / pi) spacings, the
     // radius of the disc that holds K sites at the mean density.
     const double dEst = std::sqrt(double(s.K) / 3.141592653589793) * spacing;
+    // Three rings (7 x 7 cells) only where they hold at most 116 sites at the
+    // mean density, about 10 % inside the 128 the kernel stages.
+    const double cellSp   = double(in.cell) / spacing;
+    const int    maxRings = 49.0 * cellSp * cellSp <= 116.0 ? vmk::kStageRetry : 2;
     auto         room = [&](double qx, double qy, int rs) {
         const double c  = double(in.cell);
         const int    hx = cellCoordF(float(qx), in.inv, in.gw), hy = cellCoordF(float(qy), in.inv, in.gh);
@@ -211,9 +215,9 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
             if (e > t) {
                 const double dx = nx1 - nx0, dy = ny1 - ny0;
                 if (dx * dx + dy * dy > maxDiag * maxDiag) break;
-                // A chunk stops growing before even three rings would miss its
+                // A chunk stops growing before its largest block would miss its
                 // disc (with a fifth of D to spare).
-                if (room(0.5 * (nx0 + nx1), 0.5 * (ny0 + ny1), vmk::kStageRetry) <
+                if (room(0.5 * (nx0 + nx1), 0.5 * (ny0 + ny1), maxRings) <
                     1.2 * dEst + std::sqrt(dx * dx + dy * dy))
                     break;
             }
@@ -235,7 +239,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
         tw.qx = qx, tw.qy = qy, tw.rho = rhof, tw.gw = in.gw, tw.gh = in.gh;
         // Two rings, or three where two would likely miss the disc (the kernel
         // retries with the other count when its choice fails).
-        const int rings = in.stageR == 2 && room(qx, qy, 2) < dEst + 2.0 * rho ? vmk::kStageRetry : in.stageR;
+        const int rings = in.stageR == 2 && maxRings > 2 && room(qx, qy, 2) < dEst + 2.0 * rho ? maxRings : in.stageR;
         tw.flags   = (rings & vmk::kWorkRingsMask) | (in.fraction < 1.0f ? vmk::kWorkSubsample : 0);
         tw.tBegin  = targetOff + t, tw.cellOff = cellOff, tw.siteOff = siteOff;
         targetWork.push_back(tw);

@@ -15,21 +15,21 @@ struct TuneRow {
 constexpr int     kTuneLog2Min = 8;
 constexpr int     kTuneLog2Max = 22;
 constexpr TuneRow kTuneTable[kTuneLog2Max - kTuneLog2Min + 1] = {
-    {1.00f, 2.20f, 3, 8}, // 2^8: 1.991e+09 site-iters/s at k=16, 2.035e+09 at k=8
-    {1.00f, 2.20f, 3, 8}, // 2^9: 2.751e+09 site-iters/s at k=16, 2.847e+09 at k=8
-    {1.00f, 2.20f, 3, 12}, // 2^10: 3.462e+09 site-iters/s at k=16, 3.578e+09 at k=12
-    {1.00f, 2.20f, 3, 12}, // 2^11: 3.890e+09 site-iters/s at k=16, 4.057e+09 at k=12
-    {1.00f, 1.60f, 3, 12}, // 2^12: 4.371e+09 site-iters/s at k=16, 4.505e+09 at k=12
-    {1.00f, 2.20f, 3, 16}, // 2^13: 4.435e+09 site-iters/s at k=16, 4.435e+09 at k=16
-    {1.00f, 1.60f, 3, 12}, // 2^14: 3.865e+09 site-iters/s at k=16, 4.076e+09 at k=12
-    {1.00f, 2.20f, 3, 12}, // 2^15: 4.808e+09 site-iters/s at k=16, 5.103e+09 at k=12
-    {1.00f, 2.20f, 3, 16}, // 2^16: 4.968e+09 site-iters/s at k=16, 4.968e+09 at k=16
-    {0.90f, 2.20f, 3, 12}, // 2^17: 5.113e+09 site-iters/s at k=16, 5.255e+09 at k=12
-    {1.00f, 2.20f, 3, 16}, // 2^18: 4.358e+09 site-iters/s at k=16, 4.358e+09 at k=16
-    {0.90f, 2.20f, 3, 16}, // 2^19: 4.632e+09 site-iters/s at k=16, 4.632e+09 at k=16
-    {0.90f, 2.20f, 3, 16}, // 2^20: 5.726e+09 site-iters/s at k=16, 5.726e+09 at k=16
-    {0.90f, 2.20f, 3, 16}, // 2^21: largest measured class
-    {0.90f, 2.20f, 3, 16}, // 2^22: largest measured class
+    {0.90f, 2.20f, 3, 16}, // 2^8: 2.916e+09 site-iters/s at k=16, 2.917e+09 at k=16
+    {1.00f, 2.20f, 3, 12}, // 2^9: 3.940e+09 site-iters/s at k=16, 3.981e+09 at k=12
+    {1.00f, 2.20f, 3, 12}, // 2^10: 4.847e+09 site-iters/s at k=16, 4.809e+09 at k=12
+    {1.00f, 2.20f, 3, 16}, // 2^11: 5.743e+09 site-iters/s at k=16, 5.728e+09 at k=16
+    {1.00f, 2.20f, 3, 12}, // 2^12: 6.334e+09 site-iters/s at k=16, 6.331e+09 at k=12
+    {1.00f, 1.60f, 3, 12}, // 2^13: 6.854e+09 site-iters/s at k=16, 6.974e+09 at k=12
+    {0.90f, 2.20f, 3, 16}, // 2^14: 6.684e+09 site-iters/s at k=16, 6.761e+09 at k=16
+    {1.30f, 1.60f, 3, 16}, // 2^15: 6.845e+09 site-iters/s at k=16, 6.860e+09 at k=16
+    {0.90f, 1.60f, 4, 12}, // 2^16: 6.796e+09 site-iters/s at k=16, 6.998e+09 at k=12
+    {1.30f, 1.60f, 4, 16}, // 2^17: 7.113e+09 site-iters/s at k=16, 7.088e+09 at k=16
+    {1.15f, 2.20f, 3, 16}, // 2^18: 6.622e+09 site-iters/s at k=16, 6.642e+09 at k=16
+    {1.15f, 2.20f, 3, 16}, // 2^19: 5.799e+09 site-iters/s at k=16, 5.811e+09 at k=16
+    {1.15f, 2.20f, 3, 16}, // 2^20: 5.815e+09 site-iters/s at k=16, 5.836e+09 at k=16
+    {1.15f, 2.20f, 3, 16}, // 2^21: largest measured class
+    {1.15f, 2.20f, 3, 16}, // 2^22: largest measured class
 };
 
 } // namespace vmrt

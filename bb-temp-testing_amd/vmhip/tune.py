@@ -6,9 +6,10 @@ written to csrc/vm_tune_table.h, which the library compiles in.  Three stages pe
 
   1. hash-grid cell scale at k = 16, chunk extent 2.2 and 3 targets per lane;
   2. targets per lane (3, 4) x chunk extent at the chosen cell scale;
-  3. k for fits that leave it to the tuner: among 8, 12 and 16, the fastest whose per-target loss terms (nearest
+  3. k for fits that leave it to the tuner: among 8, 12 and 16 the ones whose per-target loss terms (nearest
      site, bisector partner, term) equal those of k = 32 on every target of the class's check fits, both at the
-     Poisson start and after 50 iterations (so the choice cannot change a result on them); 16 when none does.
+     Poisson start and after 50 iterations (so the choice cannot change a result on them), 16 always; the
+     smallest of them (the least work per target), unless a larger one runs faster.
 
 Classes above the largest measured one reuse its row.  Run on a GPU box, then rebuild:
 
@@ -24,7 +25,7 @@ import numpy as np
 import vmhip
 from vmhip import workloads as wl
 
-CELL_SCALES = (0.8, 0.9, 1.0, 1.15, 1.3)
+CELL_SCALES = (0.7, 0.8, 0.9, 1.0, 1.15, 1.3)
 CHUNK_EXTENTS = (1.6, 2.2, 3.0, 4.0)
 # Two targets per lane only pays in a batch of nothing but tiny fits (more, smaller chunks); in mixed batches the
 # larger fits' chunks dominate, so the sweep keeps to 3 and 4.
@@ -56,19 +57,29 @@ def make_class(c, budget_sites, max_fits, seed):
     return fits
 
 
-def measure(fits, K, scale, extent, tpl, iters, repeats=3):
-    """Site-iterations/s of the class batch with these choices: the median of `repeats` timed runs of `iters`
-    iterations (graph replays of the whole batch)."""
-    b = vmhip.Batch()
-    for f in fits:
-        a = math.sqrt(1.0 / len(f["sites"]))
-        b.add(1.0, 1.0, f["sites"], f["targets"], 1e-3, loss_k=K, grid_cell=scale * base_cell(K) * a, stage_rings=2,
-              chunk_extent=extent, targets_per_lane=tpl)
-    b.commit()
-    b.step(3)
-    ms = sorted(b.step(iters) for _ in range(repeats))[repeats // 2]
-    b.close()
-    return sum(len(f["sites"]) for f in fits) * iters / (ms / 1e3)
+def measure(fits, configs, iters, rounds=5):
+    """Site-iterations/s of the class batch under each configuration (K, cell scale, extent, targets per lane).
+    Every configuration's batch is built first; then `rounds` rounds time `iters` iterations (graph replays of
+    the whole batch) of each batch in turn, so drift of the clocks between configurations cancels, and each
+    configuration takes the median of its rounds."""
+    batches = []
+    for K, scale, extent, tpl in configs:
+        b = vmhip.Batch()
+        for f in fits:
+            a = math.sqrt(1.0 / len(f["sites"]))
+            b.add(1.0, 1.0, f["sites"], f["targets"], 1e-3, loss_k=K, grid_cell=scale * base_cell(K) * a,
+                  stage_rings=2, chunk_extent=extent, targets_per_lane=tpl)
+        b.commit()
+        b.step(3)
+        batches.append(b)
+    times = [[] for _ in batches]
+    for _ in range(rounds):
+        for i, b in enumerate(batches):
+            times[i].append(b.step(iters))
+    for b in batches:
+        b.close()
+    sites = sum(len(f["sites"]) for f in fits)
+    return [sites * iters / (sorted(t)[len(t) // 2] / 1e3) for t in times]
 
 
 def terms(sites_slots, sw, sh, cell, targets, K, radius):
@@ -128,40 +139,41 @@ def write_table(path, rows):
 
 def tune_class(c, args):
     """Coordinate search from the default (cell x1.0, extent 2.2, 3 targets per lane, k = 16); a candidate replaces
-    the current choice only when it is faster by more than MIN_GAIN (run-to-run noise is about 1 %)."""
+    the current choice only when it is faster by more than MIN_GAIN (run-to-run noise is about 1 %).  The
+    candidates of each stage are timed in interleaved rounds."""
     t0 = time.time()
     budget = max(args.budget, int(1.5 * 2 ** c))
     fits = make_class(c, budget, args.max_fits, seed=c)
     say = lambda m: print(f"class 2^{c} ({len(fits)} fits): {m}", flush=True)  # noqa: E731
-    rate = measure(fits, 16, 1.0, 2.2, 3, args.iters)
-    best = dict(scale=1.0, extent=2.2, tpl=3, rate=rate)
-    say(f"default cell x1.00, extent 2.2, tpl 3: {rate:.3e}")
-    for s in CELL_SCALES:
-        if s == 1.0:
-            continue
-        rate = measure(fits, 16, s, 2.2, 3, args.iters)
-        say(f"cell x{s:.2f}, extent 2.2, tpl 3: {rate:.3e}")
-        if rate > best["rate"] * (1 + MIN_GAIN):
-            best.update(scale=s, rate=rate)
-    for tpl in TARGETS_PER_LANE:
-        for e in CHUNK_EXTENTS:
-            if tpl == best["tpl"] and e == best["extent"]:
-                continue
-            rate = measure(fits, 16, best["scale"], e, tpl, args.iters)
-            say(f"cell x{best['scale']:.2f}, extent {e:.1f}, tpl {tpl}: {rate:.3e}")
-            if rate > best["rate"] * (1 + MIN_GAIN):
-                best.update(extent=e, tpl=tpl, rate=rate)
+    rates = measure(fits, [(16, sc, 2.2, 3) for sc in CELL_SCALES], args.iters)
+    rate1 = dict(zip(CELL_SCALES, rates))
+    best = dict(scale=1.0, extent=2.2, tpl=3, rate=rate1[1.0])
+    for sc in CELL_SCALES:
+        say(f"cell x{sc:.2f}, extent 2.2, tpl 3: {rate1[sc]:.3e}")
+        if rate1[sc] > best["rate"] * (1 + MIN_GAIN):
+            best.update(scale=sc, rate=rate1[sc])
+    shapes = [(tpl, e) for tpl in TARGETS_PER_LANE for e in CHUNK_EXTENTS]
+    rates = measure(fits, [(16, best["scale"], e, tpl) for tpl, e in shapes], args.iters)
+    rate2 = dict(zip(shapes, rates))
+    best.update(rate=rate2[(3, 2.2)])
+    for (tpl, e) in shapes:
+        say(f"cell x{best['scale']:.2f}, extent {e:.1f}, tpl {tpl}: {rate2[(tpl, e)]:.3e}")
+        if rate2[(tpl, e)] > best["rate"] * (1 + MIN_GAIN):
+            best.update(extent=e, tpl=tpl, rate=rate2[(tpl, e)])
     # Check fits: a few of the class (one for classes of a million sites and more: each pass holds its k-NN lists).
     check = fits[:min(len(fits), args.check_fits if c < 19 else 1)]
     agree = k_agreement(check, [K for K in K_CANDIDATES if K != 16])
-    best.update(k=16, rate_k=best["rate"])
+    agree[16] = True
+    ks = [K for K in K_CANDIDATES if agree[K]]
+    rates = dict(zip(ks, measure(fits, [(K, best["scale"], best["extent"], best["tpl"]) for K in ks], args.iters)))
     for K in K_CANDIDATES:
-        if K == 16:
-            continue
-        rate = measure(fits, K, best["scale"], best["extent"], best["tpl"], args.iters)
-        say(f"k {K}: {rate:.3e}, terms equal to k={K_REFERENCE}'s: {agree[K]}")
-        if agree[K] and rate > best["rate_k"] * (1 + MIN_GAIN):
-            best.update(k=K, rate_k=rate)
+        say(f"k {K}: {rates.get(K, float('nan')):.3e}, terms equal to k={K_REFERENCE}'s: {agree[K]}")
+    # The smallest agreeing k; a larger one replaces it only when faster by more than MIN_GAIN.
+    ks = sorted(rates)
+    best.update(k=ks[0], rate_k=rates[ks[0]])
+    for K in ks[1:]:
+        if rates[K] > best["rate_k"] * (1 + MIN_GAIN):
+            best.update(k=K, rate_k=rates[K])
     say(f"best cell x{best['scale']:.2f}, extent {best['extent']:.1f}, tpl {best['tpl']}, k {best['k']} "
         f"({time.time() - t0:.1f} s)")
     return best
@@ -171,10 +183,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
     ap.add_argument("--classes", default="8-20", help="measured size classes (log2 of the site count)")
-    ap.add_argument("--budget", type=int, default=600_000, help="sites per class")
-    ap.add_argument("--max-fits", type=int, default=512)
+    ap.add_argument("--budget", type=int, default=2_000_000, help="sites per class (enough to fill the GPU, as a mixed batch does)")
+    ap.add_argument("--max-fits", type=int, default=2048)
     ap.add_argument("--check-fits", type=int, default=3, help="fits per class for the k agreement check")
-    ap.add_argument("--iters", type=int, default=30)
+    ap.add_argument("--iters", type=int, default=100, help="iterations per timed run (about 30 ms of a class batch)")
     args = ap.parse_args()
     c0, c1 = (int(x) for x in args.classes.split("-"))
     rows = {c: tune_class(c, args) for c in range(c0, c1 + 1)}
