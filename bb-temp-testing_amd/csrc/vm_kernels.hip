@@ -603,7 +603,7 @@ constexpr uint32_t kBisectSpan  = 8u << 23; // eight binades
 // fma(ux, nx, fma(uy, ny, off)) with u = p - c_r.  Rows are kTblPad entries
 // longer than the columns, which puts the rows a wavefront reads at once in
 // different LDS banks.
-constexpr int kTblEntries = 160;
+constexpr int kTblEntries = 128;
 constexpr int kTblPad     = 2;
 
 template <class R> struct LossLds {
@@ -1057,7 +1057,7 @@ __device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float
 // k comes from each work item's instance record, so one launch serves fits of
 // every k.
 template <int TPL, class R, bool Diag>
-__global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(7))) void lossKernel(LossArgs<R> a)
+__global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8))) void lossKernel(LossArgs<R> a)
 {
     // Diagnostics (path counters, k-NN lists, per-target terms, exact-only
     // runs) exist only in the Diag instantiation.
