@@ -56,7 +56,8 @@ typedef struct vm_params {
     unsigned seed;              /* 1: subsampling hash seed */
     int      loss_k;            /* 12: neighbours per target (reference kLossK, voro_optimizer.cu:54) */
     float    grid_cell;         /* hash-grid cell in world units; 0: 2 x spacing */
-    int      stage_rings;       /* cell rings staged in LDS by the loss kernel (1 or 2); 0: auto */
+    int      stage_rings;       /* cell rings the loss kernel stages in LDS first (1 or 2; with 2, chunks whose
+                                   candidate disc two rings would likely miss start at 3); 0: auto */
     int      precision;         /* 0: FP32 sites/gradients/Adam; 1: all FP64 */
 } vm_params;
 
