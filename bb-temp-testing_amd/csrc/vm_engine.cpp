@@ -263,7 +263,7 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
 
 namespace {
 
-constexpr int kMaxLossGrid = 65536; // wavefronts (measured: beats both the resident count and one per chunk)
+constexpr int kMaxLossGrid = 131072; // wavefronts (measured on the batch: 0.4 % faster than 65,536, 2.7 % faster than one per chunk)
 constexpr int kMaxSiteGrid = 65536; // one site chunk per workgroup up to 16.7 M sites
 constexpr int kMaxScanGrid = 4096;
 
