@@ -122,7 +122,8 @@ void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff,
     in.maxIter = s.maxIter, in.tol = s.tol;
     in.gradShift = gradShiftFor(T, in.ext);
 
-    for (int b = 0; b < S; b += vmk::kSiteChunk) siteWork.push_back({id, b, std::min(vmk::kSiteChunk, S - b), 0});
+    for (int b = 0; b < S; b += vmk::kSiteChunk)
+        siteWork.push_back({id, b, std::min(vmk::kSiteChunk, S - b), 0, siteOff, cellOff});
 
     // Targets sorted by a Morton key over the grid at 1/16 of a cell (the bits
     // of the fine x and y coordinates interleaved), input order breaking ties.

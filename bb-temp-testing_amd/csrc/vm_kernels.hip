@@ -543,19 +543,22 @@ __global__ __launch_bounds__(kSiteChunk) void scatterKernel(const Inst *__restri
         ctl->next  = 0;
         ctl->epoch = ctl->epoch + 1;
     }
+    (void)insts; // the work items carry the offsets
     const int nWork = *nWorkPtr;
     for (int w = blockIdx.x; w < nWork; w += gridDim.x) {
-        const SiteWork wk = work[w];
-        const Inst    &in = insts[wk.inst];
-        if (mode == kModeIterate && state[wk.inst].done) continue;
-        int i = wk.begin + int(threadIdx.x);
+        const SiteWork wk = loadUniform(work + w);
+        const int      i  = wk.begin + int(threadIdx.x);
         if (i >= wk.begin + wk.count) continue;
-        int64_t   g  = in.siteOff + i;
-        const int ce = cellOf[g];
+        // The site loads go out with the instance's done flag.
+        const int64_t g  = wk.siteOff + i;
+        const int     ce = cellOf[g];
+        const int     rk = rank[g];
+        const V2<R>   p  = pos[g];
+        if (mode == kModeIterate && loadUniform(&state[wk.inst].done)) continue;
         if (ce < 0) continue; // empty slot
-        int s = start[in.cellOff + ce] + rank[g];
-        spos[in.siteOff + s] = pos[g];
-        sidx[in.siteOff + s] = i;
+        const int s = start[wk.cellOff + ce] + rk;
+        spos[wk.siteOff + s] = p;
+        sidx[wk.siteOff + s] = i;
     }
 }
 
@@ -1694,7 +1697,7 @@ __global__ __launch_bounds__(kSiteChunk) void adamBinKernel(AdamArgs<R> a)
         if (a.doAdam && loadUniform(&st->done)) continue;
         const int     i    = wk.begin + int(threadIdx.x);
         const bool    mine = i < wk.begin + wk.count;
-        const int64_t g    = in.siteOff + i;
+        const int64_t g    = wk.siteOff + i;
         // The site's state is loaded before the loss partials are folded, so
         // the two latencies overlap.
         V2<R>    p{}, mo{}, vo{};

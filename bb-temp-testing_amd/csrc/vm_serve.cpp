@@ -616,7 +616,7 @@ public:
             const int b = k * vmk::kScanTileCells;
             si.tiles.push_back({slot, b, int(std::min<int64_t>(vmk::kScanTileCells, p.cells - b)), k == 0});
         }
-        for (auto w : p.sw) si.sw.push_back({slot, w.begin, w.count, 0});
+        for (auto w : p.sw) si.sw.push_back({slot, w.begin, w.count, 0, so, co});
         for (auto w : p.tw) { // laid out at offset 0: move to the pool ranges
             w.inst = slot, w.tBegin += to, w.cellOff = co, w.siteOff = so;
             si.tw.push_back(w);

@@ -76,9 +76,14 @@ struct LossPartial {
     char               pad[112];
 };
 
+// Sites [begin, begin+count) of instance `inst`; its pooled site and cell
+// offsets are repeated from the record, so a workgroup can issue its site
+// loads as soon as it holds the work item.
 struct SiteWork {
-    int inst, begin, count, pad;
+    int     inst, begin, count, pad;
+    int64_t siteOff, cellOff;
 };
+static_assert(sizeof(SiteWork) == 32, "site work items are two 16-byte words");
 
 // Targets [tBegin, tBegin+count) of the pooled sorted target array, all of
 // instance `inst` and in its hash cell (cx, cy); every one lies within `rho`
