@@ -88,9 +88,10 @@ static int ceilLog2(double x)
 int gradShiftFor(int numTargets, float ext)
 {
     // A site collects at most T terms of magnitude ~ few * ext, so sums stay
-    // below 2^60 with this shift; clamped to [8, 52].
-    int g = 58 - ceilLog2(double(std::max(numTargets, 1))) - ceilLog2(double(std::max(ext, 1e-30f)));
-    return std::min(std::max(g, 8), 52);
+    // below 2^55 with this shift; clamped to [3, 47] (each scaled term is
+    // clamped to 2^51, the range of the kernels' one-addition rounding).
+    int g = 53 - ceilLog2(double(std::max(numTargets, 1))) - ceilLog2(double(std::max(ext, 1e-30f)));
+    return std::min(std::max(g, 3), 47);
 }
 
 void layoutInstance(const InstSpec &s, int id, int64_t siteOff, int64_t cellOff, int64_t targetOff, vmk::Inst &in,

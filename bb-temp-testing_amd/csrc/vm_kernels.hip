@@ -145,20 +145,30 @@ __device__ __forceinline__ double powi(double b, int e)
 }
 
 // Gradient accumulation.  FP32: a term c becomes round(c * 2^shift) (double
-// scaling is exact, rounding to nearest even), clamped to +-2^56, and is summed
+// scaling is exact, rounding to nearest even), clamped to +-2^51, and is summed
 // with 64-bit integer atomics; the sum is order-independent.  FP64: double atomics.
-// scale = 2^shift, shift in [8, 52].  The (finite) term is clamped to
-// +-2^(56 - shift) in FP32 before the exact scaling, which is the same as
-// clamping c * scale to +-2^56 after it.
-__device__ __forceinline__ float quantLimit(double scale) // 2^(56 - shift)
+// scale = 2^shift, shift in [3, 47].  The (finite) term is clamped to
+// +-2^(51 - shift) in FP32 before the exact scaling, which is the same as
+// clamping c * scale to +-2^51 after it.
+__device__ __forceinline__ float quantLimit(double scale) // 2^(51 - shift)
 {
     const int g = int(uint64_t(__double_as_longlong(scale)) >> 52) - 1023;
-    return __uint_as_float(uint32_t(127 + 56 - g) << 23);
+    return __uint_as_float(uint32_t(127 + 51 - g) << 23);
 }
+// The rounding to an integer is one FP64 addition: for |x| <= 2^51 the sum
+// x + 1.5 * 2^52 lies in [2^52, 2^53], where doubles are spaced 1 apart, so it
+// is 1.5 * 2^52 + round(x) (ties to even: 1.5 * 2^52 is even), and its bit
+// pattern minus that of 1.5 * 2^52 is round(x) in two's complement.  The
+// subtrahend's low word is zero, so only the high word changes.  x = c * scale
+// is exact, so one fma forms the same sum.
 __device__ __forceinline__ long long quantize(float c, double scale)
 {
-    const float lim = quantLimit(scale);
-    return __double2ll_rn(double(__builtin_amdgcn_fmed3f(c, -lim, lim)) * scale);
+    constexpr double   kMagic   = 0x1.8p52;
+    constexpr uint32_t kMagicHi = 0x43380000u; // high word of kMagic's bit pattern
+    const float        lim      = quantLimit(scale);
+    const double       t        = __builtin_fma(double(__builtin_amdgcn_fmed3f(c, -lim, lim)), scale, kMagic);
+    const uint64_t     b        = uint64_t(__double_as_longlong(t));
+    return (long long)((uint64_t(uint32_t(b >> 32) - kMagicHi) << 32) | (b & 0xFFFFFFFFull));
 }
 
 template <class R> struct GradOps;

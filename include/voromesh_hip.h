@@ -141,8 +141,8 @@ VM_API int vm_loss_pass(vm_handle *h, double *grad_xy, double *loss_sum, int64_t
  * layout with the same launches) and execs instantiated (first use, or launches of a new shape).  No
  * counterpart in the reference (it re-instantiates after every capacity change, voro_optimizer.cu:554,618). */
 VM_API int vm_graph_stats(vm_handle *h, int64_t *updates, int64_t *instantiations);
-/* FP32 gradient sums are fixed point: a term c is clamped to +-2^(56 - g), g the instance's shift
- * (58 - ceil(log2 T) - ceil(log2 max(W, H)), within [8, 52]), before it is scaled by 2^g and added.  This
+/* FP32 gradient sums are fixed point: a term c is clamped to +-2^(51 - g), g the instance's shift
+ * (53 - ceil(log2 T) - ceil(log2 max(W, H)), within [3, 47]), before it is scaled by 2^g and added.  This
  * returns how many target terms (since the optimizer was created) had a clamped component: 0 unless two
  * neighbouring sites nearly coincide.  FP64 sums are never clamped.  No counterpart in the reference (its FP32
  * float atomics have no clamp). */

@@ -265,17 +265,18 @@ inline int ceilLog2(double x)
 }
 
 // Fixed-point shift of the FP32 gradient sums: a site collects at most T
-// terms of magnitude ~ few * extent, so the sums stay below 2^60.
+// terms of magnitude ~ few * extent, so the sums stay below 2^55; each scaled
+// term is clamped to +-2^51.
 inline int gradShift(int liveTargets, float extent)
 {
-    int g = 58 - ceilLog2(double(std::max(liveTargets, 1))) - ceilLog2(double(std::max(extent, 1e-30f)));
-    return std::min(std::max(g, 8), 52);
+    int g = 53 - ceilLog2(double(std::max(liveTargets, 1))) - ceilLog2(double(std::max(extent, 1e-30f)));
+    return std::min(std::max(g, 3), 47);
 }
 
 inline int64_t quantizeTerm(float c, int shift)
 {
     double x = std::ldexp(double(c), shift);
-    x        = std::fmin(std::fmax(x, -72057594037927936.0), 72057594037927936.0); // +-2^56
+    x        = std::fmin(std::fmax(x, -2251799813685248.0), 2251799813685248.0); // +-2^51
     return std::llrint(x);
 }
 

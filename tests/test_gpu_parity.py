@@ -702,7 +702,7 @@ def test_loss_pass_matches_committed_fixture():
 
 @pytest.mark.parametrize("gap", [1e-3, 1e-6])
 def test_gradient_clamp_counted(gap):
-    """FP32 gradient sums are fixed point, each term clamped to 2^(56 - shift) first (shift 52 for 64 targets in
+    """FP32 gradient sums are fixed point, each term clamped to 2^(51 - shift) first (shift 47 for 64 targets in
     the unit square, so |term| <= 16).  A site pair `gap` apart with 64 targets on a ring of radius 0.03 around it:
     at 1e-3 no term reaches the clamp; at 1e-6 the pair's terms (about 2 d |u_perp| / gap) pass it.  The reported
     count equals the targets whose FP32 terms (the oracle's per-target terms, which the kernel computes bit for bit)
@@ -725,7 +725,7 @@ def test_gradient_clamp_counted(gap):
     sat = opt.grad_saturation()
     opt.close()
     ref = O.loss_grad(sites, targets, K, W, H, 0.1, per_target=True)
-    expect = int((np.abs(ref["term_grad"]).max(axis=1) > 2.0 ** (56 - 52)).sum())
+    expect = int((np.abs(ref["term_grad"]).max(axis=1) > 2.0 ** (51 - 47)).sum())
     print(f"case: gap {gap}: {sat} clamped targets (oracle terms: {expect})", flush=True)
     assert sat == expect
     assert (sat > 0) == (gap < 1e-5)
