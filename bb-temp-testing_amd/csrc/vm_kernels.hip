@@ -996,13 +996,16 @@ template <class R> __device__ __forceinline__ R waveMinR(R v)
     return -waveMaxR<R>(-v);
 }
 
-// The LDS gradient of a wavefront holds `copies` (1, 2 or 4) copies of
-// kLossStage / copies slots; lane l adds into copy l % copies, so lanes whose
-// terms touch the same site collide less.  Copy c is rotated by c slots, which
-// puts the copies of one slot in different LDS banks.  The flush sums them.
-__device__ __forceinline__ int gradSlot(int s, int c, int M) // M = kLossStage / copies
+// The LDS gradient of a wavefront holds `copies` (1, 2 or 4) interleaved
+// copies of the accumulators of its staged sites: slot s * copies + c, 16
+// bytes (x, y) each.  Lane l adds into copy l % copies, so lanes whose terms
+// touch the same site collide less, and the copies of one site lie in
+// different LDS banks.  The flush sums them.  A lane's accumulator of site s is
+// one shift-add from its copy's base: gradAt(base, s, shift), shift =
+// log2(copies) + 4.
+template <class T> __device__ __forceinline__ T *gradAt(T *laneBase, int s, int shift)
 {
-    return c * M + ((s + c) & (M - 1));
+    return reinterpret_cast<T *>(reinterpret_cast<char *>(laneBase) + (s << shift));
 }
 
 // Number of set bits of `mask` below this lane.
@@ -1020,8 +1023,8 @@ __device__ __forceinline__ int laneRank(uint64_t mask)
 // accumulators.  Returns the winning lane (-1: every pair degenerate); that
 // lane's outputs are the term's loss, the nearest site's and the partner's
 // instance indices and whether a gradient component was clamped.
-__device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float> pt, int lane, int K, int gcopies, int gM,
-                                            bool grad, double gscale, float &lossT, int &i0, int &j, int &sat)
+__device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float> pt, int lane, int K, long long *gLane,
+                                            int gShift, bool grad, double gscale, float &lossT, int &i0, int &j, int &sat)
 {
     for (int s = lane; s < m; s += 64)
         lds.u.x.xkey[s] = Key<float>::make(dist2(lds.pos[s], pt), (uint32_t(lds.idx[s]) << kLdsSlotBits) | uint32_t(s)).k;
@@ -1057,15 +1060,13 @@ __device__ __forceinline__ int exactInChunk(LossLds<float> &lds, int m, V2<float
     const int      wl   = win ? __builtin_ctzll(win) : -1;
     if (lane == wl) {
         lossT = best, i0 = lds.idx[s0], j = lds.idx[sj];
-        sat   = grad ? lossGrad(u, ev, 1.0f / l, d, &lds.grad[2 * gradSlot(sj, lane & (gcopies - 1), gM)],
-                              &lds.grad[2 * gradSlot(s0, lane & (gcopies - 1), gM)], gscale)
-                     : 0;
+        sat   = grad ? lossGrad(u, ev, 1.0f / l, d, gradAt(gLane, sj, gShift), gradAt(gLane, s0, gShift), gscale) : 0;
     }
     waveSync();
     return wl;
 }
 
-// At most 72 VGPRs, so 7 wavefronts share a SIMD: the kernel is bound by issue
+// At most 64 VGPRs, so 8 wavefronts share a SIMD: the kernel is bound by issue
 // and by the latency of its per-chunk loads, and occupancy hides the latter.
 // k comes from each work item's instance record, so one launch serves fits of
 // every k.
@@ -1154,7 +1155,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
 
         bool         fast  = false;
         int          m     = 0; // compacted candidates (fast path)
-        int          gcopies = 1, gM = kLossStage; // gradient copies, slots per copy
+        int          gcopies = 1, gShift = 4; // gradient copies, log2(copies) + 4
         int          total = 0; // sites of the staged block
         ScreenBounds sb{};
         if constexpr (sizeof(R) == 4) {
@@ -1288,7 +1289,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
                 if (lane < mp - m) lds.pos[m + lane] = V2<R>{R(kFarSite), R(kFarSite)};
                 // Clear the gradient copies.
                 gcopies = m <= kLossStage / 4 ? 4 : (m <= kLossStage / 2 ? 2 : 1);
-                gM      = m <= kLossStage / 4 ? kLossStage / 4 : (m <= kLossStage / 2 ? kLossStage / 2 : kLossStage);
+                gShift  = m <= kLossStage / 4 ? 6 : (m <= kLossStage / 2 ? 5 : 4);
                 if (a.grad) {
 #pragma unroll
                     for (int i = lane; i < 2 * kLossStage; i += 64) lds.grad[i] = GradT<R>(0);
@@ -1346,6 +1347,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
         int      openCnt = 0;
         int      sat     = 0; // targets whose gradient terms were clamped
         uint64_t openM[TPL]; // open targets of each pass (exact selection inside the chunk)
+        GradT<R> *const gLane = lds.grad + 2 * (lane & (gcopies - 1)); // this lane's gradient copy
 #pragma unroll
         for (int h = 0; h < TPL; ++h) {
             openM[h] = 0;
@@ -1374,8 +1376,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
             const R lossT = tr.used ? tr.d * tr.d : R(0);
             if (tr.used && a.grad)
                 sat += lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.rl, tr.d,
-                         &lds.grad[2 * gradSlot(tr.slj, lane & (gcopies - 1), gM)],
-                         &lds.grad[2 * gradSlot(tr.sl0, lane & (gcopies - 1), gM)], __builtin_ldexp(1.0, in.gradShift));
+                         gradAt(gLane, tr.slj, gShift), gradAt(gLane, tr.sl0, gShift), __builtin_ldexp(1.0, in.gradShift));
             if (dbgKnn && wk.inst == a.dbgInst && lane + 64 * h < wk.count && !take[h]) {
                 for (int k = 0; k < K; ++k) dbgKnn[(t - in.targetOff) * K + k] = -1;
             }
@@ -1402,7 +1403,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
                                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p[h].y), l))};
                     float     xl = 0.0f;
                     int       xi0 = -1, xj = -1, xs = 0;
-                    const int wl = exactInChunk(lds, m, pt, lane, K, gcopies, gM, a.grad != nullptr,
+                    const int wl = exactInChunk(lds, m, pt, lane, K, gLane, gShift, a.grad != nullptr,
                                                    __builtin_ldexp(1.0, in.gradShift), xl, xi0, xj, xs);
                     if (lane == wl) lossSum += double(xl), used += 1, sat += xs;
                     if (dbgTerm && lane == (wl >= 0 ? wl : 0)) {
@@ -1448,10 +1449,8 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
             waveSync();
             for (int s = lane; s < m; s += 64) {
                 GradT<R> gx = GradT<R>(0), gy = GradT<R>(0);
-                for (int c = 0; c < gcopies; ++c) {
-                    const int g = gradSlot(s, c, gM);
-                    gx += lds.grad[2 * g], gy += lds.grad[2 * g + 1];
-                }
+                const GradT<R> *gs = gradAt(lds.grad, s, gShift);
+                for (int c = 0; c < gcopies; ++c) gx += gs[2 * c], gy += gs[2 * c + 1];
                 if (gx != GradT<R>(0) || gy != GradT<R>(0)) {
                     const int i = lds.idx[s];
                     GradOps<R>::addRaw(&a.grad[2 * (so + i)], gx);
