@@ -1079,6 +1079,11 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
     int *const                dbgKnn    = Diag ? a.dbgKnn : nullptr;
     TermDbg *const            dbgTerm   = Diag ? a.dbgTerm : nullptr;
     const bool                exactOnly = Diag && a.exactOnly;
+    // The plain instantiation runs optimizer iterations with a gradient only
+    // (the launcher sends evaluation passes to the other one), so these are
+    // constants there.
+    const bool iterate  = !Diag || a.mode == kModeIterate;
+    const bool wantGrad = !Diag || a.grad != nullptr;
     __shared__ LossLds<R> ldsAll[kLossWaves];
     const int  lane  = threadIdx.x & 63;
     // The wavefront's index is uniform: read it into a scalar register so that
@@ -1135,16 +1140,16 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
         const Inst in = loadUniform(a.insts + wk.inst);
         const int  K  = in.K; // neighbours of the instance's terms
         InstState *st = a.state + wk.inst;
-        if (a.mode == kModeIterate && loadUniform(&st->done)) continue;
+        if (iterate && loadUniform(&st->done)) continue;
         bool anyTake = false;
 #pragma unroll
         for (int h = 0; h < TPL; ++h) {
             const bool valid = lane + 64 * h < wk.count;
             if (!valid) p[h] = V2<R>{R(-1), R(-1)};
             take[h] = valid && p[h].x >= R(0);
-            if (take[h] && in.useBox && a.mode == kModeIterate)
+            if (take[h] && in.useBox && iterate)
                 take[h] = p[h].x >= R(in.bx0) && p[h].x <= R(in.bx1) && p[h].y >= R(in.by0) && p[h].y <= R(in.by1);
-            if (take[h] && a.mode == kModeIterate && in.fraction < 1.0f)
+            if (take[h] && iterate && in.fraction < 1.0f)
                 take[h] = rand01_loss(tsl[h], uint32_t(loadUniform(&st->iter)), in.seed) < in.fraction;
             anyTake = anyTake || take[h];
         }
@@ -1290,7 +1295,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
                 // Clear the gradient copies.
                 gcopies = m <= kLossStage / 4 ? 4 : (m <= kLossStage / 2 ? 2 : 1);
                 gShift  = m <= kLossStage / 4 ? 6 : (m <= kLossStage / 2 ? 5 : 4);
-                if (a.grad) {
+                if (wantGrad) {
 #pragma unroll
                     for (int i = lane; i < 2 * kLossStage; i += 64) lds.grad[i] = GradT<R>(0);
                 }
@@ -1374,7 +1379,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
             }
             openCnt += __popcll(__ballot(open));
             const R lossT = tr.used ? tr.d * tr.d : R(0);
-            if (tr.used && a.grad)
+            if (tr.used && wantGrad)
                 sat += lossGrad(V2<R>{p[h].x - tr.c0.x, p[h].y - tr.c0.y}, tr.ev, tr.rl, tr.d,
                          gradAt(gLane, tr.slj, gShift), gradAt(gLane, tr.sl0, gShift), __builtin_ldexp(1.0, in.gradShift));
             if (dbgKnn && wk.inst == a.dbgInst && lane + 64 * h < wk.count && !take[h]) {
@@ -1403,7 +1408,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
                                        __int_as_float(__builtin_amdgcn_readlane(__float_as_int(p[h].y), l))};
                     float     xl = 0.0f;
                     int       xi0 = -1, xj = -1, xs = 0;
-                    const int wl = exactInChunk(lds, m, pt, lane, K, gLane, gShift, a.grad != nullptr,
+                    const int wl = exactInChunk(lds, m, pt, lane, K, gLane, gShift, wantGrad,
                                                    __builtin_ldexp(1.0, in.gradShift), xl, xi0, xj, xs);
                     if (lane == wl) lossSum += double(xl), used += 1, sat += xs;
                     if (dbgTerm && lane == (wl >= 0 ? wl : 0)) {
@@ -1434,7 +1439,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
         const double ls = waveSumDouble(lossSum);
         const int    uc = waveReduce32<int>(used, 0, [](int x, int y) { return x + y; });
         if (lane == 0 && uc > 0) {
-            if (a.mode == kModeIterate) {
+            if (iterate) {
                 LossPartial *pp = a.partial + int64_t(wk.inst) * kLossPartials + (blockIdx.x & (kLossPartials - 1));
                 unsafeAtomicAdd(&pp->loss, ls);
                 atomicAdd(&pp->count, (unsigned long long)uc);
@@ -1445,7 +1450,7 @@ __global__ __launch_bounds__(kLossThreads) __attribute__((amdgpu_waves_per_eu(8)
         }
 
         // ---- flush the LDS gradient: one atomic per touched site ----------
-        if (fast && a.grad) {
+        if (fast && wantGrad) {
             waveSync();
             for (int s = lane; s < m; s += 64) {
                 GradT<R> gx = GradT<R>(0), gy = GradT<R>(0);
@@ -2326,7 +2331,9 @@ hipError_t launchScatter(int grid, hipStream_t s, const Inst *insts, const InstS
 
 template <class R> hipError_t launchLoss(int tpl, int grid, hipStream_t s, const LossArgs<R> &a)
 {
-    const bool diag   = a.stats || a.dbgKnn || a.dbgTerm || a.exactOnly;
+    // Evaluation passes and passes without a gradient take the instantiation
+    // that also serves the diagnostics.
+    const bool diag   = a.stats || a.dbgKnn || a.dbgTerm || a.exactOnly || a.mode != kModeIterate || !a.grad;
     const int  blocks = ((grid + kLossWaves - 1) / kLossWaves + 2 * kXcdRun - 1) / (2 * kXcdRun) * (2 * kXcdRun);
     if (!lossTplSupported(tpl, sizeof(R) == 8)) return hipErrorInvalidValue;
 #define VM_T(t)                                                                                                   \
