@@ -557,7 +557,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="arrival window of the serving run (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-cap", type=float, default=8e6, help="resident-site cap of the serving loop")
-    ap.add_argument("--serve-sweep", default="500,650,750,800,850,900", help="arrival rates of the sustainable-rate sweep")
+    ap.add_argument("--serve-sweep", default="500,700,800,900,950,1000", help="arrival rates of the sustainable-rate sweep")
     ap.add_argument("--serve-sweep-seconds", type=float, default=14.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the large/small/precision/local-edit sections")
     ap.add_argument("--only", choices=["large", "small", "precision", "local_edit", "batch", "serving"], default=None,
